@@ -1,0 +1,204 @@
+/* qdfn — MI355X (gfx950) earthquake-cycle hot path of Quake-DFN behind a C ABI.
+ *
+ * libqdfn_hip.so exports the functions below.  Every pointer argument is a
+ * HOST pointer read or written during the call only, unless stated otherwise;
+ * matrices are column-major (Julia layout) with an explicit leading
+ * dimension.  No function throws or aborts across the ABI: each returns
+ * QDFN_OK (0) or a negative QDFN_ERR_* code and leaves a thread-local
+ * message for qdfn_last_error().  Handles are not thread-safe; each owns one
+ * HIP stream and calls on it are synchronous.
+ *
+ * Reference = limkjae/Quake-DFN (Julia).  Each declaration cites the
+ * reference interface it replaces.
+ */
+#ifndef QDFN_H
+#define QDFN_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QDFN_OK 0
+#define QDFN_ERR_ARG (-1)
+#define QDFN_ERR_HIP (-2)
+#define QDFN_ERR_NOMEM (-3)
+#define QDFN_ERR_STATE (-4)
+/* more than 100 retries of one step: the reference blocks in readline()
+ * (scripts/Functions_RSFDFN3DMain_D.jl:255-258, _H.jl same loop) */
+#define QDFN_ERR_RETRIES (-10)
+/* empty candidate set for min |V/Accel|: Julia's minimum() of an empty
+ * collection throws (Functions_RSFDFN3DMain_D.jl:165-168) */
+#define QDFN_ERR_EMPTY_DTMIN (-11)
+
+const char* qdfn_last_error(void);
+
+/* Device name, compute-unit count and HBM bytes of the current HIP device. */
+int qdfn_device_info(char* name, int64_t name_len, int32_t* cu_count, int64_t* hbm_bytes);
+
+/* ---- Okada rectangles -------------------------------------------------- */
+
+/* Replaces Okada_DC3D_Vector(ALPHA,X,Y,Z,DEPTH,DIP,AL1,AL2,AW1,AW2,DISL1,DISL2,DISL3)
+ * scripts/Functions_OKADA3D.jl:464.  U is n x 12 column-major (columns
+ * UX,UY,UZ,UXX,UYX,UZX,UXY,UYY,UZY,UXZ,UYZ,UZZ); iret[n] as IRET. */
+int qdfn_okada_dc3d(double alpha, int64_t n, const double* X, const double* Y, const double* Z, double depth,
+                    double dip, double al1, double al2, double aw1, double aw2, double disl1, double disl2,
+                    double disl3, double* U, double* iret);
+
+/* Replaces StiffnessMatrix_ByParts_Calculation_Rec(Input_SegmentSource,
+ * Input_SegmentReceiver, ShearModulus, PoissonRatio, ...) and
+ * BuildMatrixByParts_Even_Rec, scripts/Functions_BuildInputFile.jl:386,516.
+ * Segment tables use the reference's Input_Segment columns (0-based here):
+ * 0-2 centre (z = +depth), 3 strike length, 4 dip length, 5 strike deg,
+ * 6 dip deg, 7 rake deg (further columns ignored).  KS/KN are nr x ns. */
+int qdfn_okada_stiffness_rect(const double* src_seg, int64_t ns, int64_t ld_src, const double* rcv_seg,
+                              int64_t nr, int64_t ld_rcv, double mu, double nu, double* KS, double* KN,
+                              int64_t ldk);
+
+/* ---- triangular dislocations (Nikkhoo & Walter 2015, half-space) -------- */
+
+/* Replaces TDstressHS(X,Y,Z,P1,P2,P3,Ss,Ds,Ts,mu,lambda),
+ * scripts/Functions_TDstressHS.jl:2.  Stress and Strain are n x 6
+ * column-major (xx,yy,zz,xy,xz,yz).  P1..P3 are 3-vectors (z <= 0). */
+int qdfn_tde_stress_hs(int64_t n, const double* X, const double* Y, const double* Z, const double* P1,
+                       const double* P2, const double* P3, double Ss, double Ds, double Ts, double mu,
+                       double lambda, double* Stress, double* Strain);
+
+/* Replaces StiffnessMatrix_ByParts_Calculation_Tri(P1,P2,P3,Rake,FaultCenter,
+ * ShearModulus,lambda,UnitVector_Normal,UnitVector_Slip) and
+ * BuildMatrixByParts_Even_Tri, scripts/Functions_BuildInputFile.jl:551,599.
+ * P1..P3, ctr, nrm, slip are n x 3 column-major, already processed by
+ * RotVerts_UnitVectors (qdfn_rotverts_unitvectors). */
+int qdfn_tde_stiffness_tri(int64_t ns, const double* P1, const double* P2, const double* P3, const double* rake,
+                           int64_t nr, const double* ctr, const double* nrm, const double* slip, double mu,
+                           double lambda, double* KS, double* KN, int64_t ldk);
+
+/* Replaces RotVerts_UnitVectors(Input_Segment, FaultCount, Rake),
+ * scripts/Functions_BuildInputFile.jl:338.  P1..P3 (n x 3) are reordered in
+ * place; outputs n x 3.  Returns the number of reversed triangles (>= 0). */
+int qdfn_rotverts_unitvectors(int64_t n, double* P1, double* P2, double* P3, const double* rake, double* nrm,
+                              double* strike, double* dip, double* slip);
+
+/* ---- dense stiffness operator (device resident) ------------------------- */
+
+#define QDFN_PREC_FP64 0 /* FP64 storage, FP64 accumulation */
+#define QDFN_PREC_FP32 1 /* FP32 storage, FP64 accumulation */
+
+typedef struct qdfn_dense qdfn_dense_t;
+
+/* Upload K_S, K_N (n x n, leading dimension ld) and build the operator of
+ * scripts/Functions_RSFDFN3DMain_D.jl:54-63: E_ij = K_S,ij/K_S,ii (i != j),
+ * E_ii = 0, K_Self = |K_S,ii|, stored at precision prec. */
+qdfn_dense_t* qdfn_dense_create(const double* KS, const double* KN, int64_t n, int64_t ld, int32_t prec);
+/* Assemble the stiffness of n rectangular segments on the device (no host
+ * round trip) and build the operator as above. */
+qdfn_dense_t* qdfn_dense_create_rect(const double* seg, int64_t n, int64_t ld_seg, double mu, double nu,
+                                     int32_t prec);
+/* Assemble the stiffness of n triangles (inputs as qdfn_tde_stiffness_tri). */
+qdfn_dense_t* qdfn_dense_create_tri(int64_t n, const double* P1, const double* P2, const double* P3,
+                                    const double* rake, const double* ctr, const double* nrm, const double* slip,
+                                    double mu, double lambda, int32_t prec);
+void qdfn_dense_destroy(qdfn_dense_t* op);
+/* K_Self (n) as the reference computes it (|K_S,ii|); diag = signed K_S,ii. */
+int qdfn_dense_kself(qdfn_dense_t* op, double* kself, double* diag);
+/* 1 if every K_N entry is zero (the reference's NormalStiffnessZero). */
+int qdfn_dense_normal_zero(qdfn_dense_t* op);
+
+/* Coupling product (Functions_RSFDFN3DMain_D.jl:199-202).  loading_form=1:
+ * yS = E*(0-x) (Elastic_Load_Disp), yN = K_N*x.  loading_form=0: yS = K_S*x,
+ * yN = K_N*x. */
+int qdfn_matvec(qdfn_dense_t* op, const double* x, double* yS, double* yN, int32_t loading_form);
+
+/* Average duration (ms, HIP events on the operator's stream) of `reps`
+ * launches of the fused coupling kernel. */
+int qdfn_dense_time_coupling(qdfn_dense_t* op, int32_t reps, double* avg_ms);
+
+/* ---- rate-and-state local solve ---------------------------------------- */
+
+typedef struct {
+    double conv_crit; /* ConvergenceCrit, 1e-9 in the reference */
+    double v0;        /* V0, 1e-9 */
+    double mu, rho, switch_v, alpha_evo;
+    int32_t evolution_dr; /* 1: aging (Dieterich) law, else Ruina */
+    int32_t pad_;
+} qdfn_step_params;
+
+/* Replaces SolveOneTimeStep(...), scripts/Functions_Solvers.jl:334-380, with
+ * Solver_HighV_D (:26) / Solver_LowV_D (:185) / EvolutionLaw (:2).  Inputs
+ * (length n): DispOld, FrictionOld, ThetaOld, VOld, Friction0, a, b, Dc,
+ * K_Self, Omega, Mass, EffNormalStress_i, Total_Loading_Disp, SolverSwitch,
+ * LoadingRate, D_EffStress_Shear, EffNormalStress_Old, InitialShearStress and
+ * the scalar Dt.  Outputs (length n): V, Friction, Disp, Theta,
+ * EffNormalStress, Dt_All, InstabilityThistime, FLAG_GoodToGo; iters
+ * (optional) = secant iterations used.  The last n_loading rows are loading
+ * faults.  Returns the number of readline() stalls of Solver_HighV_D (>= 0). */
+int64_t qdfn_solve_one_timestep(const qdfn_step_params* p, int64_t n, int64_t n_loading, const double* DispOld,
+                                const double* FrictionOld, const double* ThetaOld, const double* VOld,
+                                const double* Friction0, const double* a, const double* b, const double* Dc,
+                                const double* K_Self, double Dt, const double* Omega, const double* Mass,
+                                const double* EffNormalStress_i, const double* Total_Loading_Disp,
+                                const int32_t* SolverSwitch, const double* LoadingRate,
+                                const double* D_EffStress_Shear, const double* EffNormalStress_Old,
+                                const double* InitialShearStress, double* V, double* Friction, double* Disp,
+                                double* Theta, double* EffNormalStress, double* Dt_All,
+                                int32_t* InstabilityThistime, int32_t* FLAG_GoodToGo, int32_t* iters);
+
+/* ---- device-resident earthquake-cycle run (main / main_H) --------------- */
+
+typedef struct {
+    int64_t n, n_loading;
+    double mu, rho;
+    double switch_v, dt_cut, rupture_dt, maximum_dt, min_normal_stress, alpha_evo;
+    int32_t evolution_dr;
+    int32_t planar;       /* NormalStiffnessZero: skip the K_N product */
+    int32_t ext_at_told;  /* 0: external stress at TOld+Dt (main), 1: at TOld (main_H) */
+    int32_t pad_;
+    int64_t record_step;  /* RecordStep; histories every record_step accepted steps */
+    int64_t record_capacity; /* rows of history kept on the device (0: none) */
+    int64_t trace_capacity;  /* per-step (TOld, Dt, max V) entries kept (0: none) */
+} qdfn_sim_params;
+
+typedef struct qdfn_sim qdfn_sim_t;
+
+typedef struct {
+    int64_t steps, rhs_evals, retry_passes, stalls, n_recorded, n_traced;
+    int32_t status; /* QDFN_OK or QDFN_ERR_RETRIES / QDFN_ERR_EMPTY_DTMIN */
+    int32_t pad_;
+    double t, dt;   /* simulated time after the last accepted step, last Dt */
+    double coupling_ms_avg; /* mean coupling-kernel duration (timed runs) */
+    int64_t coupling_timed; /* launches averaged */
+} qdfn_sim_stats;
+
+/* Create a run over a dense operator (which must outlive the run).  Element
+ * arrays (length n): Mass, a, b, Dc, Theta_i, V_i, Friction_i,
+ * InitialNormalStress, LoadingRate (V_Const).  Mirrors the arguments of
+ * main(), scripts/Functions_RSFDFN3DMain_D.jl:2. */
+qdfn_sim_t* qdfn_sim_create_dense(qdfn_dense_t* op, const qdfn_sim_params* p, const double* mass, const double* a,
+                                  const double* b, const double* dc, const double* theta_i, const double* v_i,
+                                  const double* friction_i, const double* sigma0, const double* loading_rate);
+/* External stress tables (Input_ExternalStressChange.jld2; InterpolateFromStressChange,
+ * Functions_Solvers.jl:465): times[nt]; ext_n/ext_s are nt x n column-major. */
+int qdfn_sim_set_external(qdfn_sim_t* sim, int64_t nt, const double* times, const double* ext_n,
+                          const double* ext_s);
+/* Advance n_steps accepted steps (the reference's TotalStep loop body).
+ * mode 0: attempts replayed from a captured HIP graph (graph_attempts per
+ * replay); mode 1: direct launches with HIP events around every coupling
+ * kernel (fills coupling_ms_avg).  Returns QDFN_OK or the run's status. */
+int qdfn_sim_run(qdfn_sim_t* sim, int64_t n_steps, int32_t mode, int32_t graph_attempts);
+int qdfn_sim_get_stats(qdfn_sim_t* sim, qdfn_sim_stats* st);
+/* Current state (after the last accepted step): V, Disp, Theta, Friction,
+ * EffNormalStress, Accel (each length n, any may be NULL). */
+int qdfn_sim_get_state(qdfn_sim_t* sim, double* V, double* Disp, double* Theta, double* Friction, double* Sigma,
+                       double* Accel);
+/* Histories recorded so far: time/dt [count], V/Disp/Theta/Sigma count x n
+ * column-major (History_* of the reference, rows = records). */
+int qdfn_sim_get_history(qdfn_sim_t* sim, int64_t count, double* time, double* dt, double* V, double* Disp,
+                         double* Theta, double* Sigma);
+/* Per-step trace: TOld, Dt and max V over non-loading elements. */
+int qdfn_sim_get_trace(qdfn_sim_t* sim, int64_t count, double* told, double* dt, double* vmax);
+void qdfn_sim_destroy(qdfn_sim_t* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,325 @@
+"""ORACLE — test infrastructure only.
+
+ctypes front end of liboracle.so, the CPU FP64 restatement of the reference
+(limkjae/Quake-DFN).  Only tests/, ``__graft_entry__.smoke()`` and the CPU
+baseline leg of bench.py import this module, and only as the checker; the
+product package never does.
+
+Function names follow the reference functions they restate:
+  dc3d                  Okada_DC3D_Vector          scripts/Functions_OKADA3D.jl:464-654
+  stiffness_rect        StiffnessMatrix_ByParts_Calculation_Rec  Functions_BuildInputFile.jl:386-512
+  tdstress_hs           TDstressHS                 scripts/Functions_TDstressHS.jl:2-130
+  rotverts_unitvectors  RotVerts_UnitVectors       Functions_BuildInputFile.jl:338-377
+  stiffness_tri         StiffnessMatrix_ByParts_Calculation_Tri  Functions_BuildInputFile.jl:551-594
+  evolution_law / solver_highv / solver_lowv / solve_one_timestep
+                        Functions_Solvers.jl:2-380
+  interp_external       InterpolateFromStressChange Functions_Solvers.jl:465-540
+  sim_dense / sim_hmat  main / main_H              Functions_RSFDFN3DMain_{D,H}.jl
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+D = C.c_double
+I64 = C.c_int64
+I32 = C.c_int32
+PD = C.POINTER(C.c_double)
+PI32 = C.POINTER(C.c_int32)
+PI64 = C.POINTER(C.c_int64)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        _declare(_LIB)
+    return _LIB
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(PD)
+
+
+def _pi32(a):
+    return a.ctypes.data_as(PI32)
+
+
+class ElemIn(C.Structure):
+    _fields_ = [(n, D) for n in ("conv_crit disp_old friction_old theta_old v_old friction0 a b dc v0 "
+                                 "k_self omega mass mu sigma tl rho dtau switch_v alpha_evo sigma_old "
+                                 "tau0").split()] + [("evolution_dr", C.c_int)]
+
+
+class ElemOut(C.Structure):
+    _fields_ = [(n, D) for n in "v friction disp theta sigma dt".split()] + \
+               [(n, C.c_int) for n in "instab flag iters stall".split()]
+
+
+class StepParams(C.Structure):
+    _fields_ = [(n, D) for n in "conv_crit v0 mu rho switch_v alpha_evo".split()] + \
+               [("evolution_dr", C.c_int)]
+
+
+class DenseOp(C.Structure):
+    _fields_ = [("n", I64), ("E", PD), ("KN", PD)]
+
+
+class SimIn(C.Structure):
+    _fields_ = ([(n, I64) for n in "n n_loading total_step record_step n_record".split()] +
+                [(n, PD) for n in "mass a b dc theta_i v_i friction_i sigma0 loading_rate k_self".split()] +
+                [(n, D) for n in ("mu rho switch_v dt_cut rupture_dt maximum_dt min_normal_stress "
+                                  "alpha_evo").split()] +
+                [(n, C.c_int) for n in "evolution_dr planar ext_at_told".split()] +
+                [("ext_nt", I64)] +
+                [(n, PD) for n in "ext_t ext_n ext_s ext_p".split()] +
+                [(n, PD) for n in "hist_time hist_dt hist_v hist_disp hist_theta hist_sigma".split()] +
+                [(n, PD) for n in "trace_told trace_dt trace_vmax".split()])
+
+
+class SimOut(C.Structure):
+    _fields_ = ([(n, I64) for n in "rhs_evals retry_passes stalls steps_done n_recorded".split()] +
+                [(n, D) for n in "t_end dt_last".split()] + [("final_state", PD)])
+
+
+class HmatOp(C.Structure):
+    _fields_ = [("n", I64), ("nblk", I64), ("range", PI64), ("kind", PI64), ("rank", PI64),
+                ("a", C.POINTER(PD)), ("b", C.POINTER(PD))]
+
+
+def _declare(L):
+    L.orc_dc3d.argtypes = [D, I64, PD, PD, PD, D, D, D, D, D, D, D, D, D, PD, PD]
+    L.orc_stiffness_rect.argtypes = [PD, I64, I64, PD, I64, I64, D, D, PD, PD, I64]
+    L.orc_evolution_law.argtypes = [D, D, D, D, D, D, D, D, C.c_int, C.c_int]
+    L.orc_evolution_law.restype = D
+    L.orc_solver_highv.argtypes = [C.POINTER(ElemIn), D, C.POINTER(ElemOut)]
+    L.orc_solver_lowv.argtypes = [C.POINTER(ElemIn), D, C.POINTER(ElemOut)]
+    L.orc_solve_one_timestep.argtypes = ([C.POINTER(StepParams), I64, I64] + [PD] * 9 + [D] +
+                                         [PD] * 4 + [PI32] + [PD] * 4 + [PD] * 6 + [PI32] * 3)
+    L.orc_solve_one_timestep.restype = I64
+    L.orc_interp_external.argtypes = [D, I64, I64, PD, PD, PD, PD, PD, PD, PD]
+    L.orc_dense_prepare.argtypes = [I64, PD, PD, PD]
+    L.orc_dense_coupling.argtypes = [C.POINTER(DenseOp), PD, PD, PD, C.c_int]
+    L.orc_sim_dense.argtypes = [C.POINTER(DenseOp), C.POINTER(SimIn), C.POINTER(SimOut)]
+    L.orc_sim_dense.restype = C.c_int
+    if hasattr(L, "orc_tdstress_hs"):
+        L.orc_tdstress_hs.argtypes = [I64, PD, PD, PD, PD, PD, PD, D, D, D, D, D, PD, PD]
+        L.orc_rotverts_unitvectors.argtypes = [I64, PD, PD, PD, PD, PD, PD, PD, PD]
+        L.orc_stiffness_tri.argtypes = [I64, PD, PD, PD, PD, I64, PD, PD, PD, D, D, PD, PD, I64]
+    if hasattr(L, "orc_hmat_matvec"):
+        L.orc_hmat_matvec.argtypes = [C.POINTER(HmatOp), PD, PD]
+        L.orc_sim_hmat.argtypes = [C.POINTER(HmatOp), C.POINTER(HmatOp), C.POINTER(SimIn),
+                                   C.POINTER(SimOut)]
+        L.orc_sim_hmat.restype = C.c_int
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def dc3d(alpha, X, Y, Z, depth, dip, al1, al2, aw1, aw2, disl1, disl2, disl3):
+    """Okada_DC3D_Vector: returns (U[n,12], IRET[n]); U columns are
+    UX,UY,UZ,UXX,UYX,UZX,UXY,UYY,UZY,UXZ,UYZ,UZZ."""
+    X, Y, Z = _f64(X).ravel(), _f64(Y).ravel(), _f64(Z).ravel()
+    n = X.size
+    U = np.zeros((12, n))
+    iret = np.zeros(n)
+    lib().orc_dc3d(alpha, n, _p(X), _p(Y), _p(Z), depth, dip, al1, al2, aw1, aw2,
+                   disl1, disl2, disl3, _p(U), _p(iret))
+    return U.T.copy(), iret
+
+
+def stiffness_rect(src_seg, rcv_seg, mu, nu):
+    """K_S, K_N (nr x ns) for rectangular segments (Input_Segment rows)."""
+    src = np.asfortranarray(src_seg, dtype=np.float64)
+    rcv = np.asfortranarray(rcv_seg, dtype=np.float64)
+    ns, nr = src.shape[0], rcv.shape[0]
+    KS = np.zeros((nr, ns), order="F")
+    KN = np.zeros((nr, ns), order="F")
+    lib().orc_stiffness_rect(_p(src), ns, ns, _p(rcv), nr, nr, mu, nu,
+                             KS.ctypes.data_as(PD), KN.ctypes.data_as(PD), nr)
+    return KS, KN
+
+
+def tdstress_hs(X, Y, Z, P1, P2, P3, Ss, Ds, Ts, mu, lam):
+    X, Y, Z = _f64(X).ravel(), _f64(Y).ravel(), _f64(Z).ravel()
+    n = X.size
+    S = np.zeros((6, n))
+    E = np.zeros((6, n))
+    lib().orc_tdstress_hs(n, _p(X), _p(Y), _p(Z), _p(_f64(P1)), _p(_f64(P2)), _p(_f64(P3)),
+                          Ss, Ds, Ts, mu, lam, _p(S), _p(E))
+    return S.T.copy(), E.T.copy()
+
+
+def rotverts_unitvectors(P1, P2, P3, rake):
+    """Returns P1,P2,P3 (possibly swapped), normal, strike, dip, slip (all n x 3)."""
+    n = P1.shape[0]
+    q1 = np.asfortranarray(P1, dtype=np.float64).copy(order="F")
+    q2 = np.asfortranarray(P2, dtype=np.float64).copy(order="F")
+    q3 = np.asfortranarray(P3, dtype=np.float64).copy(order="F")
+    outs = [np.zeros((n, 3), order="F") for _ in range(4)]
+    lib().orc_rotverts_unitvectors(n, _p(q1), _p(q2), _p(q3), _p(_f64(rake)),
+                                   *[o.ctypes.data_as(PD) for o in outs])
+    return (q1, q2, q3, *outs)
+
+
+def stiffness_tri(P1, P2, P3, rake, ctr, nrm, slip, mu, lam):
+    ns, nr = P1.shape[0], ctr.shape[0]
+    f = lambda a: np.asfortranarray(a, dtype=np.float64)
+    P1, P2, P3, ctr, nrm, slip = map(f, (P1, P2, P3, ctr, nrm, slip))
+    KS = np.zeros((nr, ns), order="F")
+    KN = np.zeros((nr, ns), order="F")
+    lib().orc_stiffness_tri(ns, _p(P1), _p(P2), _p(P3), _p(_f64(rake)), nr, _p(ctr), _p(nrm),
+                            _p(slip), mu, lam, KS.ctypes.data_as(PD), KN.ctypes.data_as(PD), nr)
+    return KS, KN
+
+
+def evolution_law(theta_old, dt, v, dc, alpha, sig, sig_old, b, fast, dr=1):
+    return lib().orc_evolution_law(theta_old, dt, v, dc, alpha, sig, sig_old, b, fast, dr)
+
+
+def _elem_in(**kw):
+    e = ElemIn()
+    for k, v in kw.items():
+        setattr(e, k, v)
+    return e
+
+
+def solver_highv(dt, **kw):
+    o = ElemOut()
+    lib().orc_solver_highv(C.byref(_elem_in(**kw)), dt, C.byref(o))
+    return o
+
+
+def solver_lowv(dt, **kw):
+    o = ElemOut()
+    lib().orc_solver_lowv(C.byref(_elem_in(**kw)), dt, C.byref(o))
+    return o
+
+
+def solve_one_timestep(params, n_loading, disp_old, friction_old, theta_old, v_old, friction0, a, b,
+                       dc, k_self, dt, omega, mass, sig_i, tl, switch, loading_rate, dtau, sig_old, tau0):
+    """SolveOneTimeStep; returns dict of the 8 outputs plus per-element iteration counts."""
+    arrs = [_f64(x) for x in (disp_old, friction_old, theta_old, v_old, friction0, a, b, dc, k_self)]
+    n = arrs[0].size
+    omega, mass, sig_i, tl = (_f64(x) for x in (omega, mass, sig_i, tl))
+    sw = np.ascontiguousarray(switch, dtype=np.int32)
+    lr, dtau, sig_old, tau0 = (_f64(x) for x in (loading_rate, dtau, sig_old, tau0))
+    outs = [np.zeros(n) for _ in range(6)]
+    ints = [np.zeros(n, dtype=np.int32) for _ in range(3)]
+    sp = StepParams(**{k: params[k] for k in ("conv_crit", "v0", "mu", "rho", "switch_v", "alpha_evo",
+                                              "evolution_dr")})
+    stalls = lib().orc_solve_one_timestep(
+        C.byref(sp), n, n_loading, *[_p(x) for x in arrs[:9]], dt, _p(omega), _p(mass), _p(sig_i),
+        _p(tl), _pi32(sw), _p(lr), _p(dtau), _p(sig_old), _p(tau0), *[_p(o) for o in outs],
+        *[_pi32(i) for i in ints])
+    keys = ("V", "Friction", "Disp", "Theta", "EffNormalStress", "Dt_All")
+    res = dict(zip(keys, outs))
+    res.update(InstabilityThistime=ints[0], FLAG_GoodToGo=ints[1], iterations=ints[2], stalls=stalls)
+    return res
+
+
+def interp_external(time, ext_t, ext_n, ext_s, ext_p):
+    ext_t = _f64(ext_t)
+    nt = ext_t.size
+    ext_n, ext_s, ext_p = (np.asfortranarray(x, dtype=np.float64) for x in (ext_n, ext_s, ext_p))
+    n = ext_n.shape[1]
+    o = [np.zeros(n) for _ in range(3)]
+    lib().orc_interp_external(time, n, nt, _p(ext_t), ext_n.ctypes.data_as(PD),
+                              ext_s.ctypes.data_as(PD), ext_p.ctypes.data_as(PD), *[_p(x) for x in o])
+    return tuple(o)
+
+
+def dense_prepare(KS):
+    KS = np.asfortranarray(KS, dtype=np.float64)
+    n = KS.shape[0]
+    E = np.zeros((n, n), order="F")
+    ks = np.zeros(n)
+    lib().orc_dense_prepare(n, KS.ctypes.data_as(PD), E.ctypes.data_as(PD), _p(ks))
+    return E, ks
+
+
+def _sim_in(prob, total_step, record_step, trace=True):
+    """prob: dict from the problem generator (element arrays + run parameters)."""
+    n = prob["n"]
+    keep = []
+
+    def arr(x):
+        a = _f64(x)
+        keep.append(a)
+        return _p(a)
+
+    s = SimIn()
+    s.n, s.n_loading, s.total_step, s.record_step = n, prob.get("n_loading", 0), total_step, record_step
+    n_rec = total_step // record_step if record_step > 0 else 0
+    s.n_record = n_rec
+    for k in ("mass a b dc theta_i v_i friction_i sigma0 loading_rate k_self".split()):
+        setattr(s, k, arr(prob[k]))
+    for k in "mu rho switch_v dt_cut rupture_dt maximum_dt min_normal_stress".split():
+        setattr(s, k, float(prob[k]))
+    s.alpha_evo = float(prob.get("alpha_evo", 0.0))
+    s.evolution_dr = int(prob.get("evolution_dr", 1))
+    s.planar = int(prob.get("planar", 0))
+    s.ext_at_told = int(prob.get("ext_at_told", 0))
+    ext = prob.get("ext")
+    if ext is not None:
+        s.ext_nt = len(ext["t"])
+        s.ext_t = arr(ext["t"])
+        for k, name in (("n", "ext_n"), ("s", "ext_s"), ("p", "ext_p")):
+            a = np.asfortranarray(ext[k] if k in ext else np.zeros_like(ext["n"]), dtype=np.float64)
+            keep.append(a)
+            setattr(s, name, a.ctypes.data_as(PD))
+    else:
+        s.ext_nt = 0
+    res = {"hist_time": np.zeros(n_rec), "hist_dt": np.zeros(n_rec)}
+    for k in "hist_v hist_disp hist_theta hist_sigma".split():
+        res[k] = np.zeros((n_rec, n), order="F")
+    if trace:
+        for k in "trace_told trace_dt trace_vmax".split():
+            res[k] = np.zeros(total_step)
+    for k, v in res.items():
+        setattr(s, k, v.ctypes.data_as(PD))
+    return s, res, keep
+
+
+def sim_dense(prob, KS, KN, total_step, record_step=1, trace=True):
+    """main() with dense stiffness; returns (status, dict of histories/trace/counters)."""
+    E, ks = dense_prepare(KS)
+    prob = dict(prob)
+    prob["k_self"] = ks
+    KN = np.asfortranarray(KN, dtype=np.float64)
+    op = DenseOp(KS.shape[0], E.ctypes.data_as(PD), KN.ctypes.data_as(PD))
+    s, res, keep = _sim_in(prob, total_step, record_step, trace)
+    n = prob["n"]
+    final = np.zeros(6 * n)
+    o = SimOut()
+    o.final_state = _p(final)
+    st = lib().orc_sim_dense(C.byref(op), C.byref(s), C.byref(o))
+    res.update(rhs_evals=o.rhs_evals, retry_passes=o.retry_passes, stalls=o.stalls,
+               steps_done=o.steps_done, n_recorded=o.n_recorded, t_end=o.t_end, dt_last=o.dt_last)
+    res["final"] = dict(zip(("V", "Disp", "Theta", "Friction", "Sigma", "Accel"), final.reshape(6, n)))
+    return st, res
+
+
+def dense_coupling(E, KN, x, planar=0):
+    E = np.asfortranarray(E, dtype=np.float64)
+    KN = np.asfortranarray(KN, dtype=np.float64)
+    n = E.shape[0]
+    op = DenseOp(n, E.ctypes.data_as(PD), KN.ctypes.data_as(PD))
+    x = _f64(x)
+    tl, sp = np.zeros(n), np.zeros(n)
+    lib().orc_dense_coupling(C.byref(op), _p(x), _p(tl), _p(sp), planar)
+    return tl, sp

@@ -1,0 +1,39 @@
+// Dense coupling operator: device layout and launch helpers.
+//
+// HBM layout: E (zero-diagonal K_S,ij/K_S,ii) and K_N are stored
+// column-major with a padded leading dimension ld = round_up(n, 64), in FP64
+// or FP32.  The coupling kernel streams both matrices once per RHS
+// evaluation, each lane owning 2 (FP64) or 4 (FP32) consecutive rows and
+// reading them with one 16-byte load per column, so a wavefront moves a
+// contiguous 1 KiB per load instruction.  The column range is split across
+// workgroups (split-K); each split writes its partial sums and a second,
+// small kernel adds the splits in a fixed order, so the result is
+// deterministic run to run.
+#pragma once
+#include "qdfn_internal.h"
+
+struct qdfn_dense {
+    int64_t n = 0, ld = 0;
+    int32_t prec = QDFN_PREC_FP64;
+    int32_t normal_zero = 0;
+    int32_t splits = 1;
+    int64_t cols_per_split = 0;
+    qdfn::DevBuf E, KN;         // ld x n, element type by prec
+    qdfn::DevBuf kself, diag;   // n doubles: |K_ii| and signed K_ii
+    qdfn::DevBuf partS, partN;  // splits x ld doubles
+    qdfn::DevBuf x, yS, yN;     // scratch vectors for qdfn_matvec
+    hipStream_t stream = nullptr;
+};
+
+namespace qdfn {
+
+// Launch the split coupling kernel: partial sums of E*x and K_N*x.
+// gate (device int, may be null): when non-null and zero the kernels return
+// immediately (the step controller reuses the previous coupling on retries).
+int launch_coupling(const qdfn_dense* op, const double* d_x, const int* d_gate, hipStream_t s);
+// Finish: TL = -(sum of E*x partials), SigProd = sum of K_N*x partials
+// (TL = E*(0-x) as Functions_RSFDFN3DMain_D.jl:199).
+int launch_coupling_finish(const qdfn_dense* op, double* d_tl, double* d_sigprod, const int* d_gate,
+                           hipStream_t s);
+
+}  // namespace qdfn

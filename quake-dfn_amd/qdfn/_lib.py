@@ -1,0 +1,135 @@
+"""Loader for libqdfn_hip.so (the HIP/gfx950 product library) and its C-ABI.
+
+The library is built in-tree by ``quake-dfn_amd/csrc/Makefile`` (or
+``__graft_entry__.build()``).  There is no fallback: if the shared object is
+missing or cannot be loaded, every entry point raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqdfn_hip.so")
+_LIB = None
+
+D = C.c_double
+I32 = C.c_int32
+I64 = C.c_int64
+PD = C.POINTER(C.c_double)
+PI32 = C.POINTER(C.c_int32)
+VP = C.c_void_p
+
+QDFN_OK = 0
+QDFN_ERR_ARG = -1
+QDFN_ERR_HIP = -2
+QDFN_ERR_NOMEM = -3
+QDFN_ERR_STATE = -4
+QDFN_ERR_RETRIES = -10
+QDFN_ERR_EMPTY_DTMIN = -11
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+
+class QdfnError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qdfn error {code}: {msg}")
+        self.code = code
+
+
+class StepParams(C.Structure):
+    _fields_ = [("conv_crit", D), ("v0", D), ("mu", D), ("rho", D), ("switch_v", D), ("alpha_evo", D),
+                ("evolution_dr", I32), ("pad_", I32)]
+
+
+class SimParams(C.Structure):
+    _fields_ = [("n", I64), ("n_loading", I64), ("mu", D), ("rho", D), ("switch_v", D), ("dt_cut", D),
+                ("rupture_dt", D), ("maximum_dt", D), ("min_normal_stress", D), ("alpha_evo", D),
+                ("evolution_dr", I32), ("planar", I32), ("ext_at_told", I32), ("pad_", I32),
+                ("record_step", I64), ("record_capacity", I64), ("trace_capacity", I64)]
+
+
+class SimStats(C.Structure):
+    _fields_ = [("steps", I64), ("rhs_evals", I64), ("retry_passes", I64), ("stalls", I64),
+                ("n_recorded", I64), ("n_traced", I64), ("status", I32), ("pad_", I32), ("t", D), ("dt", D),
+                ("coupling_ms_avg", D), ("coupling_timed", I64)]
+
+
+# name -> (restype, argtypes); mirrors include/qdfn.h
+SIGNATURES = {
+    "qdfn_last_error": (C.c_char_p, []),
+    "qdfn_device_info": (C.c_int, [C.c_char_p, I64, PI32, C.POINTER(I64)]),
+    "qdfn_okada_dc3d": (C.c_int, [D, I64, PD, PD, PD, D, D, D, D, D, D, D, D, D, PD, PD]),
+    "qdfn_okada_stiffness_rect": (C.c_int, [PD, I64, I64, PD, I64, I64, D, D, PD, PD, I64]),
+    "qdfn_tde_stress_hs": (C.c_int, [I64, PD, PD, PD, PD, PD, PD, D, D, D, D, D, PD, PD]),
+    "qdfn_tde_stiffness_tri": (C.c_int, [I64, PD, PD, PD, PD, I64, PD, PD, PD, D, D, PD, PD, I64]),
+    "qdfn_rotverts_unitvectors": (C.c_int, [I64, PD, PD, PD, PD, PD, PD, PD, PD]),
+    "qdfn_dense_create": (VP, [PD, PD, I64, I64, I32]),
+    "qdfn_dense_create_rect": (VP, [PD, I64, I64, D, D, I32]),
+    "qdfn_dense_create_tri": (VP, [I64, PD, PD, PD, PD, PD, PD, PD, D, D, I32]),
+    "qdfn_dense_destroy": (None, [VP]),
+    "qdfn_dense_kself": (C.c_int, [VP, PD, PD]),
+    "qdfn_dense_normal_zero": (C.c_int, [VP]),
+    "qdfn_matvec": (C.c_int, [VP, PD, PD, PD, I32]),
+    "qdfn_dense_time_coupling": (C.c_int, [VP, I32, PD]),
+    "qdfn_solve_one_timestep": (I64, [C.POINTER(StepParams), I64, I64] + [PD] * 9 + [D] + [PD] * 4 + [PI32] +
+                                [PD] * 4 + [PD] * 6 + [PI32] * 3),
+    "qdfn_sim_create_dense": (VP, [VP, C.POINTER(SimParams)] + [PD] * 9),
+    "qdfn_sim_set_external": (C.c_int, [VP, I64, PD, PD, PD]),
+    "qdfn_sim_run": (C.c_int, [VP, I64, I32, I32]),
+    "qdfn_sim_get_stats": (C.c_int, [VP, C.POINTER(SimStats)]),
+    "qdfn_sim_get_state": (C.c_int, [VP, PD, PD, PD, PD, PD, PD]),
+    "qdfn_sim_get_history": (C.c_int, [VP, I64, PD, PD, PD, PD, PD, PD]),
+    "qdfn_sim_get_trace": (C.c_int, [VP, I64, PD, PD, PD]),
+    "qdfn_sim_destroy": (None, [VP]),
+}
+
+
+def lib():
+    """The loaded C-ABI library (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C quake-dfn_amd/csrc` "
+                              "or __graft_entry__.build(); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def last_error():
+    return lib().qdfn_last_error().decode(errors="replace")
+
+
+def check(rc):
+    if rc < 0:
+        raise QdfnError(rc, last_error())
+    return rc
+
+
+def check_handle(h, what):
+    if not h:
+        raise QdfnError(QDFN_ERR_STATE, f"{what}: {last_error()}")
+    return h
+
+
+def ptr(a):
+    """Pointer to a numpy float64 array (None passes NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(PD)
+
+
+def ptr_i32(a):
+    return a.ctypes.data_as(PI32)
+
+
+def device_info():
+    name = C.create_string_buffer(128)
+    cu = I32()
+    hbm = I64()
+    check(lib().qdfn_device_info(name, 128, C.byref(cu), C.byref(hbm)))
+    return {"arch": name.value.decode(), "cu_count": cu.value, "hbm_bytes": hbm.value}

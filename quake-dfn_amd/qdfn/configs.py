@@ -1,0 +1,216 @@
+"""Input-format helpers and the seeded synthetic workloads.
+
+Format layer (reference file:line):
+  BulkToSegment      scripts/Functions_BuildInputFile.jl:82-179 (rectangle subdivision)
+  LRtoRake           scripts/Functions_BuildInputFile.jl:182-213
+  run_setup          scripts/QUAKEDFN_RunARG.jl:159-201 (mass, RecTimeStep, RuptureDt)
+  backslip_tables    Tools/ExternalStress/UniformBackslip.jl:75-94 (uniform back-slip loading)
+
+Workloads (all arrays seeded with numpy PCG64):
+  single_fault   one vertical strike-slip fault 3.2 x 1.6 km, 32 x 16 elements of 100 m
+  fault_network  8 rectangular faults of 32 x 32 elements (8,192 elements)
+"""
+import numpy as np
+
+SEC_PER_YEAR = 3.15576e7
+
+# RUN3_QUAKEDFN.jl:13-17 defaults
+RUN3_DEFAULTS = dict(dt_cut=5.0, switch_v=1e-2, rupture_multiple=3.0, maximum_dt=1e7, vertical_length_scale=0.0)
+
+
+def _cosd(x):
+    return np.cos(np.deg2rad(x))
+
+
+def _sind(x):
+    return np.sin(np.deg2rad(x))
+
+
+def LRtoRake(Switch_StrikeSlip_or_ReverseNormal, Input_Bulk):
+    """Column 8 (LR/RN flag, -1 or +1) -> rake angle in degrees."""
+    B = np.array(Input_Bulk, dtype=np.float64, copy=True)
+    if Switch_StrikeSlip_or_ReverseNormal == 1:
+        B[:, 7] = np.where(B[:, 7] == -1.0, 0.0, 180.0)
+    elif Switch_StrikeSlip_or_ReverseNormal == 2:
+        shallow = B[:, 6] < 90.0
+        ll = B[:, 7] == -1.0
+        B[:, 7] = np.where(shallow, np.where(ll, 90.0, 270.0), np.where(ll, 270.0, 90.0))
+    return B
+
+
+def BulkToSegment(Input_Bulk, counts=None):
+    """Subdivide bulk rectangles (18 columns) into Input_Segment rows (19 columns):
+    1-3 centre (z = depth), 4 strike length, 5 dip length, 6 strike, 7 dip, 8 rake,
+    9 a, 10 b, 11 Dc, 12 Theta_i, 13 V_i, 14 Friction_i, 15 normal stress,
+    16 V_Const, 17 bulk index, 18-19 bulk lengths.  Element counts along strike
+    and dip are ceil(length / column 18) as in the reference, unless `counts`
+    gives them explicitly per bulk fault."""
+    B = np.asarray(Input_Bulk, dtype=np.float64)
+    if counts is None:
+        nl = np.ceil(B[:, 3] / B[:, 17]).astype(int)
+        nw = np.ceil(B[:, 4] / B[:, 17]).astype(int)
+    else:
+        nl = np.array([c[0] for c in counts], dtype=int)
+        nw = np.array([c[1] for c in counts], dtype=int)
+    rows = []
+    for k in range(B.shape[0]):
+        L, W = B[k, 3], B[k, 4]
+        st, dp = B[k, 5], B[k, 6]
+        Rs = np.array([[_cosd(st), -_sind(st), 0.0], [_sind(st), _cosd(st), 0.0], [0.0, 0.0, 1.0]])
+        Rd = np.array([[1.0, 0.0, 0.0], [0.0, _cosd(dp), -_sind(dp)], [0.0, _sind(dp), _cosd(dp)]])
+        R = Rs @ Rd
+        for li in range(1, nl[k] + 1):
+            for wi in range(1, nw[k] + 1):
+                if nl[k] * nw[k] == 1:
+                    c = B[k, 0:3].copy()
+                else:
+                    loc = np.array([-L / 2 + L * (2 * li - 1) / nl[k] / 2, -W / 2 + W * (2 * wi - 1) / nw[k] / 2, 0.0])
+                    c = R @ loc + np.array([B[k, 0], B[k, 1], -B[k, 2]])
+                    c[2] = -c[2]
+                row = np.zeros(19)
+                row[0:3] = c
+                row[3] = L / nl[k]
+                row[4] = W / nw[k]
+                row[5:14] = B[k, 5:14]
+                row[14] = B[k, 14] + B[k, 15] * c[2]
+                row[15] = B[k, 16]
+                row[16] = k + 1
+                row[17] = L
+                row[18] = W
+                rows.append(row)
+    return np.array(rows)
+
+
+def run_setup(seg, rho, kself, rupture_multiple=3.0, vertical_length_scale=0.0):
+    """FaultMass, RecTimeStep and RuptureDt as RunRSFDFN3D computes them."""
+    n = seg.shape[0]
+    vs = vertical_length_scale if vertical_length_scale != 0 else min(seg[:, 3].min(), seg[:, 4].min())
+    mass = np.ones(n) * vs * rho / 2
+    period = np.sqrt(mass / np.abs(kself))
+    rec = period.min() / 10
+    return mass, rec, rec * rupture_multiple
+
+
+def backslip_tables(ks_row_sums, kn_row_sums, rate=1e-9, t_end=1e11):
+    """Back-slip loading tables: stress after t_end seconds of uniform slip
+    deficit at `rate` (UniformBackslip.jl:81-86), linearly interpolated."""
+    n = ks_row_sums.size
+    ext_s = np.zeros((2, n))
+    ext_n = np.zeros((2, n))
+    ext_s[1] = -ks_row_sums * t_end * rate
+    ext_n[1] = -kn_row_sums * t_end * rate
+    return {"t": np.array([0.0, t_end]), "n": ext_n, "s": ext_s}
+
+
+def _friction_init(a, b, v_i=1e-9, f_ref=0.6, v_ref=1e-6):
+    """Friction_i at steady state for V_i, expressing f0 = f_ref at V_ref under
+    the reference's hard-coded V0 = 1e-9."""
+    return f_ref + (a - b) * np.log(v_i / v_ref)
+
+
+def single_fault(nx=32, nz=16, elem=100.0, mu=32e9, nu=0.25, vs=3464.0, sigma=50e6):
+    """Config: one vertical strike-slip fault 3.2 km x 1.6 km, top at the surface."""
+    L, W = nx * elem, nz * elem
+    bulk = np.zeros((1, 18))
+    bulk[0, :8] = [0.0, 0.0, W / 2, L, W, 90.0, 90.0, 1.0]
+    bulk[0, 8:18] = [0.010, 0.006, 1e-3, 1e6, 1e-9, 0.6, sigma, 0.0, 0.0, elem]
+    seg = BulkToSegment(LRtoRake(1, bulk))
+    n = seg.shape[0]
+    # velocity-weakening 2 km x 1 km central patch
+    x_along = seg[:, 1]  # strike 90 deg: along-strike is the y axis
+    depth = seg[:, 2]
+    b = np.full(n, 0.006)
+    vw = (np.abs(x_along) <= 1000.0) & (np.abs(depth - W / 2) <= 500.0)
+    b[vw] = 0.015
+    a = np.full(n, 0.010)
+    seg[:, 8], seg[:, 9] = a, b
+    dc = np.full(n, 1e-3)
+    v_i = np.full(n, 1e-9)
+    seg[:, 10], seg[:, 12] = dc, v_i
+    seg[:, 11] = dc / v_i
+    seg[:, 13] = _friction_init(a, b)
+    seg[:, 14] = sigma
+    rho = mu / vs ** 2
+    return dict(name="single_fault_512", seg=seg, mu=mu, nu=nu, rho=rho, min_normal_stress=2e6)
+
+
+def fault_network(seed=20240611, n_faults=8, nx=32, nz=32, mu=32e9, nu=0.25, vs=3464.0, min_gap=150.0):
+    """Config: 8 rectangular faults (length U(2,5) km, depth extent U(1,3) km,
+    strike U(0,180), dip U(60,90), centres U in a 10 x 10 km box, tops buried
+    >= 100 m), each 32 x 32 elements -> 8,192 elements.  Faults closer than
+    min_gap to an already placed fault are redrawn (seeded rejection), since
+    intersecting rectangles have no finite interaction."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    placed = []
+    segs = []
+    while len(segs) < n_faults:
+        L = rng.uniform(2000, 5000)
+        W = rng.uniform(1000, 3000)
+        st = rng.uniform(0, 180)
+        dp = rng.uniform(60, 90)
+        cx, cy = rng.uniform(-5000, 5000, size=2)
+        top = rng.uniform(100, 1000)
+        cz = top + W / 2 * _sind(dp)
+        bulk = np.zeros((1, 18))
+        bulk[0, :8] = [cx, cy, cz, L, W, st, dp, rng.choice([-1.0, 1.0])]
+        bulk[0, 8:18] = [0.01, 0.015, 1e-3, 1e6, 1e-9, 0.6, 50e6, 0.0, 0.0, max(L / nx, W / nz)]
+        seg = BulkToSegment(LRtoRake(1, bulk), counts=[(nx, nz)])
+        ok = True
+        for other in placed:
+            d = np.sqrt(((seg[:, None, :3] - other[None, :, :3]) ** 2).sum(-1)).min()
+            if d < min_gap:
+                ok = False
+                break
+        if not ok:
+            continue
+        placed.append(seg)
+        segs.append(seg)
+    seg = np.vstack(segs)
+    n = seg.shape[0]
+    a = rng.uniform(0.008, 0.012, n)
+    b = a + rng.uniform(-0.004, 0.006, n)
+    sigma = rng.uniform(30e6, 80e6, n)
+    seg[:, 8], seg[:, 9] = a, b
+    seg[:, 10] = 1e-3
+    seg[:, 12] = 1e-9
+    seg[:, 11] = seg[:, 10] / seg[:, 12]
+    seg[:, 13] = _friction_init(a, b)
+    seg[:, 14] = sigma
+    rho = mu / vs ** 2
+    return dict(name="fault_network_8192", seg=seg, mu=mu, nu=nu, rho=rho, min_normal_stress=2e6)
+
+
+def problem_from_segments(cfg, kself, ks_row_sums, kn_row_sums, rate=1e-9, **overrides):
+    """Element arrays and run parameters for a rectangle workload: the
+    Input_Segment columns plus RUN3 defaults and back-slip loading."""
+    seg = cfg["seg"]
+    n = seg.shape[0]
+    d = dict(RUN3_DEFAULTS)
+    d.update(overrides)
+    mass, rec, rup = run_setup(seg, cfg["rho"], kself, d["rupture_multiple"], d["vertical_length_scale"])
+    prob = dict(n=n, n_loading=0, mass=mass, a=seg[:, 8].copy(), b=seg[:, 9].copy(), dc=seg[:, 10].copy(),
+                theta_i=seg[:, 11].copy(), v_i=seg[:, 12].copy(), friction_i=seg[:, 13].copy(),
+                sigma0=seg[:, 14].copy(), loading_rate=seg[:, 15].copy(), mu=cfg["mu"], rho=cfg["rho"],
+                switch_v=d["switch_v"], dt_cut=d["dt_cut"], rupture_dt=rup, maximum_dt=d["maximum_dt"],
+                min_normal_stress=cfg["min_normal_stress"], alpha_evo=0.0, evolution_dr=1, planar=0,
+                ext_at_told=0, rec_time_step=rec)
+    prob["ext"] = backslip_tables(ks_row_sums, kn_row_sums, rate)
+    return prob
+
+
+def first_event_time(told, dt, vmax, threshold=1e-3):
+    """Onset of the first event: the first accepted step whose max V reaches
+    `threshold`, placed by linear interpolation in log V between that step and
+    the one before (time axis = end of each step)."""
+    t_end = np.asarray(told) + np.asarray(dt)
+    v = np.asarray(vmax)
+    idx = np.nonzero(v >= threshold)[0]
+    if idx.size == 0:
+        return None
+    k = idx[0]
+    if k == 0:
+        return float(t_end[0])
+    l0, l1 = np.log10(v[k - 1]), np.log10(v[k])
+    lt = np.log10(threshold)
+    f = (lt - l0) / (l1 - l0) if l1 != l0 else 1.0
+    return float(t_end[k - 1] + f * (t_end[k] - t_end[k - 1]))

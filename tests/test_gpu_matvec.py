@@ -1,0 +1,75 @@
+"""GPU parity: dense coupling product (scripts/Functions_RSFDFN3DMain_D.jl:54-63,
+:199-202) vs the oracle's FP64 product and numpy/OpenBLAS dgemv.
+
+FP64 storage: <= 1e-12 relative to sum_j |E_ij x_j| (summation order differs).
+FP32 storage: <= 2e-7 relative to the same scale (one FP32 rounding per entry,
+FP64 accumulation).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _mats(n, seed):
+    rng = np.random.default_rng(seed)
+    KS = rng.standard_normal((n, n))
+    KS[np.diag_indices(n)] = -np.abs(KS[np.diag_indices(n)]) - n  # dominant negative self term
+    KN = rng.standard_normal((n, n))
+    return np.asfortranarray(KS), np.asfortranarray(KN)
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 513, 2048, 3001])
+def test_fp64_matches_oracle_and_blas(qd, orc, n):
+    KS, KN = _mats(n, n)
+    x = np.random.default_rng(n + 1).standard_normal(n)
+    op = qd.DenseOperator.from_matrices(KS, KN, qd.PREC_FP64)
+    tl, sp = op.matvec(x, loading_form=True)
+    E, ks = orc.dense_prepare(KS)
+    tlo, spo = orc.dense_coupling(E, KN, x)
+    scaleS = np.abs(E) @ np.abs(x) + 1e-300
+    scaleN = np.abs(KN) @ np.abs(x) + 1e-300
+    assert (np.abs(tl - tlo) / scaleS).max() <= 1e-12
+    assert (np.abs(sp - spo) / scaleN).max() <= 1e-12
+    assert (np.abs(sp - KN @ x) / scaleN).max() <= 1e-12  # numpy/OpenBLAS dgemv
+    yS, _ = op.matvec(x, loading_form=False)
+    assert (np.abs(yS - KS @ x) / (np.abs(KS) @ np.abs(x))).max() <= 1e-12
+    op.close()
+
+
+@pytest.mark.parametrize("n", [64, 1000, 4096])
+def test_fp32_storage_error_bound(qd, orc, n):
+    KS, KN = _mats(n, 100 + n)
+    x = np.random.default_rng(n).standard_normal(n)
+    op = qd.DenseOperator.from_matrices(KS, KN, qd.PREC_FP32)
+    tl, sp = op.matvec(x)
+    E, _ = orc.dense_prepare(KS)
+    tlo, spo = orc.dense_coupling(E, KN, x)
+    # exact product of the FP32-rounded matrices, accumulated in FP64
+    E32 = E.astype(np.float32).astype(np.float64)
+    K32 = KN.astype(np.float32).astype(np.float64)
+    scaleS = np.abs(E) @ np.abs(x) + 1e-300
+    scaleN = np.abs(KN) @ np.abs(x) + 1e-300
+    assert (np.abs(tl - (-(E32 @ x))) / scaleS).max() <= 1e-12
+    assert (np.abs(sp - K32 @ x) / scaleN).max() <= 1e-12
+    assert (np.abs(tl - tlo) / scaleS).max() <= 2e-7
+    assert (np.abs(sp - spo) / scaleN).max() <= 2e-7
+    op.close()
+
+
+def test_normal_zero_flag(qd):
+    n = 100
+    KS, _ = _mats(n, 5)
+    op = qd.DenseOperator.from_matrices(KS, np.zeros((n, n)))
+    assert op.normal_zero()
+    _, sp = op.matvec(np.ones(n))
+    assert np.all(sp == 0.0)
+    KN = np.zeros((n, n))
+    KN[3, 7] = 1e-300
+    op2 = qd.DenseOperator.from_matrices(KS, KN)
+    assert not op2.normal_zero()
+
+
+def test_bad_args_raise(qd):
+    with pytest.raises(qd.QdfnError):
+        qd.DenseOperator.from_matrices(np.zeros((0, 0)), np.zeros((0, 0)))
