@@ -1,0 +1,44 @@
+"""Launch the fused coupling kernel alone, for counter collection.
+
+    python -m qdfn.profile_coupling [--prec fp32|fp64] [--reps 50] [--copy-mib 512]
+
+Builds the 8,192-element fault-network operator on the device, runs `reps`
+coupling products, then (optionally) one plain device-to-device copy of
+--copy-mib MiB through torch so that HBM counters (FETCH_SIZE / WRITE_SIZE)
+can be calibrated against a kernel of known traffic.
+"""
+import argparse
+import sys
+
+import numpy as np
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--prec", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--copy-mib", type=int, default=512)
+    a = ap.parse_args()
+    import qdfn
+    from qdfn import configs
+    cfg = configs.fault_network()
+    prec = qdfn.PREC_FP32 if a.prec == "fp32" else qdfn.PREC_FP64
+    op = qdfn.DenseOperator.from_rect_segments(cfg["seg"], cfg["mu"], cfg["nu"], prec)
+    op.matvec(np.ones(op.n))
+    ms = op.time_coupling(a.reps)
+    gbs = op.bytes_per_product() / (ms * 1e-3) / 1e9
+    print(f"coupling {a.prec}: {ms * 1e3:.1f} us/launch, {op.bytes_per_product() / 1e6:.1f} MB algorithmic, "
+          f"{gbs:.0f} GB/s", flush=True)
+    if a.copy_mib > 0:
+        import torch
+        x = torch.empty(a.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"copy {a.copy_mib} MiB done", flush=True)
+    op.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
