@@ -27,10 +27,22 @@ struct qdfn_dense {
 
 namespace qdfn {
 
+// Optional device-side control of a coupling launch:
+//   gate   - when non-null and *gate == 0 the kernel returns at once (the
+//            step controller reuses the previous coupling on retries);
+//   parity - when non-null the input vector is x + (*parity) * parity_stride
+//            (the double-buffered DispOld of a run);
+//   ran_log[slot] - set to 1/0 by the kernel: whether it streamed the matrices.
+struct CouplingGate {
+    const int* gate = nullptr;
+    const int* parity = nullptr;
+    int64_t parity_stride = 0;
+    int* ran_log = nullptr;
+    int slot = 0;
+};
+
 // Launch the split coupling kernel: partial sums of E*x and K_N*x.
-// gate (device int, may be null): when non-null and zero the kernels return
-// immediately (the step controller reuses the previous coupling on retries).
-int launch_coupling(const qdfn_dense* op, const double* d_x, const int* d_gate, hipStream_t s);
+int launch_coupling(const qdfn_dense* op, const double* d_x, const CouplingGate& g, hipStream_t s);
 // Finish: TL = -(sum of E*x partials), SigProd = sum of K_N*x partials
 // (TL = E*(0-x) as Functions_RSFDFN3DMain_D.jl:199).
 int launch_coupling_finish(const qdfn_dense* op, double* d_tl, double* d_sigprod, const int* d_gate,

@@ -30,10 +30,13 @@ constexpr int kUnroll = 8;
 // partS[split, row] = sum_{j in split} E[row, j] * x[j]; partN likewise with K_N.
 template <typename T>
 __global__ __launch_bounds__(256) void k_coupling(const T* __restrict__ E, const T* __restrict__ KN, int64_t ld,
-                                                  int64_t n, int64_t cps, const double* __restrict__ x,
+                                                  int64_t n, int64_t cps, const double* __restrict__ xbase,
                                                   double* __restrict__ partS, double* __restrict__ partN,
-                                                  int planar, const int* __restrict__ gate) {
-    if (gate && *gate == 0) return;
+                                                  int planar, CouplingGate g) {
+    const int run = g.gate ? *g.gate : 1;
+    if (g.ran_log && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) g.ran_log[g.slot] = run;
+    if (!run) return;
+    const double* __restrict__ x = xbase + (g.parity ? (int64_t)(*g.parity) * g.parity_stride : 0);
     using P = RowPack<T>;
     using V = typename P::V;
     constexpr int R = P::R;
@@ -109,41 +112,54 @@ __global__ __launch_bounds__(256) void k_coupling(const T* __restrict__ E, const
     }
 }
 
+// Sum of the split partials in a fixed order: each workgroup owns 64 rows;
+// its 4 wavefronts sum interleaved quarters of the splits, then the four
+// quarter sums are added in wavefront order.
 __global__ __launch_bounds__(256) void k_coupling_finish(const double* __restrict__ partS,
                                                          const double* __restrict__ partN, int64_t ld, int64_t n,
                                                          int splits, int planar, double* __restrict__ tl,
                                                          double* __restrict__ sigprod, const int* __restrict__ gate) {
     if (gate && *gate == 0) return;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.0, q = 0.0;
-    for (int k = 0; k < splits; ++k) {
-        s += partS[(int64_t)k * ld + i];
-        q += partN[(int64_t)k * ld + i];
+    __shared__ double sh[2][4][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    double s = 0.0, r = 0.0;
+    if (i < n) {
+        for (int k = q; k < splits; k += 4) {
+            s += partS[(int64_t)k * ld + i];
+            r += partN[(int64_t)k * ld + i];
+        }
     }
-    tl[i] = -s;
-    sigprod[i] = planar ? 0.0 : q;
+    sh[0][q][lane] = s;
+    sh[1][q][lane] = r;
+    __syncthreads();
+    if (q == 0 && i < n) {
+        const double ts = ((sh[0][0][lane] + sh[0][1][lane]) + sh[0][2][lane]) + sh[0][3][lane];
+        const double tn = ((sh[1][0][lane] + sh[1][1][lane]) + sh[1][2][lane]) + sh[1][3][lane];
+        tl[i] = -ts;
+        sigprod[i] = planar ? 0.0 : tn;
+    }
 }
 
-int launch_coupling(const qdfn_dense* op, const double* d_x, const int* d_gate, hipStream_t s) {
+int launch_coupling(const qdfn_dense* op, const double* d_x, const CouplingGate& g, hipStream_t s) {
     const int R = op->prec == QDFN_PREC_FP64 ? 2 : 4;
     const int64_t rb = (op->ld + 256 * R - 1) / (256 * R);
     dim3 grid((unsigned)rb, (unsigned)op->splits);
     if (op->prec == QDFN_PREC_FP64)
         hipLaunchKernelGGL(k_coupling<double>, grid, dim3(256), 0, s, op->E.as<double>(), op->KN.as<double>(), op->ld,
                            op->n, op->cols_per_split, d_x, op->partS.as<double>(), op->partN.as<double>(),
-                           op->normal_zero, d_gate);
+                           op->normal_zero, g);
     else
         hipLaunchKernelGGL(k_coupling<float>, grid, dim3(256), 0, s, op->E.as<float>(), op->KN.as<float>(), op->ld,
                            op->n, op->cols_per_split, d_x, op->partS.as<double>(), op->partN.as<double>(),
-                           op->normal_zero, d_gate);
+                           op->normal_zero, g);
     QDFN_HIP(hipGetLastError());
     return QDFN_OK;
 }
 
 int launch_coupling_finish(const qdfn_dense* op, double* d_tl, double* d_sigprod, const int* d_gate,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_coupling_finish, dim3((unsigned)((op->n + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_coupling_finish, dim3((unsigned)((op->n + 63) / 64)), dim3(256), 0, s,
                        op->partS.as<double>(), op->partN.as<double>(), op->ld, op->n, op->splits, op->normal_zero,
                        d_tl, d_sigprod, d_gate);
     QDFN_HIP(hipGetLastError());
@@ -189,7 +205,7 @@ __global__ __launch_bounds__(256) void k_build(const double* __restrict__ KS, co
 static int choose_splits(qdfn_dense* op) {
     const int R = op->prec == QDFN_PREC_FP64 ? 2 : 4;
     const int64_t rb = (op->ld + 256 * R - 1) / (256 * R);
-    int64_t target = 1024;
+    int64_t target = 512;
     if (const char* env = getenv("QDFN_COUPLING_WGS")) target = std::max<int64_t>(1, atoll(env));
     int64_t splits = std::max<int64_t>(1, (target + rb - 1) / rb);
     splits = std::min<int64_t>(splits, std::max<int64_t>(1, op->n / 32));
@@ -376,7 +392,7 @@ extern "C" int qdfn_matvec(qdfn_dense_t* op, const double* x, double* yS, double
     if (!op || !x) return fail(QDFN_ERR_ARG, "qdfn_matvec: null argument");
     hipStream_t s = op->stream;
     QDFN_HIP(hipMemcpyAsync(op->x.p, x, sizeof(double) * op->n, hipMemcpyHostToDevice, s));
-    int rc = launch_coupling(op, op->x.as<double>(), nullptr, s);
+    int rc = launch_coupling(op, op->x.as<double>(), CouplingGate{}, s);
     if (rc) return rc;
     rc = launch_coupling_finish(op, op->yS.as<double>(), op->yN.as<double>(), nullptr, s);
     if (rc) return rc;
@@ -398,9 +414,9 @@ extern "C" int qdfn_dense_time_coupling(qdfn_dense_t* op, int32_t reps, double* 
     hipEvent_t a, b;
     QDFN_HIP(hipEventCreate(&a));
     QDFN_HIP(hipEventCreate(&b));
-    int rc = launch_coupling(op, op->x.as<double>(), nullptr, s);  // warm
+    int rc = launch_coupling(op, op->x.as<double>(), CouplingGate{}, s);  // warm
     QDFN_HIP(hipEventRecord(a, s));
-    for (int r = 0; r < reps && rc == QDFN_OK; ++r) rc = launch_coupling(op, op->x.as<double>(), nullptr, s);
+    for (int r = 0; r < reps && rc == QDFN_OK; ++r) rc = launch_coupling(op, op->x.as<double>(), CouplingGate{}, s);
     QDFN_HIP(hipEventRecord(b, s));
     QDFN_HIP(hipEventSynchronize(b));
     float ms = 0.f;

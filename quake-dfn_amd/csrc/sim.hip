@@ -2,18 +2,23 @@
 // scripts/Functions_RSFDFN3DMain_D.jl:2-324 (and main_H's identical control
 // logic, scripts/Functions_RSFDFN3DMain_H.jl:180-383) without host round trips.
 //
-// One "attempt" = four launches on the run's stream:
-//   k_coupling        (split coupling product; gated: only on the first
-//                      attempt of a step, retries reuse it as the reference does)
-//   k_coupling_finish (fixed-order sum of the splits -> TL, SigProd; gated)
-//   k_solve           (one lane per element: HighV / LowV secant solve, FP64)
-//   k_control         (one workgroup: retry-or-accept decision, history
-//                      recording, Old<-New swap by buffer parity, and the next
-//                      step's SolverSwitch / Vmax / Dtmin / Dt selection)
-// All decisions live in a device SimState word block, so attempts can be
-// captured once into a HIP graph and replayed; the host reads 1 state block
-// per replay.  Kernels return at once when the state says the requested
-// number of steps is done, so over-issued attempts are harmless.
+// One "attempt" = three launches on the run's stream:
+//   k_coupling        split coupling product of DispOld (gated: only on the
+//                     first attempt of a step; retries reuse it, as the
+//                     reference does), reading the parity-selected buffer
+//   k_coupling_finish fixed-order sum of the splits -> TL, SigProd (gated)
+//   k_step            one lane per element: HighV / LowV secant solve (FP64),
+//                     then per-workgroup partial reductions; the last
+//                     workgroup to arrive takes the retry-or-accept decision
+//                     of main() (:240-258), records history / trace, swaps
+//                     Old<-New by flipping the buffer parity, and selects the
+//                     next step's Dt (:162-187) from quantities every lane
+//                     computed on the assumption that its attempt is accepted
+//                     (Accel, V/Accel, SolverSwitch inputs).
+// All decisions live in a device SimState block, so attempts can be captured
+// once into a HIP graph and replayed; the host reads the state once per
+// replay.  Kernels return at once when the state says the requested number of
+// steps is done, so over-issued attempts are harmless.
 #include <string.h>
 
 #include <algorithm>
@@ -36,6 +41,12 @@ struct SimState {
     int32_t pad;
 };
 
+// Per-workgroup partial results of one attempt.
+struct BlockPart {
+    int32_t ok, any4, found, pad;
+    double mindt, vmax, vmax_next, dtmin_next;
+};
+
 // Device arrays of a run.  Double-buffered state arrays are [2][n].
 struct SimArrays {
     int64_t n, n_loading;
@@ -45,9 +56,11 @@ struct SimArrays {
     // double-buffered state
     double *V, *D, *TH, *F, *SIG;
     // single
-    double *accel, *tl, *sigprod, *dt_all, *dold;
-    int32_t *instab, *flag, *sw;
+    double *accel, *tl, *sigprod;
+    int32_t* sw;  // SolverSwitch to use if the next attempt retries this step
     int32_t* ran_log;  // per-attempt "coupling ran" marks of a timed batch
+    BlockPart* parts;
+    unsigned int* arrive;
     // external stress (nt == 0: none)
     int64_t nt;
     const double *ext_t, *ext_n, *ext_s;
@@ -83,55 +96,19 @@ __device__ inline void ext_interp(const SimArrays& A, double time, int64_t i, do
     }
 }
 
-// SolveOneTimeStep over all elements (Functions_Solvers.jl:334-380) with the
-// per-attempt effective normal stress of main() (:212-226).
-__global__ __launch_bounds__(256) void k_solve(SimArrays A, SimState* __restrict__ st) {
-    if (st->done) return;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = A.n;
-    if (i >= n) return;
-    const int p = st->parity;
-    const int64_t o = (int64_t)p * n, w = (int64_t)(1 - p) * n;
-    const double dt = st->dt;
-    double dsn = 0.0, dtau = 0.0;
-    if (A.nt > 0) ext_interp(A, A.ext_at_told ? st->told : st->told + dt, i, dsn, dtau);
-    double sig = (A.planar ? 0.0 : A.sigprod[i]) + A.sigma0[i] + dsn;
-    if (sig < A.min_sig) sig = A.min_sig;
-    if (i < n - A.n_loading) {
-        ElemIn e;
-        e.conv = A.conv; e.d_old = A.D[o + i]; e.f_old = A.F[o + i]; e.th_old = A.TH[o + i];
-        e.v_old = A.V[o + i]; e.f0 = A.f0[i]; e.a = A.a[i]; e.b = A.b[i]; e.dc = A.dc[i]; e.v0 = A.v0;
-        e.k = A.kself[i]; e.omega = A.omega[i]; e.mass = A.mass[i]; e.mu = A.mu; e.sig = sig;
-        e.tl = 0.0 + A.tl[i]; e.rho = A.rho; e.dtau = dtau; e.switch_v = A.switch_v; e.alpha_evo = A.alpha_evo;
-        e.sig_old = A.SIG[o + i]; e.tau0 = A.tau0[i]; e.dr = A.dr;
-        const ElemOut r = (A.sw[i] == 1) ? solve_high_v(e, dt) : solve_low_v(e, dt);
-        A.V[w + i] = r.v; A.F[w + i] = r.f; A.D[w + i] = r.d; A.TH[w + i] = r.th; A.SIG[w + i] = r.sig;
-        A.dt_all[i] = r.dt; A.instab[i] = r.instab; A.flag[i] = r.ok;
-        if (r.stall) atomicAdd((unsigned long long*)&st->stalls, (unsigned long long)r.stall);
-    } else {
-        const double v = A.lrate[i];
-        A.V[w + i] = v; A.F[w + i] = A.F[o + i]; A.D[w + i] = A.D[o + i] + v * dt; A.TH[w + i] = A.TH[o + i];
-        A.SIG[w + i] = 1e10; A.dt_all[i] = dt; A.instab[i] = 0; A.flag[i] = 1;
-    }
-}
-
-constexpr int kCtl = 1024;
+constexpr int kStepBlock = 256;
+constexpr int kSetupBlock = 1024;
 constexpr int kTimedBatch = 64;
 
-template <typename T, typename Op>
+template <int NT, typename T, typename Op>
 __device__ inline T block_reduce(T v, Op op, T* sh) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
     __syncthreads();
     if (lane == 0) sh[wid] = v;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        T r = sh[0];
-        for (int k = 1; k < kCtl / 64; ++k) r = op(r, sh[k]);
-        sh[0] = r;
-    }
-    __syncthreads();
-    const T r = sh[0];
+    T r = sh[0];
+    for (int k = 1; k < NT / 64; ++k) r = op(r, sh[k]);
     __syncthreads();
     return r;
 }
@@ -141,16 +118,28 @@ struct MaxOp { __device__ double operator()(double a, double b) const { return b
 struct AndOp { __device__ int operator()(int a, int b) const { return a & b; } };
 struct OrOp { __device__ int operator()(int a, int b) const { return a | b; } };
 
-// Step setup (Functions_RSFDFN3DMain_D.jl:148-187): SolverSwitch from VOld,
-// Vmax and min |V/Accel| over the candidate set, DtRef and Dt.
-__device__ void step_setup(const SimArrays& A, SimState* st, int parity, double* shd, int* shi) {
+// DtRef / Dt selection of main() (:171-187); returns the new Dt.
+__device__ inline double select_dt(const SimArrays& A, double dt_prev, double vmax, double dtmin) {
+    double dtref;
+    if (vmax > A.switch_v) dtref = A.rupture_dt;
+    else if (dtmin > A.maximum_dt) dtref = A.maximum_dt;
+    else dtref = dtmin;
+    if (dtref <= A.rupture_dt) return dtref;
+    if (dt_prev < dtref / 1.2) return dt_prev * 1.2;
+    return dtref;
+}
+
+// Run-start step setup (:148-187) from the stored Old state and Accel.
+__global__ __launch_bounds__(kSetupBlock) void k_setup(SimArrays A, SimState* __restrict__ st) {
+    __shared__ double shd[kSetupBlock / 64];
+    __shared__ int shi[kSetupBlock / 64];
+    if (st->done) return;
     const int64_t n = A.n;
-    const int64_t o = (int64_t)parity * n;
+    const int64_t o = (int64_t)st->parity * n;
     double vmax = -INFINITY, dmin = INFINITY;
     int found = 0;
-    for (int64_t i = threadIdx.x; i < n; i += kCtl) {
+    for (int64_t i = threadIdx.x; i < n; i += kSetupBlock) {
         const double v = A.V[o + i];
-        A.sw[i] = v > A.switch_v ? 1 : 0;
         if (A.cand[i]) {
             vmax = v > vmax ? v : vmax;
             const double r = fabs(v / A.accel[i]);
@@ -160,121 +149,172 @@ __device__ void step_setup(const SimArrays& A, SimState* st, int parity, double*
             }
         }
     }
-    vmax = block_reduce(vmax, MaxOp(), shd);
-    dmin = block_reduce(dmin, MinOp(), shd);
-    found = block_reduce(found, OrOp(), shi);
+    vmax = block_reduce<kSetupBlock>(vmax, MaxOp(), shd);
+    dmin = block_reduce<kSetupBlock>(dmin, MinOp(), shd);
+    found = block_reduce<kSetupBlock>(found, OrOp(), shi);
     if (threadIdx.x == 0) {
         if (!found) {
             st->status = QDFN_ERR_EMPTY_DTMIN;
             st->done = 1;
+            st->need_coupling = 0;
         } else {
-            const double dtmin = dmin / A.dt_cut;
-            double dtref;
-            if (vmax > A.switch_v) dtref = A.rupture_dt;
-            else if (dtmin > A.maximum_dt) dtref = A.maximum_dt;
-            else dtref = dtmin;
-            double dt = st->dt;
-            if (dtref <= A.rupture_dt) dt = dtref;
-            else if (dt < dtref / 1.2) dt = dt * 1.2;
-            else dt = dtref;
-            st->dt = dt;
+            st->dt = select_dt(A, st->dt, vmax, dmin / A.dt_cut);
             st->iteration = 1;
             st->need_coupling = 1;
             st->rhs_evals += 1;
         }
     }
-    __syncthreads();
 }
 
-// mode 0: setup only (start of a run); mode 1: judge the attempt just solved.
-__global__ __launch_bounds__(kCtl) void k_control(SimArrays A, SimState* __restrict__ st, int mode) {
-    __shared__ double shd[kCtl / 64];
-    __shared__ int shi[kCtl / 64];
-    __shared__ int s_skip;
-    if (threadIdx.x == 0) s_skip = st->done;
-    __syncthreads();
-    if (s_skip) return;
+// One attempt of SolveOneTimeStep (Functions_Solvers.jl:334-380) with the
+// effective normal stress of main() (:212-226), plus the step decision.
+__global__ __launch_bounds__(kStepBlock) void k_step(SimArrays A, SimState* __restrict__ st) {
+    __shared__ double shd[kStepBlock / 64];
+    __shared__ int shi[kStepBlock / 64];
+    __shared__ int s_last;
+    if (st->done) return;
     const int64_t n = A.n;
-    if (mode == 0) {
-        step_setup(A, st, st->parity, shd, shi);
-        return;
-    }
-    int ok = 1, any4 = 0;
-    double mdt = INFINITY;
-    for (int64_t i = threadIdx.x; i < n; i += kCtl) {
-        ok &= A.flag[i] != 0;
-        any4 |= A.instab[i] == 4;
-        const double d = A.dt_all[i];
-        mdt = d < mdt ? d : mdt;
-    }
-    ok = block_reduce(ok, AndOp(), shi);
-    any4 = block_reduce(any4, OrOp(), shi);
-    mdt = block_reduce(mdt, MinOp(), shd);
-    if (!ok) {  // retry with the reduced Dt (:240-258)
-        for (int64_t i = threadIdx.x; i < n; i += kCtl) {
-            if (A.instab[i] == 3) A.sw[i] = 1;
-            else if (A.instab[i] == 4) A.sw[i] = 0;
-        }
-        if (threadIdx.x == 0) {
-            st->dt = any4 ? 1e-3 : mdt;
-            st->need_coupling = 0;
-            st->retry_passes += 1;
-            if (st->iteration > 100) {
-                st->status = QDFN_ERR_RETRIES;
-                st->done = 1;
-            }
-            st->iteration += 1;
-        }
-        return;
-    }
-    // accept (:262-319)
+    const int64_t i = (int64_t)blockIdx.x * kStepBlock + threadIdx.x;
     const int p = st->parity;
     const int64_t o = (int64_t)p * n, w = (int64_t)(1 - p) * n;
-    const double dt = mdt;
-    const double told = st->told;
-    const int64_t step = st->step + 1;
-    double vmax = -INFINITY;
-    for (int64_t i = threadIdx.x; i < n - A.n_loading; i += kCtl) {
-        const double v = A.V[w + i];
-        vmax = v > vmax ? v : vmax;
-    }
-    vmax = block_reduce(vmax, MaxOp(), shd);
+    const double dt = st->dt, told = st->told;
+    const int fresh = st->iteration == 1;
+    const int64_t step = st->step;
+    const bool record = A.record_step > 0 && (step + 1) % A.record_step == 0 && st->rec < A.rec_cap;
     const int64_t rec = st->rec;
-    const bool record = A.record_step > 0 && step % A.record_step == 0 && rec < A.rec_cap;
-    for (int64_t i = threadIdx.x; i < n; i += kCtl) {
-        const double v = A.V[w + i];
-        if (record) {
-            A.h_v[rec + i * A.rec_cap] = v;
-            A.h_d[rec + i * A.rec_cap] = A.D[w + i];
-            A.h_th[rec + i * A.rec_cap] = A.TH[w + i];
-            A.h_sig[rec + i * A.rec_cap] = A.SIG[w + i];
+
+    int ok = 1, any4 = 0, found = 0;
+    double mindt = INFINITY, vmax = -INFINITY, vmax_next = -INFINITY, dtmin_next = INFINITY;
+    if (i < n) {
+        const double v_old = A.V[o + i];
+        double dsn = 0.0, dtau = 0.0;
+        if (A.nt > 0) ext_interp(A, A.ext_at_told ? told : told + dt, i, dsn, dtau);
+        double sig = (A.planar ? 0.0 : A.sigprod[i]) + A.sigma0[i] + dsn;
+        if (sig < A.min_sig) sig = A.min_sig;
+        double vn;
+        if (i < n - A.n_loading) {
+            const int sw = fresh ? (v_old > A.switch_v ? 1 : 0) : A.sw[i];
+            ElemIn e;
+            e.conv = A.conv; e.d_old = A.D[o + i]; e.f_old = A.F[o + i]; e.th_old = A.TH[o + i];
+            e.v_old = v_old; e.f0 = A.f0[i]; e.a = A.a[i]; e.b = A.b[i]; e.dc = A.dc[i]; e.v0 = A.v0;
+            e.k = A.kself[i]; e.omega = A.omega[i]; e.mass = A.mass[i]; e.mu = A.mu; e.sig = sig;
+            e.tl = 0.0 + A.tl[i]; e.rho = A.rho; e.dtau = dtau; e.switch_v = A.switch_v; e.alpha_evo = A.alpha_evo;
+            e.sig_old = A.SIG[o + i]; e.tau0 = A.tau0[i]; e.dr = A.dr;
+            const ElemOut r = (sw == 1) ? solve_high_v(e, dt) : solve_low_v(e, dt);
+            A.V[w + i] = r.v; A.F[w + i] = r.f; A.D[w + i] = r.d; A.TH[w + i] = r.th; A.SIG[w + i] = r.sig;
+            ok = r.ok;
+            any4 = r.instab == 4;
+            mindt = r.dt;
+            // SolverSwitch if the step is retried (:242-249)
+            A.sw[i] = r.instab == 3 ? 1 : (r.instab == 4 ? 0 : sw);
+            if (r.stall) atomicAdd((unsigned long long*)&st->stalls, (unsigned long long)r.stall);
+            vn = r.v;
+            vmax = vn;
+            if (record) {
+                A.h_v[rec + i * A.rec_cap] = vn;
+                A.h_d[rec + i * A.rec_cap] = r.d;
+                A.h_th[rec + i * A.rec_cap] = r.th;
+                A.h_sig[rec + i * A.rec_cap] = r.sig;
+            }
+        } else {
+            vn = A.lrate[i];
+            const double d = A.D[o + i] + vn * dt;
+            A.V[w + i] = vn; A.F[w + i] = A.F[o + i]; A.D[w + i] = d; A.TH[w + i] = A.TH[o + i];
+            A.SIG[w + i] = 1e10;
+            mindt = dt;
+            if (record) {
+                A.h_v[rec + i * A.rec_cap] = vn;
+                A.h_d[rec + i * A.rec_cap] = d;
+                A.h_th[rec + i * A.rec_cap] = A.TH[o + i];
+                A.h_sig[rec + i * A.rec_cap] = 1e10;
+            }
         }
-        A.accel[i] = (v - A.V[o + i]) / dt;
+        // quantities of the next step's setup, valid if this attempt is accepted
+        const double acc = (vn - v_old) / dt;
+        A.accel[i] = acc;
+        if (A.cand[i]) {
+            vmax_next = vn;
+            const double r = fabs(vn / acc);
+            if (!(isnan(r) || r == 0.0)) {
+                dtmin_next = r;
+                found = 1;
+            }
+        }
     }
-    __syncthreads();
+    ok = block_reduce<kStepBlock>(ok, AndOp(), shi);
+    any4 = block_reduce<kStepBlock>(any4, OrOp(), shi);
+    found = block_reduce<kStepBlock>(found, OrOp(), shi);
+    mindt = block_reduce<kStepBlock>(mindt, MinOp(), shd);
+    vmax = block_reduce<kStepBlock>(vmax, MaxOp(), shd);
+    vmax_next = block_reduce<kStepBlock>(vmax_next, MaxOp(), shd);
+    dtmin_next = block_reduce<kStepBlock>(dtmin_next, MinOp(), shd);
     if (threadIdx.x == 0) {
-        if (st->ntrace < A.trace_cap) {
-            A.t_told[st->ntrace] = told;
-            A.t_dt[st->ntrace] = dt;
-            A.t_vmax[st->ntrace] = vmax;
-            st->ntrace += 1;
-        }
-        if (record) {
-            A.h_time[rec] = told;
-            A.h_dt[rec] = dt;
-            st->rec = rec + 1;
-        }
-        st->dt = dt;
-        st->told = told + dt;
-        st->parity = 1 - p;
-        st->step = step;
-        st->need_coupling = 0;
-        s_skip = step >= st->target;
-        if (s_skip) st->done = 1;
+        BlockPart& bp = A.parts[blockIdx.x];
+        bp.ok = ok; bp.any4 = any4; bp.found = found;
+        bp.mindt = mindt; bp.vmax = vmax; bp.vmax_next = vmax_next; bp.dtmin_next = dtmin_next;
+        __threadfence();
+        const unsigned prev = atomicAdd(A.arrive, 1u);
+        s_last = prev == gridDim.x - 1;
     }
     __syncthreads();
-    if (s_skip) return;
-    step_setup(A, st, 1 - p, shd, shi);
+    if (!s_last || threadIdx.x != 0) return;
+    // ---- last workgroup: the decision of main() ----
+    __threadfence();
+    *A.arrive = 0u;
+    int OK = 1, ANY4 = 0, FOUND = 0;
+    double MINDT = INFINITY, VMAX = -INFINITY, VNEXT = -INFINITY, DNEXT = INFINITY;
+    for (unsigned k = 0; k < gridDim.x; ++k) {
+        const BlockPart bp = A.parts[k];
+        OK &= bp.ok; ANY4 |= bp.any4; FOUND |= bp.found;
+        MINDT = bp.mindt < MINDT ? bp.mindt : MINDT;
+        VMAX = bp.vmax > VMAX ? bp.vmax : VMAX;
+        VNEXT = bp.vmax_next > VNEXT ? bp.vmax_next : VNEXT;
+        DNEXT = bp.dtmin_next < DNEXT ? bp.dtmin_next : DNEXT;
+    }
+    if (!OK) {  // retry the step with the reduced Dt (:240-258)
+        st->dt = ANY4 ? 1e-3 : MINDT;
+        st->need_coupling = 0;
+        st->retry_passes += 1;
+        if (st->iteration > 100) {
+            st->status = QDFN_ERR_RETRIES;
+            st->done = 1;
+        }
+        st->iteration += 1;
+        return;
+    }
+    // accept (:262-319); Dt = min(Dt_All) == dt when every element converged
+    const double dta = MINDT;
+    if (st->ntrace < A.trace_cap) {
+        A.t_told[st->ntrace] = told;
+        A.t_dt[st->ntrace] = dta;
+        A.t_vmax[st->ntrace] = VMAX;
+        st->ntrace += 1;
+    }
+    if (record) {
+        A.h_time[rec] = told;
+        A.h_dt[rec] = dta;
+        st->rec = rec + 1;
+    }
+    st->told = told + dta;
+    st->parity = 1 - p;
+    st->step = step + 1;
+    if (step + 1 >= st->target) {
+        st->dt = dta;
+        st->done = 1;
+        st->need_coupling = 0;
+        return;
+    }
+    if (!FOUND) {
+        st->dt = dta;
+        st->status = QDFN_ERR_EMPTY_DTMIN;
+        st->done = 1;
+        st->need_coupling = 0;
+        return;
+    }
+    st->dt = select_dt(A, dta, VNEXT, DNEXT / A.dt_cut);
+    st->iteration = 1;
+    st->need_coupling = 1;
+    st->rhs_evals += 1;
 }
 
 }  // namespace qdfn
@@ -285,7 +325,7 @@ struct qdfn_sim {
     qdfn_dense* op = nullptr;
     qdfn_sim_params P{};
     hipStream_t stream = nullptr;
-    DevBuf consts, state2, single, ints, cand, ext, hist, trace, dstate, ranlog;
+    DevBuf consts, state2, single, ints, cand, ext, hist, trace, dstate, ranlog, parts;
     SimArrays A{};
     SimState* d_st = nullptr;
     SimState h_st{};
@@ -299,34 +339,22 @@ struct qdfn_sim {
     int64_t coup_count = 0;
 };
 
-// The coupling input is DispOld, which lives in the D buffer selected by the
-// device parity.  To keep the captured launches parity-agnostic, the coupling
-// reads a dedicated vector refreshed here on the first attempt of each step.
-// slot >= 0 marks in ran_log whether this attempt's coupling will run.
-__global__ void k_copy_dold(SimArrays A, SimState* st, int slot) {
-    const int run = !st->done && st->need_coupling;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= 0 && i == 0) A.ran_log[slot] = run;
-    if (!run || i >= A.n) return;
-    A.dold[i] = A.D[(int64_t)st->parity * A.n + i];
-}
-
 static int enqueue_attempt(qdfn_sim* s, int slot, hipEvent_t ev0, hipEvent_t ev1) {
     const int64_t n = s->P.n;
-    const unsigned eb = (unsigned)((n + 255) / 256);
-    double* d_dold = s->A.dold;
-    hipLaunchKernelGGL(k_copy_dold, dim3(eb), dim3(256), 0, s->stream, s->A, s->d_st, slot);
-    QDFN_HIP(hipGetLastError());
-    const int* gate = &s->d_st->need_coupling;
+    CouplingGate g;
+    g.gate = &s->d_st->need_coupling;
+    g.parity = &s->d_st->parity;
+    g.parity_stride = n;
+    g.ran_log = slot >= 0 ? s->A.ran_log : nullptr;
+    g.slot = slot;
     if (ev0) QDFN_HIP(hipEventRecord(ev0, s->stream));
-    int rc = launch_coupling(s->op, d_dold, gate, s->stream);
+    int rc = launch_coupling(s->op, s->A.D, g, s->stream);
     if (rc) return rc;
     if (ev1) QDFN_HIP(hipEventRecord(ev1, s->stream));
-    rc = launch_coupling_finish(s->op, s->A.tl, s->A.sigprod, gate, s->stream);
+    rc = launch_coupling_finish(s->op, s->A.tl, s->A.sigprod, g.gate, s->stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_solve, dim3(eb), dim3(256), 0, s->stream, s->A, s->d_st);
-    QDFN_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_control, dim3(1), dim3(kCtl), 0, s->stream, s->A, s->d_st, 1);
+    hipLaunchKernelGGL(k_step, dim3((unsigned)((n + kStepBlock - 1) / kStepBlock)), dim3(kStepBlock), 0, s->stream,
+                       s->A, s->d_st);
     QDFN_HIP(hipGetLastError());
     return QDFN_OK;
 }
@@ -401,11 +429,14 @@ extern "C" qdfn_sim_t* qdfn_sim_create_dense(qdfn_dense_t* op, const qdfn_sim_pa
     if (s->state2.alloc(sizeof(double) * 10 * n) ||
         hipMemcpy(s->state2.p, h0.data(), sizeof(double) * 10 * n, hipMemcpyHostToDevice))
         return bail("state upload");
-    // single: accel, tl, sigprod, dt_all, dold
-    if (s->single.alloc(sizeof(double) * 5 * n) || hipMemset(s->single.p, 0, sizeof(double) * 5 * n))
+    // single: accel, tl, sigprod
+    if (s->single.alloc(sizeof(double) * 3 * n) || hipMemset(s->single.p, 0, sizeof(double) * 3 * n))
         return bail("single alloc");
-    if (s->ints.alloc(sizeof(int32_t) * 3 * n) || hipMemset(s->ints.p, 0, sizeof(int32_t) * 3 * n))
+    // sw[n] + arrival counter
+    if (s->ints.alloc(sizeof(int32_t) * (n + 1)) || hipMemset(s->ints.p, 0, sizeof(int32_t) * (n + 1)))
         return bail("int alloc");
+    const int64_t nblk = (n + kStepBlock - 1) / kStepBlock;
+    if (s->parts.alloc(sizeof(BlockPart) * nblk)) return bail("partials alloc");
     if (s->cand.alloc(n) || hipMemcpy(s->cand.p, cand.data(), n, hipMemcpyHostToDevice)) return bail("cand");
     const int64_t rc_cap = p->record_capacity, tc = p->trace_capacity;
     if (s->hist.alloc(sizeof(double) * (2 * rc_cap + 4 * rc_cap * n + 1))) return bail("history alloc");
@@ -426,12 +457,14 @@ extern "C" qdfn_sim_t* qdfn_sim_create_dense(qdfn_dense_t* op, const qdfn_sim_pa
     double* sd = s->state2.as<double>();
     A.V = sd; A.D = sd + 2 * n; A.TH = sd + 4 * n; A.F = sd + 6 * n; A.SIG = sd + 8 * n;
     double* sg = s->single.as<double>();
-    A.accel = sg; A.tl = sg + n; A.sigprod = sg + 2 * n; A.dt_all = sg + 3 * n; A.dold = sg + 4 * n;
+    A.accel = sg; A.tl = sg + n; A.sigprod = sg + 2 * n;
     if (s->ranlog.alloc(sizeof(int32_t) * kTimedBatch) || hipMemset(s->ranlog.p, 0, sizeof(int32_t) * kTimedBatch))
         return bail("log alloc");
     A.ran_log = s->ranlog.as<int32_t>();
     int32_t* si = s->ints.as<int32_t>();
-    A.instab = si; A.flag = si + n; A.sw = si + 2 * n;
+    A.sw = si;
+    A.arrive = reinterpret_cast<unsigned int*>(si + n);
+    A.parts = s->parts.as<BlockPart>();
     A.nt = 0; A.ext_t = A.ext_n = A.ext_s = nullptr;
     A.rec_cap = rc_cap; A.trace_cap = tc; A.record_step = p->record_step;
     double* hh = s->hist.as<double>();
@@ -478,7 +511,7 @@ extern "C" int qdfn_sim_run(qdfn_sim_t* s, int64_t n_steps, int32_t mode, int32_
     QDFN_HIP(hipMemcpy(s->d_st, &s->h_st, sizeof(SimState), hipMemcpyHostToDevice));
     s->started = true;
     // step setup for the first step of this call
-    hipLaunchKernelGGL(k_control, dim3(1), dim3(kCtl), 0, s->stream, s->A, s->d_st, 0);
+    hipLaunchKernelGGL(k_setup, dim3(1), dim3(kSetupBlock), 0, s->stream, s->A, s->d_st);
     QDFN_HIP(hipGetLastError());
     if (mode == 0) {
         const int K = std::max(1, graph_attempts > 0 ? graph_attempts : 32);

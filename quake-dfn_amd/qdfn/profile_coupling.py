@@ -33,7 +33,7 @@ def main():
         import torch
         x = torch.empty(a.copy_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
-        y.copy_(x)
+        torch.mul(x, 1.0, out=y)  # elementwise kernel: reads and writes copy_mib MiB
         torch.cuda.synchronize()
         print(f"copy {a.copy_mib} MiB done", flush=True)
     op.close()

@@ -44,7 +44,10 @@ def test_displacement_jump_equals_slip(orc, dip, disl):
     Up = dc3d(orc, *(pts + h * n).T, dip, disl)
     Um = dc3d(orc, *(pts - h * n).T, dip, disl)
     jump = Up[:, :3] - Um[:, :3]
-    b = -(disl[0] * np.array([1.0, 0, 0]) + disl[1] * s_dip + disl[2] * n)
+    # n points into the footwall, so the jump is footwall minus hanging wall:
+    # Okada's positive DISL1 is left-lateral (hanging wall moves +x), positive
+    # DISL2 is reverse (hanging wall moves up-dip), positive DISL3 is opening.
+    b = -disl[0] * np.array([1.0, 0, 0]) - disl[1] * s_dip + disl[2] * n
     np.testing.assert_allclose(jump, np.broadcast_to(b, jump.shape), atol=2e-6)
 
 
@@ -133,11 +136,18 @@ def _plane(n1, n2, ds, strike=30.0, dip=70.0, center=(100.0, -50.0, 1500.0), rak
     return configs.BulkToSegment(bulk)
 
 
-def test_stiffness_reciprocity_coplanar(orc):
-    seg = _plane(6, 4, 100.0)
+@pytest.mark.parametrize("rake", [180.0, 90.0])
+def test_stiffness_reciprocity_along_strike(orc, rake):
+    """Identical elements in one row along strike: translation along strike and
+    the reflection x -> -x are symmetries of the half-space, so collocated
+    K_ij = K_ji exactly (up to rounding)."""
+    seg = _plane(10, 1, 100.0, rake=rake)
     KS, KN = orc.stiffness_rect(seg, seg, MU, NU)
     assert np.all(np.diag(KS) < 0)
-    np.testing.assert_allclose(KS, KS.T, rtol=1e-8, atol=1e-9 * np.abs(KS).max())
+    np.testing.assert_allclose(KS, KS.T, rtol=1e-9, atol=1e-10 * np.abs(KS).max())
+    # normal traction from strike slip is odd under the reflection, from dip slip even
+    sign = -1.0 if rake == 180.0 else 1.0
+    np.testing.assert_allclose(KN, sign * KN.T, rtol=1e-9, atol=1e-10 * np.abs(KS).max())
 
 
 def test_stiffness_translation_and_rotation_invariance(orc):
