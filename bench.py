@@ -45,21 +45,31 @@ def build_problem(qd, cfg):
     return ops, prob
 
 
-def run_timed(qd, op, prob, steps, warmup, graph_attempts):
-    sim = qd.Simulation(op, prob, record_step=0, record_capacity=0, trace_capacity=steps + warmup)
+def run_timed(qd, op, prob, steps, warmup, graph_attempts, kernel_steps):
+    """Warm up, then the timed region: `steps` accepted steps replayed from the
+    captured attempt graph (wall clock around a synchronous call), followed by
+    `kernel_steps` steps launched directly with HIP events around every
+    coupling kernel on the run's stream (per-kernel durations)."""
+    sim = qd.Simulation(op, prob, record_step=0, record_capacity=0,
+                        trace_capacity=steps + warmup + kernel_steps)
     if warmup:
         sim.run(warmup, graph_attempts=graph_attempts)
     s0 = sim.stats()
     t0 = time.perf_counter()
-    sim.run(steps, timed=True)
+    sim.run(steps, graph_attempts=graph_attempts)
     el = time.perf_counter() - t0
     s1 = sim.stats()
+    t1 = time.perf_counter()
+    sim.run(kernel_steps, timed=True)
+    el_k = time.perf_counter() - t1
+    s2 = sim.stats()
     tr = sim.trace()
     sim.close()
     rhs = s1["rhs_evals"] - s0["rhs_evals"]
     return dict(elapsed=el, rhs=rhs, retries=s1["retry_passes"] - s0["retry_passes"],
-                sim_seconds=s1["t"] - s0["t"], coupling_ms=s1["coupling_ms_avg"],
-                coupling_timed=s1["coupling_timed"], trace=tr, status=s1["status"])
+                sim_seconds=s1["t"] - s0["t"], coupling_ms=s2["coupling_ms_avg"],
+                coupling_timed=s2["coupling_timed"], trace=tr, status=s2["status"],
+                direct_evals_per_s=(s2["rhs_evals"] - s1["rhs_evals"]) / el_k)
 
 
 def cpu_baseline(ops, prob, n_steps):
@@ -88,6 +98,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--graph-attempts", type=int, default=32)
+    ap.add_argument("--kernel-steps", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -108,14 +119,15 @@ def main():
     res = {}
     for name in ("fp64", "fp32"):
         log(f"[bench] {name}: warmup {args.warmup} steps, timed {args.steps} steps")
-        r = run_timed(qdfn, ops[name], prob, args.steps, args.warmup, args.graph_attempts)
+        r = run_timed(qdfn, ops[name], prob, args.steps, args.warmup, args.graph_attempts, args.kernel_steps)
         if r["status"] != 0:
             raise SystemExit(f"{name} run failed with status {r['status']}")
         r["evals_per_s"] = r["rhs"] / r["elapsed"]
         r["sim_years_per_hour"] = r["sim_seconds"] / configs.SEC_PER_YEAR / (r["elapsed"] / 3600.0)
         log(f"[bench] {name}: {r['rhs']} RHS evals in {r['elapsed']:.3f}s = {r['evals_per_s']:.1f}/s, "
             f"coupling {r['coupling_ms']*1e3:.1f} us avg over {r['coupling_timed']}, "
-            f"sim {r['sim_seconds']/configs.SEC_PER_YEAR:.4f} yr, retries {r['retries']}")
+            f"sim {r['sim_seconds']/configs.SEC_PER_YEAR:.4f} yr, retries {r['retries']}, "
+            f"direct-launch {r['direct_evals_per_s']:.1f}/s")
         res[name] = r
 
     # event-time error between mixed and FP64 over the whole run

@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_build(const double* __restrict__ KS, co
 static int choose_splits(qdfn_dense* op) {
     const int R = op->prec == QDFN_PREC_FP64 ? 2 : 4;
     const int64_t rb = (op->ld + 256 * R - 1) / (256 * R);
-    int64_t target = 512;
+    int64_t target = 256;
     if (const char* env = getenv("QDFN_COUPLING_WGS")) target = std::max<int64_t>(1, atoll(env));
     int64_t splits = std::max<int64_t>(1, (target + rb - 1) / rb);
     splits = std::min<int64_t>(splits, std::max<int64_t>(1, op->n / 32));

@@ -321,6 +321,19 @@ __global__ __launch_bounds__(kStepBlock) void k_step(SimArrays A, SimState* __re
 
 using namespace qdfn;
 
+struct GraphSlot {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int attempts = 0;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        attempts = 0;
+    }
+};
+
 struct qdfn_sim {
     qdfn_dense* op = nullptr;
     qdfn_sim_params P{};
@@ -330,10 +343,7 @@ struct qdfn_sim {
     SimState* d_st = nullptr;
     SimState h_st{};
     bool started = false;
-    // graph
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_attempts = 0;
+    GraphSlot graph;  // captured attempt sequence
     // timing of coupling kernels (mode 1)
     double coup_ms_sum = 0.0;
     int64_t coup_count = 0;
@@ -362,8 +372,7 @@ static int enqueue_attempt(qdfn_sim* s, int slot, hipEvent_t ev0, hipEvent_t ev1
 extern "C" void qdfn_sim_destroy(qdfn_sim_t* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->graph.reset();
     hipStream_t st = s->stream;
     delete s;
     if (st) (void)hipStreamDestroy(st);
@@ -513,13 +522,22 @@ extern "C" int qdfn_sim_run(qdfn_sim_t* s, int64_t n_steps, int32_t mode, int32_
     // step setup for the first step of this call
     hipLaunchKernelGGL(k_setup, dim3(1), dim3(kSetupBlock), 0, s->stream, s->A, s->d_st);
     QDFN_HIP(hipGetLastError());
+    auto take_times = [&](const std::vector<hipEvent_t>& ev, const std::vector<int32_t>& ran, int K) -> int {
+        for (int k = 0; k < K; ++k) {
+            if (!ran[k]) continue;  // gated off: the kernel returned at once
+            float ms = 0.f;
+            QDFN_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+            s->coup_ms_sum += ms;
+            s->coup_count += 1;
+        }
+        return QDFN_OK;
+    };
+    if (mode != 0 && mode != 1) return fail(QDFN_ERR_ARG, "qdfn_sim_run: mode must be 0 or 1");
     if (mode == 0) {
         const int K = std::max(1, graph_attempts > 0 ? graph_attempts : 32);
-        if (!s->gexec || s->graph_attempts != K) {
-            if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-            if (s->graph) (void)hipGraphDestroy(s->graph);
-            s->gexec = nullptr;
-            s->graph = nullptr;
+        GraphSlot& gs = s->graph;
+        if (!gs.exec || gs.attempts != K) {
+            gs.reset();
             QDFN_HIP(hipStreamSynchronize(s->stream));
             QDFN_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
             int rc = QDFN_OK;
@@ -528,12 +546,12 @@ extern "C" int qdfn_sim_run(qdfn_sim_t* s, int64_t n_steps, int32_t mode, int32_
             hipError_t e = hipStreamEndCapture(s->stream, &g);
             if (rc != QDFN_OK) return rc;
             if (e != hipSuccess) return fail(QDFN_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-            s->graph = g;
-            QDFN_HIP(hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0));
-            s->graph_attempts = K;
+            gs.graph = g;
+            QDFN_HIP(hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0));
+            gs.attempts = K;
         }
         for (;;) {
-            QDFN_HIP(hipGraphLaunch(s->gexec, s->stream));
+            QDFN_HIP(hipGraphLaunch(gs.exec, s->stream));
             QDFN_HIP(hipMemcpyAsync(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost, s->stream));
             QDFN_HIP(hipStreamSynchronize(s->stream));
             if (s->h_st.done) break;
@@ -551,14 +569,8 @@ extern "C" int qdfn_sim_run(qdfn_sim_t* s, int64_t n_steps, int32_t mode, int32_
             QDFN_HIP(hipMemcpyAsync(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost, s->stream));
             QDFN_HIP(hipMemcpyAsync(ran.data(), s->A.ran_log, sizeof(int32_t) * K, hipMemcpyDeviceToHost, s->stream));
             QDFN_HIP(hipStreamSynchronize(s->stream));
-            for (int k = 0; k < K; ++k) {
-                if (!ran[k]) continue;  // gated off: the kernel returned at once
-                float ms = 0.f;
-                QDFN_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
-                s->coup_ms_sum += ms;
-                s->coup_count += 1;
-            }
-            if (s->h_st.done) break;
+            rc = take_times(ev, ran, K);
+            if (rc || s->h_st.done) break;
         }
         for (auto& e : ev) (void)hipEventDestroy(e);
         if (rc) return rc;

@@ -136,9 +136,11 @@ class Simulation:
             check(lib().qdfn_sim_set_external(self._h, t.size, ptr(t), ptr(en), ptr(es)))
 
     def run(self, n_steps, timed=False, graph_attempts=32):
-        """Advance n_steps accepted steps.  timed=True launches attempts directly
-        with HIP events around every coupling kernel."""
-        rc = lib().qdfn_sim_run(self._h, int(n_steps), 1 if timed else 0, int(graph_attempts))
+        """Advance n_steps accepted steps.  timed=False: replay a captured graph
+        of attempts; timed=True: direct launches with HIP events around every
+        coupling kernel (fills stats()["coupling_ms_avg"])."""
+        mode = 1 if timed else 0
+        rc = lib().qdfn_sim_run(self._h, int(n_steps), mode, int(graph_attempts))
         if rc < 0:
             raise _lib.QdfnError(rc, _lib.last_error())
         return self.stats()

@@ -72,7 +72,9 @@ def test_graph_and_timed_modes_agree_bitwise(qd, fault):
         np.testing.assert_array_equal(a["trace"][k], b["trace"][k])
     for k in ("V", "Disp", "Theta"):
         np.testing.assert_array_equal(a["state"][k], b["state"][k])
-    assert b["stats"]["coupling_timed"] > 0 and b["stats"]["coupling_ms_avg"] > 0
+    # every accepted step ran exactly one coupling product
+    assert b["stats"]["coupling_timed"] == b["stats"]["rhs_evals"]
+    assert b["stats"]["coupling_ms_avg"] > 0
 
 
 def test_fp32_storage_first_event(qd, fault):
