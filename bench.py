@@ -25,6 +25,19 @@ import numpy as np  # noqa: E402
 
 METRIC = "RHS evals/sec + sim-years/hour, mixed vs FP64 on GPU at stated event-time error"
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(ROOT, "profiles", "coupling_fault_network_8192.md")
+
+
+def measured_traffic():
+    """HBM bytes per FP32 coupling launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated); None if absent."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith("traffic_bytes_per_launch_fp32:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -179,7 +192,7 @@ def main():
         "event_time_rel_err": ev_err,
         "sim_time_rel_dev": time_dev,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -1,0 +1,133 @@
+# QuakeDFNHIP.jl — ccall bindings of libqdfn_hip.so with the signatures of the
+# Quake-DFN Julia functions they replace.  `include` this file instead of
+# scripts/Functions_OKADA3D.jl / the assembly part of
+# scripts/Functions_BuildInputFile.jl / scripts/Functions_Solvers.jl, or call
+# RunDense(...) in place of main(...).  Julia is not available in this build's
+# container, so this file is not exercised by the test suite; the C ABI it
+# binds is (through ctypes).
+module QuakeDFNHIP
+
+const LIB = get(ENV, "QDFN_LIB", joinpath(@__DIR__, "..", "qdfn", "libqdfn_hip.so"))
+
+struct QdfnError <: Exception
+    code::Cint
+    msg::String
+end
+
+function check(rc)
+    if rc < 0
+        msg = unsafe_string(ccall((:qdfn_last_error, LIB), Cstring, ()))
+        throw(QdfnError(Cint(rc), msg))
+    end
+    return rc
+end
+
+# Functions_OKADA3D.jl:464
+function Okada_DC3D_Vector(ALPHA, X, Y, Z, DEPTH, DIP_Angle, AL1, AL2, AW1, AW2, DISL1, DISL2, DISL3)
+    X = Vector{Float64}(X); Y = Vector{Float64}(Y); Z = Vector{Float64}(Z)
+    n = length(X)
+    U = zeros(n, 12)
+    IRET = zeros(n)
+    GC.@preserve X Y Z U IRET begin
+        check(ccall((:qdfn_okada_dc3d, LIB), Cint,
+                    (Float64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Float64,
+                     Float64, Float64, Float64, Float64, Float64, Ptr{Float64}, Ptr{Float64}),
+                    ALPHA, n, X, Y, Z, DEPTH, DIP_Angle, AL1, AL2, AW1, AW2, DISL1, DISL2, DISL3, U, IRET))
+    end
+    return (U[:, 1], U[:, 2], U[:, 3], U[:, 4], U[:, 5], U[:, 6], U[:, 7], U[:, 8], U[:, 9], U[:, 10], U[:, 11],
+            U[:, 12], IRET)
+end
+
+# Functions_BuildInputFile.jl:386
+function StiffnessMatrix_ByParts_Calculation_Rec(Input_SegmentSource, Input_SegmentReceiver, ShearModulus,
+                                                 PoissonRatio, CurrentPart=1, TotalParts=1)
+    src = Matrix{Float64}(Input_SegmentSource); rcv = Matrix{Float64}(Input_SegmentReceiver)
+    ns, nr = size(src, 1), size(rcv, 1)
+    KS = zeros(nr, ns); KN = zeros(nr, ns)
+    GC.@preserve src rcv KS KN begin
+        check(ccall((:qdfn_okada_stiffness_rect, LIB), Cint,
+                    (Ptr{Float64}, Int64, Int64, Ptr{Float64}, Int64, Int64, Float64, Float64, Ptr{Float64},
+                     Ptr{Float64}, Int64),
+                    src, ns, ns, rcv, nr, nr, ShearModulus, PoissonRatio, KS, KN, nr))
+    end
+    return KS, KN
+end
+
+# Functions_BuildInputFile.jl:516 (one device launch covers the whole matrix)
+BuildMatrixByParts_Even_Rec(FaultCount, Input_Segment, ShearModulus, PoissonRatio) =
+    StiffnessMatrix_ByParts_Calculation_Rec(Input_Segment, Input_Segment, ShearModulus, PoissonRatio)
+
+# Functions_TDstressHS.jl:2 (P1..P3 are not mutated)
+function TDstressHS(X, Y, Z, P1, P2, P3, Ss, Ds, Ts, mu, lambda)
+    X = Vector{Float64}(X); Y = Vector{Float64}(Y); Z = Vector{Float64}(Z)
+    p1 = Vector{Float64}(P1); p2 = Vector{Float64}(P2); p3 = Vector{Float64}(P3)
+    n = length(X)
+    Stress = zeros(n, 6); Strain = zeros(n, 6)
+    GC.@preserve X Y Z p1 p2 p3 Stress Strain begin
+        check(ccall((:qdfn_tde_stress_hs, LIB), Cint,
+                    (Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                     Float64, Float64, Float64, Float64, Float64, Ptr{Float64}, Ptr{Float64}),
+                    n, X, Y, Z, p1, p2, p3, Ss, Ds, Ts, mu, lambda, Stress, Strain))
+    end
+    return Stress, Strain
+end
+
+# Functions_BuildInputFile.jl:551
+function StiffnessMatrix_ByParts_Calculation_Tri(P1, P2, P3, Rake, FaultCenter, ShearModulus, lambda,
+                                                 UnitVector_Normal, UnitVector_Slip)
+    P1 = Matrix{Float64}(P1); P2 = Matrix{Float64}(P2); P3 = Matrix{Float64}(P3)
+    Rake = Vector{Float64}(vec(Rake)); C = Matrix{Float64}(FaultCenter)
+    Nrm = Matrix{Float64}(UnitVector_Normal); Slp = Matrix{Float64}(UnitVector_Slip)
+    ns, nr = size(P1, 1), size(C, 1)
+    KS = zeros(nr, ns); KN = zeros(nr, ns)
+    GC.@preserve P1 P2 P3 Rake C Nrm Slp KS KN begin
+        check(ccall((:qdfn_tde_stiffness_tri, LIB), Cint,
+                    (Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64},
+                     Ptr{Float64}, Ptr{Float64}, Float64, Float64, Ptr{Float64}, Ptr{Float64}, Int64),
+                    ns, P1, P2, P3, Rake, nr, C, Nrm, Slp, ShearModulus, lambda, KS, KN, nr))
+    end
+    return KS, KN
+end
+
+struct StepParams
+    conv_crit::Float64
+    v0::Float64
+    mu::Float64
+    rho::Float64
+    switch_v::Float64
+    alpha_evo::Float64
+    evolution_dr::Int32
+    pad::Int32
+end
+
+# Functions_Solvers.jl:334 — same argument list; fills the output arrays in place
+function SolveOneTimeStep(ConvergenceCrit, DispOld, FrictionOld, ThetaOld, VOld, Friction0, a, b, Dc, V0, K_Self,
+                          Dt, Omega, Mass, ShearModulus, EffNormalStress_i, Total_Loading_Disp, SolverSwitch, V,
+                          Friction, Disp, Theta, EffNormalStress, Dt_All, InstabilityThistime, LoadingRate,
+                          LoadingFaultCount, FaultCount, RockDensity, D_EffStress_Shear, SwitchV, Alpha_Evo,
+                          EffNormalStress_Old, EvolutionDR, InitialShearStress)
+    f(x) = Vector{Float64}(vec(x))
+    p = Ref(StepParams(ConvergenceCrit, V0, ShearModulus, RockDensity, SwitchV, Alpha_Evo, Int32(EvolutionDR), 0))
+    ins = map(f, (DispOld, FrictionOld, ThetaOld, VOld, Friction0, a, b, Dc, K_Self))
+    om, ms, si, tl = map(f, (Omega, Mass, EffNormalStress_i, Total_Loading_Disp))
+    sw = Vector{Int32}(vec(SolverSwitch))
+    lr, dt, so, t0 = map(f, (LoadingRate, D_EffStress_Shear, EffNormalStress_Old, InitialShearStress))
+    n = FaultCount
+    outs = [zeros(n) for _ in 1:6]
+    inst = zeros(Int32, n); flag = zeros(Int32, n); its = zeros(Int32, n)
+    GC.@preserve p ins om ms si tl sw lr dt so t0 outs inst flag its begin
+        check(ccall((:qdfn_solve_one_timestep, LIB), Int64,
+                    (Ptr{StepParams}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                     Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Ptr{Float64},
+                     Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Float64}, Ptr{Float64},
+                     Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                     Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}),
+                    p, n, LoadingFaultCount, ins..., Dt, om, ms, si, tl, sw, lr, dt, so, t0, outs..., inst, flag,
+                    its))
+    end
+    V .= outs[1]; Friction .= outs[2]; Disp .= outs[3]; Theta .= outs[4]; EffNormalStress .= outs[5]
+    Dt_All .= outs[6]; InstabilityThistime .= inst
+    return V, Friction, Disp, Theta, EffNormalStress, Dt_All, InstabilityThistime, Float64.(flag)
+end
+
+end # module

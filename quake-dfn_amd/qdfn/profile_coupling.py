@@ -2,10 +2,13 @@
 
     python -m qdfn.profile_coupling [--prec fp32|fp64] [--reps 50] [--copy-mib 512]
 
-Builds the 8,192-element fault-network operator on the device, runs `reps`
-coupling products, then (optionally) one plain device-to-device copy of
---copy-mib MiB through torch so that HBM counters (FETCH_SIZE / WRITE_SIZE)
-can be calibrated against a kernel of known traffic.
+Builds the 8,192-element fault-network operator on the device and runs `reps`
+coupling products.  The operator build itself launches k_build, a plain
+streaming kernel of known traffic (reads K_S and K_N in FP64, writes E and
+K_N at the stored precision), against which the HBM counters (FETCH_SIZE /
+WRITE_SIZE) of the coupling kernel are calibrated.  --copy-mib adds a torch
+elementwise copy of that size as a second calibration kernel (it does not run
+under rocprofv3 counter collection on every image).
 """
 import argparse
 import sys
@@ -17,7 +20,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--prec", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--reps", type=int, default=50)
-    ap.add_argument("--copy-mib", type=int, default=512)
+    ap.add_argument("--copy-mib", type=int, default=0)
     a = ap.parse_args()
     import qdfn
     from qdfn import configs
