@@ -9,6 +9,8 @@ Format layer (reference file:line):
 Workloads (all arrays seeded with numpy PCG64):
   single_fault   one vertical strike-slip fault 3.2 x 1.6 km, 32 x 16 elements of 100 m
   fault_network  8 rectangular faults of 32 x 32 elements (8,192 elements)
+  rough_fault    a non-planar strike-slip surface with sinusoidal roughness meshed
+                 into triangles (100 x 100 cells of 50 m -> 20,000 triangles)
 """
 import numpy as np
 
@@ -214,3 +216,53 @@ def first_event_time(told, dt, vmax, threshold=1e-3):
     lt = np.log10(threshold)
     f = (lt - l0) / (l1 - l0) if l1 != l0 else 1.0
     return float(t_end[k - 1] + f * (t_end[k] - t_end[k - 1]))
+
+
+def rough_fault(nx=100, nz=100, h=50.0, top=100.0, seed=20240612, ratio=1e-2, n_modes=3, mu=32e9, nu=0.25,
+                vs=3464.0):
+    """Config: a vertical strike-slip surface y = sum_m A_m sin(2 pi k_m . (x, z) + phi_m)
+    (A_m / lambda_m = ratio, n_modes seeded modes), nx x nz cells of h metres,
+    each split into two triangles; top edge at depth `top` (triangle vertices
+    need z <= 0).  Returns vertex arrays P1, P2, P3 (n x 3, EFCS: z up) and
+    per-element friction drawn as in the fault network."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    Lx, Lz = nx * h, nz * h
+    lam = rng.uniform(0.2, 1.0, n_modes) * max(Lx, Lz)
+    ang = rng.uniform(0, np.pi, n_modes)
+    phi = rng.uniform(0, 2 * np.pi, n_modes)
+    amp = ratio * lam
+
+    def yoff(x, z):
+        out = np.zeros_like(x)
+        for m in range(n_modes):
+            kx, kz = np.cos(ang[m]) / lam[m], np.sin(ang[m]) / lam[m]
+            out += amp[m] * np.sin(2 * np.pi * (kx * x + kz * z) + phi[m])
+        return out
+
+    xs = np.arange(nx + 1) * h - Lx / 2
+    zs = -top - np.arange(nz + 1) * h
+    X, Z = np.meshgrid(xs, zs, indexing="ij")
+    Y = yoff(X, Z)
+    V = np.stack([X, Y, Z], -1)
+    tris = []
+    for i in range(nx):
+        for j in range(nz):
+            a, b, c, d = V[i, j], V[i + 1, j], V[i + 1, j + 1], V[i, j + 1]
+            tris.append((a, b, c))
+            tris.append((a, c, d))
+    P = np.array(tris)
+    P1, P2, P3 = P[:, 0].copy(), P[:, 1].copy(), P[:, 2].copy()
+    n = P1.shape[0]
+    a = rng.uniform(0.008, 0.012, n)
+    b = a + rng.uniform(-0.004, 0.006, n)
+    sigma = rng.uniform(30e6, 80e6, n)
+    rake = np.zeros(n)
+    return dict(name=f"rough_fault_{n}", P1=P1, P2=P2, P3=P3, rake=rake, a=a, b=b, sigma=sigma,
+                dc=np.full(n, 1e-3), mu=mu, nu=nu, rho=mu / vs ** 2, min_normal_stress=2e6, h=h)
+
+
+def triangle_lengths(P1, P2, P3):
+    """FaultLengthStrike of a triangle as ReadSegmentInput defines it
+    (Functions_BuildInputFile.jl:320): the largest |component| of the three
+    edge vectors."""
+    return np.max(np.abs(np.hstack([P1 - P2, P2 - P3, P1 - P3])), axis=1)
