@@ -25,7 +25,7 @@ __device__ __host__ inline double norm3(const double a[3]) { return sqrt(a[0] * 
 // Strike/dip/normal basis of a triangle (TDstressFS :149-166 /
 // TDstress_HarFunc :307-318).  image_flip: horizontal image triangles above
 // the surface reverse their strike vector.
-__device__ inline void tri_basis(const double P1[3], const double P2[3], const double P3[3], bool image_rule,
+__device__ __forceinline__ void tri_basis(const double P1[3], const double P2[3], const double P3[3], bool image_rule,
                                  double Vn[3], double Vs[3], double Vd[3]) {
     const double u[3] = {P2[0] - P1[0], P2[1] - P1[1], P2[2] - P1[2]};
     const double w[3] = {P3[0] - P1[0], P3[1] - P1[1], P3[2] - P1[2]};
@@ -45,7 +45,7 @@ __device__ inline void tri_basis(const double P1[3], const double P2[3], const d
     cross3(Vn, Vs, Vd);
 }
 
-__device__ inline void make_frame(const double P1[3], const double P2[3], const double P3[3], TdFrame& f) {
+__device__ __forceinline__ void make_frame(const double P1[3], const double P2[3], const double P3[3], TdFrame& f) {
     double Vn[3], Vs[3], Vd[3];
     tri_basis(P1, P2, P3, true, Vn, Vs, Vd);
     for (int k = 0; k < 3; ++k) {
@@ -56,29 +56,48 @@ __device__ inline void make_frame(const double P1[3], const double P2[3], const 
     }
     double d1[3] = {P1[0] - P2[0], P1[1] - P2[1], P1[2] - P2[2]};
     double d3[3] = {P3[0] - P2[0], P3[1] - P2[1], P3[2] - P2[2]};
+    double p1[3], p3[3];
     for (int r = 0; r < 3; ++r) {
-        f.p1[r] = f.A[3 * r] * d1[0] + f.A[3 * r + 1] * d1[1] + f.A[3 * r + 2] * d1[2];
-        f.p3[r] = f.A[3 * r] * d3[0] + f.A[3 * r + 1] * d3[1] + f.A[3 * r + 2] * d3[2];
+        p1[r] = f.A[3 * r] * d1[0] + f.A[3 * r + 1] * d1[1] + f.A[3 * r + 2] * d1[2];
+        p3[r] = f.A[3 * r] * d3[0] + f.A[3 * r + 1] * d3[1] + f.A[3 * r + 2] * d3[2];
     }
-    double s12[3], s13[3], s23[3];
+    double s12[3], s13[3], s23[3], e12[3], e13[3], e23[3];
     for (int k = 0; k < 3; ++k) {
-        s12[k] = 0.0 - f.p1[k];
-        s13[k] = f.p3[k] - f.p1[k];
-        s23[k] = f.p3[k] - 0.0;
+        s12[k] = 0.0 - p1[k];
+        s13[k] = p3[k] - p1[k];
+        s23[k] = p3[k] - 0.0;
     }
     const double n12 = norm3(s12), n13 = norm3(s13), n23 = norm3(s23);
     for (int k = 0; k < 3; ++k) {
-        f.e12[k] = s12[k] / n12;
-        f.e13[k] = s13[k] / n13;
-        f.e23[k] = s23[k] / n23;
+        e12[k] = s12[k] / n12;
+        e13[k] = s13[k] / n13;
+        e23[k] = s23[k] / n23;
     }
-    f.angA = acos(f.e12[0] * f.e13[0] + f.e12[1] * f.e13[1] + f.e12[2] * f.e13[2]);
-    f.angB = acos(-f.e12[0] * f.e23[0] - f.e12[1] * f.e23[1] - f.e12[2] * f.e23[2]);
-    f.angC = acos(f.e23[0] * f.e13[0] + f.e23[1] * f.e13[1] + f.e23[2] * f.e13[2]);
+    const double ang[3] = {acos(e12[0] * e13[0] + e12[1] * e13[1] + e12[2] * e13[2]),
+                           acos(-e12[0] * e23[0] - e12[1] * e23[1] - e12[2] * e23[2]),
+                           acos(e23[0] * e13[0] + e23[1] * e13[1] + e23[2] * e13[2])};
+    for (int k = 0; k < 3; ++k) {
+        const double al = -QDFN_PI + ang[k];
+        f.sinA[k] = sin(al);
+        f.cosA[k] = cos(al);
+    }
+    f.tv[0][0] = p1[1];
+    f.tv[0][1] = p1[2];
+    f.tv[1][0] = 0.0;
+    f.tv[1][1] = 0.0;
+    f.tv[2][0] = p3[1];
+    f.tv[2][1] = p3[2];
+    f.sv[0][0] = e13[1];
+    f.sv[0][1] = e13[2];
+    f.sv[1][0] = -e12[1];
+    f.sv[1][1] = -e12[2];
+    f.sv[2][0] = -e23[1];
+    f.sv[2][1] = -e23[2];
     f.flip_xz = 0;
+    f.pad = 0;
 }
 
-__device__ inline void make_edge(const double PA[3], const double PB[3], const double b[3], FscEdge& e) {
+__device__ __forceinline__ void make_edge(const double PA[3], const double PB[3], const double b[3], FscEdge& e) {
     const double sv[3] = {PB[0] - PA[0], PB[1] - PA[1], PB[2] - PA[2]};
     const double beta = acos(-sv[2] / norm3(sv));
     e.beta = beta;
@@ -146,7 +165,7 @@ __global__ void k_tri_prepare(int64_t ns, const double* __restrict__ P1, const d
 }
 
 // TensTrans (:338-362) with the matrix given by its three columns c0, c1, c2.
-__device__ inline void tens_rot(const double T[6], const double c0[3], const double c1[3], const double c2[3],
+__device__ __forceinline__ void tens_rot(const double T[6], const double c0[3], const double c1[3], const double c2[3],
                                 double O[6]) {
     const double A[9] = {c0[0], c0[1], c0[2], c1[0], c1[1], c1[2], c2[0], c2[1], c2[2]};
     const double xx = T[0], yy = T[1], zz = T[2], xy = T[3], xz = T[4], yz = T[5];
@@ -161,17 +180,18 @@ __device__ inline void tens_rot(const double T[6], const double c0[3], const dou
            (A[7] * A[5] + A[8] * A[4]) * yz + A[4] * A[5] * yy + A[7] * A[8] * zz;
 }
 
-// one angular dislocation of TDSetupS (:433-464), strain accumulated into acc (TDCS)
-__device__ inline void setup_add(double x, double y, double z, double alpha, double bx, double by, double bz,
-                                 double nu, const double tv[3], double s1, double s2, double acc[6]) {
-    // side vector (sv1, s1, s2): A = [s2 -s1; s1 s2]
-    const double y1 = s2 * (y - tv[1]) - s1 * (z - tv[2]);
-    const double z1 = s1 * (y - tv[1]) + s2 * (z - tv[2]);
+// one angular dislocation of TDSetupS (:433-464), strain accumulated into acc (TDCS);
+// (s1, s2) is the side direction (y, z), tv the vertex (y, z).
+__device__ __forceinline__ void setup_add(double x, double y, double z, double sinA, double cosA, double bx,
+                                          double by, double bz, double nu, double tv0, double tv1, double s1,
+                                          double s2, double acc[6]) {
+    // A = [s2 -s1; s1 s2]
+    const double y1 = s2 * (y - tv0) - s1 * (z - tv1);
+    const double z1 = s1 * (y - tv0) + s2 * (z - tv1);
     const double by1 = s2 * by - s1 * bz;
     const double bz1 = s1 * by + s2 * bz;
-    const double al = -QDFN_PI + alpha;
     double e[6], o[6];
-    ang_strain(x, y1, z1, sin(al), cos(al), bx, by1, bz1, nu, e);
+    ang_strain(x, y1, z1, sinA, cosA, bx, by1, bz1, nu, e);
     // B = [1 0 0; 0 s2 s1; 0 -s1 s2] by columns
     const double c0[3] = {1.0, 0.0, 0.0}, c1[3] = {0.0, s2, -s1}, c2[3] = {0.0, s1, s2};
     tens_rot(e, c0, c1, c2, o);
@@ -179,14 +199,14 @@ __device__ inline void setup_add(double x, double y, double z, double alpha, dou
 }
 
 // TDstressFS (:133-286) strain in EFCS at one point for one frame.
-__device__ inline void frame_strain(double X, double Y, double Z, const TdFrame& f, double bx, double by, double bz,
-                                    double nu, double E[6]) {
+__device__ __forceinline__ void frame_strain(double X, double Y, double Z, const TdFrame& f, double bx, double by,
+                                             double bz, double nu, double E[6]) {
     const double d[3] = {X - f.P2[0], Y - f.P2[1], Z - f.P2[2]};
     const double x = f.A[0] * d[0] + f.A[1] * d[1] + f.A[2] * d[2];
     const double y = f.A[3] * d[0] + f.A[4] * d[1] + f.A[5] * d[2];
     const double z = f.A[6] * d[0] + f.A[7] * d[1] + f.A[8] * d[2];
     // trimodefinder(y, z, x, p1[2:3], p2[2:3], p3[2:3]) with p2 = 0 (:393-430)
-    const double a1 = f.p1[1], a2 = f.p1[2], c1 = f.p3[1], c2 = f.p3[2];
+    const double a1 = f.tv[0][0], a2 = f.tv[0][1], c1 = f.tv[2][0], c2 = f.tv[2][1];
     const double den = (0.0 - c2) * (a1 - c1) + (c1 - 0.0) * (a2 - c2);
     const double la = ((0.0 - c2) * (y - c1) + (c1 - 0.0) * (z - c2)) / den;
     const double lb = ((c2 - a2) * (y - c1) + (a1 - c1) * (z - c2)) / den;
@@ -203,76 +223,93 @@ __device__ inline void frame_strain(double X, double Y, double Z, const TdFrame&
     if (m == 0) {
         for (int k = 0; k < 6; ++k) e[k] = NAN;
     } else {
-        const double sg = (m == 1) ? -1.0 : 1.0;  // configuration I uses -e13, e12, e23; II uses e13, -e12, -e23
-        const double p2[3] = {0.0, 0.0, 0.0};
-        setup_add(x, y, z, f.angA, bx, by, bz, nu, f.p1, sg * f.e13[1], sg * f.e13[2], e);
-        setup_add(x, y, z, f.angB, bx, by, bz, nu, p2, -sg * f.e12[1], -sg * f.e12[2], e);
-        setup_add(x, y, z, f.angC, bx, by, bz, nu, f.p3, -sg * f.e23[1], -sg * f.e23[2], e);
+        // configuration I (m = 1) uses -e13, e12, e23; II uses e13, -e12, -e23
+        const double sg = (m == 1) ? -1.0 : 1.0;
+#pragma unroll 1
+        for (int k = 0; k < 3; ++k)
+            setup_add(x, y, z, f.sinA[k], f.cosA[k], bx, by, bz, nu, f.tv[k][0], f.tv[k][1], sg * f.sv[k][0],
+                      sg * f.sv[k][1], e);
     }
     const double Vn[3] = {f.A[0], f.A[1], f.A[2]}, Vs[3] = {f.A[3], f.A[4], f.A[5]}, Vd[3] = {f.A[6], f.A[7], f.A[8]};
     tens_rot(e, Vn, Vs, Vd, E);
 }
 
-// AngSetupFSC_S (:467-574) strain at one point for one edge.
-__device__ inline void edge_fsc_strain(double X, double Y, double Z, const FscEdge& g, double nu, double E[6]) {
+// AngSetupFSC_S (:467-574) strain at one point for one edge: v = v(PB) - v(PA),
+// both evaluated in the configuration chosen by the sign of beta * y1 at PA.
+__device__ __forceinline__ void edge_fsc_strain(double X, double Y, double Z, const FscEdge& g, double nu,
+                                                double E[6]) {
     if (g.degenerate) {
         for (int k = 0; k < 6; ++k) E[k] = 0.0;
         return;
     }
     const double dx = X - g.PA[0], dy = Y - g.PA[1], dz = Z - g.PA[2];
     const double yA[3] = {g.ey1x * dx + g.ey1y * dy, g.ey1y * dx - g.ey1x * dy, -dz};
-    const double yB[3] = {yA[0] - g.yAB[0], yA[1] - g.yAB[1], yA[2] - g.yAB[2]};
-    double vA[6], vB[6];
-    if (g.beta * yA[0] >= 0) {
-        ang_strain_fsc(-yA[0], -yA[1], yA[2], g.sinB2, g.cosB2, g.cotB2, -g.bb[0], -g.bb[1], g.bb[2], nu, -g.PA[2], vA);
-        ang_strain_fsc(-yB[0], -yB[1], yB[2], g.sinB2, g.cosB2, g.cotB2, -g.bb[0], -g.bb[1], g.bb[2], nu, -g.PB3, vB);
-        vA[4] = -vA[4];
-        vA[5] = -vA[5];
-        vB[4] = -vB[4];
-        vB[5] = -vB[5];
-    } else {
-        ang_strain_fsc(yA[0], yA[1], yA[2], g.sinB, g.cosB, g.cotB, g.bb[0], g.bb[1], g.bb[2], nu, -g.PA[2], vA);
-        ang_strain_fsc(yB[0], yB[1], yB[2], g.sinB, g.cosB, g.cotB, g.bb[0], g.bb[1], g.bb[2], nu, -g.PB3, vB);
+    const bool cfg1 = g.beta * yA[0] >= 0;
+    // configuration I: mirrored angle pi - beta, (y1, y2, b1, b2) negated, xz and yz negated
+    const double sgn = cfg1 ? -1.0 : 1.0;
+    const double sinB = cfg1 ? g.sinB2 : g.sinB, cosB = cfg1 ? g.cosB2 : g.cosB, cotB = cfg1 ? g.cotB2 : g.cotB;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+    for (int p = 0; p < 2; ++p) {
+        const double y0 = p ? yA[0] - g.yAB[0] : yA[0];
+        const double y1 = p ? yA[1] - g.yAB[1] : yA[1];
+        const double y2 = p ? yA[2] - g.yAB[2] : yA[2];
+        double w[6];
+        ang_strain_fsc(sgn * y0, sgn * y1, y2, sinB, cosB, cotB, sgn * g.bb[0], sgn * g.bb[1], g.bb[2], nu,
+                       p ? -g.PB3 : -g.PA[2], w);
+        if (cfg1) {
+            w[4] = -w[4];
+            w[5] = -w[5];
+        }
+        for (int k = 0; k < 6; ++k) v[k] = p ? v[k] + w[k] : v[k] - w[k];
     }
-    double v[6];
-    for (int k = 0; k < 6; ++k) v[k] = vB[k] - vA[k];
     // TensTrans with A' (= A, symmetric): columns (ey1x, ey1y, 0), (ey1y, -ey1x, 0), (0, 0, -1)
     const double c0[3] = {g.ey1x, g.ey1y, 0.0}, c1[3] = {g.ey1y, -g.ey1x, 0.0}, c2[3] = {0.0, 0.0, -1.0};
     tens_rot(v, c0, c1, c2, E);
 }
 
-// TDstressHS (:2-130): strain and stress at one point (EFCS, xx yy zz xy xz yz).
-__device__ inline void tde_point(double X, double Y, double Z, const TriSource& t, double mu, double lam, double S[6],
-                                 double E[6]) {
-    double Ems[6], Eis[6], Ef[6];
-    frame_strain(X, Y, Z, t.main, t.bx, t.by, t.bz, t.nu, Ems);
-    frame_strain(X, Y, Z, t.image, t.bx, t.by, t.bz, t.nu, Eis);
-    if (t.image.flip_xz) {
-        Eis[4] = -Eis[4];
-        Eis[5] = -Eis[5];
-    }
-    double Ek[6];
-    for (int k = 0; k < 6; ++k) Ef[k] = 0.0;
-    for (int j = 0; j < 3; ++j) {
-        edge_fsc_strain(X, Y, Z, t.edge[j], t.nu, Ek);
-        for (int k = 0; k < 6; ++k) Ef[k] += Ek[k];
-    }
-    double Sm[6] = {0}, Si[6] = {0}, Sf[6] = {0};
-    auto hooke = [&](const double e[6], double s[6]) {
-        const double tr = e[0] + e[1] + e[2];
-        s[0] = 2 * mu * e[0] + lam * tr;
-        s[1] = 2 * mu * e[1] + lam * tr;
-        s[2] = 2 * mu * e[2] + lam * tr;
-        s[3] = 2 * mu * e[3];
-        s[4] = 2 * mu * e[4];
-        s[5] = 2 * mu * e[5];
-    };
-    hooke(Ems, Sm);
-    hooke(Eis, Si);
-    hooke(Ef, Sf);
+// Hooke's law (TDstressHS :122-129 and AngSetupFSC_S :566-572), accumulated.
+__device__ __forceinline__ void add_hooke(const double e[6], double mu, double lam, double S[6], double E[6]) {
+    const double tr = e[0] + e[1] + e[2];
+    S[0] += 2 * mu * e[0] + lam * tr;
+    S[1] += 2 * mu * e[1] + lam * tr;
+    S[2] += 2 * mu * e[2] + lam * tr;
+    S[3] += 2 * mu * e[3];
+    S[4] += 2 * mu * e[4];
+    S[5] += 2 * mu * e[5];
+    for (int k = 0; k < 6; ++k) E[k] += e[k];
+}
+
+// TDstressHS (:2-130): strain and stress at one point (EFCS, xx yy zz xy xz yz)
+// = main + image (full space, per frame) + the harmonic free-surface correction
+// (the sum of the three edges' stresses and strains).
+__device__ __forceinline__ void tde_point(double X, double Y, double Z, const TriSource& t, double mu, double lam,
+                                          double S[6], double E[6]) {
     for (int k = 0; k < 6; ++k) {
-        S[k] = Sm[k] + Si[k] + Sf[k];
-        E[k] = Ems[k] + Eis[k] + Ef[k];
+        S[k] = 0.0;
+        E[k] = 0.0;
+    }
+#pragma unroll 1
+    for (int fi = 0; fi < 2; ++fi) {
+        const TdFrame& f = fi ? t.image : t.main;
+        double e[6];
+        frame_strain(X, Y, Z, f, t.bx, t.by, t.bz, t.nu, e);
+        if (f.flip_xz) {
+            e[4] = -e[4];
+            e[5] = -e[5];
+        }
+        add_hooke(e, mu, lam, S, E);
+    }
+    double Sf[6] = {0, 0, 0, 0, 0, 0}, Ef[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll 1
+    for (int j = 0; j < 3; ++j) {
+        double Ek[6];
+        edge_fsc_strain(X, Y, Z, t.edge[j], t.nu, Ek);
+        add_hooke(Ek, mu, lam, Sf, Ef);
+    }
+    for (int k = 0; k < 6; ++k) {
+        S[k] += Sf[k];
+        E[k] += Ef[k];
     }
 }
 
@@ -282,9 +319,8 @@ __global__ __launch_bounds__(256) void k_tde_points(int64_t n, const double* __r
                                                     double* __restrict__ S, double* __restrict__ E) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const TriSource t = src[0];
     double s[6], e[6];
-    tde_point(X[i], Y[i], Z[i], t, mu, lam, s, e);
+    tde_point(X[i], Y[i], Z[i], src[0], mu, lam, s, e);
     for (int k = 0; k < 6; ++k) {
         S[i + k * n] = s[k];
         E[i + k * n] = e[k];

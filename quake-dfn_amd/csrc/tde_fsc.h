@@ -10,7 +10,10 @@
 //     frames of the three edges, slip vectors) is computed once per source by
 //     k_tri_prepare into a TriSource record that a workgroup reads uniformly;
 //   * the per-pair work is the six angular-dislocation strains, the six
-//     free-surface-correction strains and the tensor transforms, in FP64;
+//     free-surface-correction strains and the tensor transforms, in FP64,
+//     written as loops over one inlined copy of each strain expression: the
+//     code stays small enough for short branches and is inlined into the
+//     kernels (no device-function call, so no return address to keep);
 //   * the free-surface-correction expressions are evaluated with their
 //     repeated brackets computed once (T*, U*, V* temporaries below).
 #pragma once
@@ -23,9 +26,10 @@ namespace qdfn {
 struct TdFrame {          // one triangle (main or image) in its own TDCS
     double A[9];          // rows Vnorm, Vstrike, Vdip (EFCS -> TDCS), row-major
     double P2[3];         // origin of the TDCS (second vertex), EFCS
-    double p1[3], p3[3];  // first and third vertices in TDCS (p2 = 0)
-    double e12[3], e13[3], e23[3];
-    double angA, angB, angC;
+    double tv[3][2];      // (y, z) of the vertices A = p1, B = p2 = 0, C = p3 in TDCS
+    double sinA[3], cosA[3];  // sin, cos of (angle - pi) at A, B, C (TDSetupS :440)
+    double sv[3][2];      // (y, z) side directions at A, B, C for configuration II:
+                          // e13, -e12, -e23 (configuration I negates all three)
     int flip_xz;          // image of a surface triangle: negate xz, yz
     int pad;
 };
@@ -51,7 +55,7 @@ struct TriSource {
 __device__ __host__ inline double dsq(double x) { return x * x; }
 
 // AngDisStrain (:577-651): strains of an angular dislocation in a full space.
-__device__ inline void ang_strain(double x, double y, double z, double sinA, double cosA, double bx, double by,
+__device__ __forceinline__ void ang_strain(double x, double y, double z, double sinA, double cosA, double bx, double by,
                                   double bz, double nu, double e[6]) {
     const double eta = y * cosA - z * sinA;
     const double zeta = y * sinA + z * cosA;
@@ -102,7 +106,7 @@ __device__ inline void ang_strain(double x, double y, double z, double sinA, dou
 
 // AngDisStrainFSC (:654-1034): harmonic free-surface correction of an angular
 // dislocation; sinB/cosB/cotB describe its angle, a its vertex depth.
-__device__ inline void ang_strain_fsc(double y1, double y2, double y3, double sinB, double cosB, double cotB,
+__device__ __forceinline__ void ang_strain_fsc(double y1, double y2, double y3, double sinB, double cosB, double cotB,
                                       double b1, double b2, double b3, double nu, double a, double v[6]) {
     const double y3b = y3 + 2 * a;
     const double z1b = y1 * cosB + y3b * sinB;
