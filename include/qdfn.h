@@ -113,6 +113,84 @@ int qdfn_matvec(qdfn_dense_t* op, const double* x, double* yS, double* yN, int32
  * launches of the fused coupling kernel. */
 int qdfn_dense_time_coupling(qdfn_dense_t* op, int32_t reps, double* avg_ms);
 
+/* ---- hierarchical (H-matrix) stiffness operator ----------------------- */
+
+/* Cluster-tree and admissibility parameters (RUN1_HMatrix_Structure.jl:28-33). */
+typedef struct {
+    int64_t min_elements_to_cut; /* MinimumElementsToCut: a cluster is split only when
+                                    (end - begin) >= this (leaf size) */
+    int64_t max_levels;          /* TotalHierarchyLevel */
+    double dist_diam_ratio;      /* DistDiamRatioCrit: admissible iff |c_i - c_j| / max(d_i, d_j) > this */
+    double arrange_point[3];     /* ArrangePoint: children ordered by distance of their centre to it */
+    int64_t n_loading;           /* loading elements: the last n_loading rows, kept out of the tree */
+    int64_t kmeans_max_iter;     /* Lloyd iterations of the 2-means split (200 in the reference) */
+} qdfn_hmat_params;
+
+/* Replaces GroupAndSort_AllLevel + BuildHierarchy (scripts/Functions_Hmatrix.jl:
+ * 1-212).  Host-only (no device work).  ctr: element centres (n x 3,
+ * column-major); dpts: the points whose bounding box gives a cluster's
+ * diameter (Input_Segment columns 1:3: the centre for rectangles, the first
+ * vertex for triangles).  Outputs:
+ *   perm[n]         element at sorted position i is input element perm[i] (0-based)
+ *   node_info       cap_nodes x 4 row-major: begin, end (1-based, inclusive, sorted order),
+ *                   level, parent node (-1 for level 1)
+ *   node_geo        cap_nodes x 4 row-major: centre xyz, diameter (Block_Ctr_Diam)
+ *   range_SR        cap_blocks x 4 row-major: src_i src_f rcv_i rcv_f (1-based,
+ *                   ElementRange_SR), admissible[cap_blocks] (Admissible)
+ * The counts are written to *n_nodes / *n_blocks; when a capacity is too small
+ * nothing else is written and QDFN_ERR_ARG is returned with the counts set. */
+int qdfn_hmat_structure(const qdfn_hmat_params* p, int64_t n, const double* ctr, const double* dpts, int64_t* perm,
+                        int64_t cap_nodes, int64_t* node_info, double* node_geo, int64_t* n_nodes,
+                        int64_t cap_blocks, int64_t* range_SR, int64_t* admissible, int64_t* n_blocks);
+
+typedef struct qdfn_hmat qdfn_hmat_t;
+
+#define QDFN_HBLK_DENSE 0
+#define QDFN_HBLK_LOWRANK 1
+
+/* Replaces HmatBuild_R (scripts/Functions_BuildInputFile.jl:725): per block the
+ * shear/normal stiffness of segment rows (already in sorted order, n x >=8,
+ * column-major, leading dimension ld); admissible blocks are compressed to
+ * U V (m x r, r x k) by adaptive cross approximation with full pivoting,
+ * stopping when max |residual| <= max(aca_atol, aca_rtol * max |block|)
+ * (the reference's pqrfact(block, atol = Tolerance)).  The whole matrix is
+ * assembled in device memory first, so n is bounded by HBM (about 130k). */
+qdfn_hmat_t* qdfn_hmat_build_rect(const double* seg, int64_t n, int64_t ld, double mu, double nu, int64_t nblk,
+                                  const int64_t* range_SR, const int64_t* admissible, double aca_rtol,
+                                  double aca_atol, int32_t prec);
+/* Replaces HmatBuild_T (:643); triangle inputs as qdfn_tde_stiffness_tri, sorted order. */
+qdfn_hmat_t* qdfn_hmat_build_tri(int64_t n, const double* P1, const double* P2, const double* P3,
+                                 const double* rake, const double* ctr, const double* nrm, const double* slip,
+                                 double mu, double lambda, int64_t nblk, const int64_t* range_SR,
+                                 const int64_t* admissible, double aca_rtol, double aca_atol, int32_t prec);
+/* Operator from given blocks (ShearStiffness_H / NormalStiffness_H as written
+ * by RUN2_BUILD_INPUT.jl:362): per block kind_S[b], rank_S[b], S_a[b] (dense
+ * m x k, or U m x r), S_b[b] (NULL, or V r x k), all column-major; the same for
+ * N.  A LowRankApprox PartialQR (A[:, p] ~ Q R) is passed as U = Q and
+ * V = R with its columns put back in place (V[:, p] = R). */
+qdfn_hmat_t* qdfn_hmat_create(int64_t n, int64_t nblk, const int64_t* range_SR, const int64_t* admissible,
+                              const int32_t* kind_S, const int64_t* rank_S, const double* const* S_a,
+                              const double* const* S_b, const int32_t* kind_N, const int64_t* rank_N,
+                              const double* const* N_a, const double* const* N_b, int32_t prec);
+void qdfn_hmat_destroy(qdfn_hmat_t* h);
+/* n, block count, device bytes of the stored operator, stored entries (both matrices). */
+int qdfn_hmat_info(qdfn_hmat_t* h, int64_t* n, int64_t* nblk, int64_t* device_bytes, int64_t* stored_entries);
+/* Per block: kind and rank of the shear and normal parts (Ranks_Shear/Ranks_Normal). */
+int qdfn_hmat_get_ranks(qdfn_hmat_t* h, int32_t* kind_S, int64_t* rank_S, int32_t* kind_N, int64_t* rank_N);
+/* One block's stored factors (which = 0 shear, 1 normal): dense m x k into a,
+ * or U (m x r) into a and V (r x k) into b; FP64 values as stored. */
+int qdfn_hmat_get_block(qdfn_hmat_t* h, int64_t b, int32_t which, double* a, double* bv);
+/* K_Self = -K_S,ii from the diagonal blocks (StiffnessTransitionToLoading,
+ * Functions_Hmatrix.jl:459); diag = K_S,ii (0 where no diagonal block). */
+int qdfn_hmat_kself(qdfn_hmat_t* h, double* kself, double* diag);
+int qdfn_hmat_normal_zero(qdfn_hmat_t* h);
+/* HmatSolver_Pararllel (Functions_Hmatrix.jl:631) for both matrices.
+ * loading_form=1: yS = (H_load * (0 - x)) ./ (-K_Self) (main_H :241-243),
+ * yN = H_N * x; loading_form=0: yS = H_S * x, yN = H_N * x. */
+int qdfn_hmat_matvec(qdfn_hmat_t* h, const double* x, double* yS, double* yN, int32_t loading_form);
+/* Average duration (ms, HIP events) of `reps` H products (both phases). */
+int qdfn_hmat_time_matvec(qdfn_hmat_t* h, int32_t reps, double* avg_ms);
+
 /* ---- rate-and-state local solve ---------------------------------------- */
 
 typedef struct {
@@ -176,6 +254,13 @@ typedef struct {
 qdfn_sim_t* qdfn_sim_create_dense(qdfn_dense_t* op, const qdfn_sim_params* p, const double* mass, const double* a,
                                   const double* b, const double* dc, const double* theta_i, const double* v_i,
                                   const double* friction_i, const double* sigma0, const double* loading_rate);
+/* main_H (scripts/Functions_RSFDFN3DMain_H.jl:2): the same loop with the
+ * H-matrix coupling; K_Self = -K_S,ii from the diagonal blocks and the
+ * external tables read at TOld (set p->ext_at_told = 1 for the reference's
+ * behaviour). */
+qdfn_sim_t* qdfn_sim_create_hmat(qdfn_hmat_t* h, const qdfn_sim_params* p, const double* mass, const double* a,
+                                 const double* b, const double* dc, const double* theta_i, const double* v_i,
+                                 const double* friction_i, const double* sigma0, const double* loading_rate);
 /* External stress tables (Input_ExternalStressChange.jld2; InterpolateFromStressChange,
  * Functions_Solvers.jl:465): times[nt]; ext_n/ext_s are nt x n column-major. */
 int qdfn_sim_set_external(qdfn_sim_t* sim, int64_t nt, const double* times, const double* ext_n,

@@ -102,6 +102,13 @@ typedef struct {
 void orc_hmat_matvec(const orc_hmat_t *h, const double *x, double *y);
 int orc_sim_hmat(const orc_hmat_t *shear_loading, const orc_hmat_t *normal, const orc_sim_in_t *in,
                  orc_sim_out_t *out);
+/* node arrays: begin/end 1-based inclusive, level, centre (n_nodes x 3 row-major),
+ * diameter; loading_begin = 0 when there is no loading fault.  Returns the block
+ * count (or -1 when cap is exceeded). */
+int64_t orc_build_hierarchy(int64_t n_nodes, const int64_t *node_begin, const int64_t *node_end,
+                            const int64_t *node_level, const double *node_ctr, const double *node_diam,
+                            int64_t total_levels, double crit, int64_t loading_begin, int64_t loading_end,
+                            int64_t cap, int64_t *range_SR, int64_t *admissible);
 
 #ifdef __cplusplus
 }

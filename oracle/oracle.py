@@ -124,6 +124,8 @@ def _declare(L):
         L.orc_sim_hmat.argtypes = [C.POINTER(HmatOp), C.POINTER(HmatOp), C.POINTER(SimIn),
                                    C.POINTER(SimOut)]
         L.orc_sim_hmat.restype = C.c_int
+        L.orc_build_hierarchy.argtypes = [I64, PI64, PI64, PI64, PD, PD, I64, D, I64, I64, I64, PI64, PI64]
+        L.orc_build_hierarchy.restype = I64
 
 
 def _f64(a):
@@ -323,3 +325,100 @@ def dense_coupling(E, KN, x, planar=0):
     tl, sp = np.zeros(n), np.zeros(n)
     lib().orc_dense_coupling(C.byref(op), _p(x), _p(tl), _p(sp), planar)
     return tl, sp
+
+
+def _hmat_struct(H):
+    """H: dict n, range (nblk x 4, 1-based src_i src_f rcv_i rcv_f), kind, rank,
+    a (list: dense m x k or U m x r), b (list: None or V r x k).  Returns the
+    ctypes struct and the arrays it points into (keep them alive)."""
+    keep = []
+    rng = np.ascontiguousarray(H["range"], dtype=np.int64)
+    kind = np.ascontiguousarray(H["kind"], dtype=np.int64)
+    rank = np.ascontiguousarray(H["rank"], dtype=np.int64)
+    nb = rng.shape[0]
+    A = (PD * nb)()
+    B = (PD * nb)()
+    for q in range(nb):
+        a = np.asfortranarray(H["a"][q], dtype=np.float64)
+        keep.append(a)
+        A[q] = a.ctypes.data_as(PD)
+        if H["b"][q] is not None:
+            b = np.asfortranarray(H["b"][q], dtype=np.float64)
+            keep.append(b)
+            B[q] = b.ctypes.data_as(PD)
+    keep += [rng, kind, rank, A, B]
+    s = HmatOp(int(H["n"]), nb, rng.ctypes.data_as(PI64), kind.ctypes.data_as(PI64), rank.ctypes.data_as(PI64),
+               C.cast(A, C.POINTER(PD)), C.cast(B, C.POINTER(PD)))
+    return s, keep
+
+
+def hmat_matvec(H, x):
+    """HmatSolver_Pararllel (Functions_Hmatrix.jl:631-646) with one thread."""
+    s, keep = _hmat_struct(H)
+    x = _f64(x)
+    y = np.zeros(int(H["n"]))
+    lib().orc_hmat_matvec(C.byref(s), _p(x), _p(y))
+    return y
+
+
+def hmat_transition_to_loading(H):
+    """StiffnessTransitionToLoading (Functions_Hmatrix.jl:459-477): copy of the
+    shear blocks with the diagonal of every block whose source and receiver
+    ranges coincide set to zero, and K_Self = -diag."""
+    n = int(H["n"])
+    ks = np.zeros(n)
+    a = []
+    for q, r in enumerate(np.asarray(H["range"])):
+        blk = np.array(H["a"][q], dtype=np.float64, copy=True)
+        if r[0] == r[2] and r[1] == r[3]:
+            for j in range(blk.shape[0]):
+                ks[r[0] - 1 + j] = -blk[j, j]
+                blk[j, j] = 0.0
+        a.append(blk)
+    out = dict(H)
+    out["a"] = a
+    return out, ks
+
+
+def sim_hmat(prob, HS, HN, total_step, record_step=1, trace=True):
+    """main_H with H-matrix stiffness (shear blocks as built; the loading form
+    and K_Self are derived here); returns (status, dict)."""
+    HL, ks = hmat_transition_to_loading(HS)
+    prob = dict(prob)
+    prob["k_self"] = ks
+    prob.setdefault("ext_at_told", 1)
+    sS, keepS = _hmat_struct(HL)
+    sN, keepN = _hmat_struct(HN)
+    s, res, keep = _sim_in(prob, total_step, record_step, trace)
+    n = prob["n"]
+    final = np.zeros(6 * n)
+    o = SimOut()
+    o.final_state = _p(final)
+    st = lib().orc_sim_hmat(C.byref(sS), C.byref(sN), C.byref(s), C.byref(o))
+    res.update(rhs_evals=o.rhs_evals, retry_passes=o.retry_passes, stalls=o.stalls,
+               steps_done=o.steps_done, n_recorded=o.n_recorded, t_end=o.t_end, dt_last=o.dt_last)
+    res["final"] = dict(zip(("V", "Disp", "Theta", "Friction", "Sigma", "Accel"), final.reshape(6, n)))
+    res["k_self"] = ks
+    return st, res
+
+
+def build_hierarchy(node_info, node_geo, total_levels, crit, loading_range=None, cap=None):
+    """BuildHierarchy (Functions_Hmatrix.jl:180-212) on a given cluster list:
+    node_info rows (begin, end, level, ...) 1-based, node_geo rows (cx, cy, cz, diam).
+    Returns (range_SR nblk x 4, admissible)."""
+    ni = np.asarray(node_info, dtype=np.int64)
+    ng = np.asarray(node_geo, dtype=np.float64)
+    nn = ni.shape[0]
+    beg, end, lev = (np.ascontiguousarray(ni[:, k]) for k in range(3))
+    ctr = np.ascontiguousarray(ng[:, :3])
+    diam = np.ascontiguousarray(ng[:, 3])
+    cap = cap or (4 * nn * nn + 16)
+    rng = np.zeros((cap, 4), dtype=np.int64)
+    adm = np.zeros(cap, dtype=np.int64)
+    lb, le = loading_range if loading_range else (0, 0)
+    nb = lib().orc_build_hierarchy(nn, beg.ctypes.data_as(PI64), end.ctypes.data_as(PI64),
+                                   lev.ctypes.data_as(PI64), _p(ctr), _p(diam), total_levels, float(crit),
+                                   lb, le, cap, rng.ctypes.data_as(PI64), adm.ctypes.data_as(PI64))
+    if nb < 0:
+        raise RuntimeError("build_hierarchy: capacity exceeded")
+    return rng[:nb].copy(), adm[:nb].copy()

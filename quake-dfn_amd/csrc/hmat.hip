@@ -1,0 +1,815 @@
+// Hierarchical (H-matrix) stiffness operator: build, device layout, product.
+//
+// Reference: HmatBuild_R / HmatBuild_T (scripts/Functions_BuildInputFile.jl:
+// 643-803) build every block densely and compress admissible ones with
+// pqrfact; HmatSolver_Pararllel (scripts/Functions_Hmatrix.jl:631-646) adds
+// y[rcv] += B * x[src] over blocks; main_H (Functions_RSFDFN3DMain_H.jl:
+// 241-248) uses the shear blocks with zeroed diagonals (StiffnessTransition-
+// ToLoading, Functions_Hmatrix.jl:459-477) divided by -K_Self.
+//
+// Build on the device: the full K_S and K_N are assembled in HBM (FP64), dense
+// blocks are read out of them and admissible blocks are compressed by an
+// adaptive cross approximation with full pivoting, one workgroup per block and
+// matrix, which stops when max |residual| <= max(atol, rtol * max |block|).
+//
+// Product layout (HBM): the rows are cut into row groups of at most 64 rows
+// whose boundaries include every block's receiver boundaries.  Each row group
+// owns one row-major panel per matrix holding, side by side, the rows of every
+// block that covers it: dense blocks contribute their columns, low-rank blocks
+// the matching rows of U.  A product is then
+//   phase 1: t = V_b x[src_b] for every low-rank factor (one wavefront per row
+//            of V, coalesced along the row);
+//   phase 2: one workgroup per panel gathers its right-hand side z from x
+//            (dense columns) and t (U columns) into LDS and streams the panel,
+//            16-byte loads along the rows, FP64 accumulation, fixed-order
+//            wavefront reductions (no atomics: results are deterministic).
+// Panels and V are stored in FP64 or FP32 (mixed precision).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "assembly.h"
+#include "dense.h"
+#include "hmat.h"
+
+namespace qdfn {
+
+namespace {
+
+constexpr int kRowsPerGroup = 64;
+constexpr int kPhase2Threads = 256;
+constexpr int kZChunk = 4096;  // doubles of the right-hand side staged in LDS per pass
+
+int64_t round4(int64_t v) { return (v + 3) / 4 * 4; }
+
+}  // namespace
+
+// ---------------------------------------------------------------- kernels ----
+
+template <class T>
+__device__ inline void load4(const T* p, double v[4]);
+template <>
+__device__ inline void load4<float>(const float* p, double v[4]) {
+    const float4 q = *reinterpret_cast<const float4*>(p);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+}
+template <>
+__device__ inline void load4<double>(const double* p, double v[4]) {
+    const double2 a = reinterpret_cast<const double2*>(p)[0];
+    const double2 b = reinterpret_cast<const double2*>(p)[1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+}
+
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// phase 1: t[j] = V_row(j) . x[src_j : src_j + k_j)
+template <class T>
+__global__ __launch_bounds__(256) void k_hphase1(const HVJob* __restrict__ jobs, int64_t n_jobs,
+                                                 const T* __restrict__ vpool, const double* __restrict__ x0,
+                                                 double* __restrict__ t, CouplingGate g) {
+    if (g.gate && *g.gate == 0) return;
+    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const int lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= n_jobs) return;
+    const HVJob jb = jobs[j];
+    const T* v = vpool + jb.off;
+    const double* xs = x + jb.src;
+    double acc = 0.0;
+    for (int64_t c = lane * 4; c < jb.k; c += 256) {
+        double w[4];
+        load4<T>(v + c, w);
+        // the row is zero-padded to a multiple of 4; guard x beyond k
+        for (int q = 0; q < 4; ++q)
+            if (c + q < jb.k) acc = fma(w[q], xs[c + q], acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) t[j] = acc;
+}
+
+// phase 2: one workgroup per panel
+template <class T>
+__global__ __launch_bounds__(kPhase2Threads) void k_hphase2(const HPanel* __restrict__ panels,
+                                                            const T* __restrict__ pool,
+                                                            const int32_t* __restrict__ idx,
+                                                            const double* __restrict__ x0,
+                                                            const double* __restrict__ t, int64_t n,
+                                                            const double* __restrict__ kdiag, int32_t form,
+                                                            double* __restrict__ yS, double* __restrict__ yN,
+                                                            CouplingGate g) {
+    if (g.gate && *g.gate == 0) return;
+    __shared__ double z[kZChunk];
+    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const HPanel P = panels[blockIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+    const T* base = pool + P.off;
+    for (int64_t c0 = 0; c0 < P.K; c0 += kZChunk) {
+        const int64_t kc = min((int64_t)kZChunk, (int64_t)P.K - c0);
+        const int64_t kc4 = (kc + 3) / 4 * 4;
+        __syncthreads();
+        for (int64_t c = threadIdx.x; c < kc4; c += kPhase2Threads) {
+            double v = 0.0;
+            if (c < kc) {
+                const int32_t j = idx[P.idx_off + c0 + c];
+                v = j < n ? x[j] : t[j - n];
+            }
+            z[c] = v;
+        }
+        __syncthreads();
+        for (int64_t c = lane * 4; c < kc4; c += 256) {
+            const double z0 = z[c], z1 = z[c + 1], z2 = z[c + 2], z3 = z[c + 3];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = wave + 4 * r;
+                if (row < P.m) {
+                    double w[4];
+                    load4<T>(base + (int64_t)row * P.ld + c0 + c, w);
+                    acc[r] = fma(w[0], z0, acc[r]);
+                    acc[r] = fma(w[1], z1, acc[r]);
+                    acc[r] = fma(w[2], z2, acc[r]);
+                    acc[r] = fma(w[3], z3, acc[r]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = wave + 4 * r;
+        const double s = wave_sum(acc[r]);
+        if (lane == 0 && row < P.m) {
+            const int64_t i = P.row0 + row;
+            if (P.which == 0) {
+                // main_H :241-243: (H_load * (0 - x)) ./ (-K_Self), -K_Self = K_ii
+                yS[i] = form ? (-s) / kdiag[i] : s + kdiag[i] * x[i];
+            } else {
+                yN[i] = s;
+            }
+        }
+    }
+    if (g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
+}
+
+__global__ void k_mark_skipped(CouplingGate g) {
+    if (g.ran_log && g.gate && *g.gate == 0) g.ran_log[g.slot] = 0;
+}
+
+// Adaptive cross approximation with full pivoting of one block (one workgroup).
+struct AcaJob {
+    const double* src;  // block in the full matrix (column-major, leading dimension ld)
+    int64_t ld;
+    double* R;          // m x k residual workspace
+    double* U;          // m x rmax
+    double* V;          // rmax x k, row-major
+    int32_t m, k, rmax, pad;
+    double rtol, atol;
+};
+
+constexpr int kAcaThreads = 1024;
+
+__device__ inline void better(double& v, int32_t& id, double& sv, double v2, int32_t id2, double sv2) {
+    if (v2 > v || (v2 == v && id2 < id)) {
+        v = v2;
+        id = id2;
+        sv = sv2;
+    }
+}
+
+__device__ void aca_reduce(double& v, int32_t& id, double& sv, double* sh_v, int32_t* sh_i, double* sh_s) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(v, o, 64);
+        const int32_t i2 = __shfl_xor(id, o, 64);
+        const double s2 = __shfl_xor(sv, o, 64);
+        better(v, id, sv, v2, i2, s2);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        sh_v[wave] = v;
+        sh_i[wave] = id;
+        sh_s[wave] = sv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kAcaThreads / 64; ++w) better(sh_v[0], sh_i[0], sh_s[0], sh_v[w], sh_i[w], sh_s[w]);
+    }
+    __syncthreads();
+    v = sh_v[0];
+    id = sh_i[0];
+    sv = sh_s[0];
+}
+
+__global__ __launch_bounds__(kAcaThreads) void k_aca(const AcaJob* __restrict__ jobs, int32_t* __restrict__ rank,
+                                                     int32_t* __restrict__ converged) {
+    __shared__ double sh_v[kAcaThreads / 64], sh_s[kAcaThreads / 64];
+    __shared__ int32_t sh_i[kAcaThreads / 64];
+    const AcaJob J = jobs[blockIdx.x];
+    const int32_t m = J.m, k = J.k, mk = m * k;
+    double amax = -1.0, piv = 0.0;
+    int32_t aid = 0x7fffffff;
+    for (int32_t e = threadIdx.x; e < mk; e += kAcaThreads) {
+        const int32_t i = e % m, j = e / m;
+        const double v = J.src[i + (int64_t)j * J.ld];
+        J.R[e] = v;
+        better(amax, aid, piv, fabs(v), e, v);
+    }
+    aca_reduce(amax, aid, piv, sh_v, sh_i, sh_s);
+    const double thr = fmax(J.atol, J.rtol * amax);
+    int32_t r = 0;
+    while (amax > thr && r < J.rmax) {
+        const int32_t is = aid % m, js = aid / m;
+        double* u = J.U + (int64_t)r * m;
+        double* v = J.V + (int64_t)r * k;
+        for (int32_t i = threadIdx.x; i < m; i += kAcaThreads) u[i] = J.R[i + (int64_t)js * m];
+        for (int32_t j = threadIdx.x; j < k; j += kAcaThreads) v[j] = J.R[is + (int64_t)j * m] / piv;
+        __syncthreads();
+        amax = -1.0;
+        aid = 0x7fffffff;
+        piv = 0.0;
+        for (int32_t e = threadIdx.x; e < mk; e += kAcaThreads) {
+            const int32_t i = e % m, j = e / m;
+            const double nv = J.R[e] - u[i] * v[j];
+            J.R[e] = nv;
+            better(amax, aid, piv, fabs(nv), e, nv);
+        }
+        aca_reduce(amax, aid, piv, sh_v, sh_i, sh_s);
+        ++r;
+    }
+    if (threadIdx.x == 0) {
+        rank[blockIdx.x] = r;
+        converged[blockIdx.x] = amax <= thr ? 1 : 0;
+    }
+}
+
+// ------------------------------------------------------ host-side layout ----
+
+static int hmat_layout(qdfn_hmat* h) {
+    const int64_t n = h->n;
+    auto& B = h->blocks;
+    // K_S,ii from the diagonal blocks (src range == rcv range)
+    h->kdiag_h.assign(n, 0.0);
+    for (const HBlock& b : B) {
+        if (b.s0 == b.r0 && b.s1 == b.r1 && b.kind[0] == QDFN_HBLK_DENSE) {
+            const int64_t m = b.r1 - b.r0 + 1;
+            for (int64_t q = 0; q < m; ++q) h->kdiag_h[b.r0 + q] = b.a[0][q + q * m];
+        }
+    }
+    int nz = 1;
+    for (const HBlock& b : B)
+        for (const auto* v : {&b.a[1], &b.b[1]})
+            for (double e : *v)
+                if (e != 0.0) nz = 0;
+    h->normal_zero = nz;
+    // row groups
+    std::vector<int64_t> cut = {0, n};
+    for (const HBlock& b : B) {
+        cut.push_back(b.r0);
+        cut.push_back(b.r1 + 1);
+    }
+    std::sort(cut.begin(), cut.end());
+    cut.erase(std::unique(cut.begin(), cut.end()), cut.end());
+    std::vector<int64_t> g0;
+    for (size_t q = 0; q + 1 < cut.size(); ++q)
+        for (int64_t r = cut[q]; r < cut[q + 1]; r += kRowsPerGroup) g0.push_back(r);
+    const int64_t G = (int64_t)g0.size();
+    g0.push_back(n);
+    std::vector<std::vector<int32_t>> cover(G);
+    for (int32_t bi = 0; bi < (int32_t)B.size(); ++bi) {
+        const HBlock& b = B[bi];
+        int64_t gi = std::lower_bound(g0.begin(), g0.begin() + G, b.r0) - g0.begin();
+        for (; gi < G && g0[gi + 1] - 1 <= b.r1; ++gi) cover[gi].push_back(bi);
+    }
+    // phase-1 jobs and t offsets
+    std::vector<HVJob> jobs;
+    std::vector<int64_t> vrow_off;
+    int64_t vtotal = 0;
+    std::vector<int64_t> toff(2 * B.size(), -1);
+    for (size_t bi = 0; bi < B.size(); ++bi)
+        for (int w = 0; w < 2; ++w) {
+            const HBlock& b = B[bi];
+            if (b.kind[w] != QDFN_HBLK_LOWRANK) continue;
+            const int64_t k = b.s1 - b.s0 + 1;
+            toff[2 * bi + w] = (int64_t)jobs.size();
+            for (int64_t r = 0; r < b.rank[w]; ++r) {
+                HVJob j;
+                j.off = vtotal;
+                j.src = b.s0;
+                j.k = (int32_t)k;
+                j.pad = 0;
+                jobs.push_back(j);
+                vtotal += round4(k);
+            }
+        }
+    h->n_jobs = (int64_t)jobs.size();
+    std::vector<double> vh(std::max<int64_t>(vtotal, 4), 0.0);
+    for (size_t bi = 0; bi < B.size(); ++bi)
+        for (int w = 0; w < 2; ++w) {
+            const HBlock& b = B[bi];
+            if (b.kind[w] != QDFN_HBLK_LOWRANK) continue;
+            const int64_t k = b.s1 - b.s0 + 1, r = b.rank[w];
+            for (int64_t q = 0; q < r; ++q) {
+                const HVJob& j = jobs[toff[2 * bi + w] + q];
+                for (int64_t c = 0; c < k; ++c) vh[j.off + c] = b.b[w][q + c * r];  // V is r x k column-major
+            }
+        }
+    // panels
+    std::vector<HPanel> panels;
+    std::vector<int32_t> idx;
+    int64_t ptotal = 0;
+    for (int64_t gi = 0; gi < G; ++gi)
+        for (int w = 0; w < 2; ++w) {
+            int64_t K = 0;
+            for (int32_t bi : cover[gi]) {
+                const HBlock& b = B[bi];
+                K += b.kind[w] == QDFN_HBLK_DENSE ? (b.s1 - b.s0 + 1) : b.rank[w];
+            }
+            HPanel P;
+            P.off = ptotal;
+            P.idx_off = (int64_t)idx.size();
+            P.row0 = g0[gi];
+            P.m = (int32_t)(g0[gi + 1] - g0[gi]);
+            P.K = (int32_t)K;
+            P.ld = (int32_t)round4(K);
+            P.which = w;
+            panels.push_back(P);
+            ptotal += (int64_t)P.m * P.ld;
+            for (int32_t bi : cover[gi]) {
+                const HBlock& b = B[bi];
+                if (b.kind[w] == QDFN_HBLK_DENSE)
+                    for (int64_t c = 0; c <= b.s1 - b.s0; ++c) idx.push_back((int32_t)(b.s0 + c));
+                else
+                    for (int64_t c = 0; c < b.rank[w]; ++c) idx.push_back((int32_t)(n + toff[2 * bi + w] + c));
+            }
+        }
+    if (n + (int64_t)jobs.size() > 0x7fffffffLL) return fail(QDFN_ERR_ARG, "H-matrix: index space exceeds int32");
+    std::vector<double> ph(std::max<int64_t>(ptotal, 4), 0.0);
+    for (size_t pi = 0; pi < panels.size(); ++pi) {
+        const HPanel& P = panels[pi];
+        const int w = P.which;
+        int64_t col = 0;
+        for (int32_t bi : cover[(int64_t)pi / 2]) {
+            const HBlock& b = B[bi];
+            const int64_t mb = b.r1 - b.r0 + 1, roff = P.row0 - b.r0;
+            const bool diag = (w == 0 && b.s0 == b.r0 && b.s1 == b.r1);
+            const int64_t nc = b.kind[w] == QDFN_HBLK_DENSE ? (b.s1 - b.s0 + 1) : b.rank[w];
+            const std::vector<double>& A = b.a[w];
+            for (int64_t i = 0; i < P.m; ++i)
+                for (int64_t c = 0; c < nc; ++c) {
+                    double v = A[(roff + i) + c * mb];
+                    if (diag && b.kind[w] == QDFN_HBLK_DENSE && P.row0 + i == b.s0 + c) v = 0.0;
+                    ph[P.off + i * P.ld + col + c] = v;
+                }
+            col += nc;
+        }
+    }
+    h->n_panels = (int64_t)panels.size();
+    h->stored_entries = 0;
+    for (const HBlock& b : B)
+        for (int w = 0; w < 2; ++w) h->stored_entries += (int64_t)(b.a[w].size() + b.b[w].size());
+    // upload
+    const size_t es = h->prec == QDFN_PREC_FP32 ? sizeof(float) : sizeof(double);
+    auto upload_vals = [&](DevBuf& d, const std::vector<double>& v) -> int {
+        QDFN_HIP(d.alloc(es * v.size()));
+        if (h->prec == QDFN_PREC_FP32) {
+            std::vector<float> f(v.begin(), v.end());
+            QDFN_HIP(hipMemcpy(d.p, f.data(), es * f.size(), hipMemcpyHostToDevice));
+        } else {
+            QDFN_HIP(hipMemcpy(d.p, v.data(), es * v.size(), hipMemcpyHostToDevice));
+        }
+        return QDFN_OK;
+    };
+    int rc = upload_vals(h->vpool, vh);
+    if (rc) return rc;
+    rc = upload_vals(h->ppool, ph);
+    if (rc) return rc;
+    QDFN_HIP(h->jobs.alloc(sizeof(HVJob) * std::max<size_t>(jobs.size(), 1)));
+    if (!jobs.empty()) QDFN_HIP(hipMemcpy(h->jobs.p, jobs.data(), sizeof(HVJob) * jobs.size(), hipMemcpyHostToDevice));
+    QDFN_HIP(h->panels.alloc(sizeof(HPanel) * std::max<size_t>(panels.size(), 1)));
+    QDFN_HIP(hipMemcpy(h->panels.p, panels.data(), sizeof(HPanel) * panels.size(), hipMemcpyHostToDevice));
+    QDFN_HIP(h->idx.alloc(sizeof(int32_t) * std::max<size_t>(idx.size(), 1)));
+    if (!idx.empty()) QDFN_HIP(hipMemcpy(h->idx.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
+    QDFN_HIP(h->t.alloc(sizeof(double) * std::max<size_t>(jobs.size(), 1)));
+    QDFN_HIP(h->kdiag.alloc(sizeof(double) * n));
+    QDFN_HIP(hipMemcpy(h->kdiag.p, h->kdiag_h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    QDFN_HIP(h->x.alloc(sizeof(double) * n));
+    QDFN_HIP(h->yS.alloc(sizeof(double) * n));
+    QDFN_HIP(h->yN.alloc(sizeof(double) * n));
+    h->device_bytes = (int64_t)(es * (vh.size() + ph.size()) + sizeof(int32_t) * idx.size() +
+                                sizeof(HVJob) * jobs.size() + sizeof(HPanel) * panels.size());
+    h->phase_bytes = (int64_t)(es * (vtotal + ptotal) + sizeof(int32_t) * idx.size() + sizeof(HVJob) * jobs.size() +
+                               sizeof(HPanel) * panels.size() + 8 * (2 * n + 2 * (int64_t)jobs.size()));
+    return QDFN_OK;
+}
+
+int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, int32_t form, double* d_yS,
+                double* d_yN, hipStream_t s) {
+    if (h->n_jobs > 0) {
+        const unsigned nb = (unsigned)((h->n_jobs + 3) / 4);
+        if (h->prec == QDFN_PREC_FP32)
+            hipLaunchKernelGGL(k_hphase1<float>, dim3(nb), dim3(256), 0, s, h->jobs.as<HVJob>(), h->n_jobs,
+                               h->vpool.as<float>(), d_x, h->t.as<double>(), g);
+        else
+            hipLaunchKernelGGL(k_hphase1<double>, dim3(nb), dim3(256), 0, s, h->jobs.as<HVJob>(), h->n_jobs,
+                               h->vpool.as<double>(), d_x, h->t.as<double>(), g);
+        QDFN_HIP(hipGetLastError());
+    }
+    if (g.ran_log) hipLaunchKernelGGL(k_mark_skipped, dim3(1), dim3(1), 0, s, g);
+    const unsigned np = (unsigned)h->n_panels;
+    if (h->prec == QDFN_PREC_FP32)
+        hipLaunchKernelGGL(k_hphase2<float>, dim3(np), dim3(kPhase2Threads), 0, s, h->panels.as<HPanel>(),
+                           h->ppool.as<float>(), h->idx.as<int32_t>(), d_x, h->t.as<double>(), h->n,
+                           h->kdiag.as<double>(), form, d_yS, d_yN, g);
+    else
+        hipLaunchKernelGGL(k_hphase2<double>, dim3(np), dim3(kPhase2Threads), 0, s, h->panels.as<HPanel>(),
+                           h->ppool.as<double>(), h->idx.as<int32_t>(), d_x, h->t.as<double>(), h->n,
+                           h->kdiag.as<double>(), form, d_yS, d_yN, g);
+    QDFN_HIP(hipGetLastError());
+    return QDFN_OK;
+}
+
+// ---------------------------------------------------------------- build ----
+
+static qdfn_hmat* new_hmat(int64_t n, int32_t prec) {
+    if (n <= 0 || (prec != QDFN_PREC_FP64 && prec != QDFN_PREC_FP32)) {
+        fail(QDFN_ERR_ARG, "H-matrix: bad size or precision");
+        return nullptr;
+    }
+    qdfn_hmat* h = new qdfn_hmat();
+    h->n = n;
+    h->prec = prec;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        fail(QDFN_ERR_HIP, "H-matrix: stream creation failed");
+        delete h;
+        return nullptr;
+    }
+    return h;
+}
+
+static int check_blocks(int64_t n, int64_t nblk, const int64_t* range, const int64_t* adm) {
+    if (nblk <= 0 || !range || !adm) return fail(QDFN_ERR_ARG, "H-matrix: empty block list");
+    for (int64_t q = 0; q < nblk; ++q) {
+        const int64_t* r = range + 4 * q;
+        if (r[0] < 1 || r[1] < r[0] || r[1] > n || r[2] < 1 || r[3] < r[2] || r[3] > n)
+            return fail(QDFN_ERR_ARG, "H-matrix: block range out of bounds");
+    }
+    return QDFN_OK;
+}
+
+// Compress / read out every block from full device matrices KS, KN (n x n, ld n).
+static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, int64_t nblk, const int64_t* range,
+                            const int64_t* adm, double rtol, double atol) {
+    const int64_t n = h->n;
+    h->blocks.resize(nblk);
+    for (int64_t q = 0; q < nblk; ++q) {
+        HBlock& b = h->blocks[q];
+        b.s0 = range[4 * q] - 1;
+        b.s1 = range[4 * q + 1] - 1;
+        b.r0 = range[4 * q + 2] - 1;
+        b.r1 = range[4 * q + 3] - 1;
+        b.adm = adm[q] > 0 ? 1 : 0;
+    }
+    const double* full[2] = {dKS, dKN};
+    auto read_dense = [&](HBlock& b, int w) -> int {
+        const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+        b.kind[w] = QDFN_HBLK_DENSE;
+        b.rank[w] = 0;
+        b.a[w].resize(m * k);
+        b.b[w].clear();
+        QDFN_HIP(hipMemcpy2D(b.a[w].data(), sizeof(double) * m, full[w] + b.r0 + b.s0 * n, sizeof(double) * n,
+                             sizeof(double) * m, k, hipMemcpyDeviceToHost));
+        return QDFN_OK;
+    };
+    // dense blocks
+    std::vector<std::pair<int64_t, int>> lr;  // (block, which) to compress
+    for (int64_t q = 0; q < nblk; ++q)
+        for (int w = 0; w < 2; ++w) {
+            HBlock& b = h->blocks[q];
+            const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+            const int64_t rmax = std::min<int64_t>(256, m * k / (m + k));
+            if (!b.adm || rmax < 1 || m * k > 0x7fffffffLL) {
+                int rc = read_dense(b, w);
+                if (rc) return rc;
+            } else {
+                lr.push_back({q, w});
+            }
+        }
+    // admissible blocks in batches bounded by a workspace budget
+    size_t free_b = 0, total_b = 0;
+    QDFN_HIP(hipMemGetInfo(&free_b, &total_b));
+    const int64_t budget = std::max<int64_t>(int64_t(256) << 20, std::min<int64_t>(int64_t(16) << 30, (int64_t)free_b / 3));
+    size_t pos = 0;
+    while (pos < lr.size()) {
+        std::vector<AcaJob> jobs;
+        std::vector<int64_t> off;  // workspace offsets (doubles): R, U, V per job
+        int64_t used = 0;
+        size_t end = pos;
+        while (end < lr.size()) {
+            const HBlock& b = h->blocks[lr[end].first];
+            const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+            const int64_t rmax = std::min<int64_t>(256, m * k / (m + k));
+            const int64_t need = m * k + m * rmax + rmax * k;
+            if (end > pos && (used + need) * 8 > budget) break;
+            off.push_back(used);
+            used += need;
+            ++end;
+        }
+        DevBuf ws, djobs, drank, dconv;
+        QDFN_HIP(ws.alloc(sizeof(double) * used));
+        for (size_t q = pos; q < end; ++q) {
+            const HBlock& b = h->blocks[lr[q].first];
+            const int w = lr[q].second;
+            const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+            const int64_t rmax = std::min<int64_t>(256, m * k / (m + k));
+            AcaJob J;
+            J.src = full[w] + b.r0 + b.s0 * n;
+            J.ld = n;
+            J.R = ws.as<double>() + off[q - pos];
+            J.U = J.R + m * k;
+            J.V = J.U + m * rmax;
+            J.m = (int32_t)m;
+            J.k = (int32_t)k;
+            J.rmax = (int32_t)rmax;
+            J.pad = 0;
+            J.rtol = rtol;
+            J.atol = atol;
+            jobs.push_back(J);
+        }
+        const size_t nj = jobs.size();
+        QDFN_HIP(djobs.alloc(sizeof(AcaJob) * nj));
+        QDFN_HIP(drank.alloc(sizeof(int32_t) * nj));
+        QDFN_HIP(dconv.alloc(sizeof(int32_t) * nj));
+        QDFN_HIP(hipMemcpy(djobs.p, jobs.data(), sizeof(AcaJob) * nj, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_aca, dim3((unsigned)nj), dim3(kAcaThreads), 0, h->stream, djobs.as<AcaJob>(),
+                           drank.as<int32_t>(), dconv.as<int32_t>());
+        QDFN_HIP(hipGetLastError());
+        QDFN_HIP(hipStreamSynchronize(h->stream));
+        std::vector<int32_t> rk(nj), cv(nj);
+        QDFN_HIP(hipMemcpy(rk.data(), drank.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost));
+        QDFN_HIP(hipMemcpy(cv.data(), dconv.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost));
+        for (size_t q = pos; q < end; ++q) {
+            HBlock& b = h->blocks[lr[q].first];
+            const int w = lr[q].second;
+            const AcaJob& J = jobs[q - pos];
+            const int64_t m = J.m, k = J.k, r = rk[q - pos];
+            if (!cv[q - pos]) {  // rank cap reached: keep the block dense
+                int rc = read_dense(b, w);
+                if (rc) return rc;
+                continue;
+            }
+            b.kind[w] = QDFN_HBLK_LOWRANK;
+            b.rank[w] = r;
+            b.a[w].resize(m * r);
+            b.b[w].resize(r * k);
+            if (r > 0) {
+                QDFN_HIP(hipMemcpy(b.a[w].data(), J.U, sizeof(double) * m * r, hipMemcpyDeviceToHost));
+                std::vector<double> vr(r * k);
+                QDFN_HIP(hipMemcpy(vr.data(), J.V, sizeof(double) * r * k, hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < r; ++i)
+                    for (int64_t c = 0; c < k; ++c) b.b[w][i + c * r] = vr[i * k + c];
+            }
+        }
+        pos = end;
+    }
+    return QDFN_OK;
+}
+
+}  // namespace qdfn
+
+using namespace qdfn;
+
+extern "C" qdfn_hmat_t* qdfn_hmat_build_rect(const double* seg, int64_t n, int64_t ld, double mu, double nu,
+                                             int64_t nblk, const int64_t* range_SR, const int64_t* admissible,
+                                             double aca_rtol, double aca_atol, int32_t prec) {
+    if (!seg || ld < n) {
+        fail(QDFN_ERR_ARG, "qdfn_hmat_build_rect: null table or ld < n");
+        return nullptr;
+    }
+    if (check_blocks(n, nblk, range_SR, admissible)) return nullptr;
+    qdfn_hmat* h = new_hmat(n, prec);
+    if (!h) return nullptr;
+    DevBuf drcv, dks, dkn;
+    const size_t b = sizeof(double) * n * n;
+    int rc = QDFN_OK;
+    if (drcv.alloc(sizeof(double) * ld * 8) || dks.alloc(b) || dkn.alloc(b))
+        rc = fail(QDFN_ERR_NOMEM, "qdfn_hmat_build_rect: out of device memory for the full matrices");
+    if (rc == QDFN_OK && hipMemcpy(drcv.p, seg, sizeof(double) * ld * 8, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(QDFN_ERR_HIP, "qdfn_hmat_build_rect: upload failed");
+    if (rc == QDFN_OK && (hipMemsetAsync(dks.p, 0, b, h->stream) || hipMemsetAsync(dkn.p, 0, b, h->stream)))
+        rc = fail(QDFN_ERR_HIP, "memset");
+    if (rc == QDFN_OK)
+        rc = assemble_rect_device(seg, n, ld, drcv.as<double>(), n, ld, mu, nu, dks.as<double>(), dkn.as<double>(), n,
+                                  h->stream);
+    if (rc == QDFN_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(QDFN_ERR_HIP, "assembly");
+    if (rc == QDFN_OK)
+        rc = blocks_from_full(h, dks.as<double>(), dkn.as<double>(), nblk, range_SR, admissible, aca_rtol, aca_atol);
+    dks.release();
+    dkn.release();
+    if (rc == QDFN_OK) rc = hmat_layout(h);
+    if (rc != QDFN_OK) {
+        qdfn_hmat_destroy(h);
+        return nullptr;
+    }
+    return h;
+}
+
+extern "C" qdfn_hmat_t* qdfn_hmat_build_tri(int64_t n, const double* P1, const double* P2, const double* P3,
+                                            const double* rake, const double* ctr, const double* nrm,
+                                            const double* slip, double mu, double lambda, int64_t nblk,
+                                            const int64_t* range_SR, const int64_t* admissible, double aca_rtol,
+                                            double aca_atol, int32_t prec) {
+    if (!P1 || !P2 || !P3 || !rake || !ctr || !nrm || !slip) {
+        fail(QDFN_ERR_ARG, "qdfn_hmat_build_tri: null input");
+        return nullptr;
+    }
+    if (check_blocks(n, nblk, range_SR, admissible)) return nullptr;
+    for (int64_t s = 0; s < n; ++s)
+        if (P1[s + 2 * n] > 0 || P2[s + 2 * n] > 0 || P3[s + 2 * n] > 0) {
+            fail(QDFN_ERR_ARG, "Half-space solution: Z coordinates must be negative!");
+            return nullptr;
+        }
+    qdfn_hmat* h = new_hmat(n, prec);
+    if (!h) return nullptr;
+    DevBuf d1, d2, d3, dr, dc, dn, dsl, dks, dkn;
+    const size_t v3 = sizeof(double) * 3 * n, b = sizeof(double) * n * n;
+    int rc = QDFN_OK;
+    if (d1.alloc(v3) || d2.alloc(v3) || d3.alloc(v3) || dr.alloc(sizeof(double) * n) || dc.alloc(v3) ||
+        dn.alloc(v3) || dsl.alloc(v3) || dks.alloc(b) || dkn.alloc(b))
+        rc = fail(QDFN_ERR_NOMEM, "qdfn_hmat_build_tri: out of device memory for the full matrices");
+    const void* src[7] = {P1, P2, P3, rake, ctr, nrm, slip};
+    DevBuf* dst[7] = {&d1, &d2, &d3, &dr, &dc, &dn, &dsl};
+    for (int k = 0; rc == QDFN_OK && k < 7; ++k)
+        if (hipMemcpy(dst[k]->p, src[k], k == 3 ? sizeof(double) * n : v3, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(QDFN_ERR_HIP, "qdfn_hmat_build_tri: upload failed");
+    if (rc == QDFN_OK)
+        rc = assemble_tri_device(n, d1.as<double>(), d2.as<double>(), d3.as<double>(), dr.as<double>(), n,
+                                 dc.as<double>(), dn.as<double>(), dsl.as<double>(), mu, lambda, dks.as<double>(),
+                                 dkn.as<double>(), n, h->stream);
+    if (rc == QDFN_OK)
+        rc = blocks_from_full(h, dks.as<double>(), dkn.as<double>(), nblk, range_SR, admissible, aca_rtol, aca_atol);
+    dks.release();
+    dkn.release();
+    if (rc == QDFN_OK) rc = hmat_layout(h);
+    if (rc != QDFN_OK) {
+        qdfn_hmat_destroy(h);
+        return nullptr;
+    }
+    return h;
+}
+
+extern "C" qdfn_hmat_t* qdfn_hmat_create(int64_t n, int64_t nblk, const int64_t* range_SR, const int64_t* admissible,
+                                         const int32_t* kind_S, const int64_t* rank_S, const double* const* S_a,
+                                         const double* const* S_b, const int32_t* kind_N, const int64_t* rank_N,
+                                         const double* const* N_a, const double* const* N_b, int32_t prec) {
+    if (check_blocks(n, nblk, range_SR, admissible)) return nullptr;
+    if (!kind_S || !rank_S || !S_a || !S_b || !kind_N || !rank_N || !N_a || !N_b) {
+        fail(QDFN_ERR_ARG, "qdfn_hmat_create: null block arrays");
+        return nullptr;
+    }
+    qdfn_hmat* h = new_hmat(n, prec);
+    if (!h) return nullptr;
+    h->blocks.resize(nblk);
+    const int32_t* kinds[2] = {kind_S, kind_N};
+    const int64_t* ranks[2] = {rank_S, rank_N};
+    const double* const* as[2] = {S_a, N_a};
+    const double* const* bs[2] = {S_b, N_b};
+    for (int64_t q = 0; q < nblk; ++q) {
+        HBlock& b = h->blocks[q];
+        b.s0 = range_SR[4 * q] - 1;
+        b.s1 = range_SR[4 * q + 1] - 1;
+        b.r0 = range_SR[4 * q + 2] - 1;
+        b.r1 = range_SR[4 * q + 3] - 1;
+        b.adm = admissible[q] > 0 ? 1 : 0;
+        const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+        for (int w = 0; w < 2; ++w) {
+            b.kind[w] = kinds[w][q];
+            if (b.kind[w] == QDFN_HBLK_DENSE) {
+                if (!as[w][q]) {
+                    fail(QDFN_ERR_ARG, "qdfn_hmat_create: null dense block");
+                    qdfn_hmat_destroy(h);
+                    return nullptr;
+                }
+                b.rank[w] = 0;
+                b.a[w].assign(as[w][q], as[w][q] + m * k);
+            } else if (b.kind[w] == QDFN_HBLK_LOWRANK) {
+                const int64_t r = ranks[w][q];
+                if (r < 0 || (r > 0 && (!as[w][q] || !bs[w][q]))) {
+                    fail(QDFN_ERR_ARG, "qdfn_hmat_create: bad low-rank block");
+                    qdfn_hmat_destroy(h);
+                    return nullptr;
+                }
+                b.rank[w] = r;
+                if (r > 0) {
+                    b.a[w].assign(as[w][q], as[w][q] + m * r);
+                    b.b[w].assign(bs[w][q], bs[w][q] + r * k);
+                }
+            } else {
+                fail(QDFN_ERR_ARG, "qdfn_hmat_create: unknown block kind");
+                qdfn_hmat_destroy(h);
+                return nullptr;
+            }
+        }
+    }
+    if (hmat_layout(h) != QDFN_OK) {
+        qdfn_hmat_destroy(h);
+        return nullptr;
+    }
+    return h;
+}
+
+extern "C" void qdfn_hmat_destroy(qdfn_hmat_t* h) {
+    if (!h) return;
+    hipStream_t s = h->stream;
+    if (s) (void)hipStreamSynchronize(s);
+    delete h;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+extern "C" int qdfn_hmat_info(qdfn_hmat_t* h, int64_t* n, int64_t* nblk, int64_t* device_bytes,
+                              int64_t* stored_entries) {
+    if (!h) return fail(QDFN_ERR_ARG, "qdfn_hmat_info: null handle");
+    if (n) *n = h->n;
+    if (nblk) *nblk = (int64_t)h->blocks.size();
+    if (device_bytes) *device_bytes = h->phase_bytes;
+    if (stored_entries) *stored_entries = h->stored_entries;
+    return QDFN_OK;
+}
+
+extern "C" int qdfn_hmat_get_ranks(qdfn_hmat_t* h, int32_t* kind_S, int64_t* rank_S, int32_t* kind_N,
+                                   int64_t* rank_N) {
+    if (!h) return fail(QDFN_ERR_ARG, "qdfn_hmat_get_ranks: null handle");
+    for (size_t q = 0; q < h->blocks.size(); ++q) {
+        const HBlock& b = h->blocks[q];
+        if (kind_S) kind_S[q] = b.kind[0];
+        if (rank_S) rank_S[q] = b.rank[0];
+        if (kind_N) kind_N[q] = b.kind[1];
+        if (rank_N) rank_N[q] = b.rank[1];
+    }
+    return QDFN_OK;
+}
+
+extern "C" int qdfn_hmat_get_block(qdfn_hmat_t* h, int64_t bi, int32_t which, double* a, double* bv) {
+    if (!h || bi < 0 || bi >= (int64_t)h->blocks.size() || which < 0 || which > 1)
+        return fail(QDFN_ERR_ARG, "qdfn_hmat_get_block: bad arguments");
+    const HBlock& b = h->blocks[bi];
+    if (a && !b.a[which].empty()) std::memcpy(a, b.a[which].data(), sizeof(double) * b.a[which].size());
+    if (bv && !b.b[which].empty()) std::memcpy(bv, b.b[which].data(), sizeof(double) * b.b[which].size());
+    return QDFN_OK;
+}
+
+extern "C" int qdfn_hmat_kself(qdfn_hmat_t* h, double* kself, double* diag) {
+    if (!h) return fail(QDFN_ERR_ARG, "qdfn_hmat_kself: null handle");
+    for (int64_t i = 0; i < h->n; ++i) {
+        if (kself) kself[i] = -h->kdiag_h[i];
+        if (diag) diag[i] = h->kdiag_h[i];
+    }
+    return QDFN_OK;
+}
+
+extern "C" int qdfn_hmat_normal_zero(qdfn_hmat_t* h) { return h ? h->normal_zero : QDFN_ERR_ARG; }
+
+extern "C" int qdfn_hmat_matvec(qdfn_hmat_t* h, const double* x, double* yS, double* yN, int32_t loading_form) {
+    if (!h || !x) return fail(QDFN_ERR_ARG, "qdfn_hmat_matvec: null argument");
+    hipStream_t s = h->stream;
+    const size_t b = sizeof(double) * h->n;
+    QDFN_HIP(hipMemcpyAsync(h->x.p, x, b, hipMemcpyHostToDevice, s));
+    int rc = launch_hmat(h, h->x.as<double>(), CouplingGate{}, loading_form ? 1 : 0, h->yS.as<double>(),
+                         h->yN.as<double>(), s);
+    if (rc) return rc;
+    if (yS) QDFN_HIP(hipMemcpyAsync(yS, h->yS.p, b, hipMemcpyDeviceToHost, s));
+    if (yN) QDFN_HIP(hipMemcpyAsync(yN, h->yN.p, b, hipMemcpyDeviceToHost, s));
+    QDFN_HIP(hipStreamSynchronize(s));
+    return QDFN_OK;
+}
+
+extern "C" int qdfn_hmat_time_matvec(qdfn_hmat_t* h, int32_t reps, double* avg_ms) {
+    if (!h || reps <= 0 || !avg_ms) return fail(QDFN_ERR_ARG, "qdfn_hmat_time_matvec: bad arguments");
+    hipStream_t s = h->stream;
+    QDFN_HIP(hipMemsetAsync(h->x.p, 0, sizeof(double) * h->n, s));
+    int rc = launch_hmat(h, h->x.as<double>(), CouplingGate{}, 1, h->yS.as<double>(), h->yN.as<double>(), s);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    QDFN_HIP(hipEventCreate(&e0));
+    QDFN_HIP(hipEventCreate(&e1));
+    QDFN_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps && rc == QDFN_OK; ++r)
+        rc = launch_hmat(h, h->x.as<double>(), CouplingGate{}, 1, h->yS.as<double>(), h->yN.as<double>(), s);
+    QDFN_HIP(hipEventRecord(e1, s));
+    QDFN_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    QDFN_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    return rc;
+}

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "dense.h"
+#include "hmat.h"
 #include "friction_device.h"
 #include "qdfn_internal.h"
 
@@ -335,7 +336,8 @@ struct GraphSlot {
 };
 
 struct qdfn_sim {
-    qdfn_dense* op = nullptr;
+    qdfn_dense* op = nullptr;   // dense coupling (main) ...
+    qdfn_hmat* hop = nullptr;   // ... or H-matrix coupling (main_H)
     qdfn_sim_params P{};
     hipStream_t stream = nullptr;
     DevBuf consts, state2, single, ints, cand, ext, hist, trace, dstate, ranlog, parts;
@@ -358,11 +360,18 @@ static int enqueue_attempt(qdfn_sim* s, int slot, hipEvent_t ev0, hipEvent_t ev1
     g.ran_log = slot >= 0 ? s->A.ran_log : nullptr;
     g.slot = slot;
     if (ev0) QDFN_HIP(hipEventRecord(ev0, s->stream));
-    int rc = launch_coupling(s->op, s->A.D, g, s->stream);
-    if (rc) return rc;
-    if (ev1) QDFN_HIP(hipEventRecord(ev1, s->stream));
-    rc = launch_coupling_finish(s->op, s->A.tl, s->A.sigprod, g.gate, s->stream);
-    if (rc) return rc;
+    int rc;
+    if (s->hop) {
+        rc = launch_hmat(s->hop, s->A.D, g, 1, s->A.tl, s->A.sigprod, s->stream);
+        if (rc) return rc;
+        if (ev1) QDFN_HIP(hipEventRecord(ev1, s->stream));
+    } else {
+        rc = launch_coupling(s->op, s->A.D, g, s->stream);
+        if (rc) return rc;
+        if (ev1) QDFN_HIP(hipEventRecord(ev1, s->stream));
+        rc = launch_coupling_finish(s->op, s->A.tl, s->A.sigprod, g.gate, s->stream);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_step, dim3((unsigned)((n + kStepBlock - 1) / kStepBlock)), dim3(kStepBlock), 0, s->stream,
                        s->A, s->d_st);
     QDFN_HIP(hipGetLastError());
@@ -378,22 +387,22 @@ extern "C" void qdfn_sim_destroy(qdfn_sim_t* s) {
     if (st) (void)hipStreamDestroy(st);
 }
 
-extern "C" qdfn_sim_t* qdfn_sim_create_dense(qdfn_dense_t* op, const qdfn_sim_params* p, const double* mass,
-                                             const double* a, const double* b, const double* dc,
-                                             const double* theta_i, const double* v_i, const double* friction_i,
-                                             const double* sigma0, const double* loading_rate) {
-    if (!op || !p || !mass || !a || !b || !dc || !theta_i || !v_i || !friction_i || !sigma0 || !loading_rate) {
-        fail(QDFN_ERR_ARG, "qdfn_sim_create_dense: null argument");
+// Shared set-up of main() / main_H(): kself = K_Self per element (host).
+static qdfn_sim* sim_create(const char* who, int64_t n_op, const double* kself_h, int normal_zero,
+                            const qdfn_sim_params* p, const double* mass, const double* a, const double* b,
+                            const double* dc, const double* theta_i, const double* v_i, const double* friction_i,
+                            const double* sigma0, const double* loading_rate) {
+    if (!p || !mass || !a || !b || !dc || !theta_i || !v_i || !friction_i || !sigma0 || !loading_rate) {
+        fail(QDFN_ERR_ARG, std::string(who) + ": null argument");
         return nullptr;
     }
-    if (p->n != op->n || p->n_loading < 0 || p->n_loading > p->n || p->record_capacity < 0 ||
+    if (p->n != n_op || p->n_loading < 0 || p->n_loading > p->n || p->record_capacity < 0 ||
         p->trace_capacity < 0) {
-        fail(QDFN_ERR_ARG, "qdfn_sim_create_dense: size mismatch");
+        fail(QDFN_ERR_ARG, std::string(who) + ": size mismatch");
         return nullptr;
     }
     const int64_t n = p->n, L = p->n_loading;
     qdfn_sim* s = new qdfn_sim();
-    s->op = op;
     s->P = *p;
     auto bail = [&](const char* msg) -> qdfn_sim_t* {
         fail(QDFN_ERR_HIP, msg);
@@ -403,9 +412,7 @@ extern "C" qdfn_sim_t* qdfn_sim_create_dense(qdfn_dense_t* op, const qdfn_sim_pa
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail("stream");
     // host-side constants (QUAKEDFN_RunARG.jl / main():42-87)
     const double V0 = 1e-9;
-    std::vector<double> kself(n), hc(10 * n);
-    if (hipMemcpy(kself.data(), op->kself.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
-        return bail("kself download");
+    std::vector<double> kself(kself_h, kself_h + n), hc(10 * n);
     std::vector<int8_t> cand(n, 0);
     int64_t n_unstable = 0;
     for (int64_t i = 0; i < n; ++i) n_unstable += (a[i] - b[i] < 0.0);
@@ -483,8 +490,44 @@ extern "C" qdfn_sim_t* qdfn_sim_create_dense(qdfn_dense_t* op, const qdfn_sim_pa
     A.t_told = tt; A.t_dt = tt + tc; A.t_vmax = tt + 2 * tc;
     A.mu = p->mu; A.rho = p->rho; A.switch_v = p->switch_v; A.dt_cut = p->dt_cut; A.rupture_dt = p->rupture_dt;
     A.maximum_dt = p->maximum_dt; A.min_sig = p->min_normal_stress; A.alpha_evo = p->alpha_evo;
-    A.conv = 1e-9; A.v0 = V0; A.dr = p->evolution_dr; A.planar = p->planar || op->normal_zero;
+    A.conv = 1e-9; A.v0 = V0; A.dr = p->evolution_dr; A.planar = p->planar || normal_zero;
     A.ext_at_told = p->ext_at_told;
+    return s;
+}
+
+extern "C" qdfn_sim_t* qdfn_sim_create_dense(qdfn_dense_t* op, const qdfn_sim_params* p, const double* mass,
+                                             const double* a, const double* b, const double* dc,
+                                             const double* theta_i, const double* v_i, const double* friction_i,
+                                             const double* sigma0, const double* loading_rate) {
+    if (!op) {
+        fail(QDFN_ERR_ARG, "qdfn_sim_create_dense: null operator");
+        return nullptr;
+    }
+    std::vector<double> kself(op->n);
+    if (hipMemcpy(kself.data(), op->kself.p, sizeof(double) * op->n, hipMemcpyDeviceToHost) != hipSuccess) {
+        fail(QDFN_ERR_HIP, "qdfn_sim_create_dense: kself download");
+        return nullptr;
+    }
+    qdfn_sim* s = sim_create("qdfn_sim_create_dense", op->n, kself.data(), op->normal_zero, p, mass, a, b, dc,
+                             theta_i, v_i, friction_i, sigma0, loading_rate);
+    if (s) s->op = op;
+    return s;
+}
+
+extern "C" qdfn_sim_t* qdfn_sim_create_hmat(qdfn_hmat_t* h, const qdfn_sim_params* p, const double* mass,
+                                            const double* a, const double* b, const double* dc,
+                                            const double* theta_i, const double* v_i, const double* friction_i,
+                                            const double* sigma0, const double* loading_rate) {
+    if (!h) {
+        fail(QDFN_ERR_ARG, "qdfn_sim_create_hmat: null operator");
+        return nullptr;
+    }
+    // K_Self = -K_S,ii from the diagonal blocks (StiffnessTransitionToLoading, Functions_Hmatrix.jl:459-477)
+    std::vector<double> kself(h->n);
+    for (int64_t i = 0; i < h->n; ++i) kself[i] = -h->kdiag_h[i];
+    qdfn_sim* s = sim_create("qdfn_sim_create_hmat", h->n, kself.data(), h->normal_zero, p, mass, a, b, dc,
+                             theta_i, v_i, friction_i, sigma0, loading_rate);
+    if (s) s->hop = h;
     return s;
 }
 

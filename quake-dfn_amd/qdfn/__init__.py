@@ -10,10 +10,11 @@ from ._lib import PREC_FP32, PREC_FP64, QdfnError, device_info, lib
 from .assembly import (BuildMatrixByParts_Even_Rec, BuildMatrixByParts_Even_Tri, Okada_DC3D_Vector,
                        RotVerts_UnitVectors, StiffnessMatrix_ByParts_Calculation_Rec,
                        StiffnessMatrix_ByParts_Calculation_Tri, TDstressHS)
+from .hmat import HMatrix, hmat_structure, main_H
 from .sim import DenseOperator, Simulation, main
 from .solvers import SolveOneTimeStep
 
 __all__ = ["PREC_FP32", "PREC_FP64", "QdfnError", "device_info", "lib", "Okada_DC3D_Vector",
            "StiffnessMatrix_ByParts_Calculation_Rec", "BuildMatrixByParts_Even_Rec", "TDstressHS",
            "RotVerts_UnitVectors", "StiffnessMatrix_ByParts_Calculation_Tri", "BuildMatrixByParts_Even_Tri",
-           "DenseOperator", "Simulation", "main", "SolveOneTimeStep"]
+           "DenseOperator", "Simulation", "main", "SolveOneTimeStep", "HMatrix", "hmat_structure", "main_H"]

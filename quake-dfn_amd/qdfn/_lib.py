@@ -54,6 +54,14 @@ class SimStats(C.Structure):
                 ("coupling_ms_avg", D), ("coupling_timed", I64)]
 
 
+class HmatParams(C.Structure):
+    _fields_ = [("min_elements_to_cut", I64), ("max_levels", I64), ("dist_diam_ratio", D),
+                ("arrange_point", D * 3), ("n_loading", I64), ("kmeans_max_iter", I64)]
+
+
+PI64 = C.POINTER(I64)
+PPD = C.POINTER(PD)
+
 # name -> (restype, argtypes); mirrors include/qdfn.h
 SIGNATURES = {
     "qdfn_last_error": (C.c_char_p, []),
@@ -73,7 +81,21 @@ SIGNATURES = {
     "qdfn_dense_time_coupling": (C.c_int, [VP, I32, PD]),
     "qdfn_solve_one_timestep": (I64, [C.POINTER(StepParams), I64, I64] + [PD] * 9 + [D] + [PD] * 4 + [PI32] +
                                 [PD] * 4 + [PD] * 6 + [PI32] * 3),
+    "qdfn_hmat_structure": (C.c_int, [C.POINTER(HmatParams), I64, PD, PD, PI64, I64, PI64, PD, PI64, I64, PI64,
+                                      PI64, PI64]),
+    "qdfn_hmat_build_rect": (VP, [PD, I64, I64, D, D, I64, PI64, PI64, D, D, I32]),
+    "qdfn_hmat_build_tri": (VP, [I64, PD, PD, PD, PD, PD, PD, PD, D, D, I64, PI64, PI64, D, D, I32]),
+    "qdfn_hmat_create": (VP, [I64, I64, PI64, PI64, PI32, PI64, PPD, PPD, PI32, PI64, PPD, PPD, I32]),
+    "qdfn_hmat_destroy": (None, [VP]),
+    "qdfn_hmat_info": (C.c_int, [VP, PI64, PI64, PI64, PI64]),
+    "qdfn_hmat_get_ranks": (C.c_int, [VP, PI32, PI64, PI32, PI64]),
+    "qdfn_hmat_get_block": (C.c_int, [VP, I64, I32, PD, PD]),
+    "qdfn_hmat_kself": (C.c_int, [VP, PD, PD]),
+    "qdfn_hmat_normal_zero": (C.c_int, [VP]),
+    "qdfn_hmat_matvec": (C.c_int, [VP, PD, PD, PD, I32]),
+    "qdfn_hmat_time_matvec": (C.c_int, [VP, I32, PD]),
     "qdfn_sim_create_dense": (VP, [VP, C.POINTER(SimParams)] + [PD] * 9),
+    "qdfn_sim_create_hmat": (VP, [VP, C.POINTER(SimParams)] + [PD] * 9),
     "qdfn_sim_set_external": (C.c_int, [VP, I64, PD, PD, PD]),
     "qdfn_sim_run": (C.c_int, [VP, I64, I32, I32]),
     "qdfn_sim_get_stats": (C.c_int, [VP, C.POINTER(SimStats)]),

@@ -99,7 +99,9 @@ class DenseOperator:
 
 
 class Simulation:
-    """One device-resident run (main() of the reference)."""
+    """One device-resident run: main() of the reference with a DenseOperator,
+    main_H() with an HMatrix (set prob["ext_at_told"] = 1 for main_H's timing
+    of the external tables)."""
 
     def __init__(self, op, prob, record_step=10, record_capacity=0, trace_capacity=0):
         self.op = op
@@ -124,8 +126,8 @@ class Simulation:
         for a in arrs:
             if a.size != n:
                 raise ValueError("element arrays must match the operator size")
-        self._h = check_handle(lib().qdfn_sim_create_dense(op.handle, p, *[ptr(a) for a in arrs]),
-                               "qdfn_sim_create_dense")
+        create = getattr(op, "SIM_CREATE", "qdfn_sim_create_dense")
+        self._h = check_handle(getattr(lib(), create)(op.handle, p, *[ptr(a) for a in arrs]), create)
         ext = prob.get("ext")
         if ext is not None:
             t = _f64(ext["t"])
