@@ -18,19 +18,22 @@ struct HBlock {
     std::vector<double> a[2], b[2];
 };
 
-// phase-1 job: one row of a low-rank factor's V (row-major, zero-padded to a
-// multiple of 4 entries at offset `off` of the V pool), applied to x[src, src + k)
-struct HVJob {
-    int64_t off, src;
-    int32_t k, pad;
+// Row block of the product: up to 64 rows of a row-major panel, one workgroup.
+//   phase 1 (kind 0): rows of V factors sharing one source range; column c
+//                     multiplies x[zoff + c]; row q writes t[out + q];
+//   phase 2 (kind 1 shear, 2 normal): one receiver row group of one matrix;
+//                     column c multiplies w[idx[zoff + c]] with w = [x; t];
+//                     row q writes row out + q of yS / yN.
+struct HRowBlock {
+    int64_t data_off, zoff, out;
+    int64_t part_base;      // split row blocks: tile q's row sums at part[part_base + q * m]
+    int32_t m, K, ld, kind;
+    int32_t ntiles, pad;
 };
 
-// phase-2 panel: rows [row0, row0 + m) of one matrix (which: 0 shear, 1 normal),
-// row-major m x ld at offset `off` of the panel pool; column c multiplies
-// w[idx[idx_off + c]] with w = [x; t]
-struct HPanel {
-    int64_t off, idx_off, row0;
-    int32_t m, K, ld, which;
+// work item: columns [c0, c1) of one row block (its q-th tile)
+struct HTile {
+    int32_t rb, c0, c1, q;
 };
 
 }  // namespace qdfn
@@ -42,10 +45,12 @@ struct qdfn_hmat {
     std::vector<qdfn::HBlock> blocks;
     std::vector<double> kdiag_h;          // K_S,ii (0 where no diagonal block)
     qdfn::DevBuf kdiag;
-    qdfn::DevBuf vpool, jobs, t;          // phase 1
-    qdfn::DevBuf ppool, panels, idx;      // phase 2
+    qdfn::DevBuf pool, idx, t;            // panel entries (V panels then row-group panels), gather indices, t
+    qdfn::DevBuf rbs, tiles, part, multi; // row blocks, tiles, partial sums, split row blocks
     qdfn::DevBuf x, yS, yN;               // vectors for qdfn_hmat_matvec
-    int64_t n_jobs = 0, n_panels = 0;
+    int64_t n_rb1 = 0, n_rb2 = 0, n_t = 0;
+    int64_t n_tiles1 = 0, n_tiles2 = 0, n_multi1 = 0, n_multi2 = 0;
+    int32_t tile_w = 1024;                // columns per tile
     int64_t stored_entries = 0;
     int64_t device_bytes = 0;
     int64_t phase_bytes = 0;              // HBM bytes one product reads and writes

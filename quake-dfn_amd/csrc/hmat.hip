@@ -10,21 +10,31 @@
 // Build on the device: the full K_S and K_N are assembled in HBM (FP64), dense
 // blocks are read out of them and admissible blocks are compressed by an
 // adaptive cross approximation with full pivoting, one workgroup per block and
-// matrix, which stops when max |residual| <= max(atol, rtol * max |block|).
+// matrix, which stops when max |residual| <= max(atol, rtol * max |K_S,b|),
+// K_S,b the block's shear entries (for the normal part too: both are stresses
+// per unit slip, and the normal entries of a near-planar fault are far smaller,
+// so a tolerance relative to their own size would spend rank on noise).  atol
+// alone gives the reference's pqrfact(block, atol = Tolerance).
 //
 // Product layout (HBM): the rows are cut into row groups of at most 64 rows
 // whose boundaries include every block's receiver boundaries.  Each row group
 // owns one row-major panel per matrix holding, side by side, the rows of every
 // block that covers it: dense blocks contribute their columns, low-rank blocks
 // the matching rows of U.  A product is then
-//   phase 1: t = V_b x[src_b] for every low-rank factor (one wavefront per row
-//            of V, coalesced along the row);
-//   phase 2: one workgroup per panel gathers its right-hand side z from x
-//            (dense columns) and t (U columns) into LDS and streams the panel,
-//            16-byte loads along the rows, FP64 accumulation, fixed-order
-//            wavefront reductions (no atomics: results are deterministic).
+//   phase 1: t = V_b x[src_b] for every low-rank factor; the V rows of all
+//            factors that share a source range form one row-major panel over
+//            that (contiguous) range of x;
+//   phase 2: y = panel * [x; t] per row group, the right-hand side gathered
+//            through an index list.
+// Both phases run the same kernel: a panel is cut into row blocks of at most
+// 64 rows and those into tiles of at most 1024 columns, one workgroup each,
+// which streams its rows with 16-byte loads, FP64 accumulation and wavefront
+// reductions; split row blocks are added tile by tile in column order by a
+// small finish kernel (no atomics: results are deterministic run to run).
 // Panels and V are stored in FP64 or FP32 (mixed precision).
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -38,7 +48,7 @@ namespace {
 
 constexpr int kRowsPerGroup = 64;
 constexpr int kPhase2Threads = 256;
-constexpr int kZChunk = 4096;  // doubles of the right-hand side staged in LDS per pass
+constexpr int kChunk = 1024;   // right-hand-side columns staged in LDS per pass (double-buffered)
 
 int64_t round4(int64_t v) { return (v + 3) / 4 * 4; }
 
@@ -66,99 +76,160 @@ __device__ inline void load4<double>(const double* p, double v[4]) {
     v[3] = b.y;
 }
 
+template <class T>
+__device__ inline void load4raw(const T* p, T v[4]) {
+    if constexpr (sizeof(T) == 4) {
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        v[0] = q.x;
+        v[1] = q.y;
+        v[2] = q.z;
+        v[3] = q.w;
+    } else {
+        const double2 a = reinterpret_cast<const double2*>(p)[0];
+        const double2 b = reinterpret_cast<const double2*>(p)[1];
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = b.x;
+        v[3] = b.y;
+    }
+}
+
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
-// phase 1: t[j] = V_row(j) . x[src_j : src_j + k_j)
-template <class T>
-__global__ __launch_bounds__(256) void k_hphase1(const HVJob* __restrict__ jobs, int64_t n_jobs,
-                                                 const T* __restrict__ vpool, const double* __restrict__ x0,
-                                                 double* __restrict__ t, CouplingGate g) {
-    if (g.gate && *g.gate == 0) return;
-    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
-    const int lane = threadIdx.x & 63;
-    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (j >= n_jobs) return;
-    const HVJob jb = jobs[j];
-    const T* v = vpool + jb.off;
-    const double* xs = x + jb.src;
-    double acc = 0.0;
-    for (int64_t c = lane * 4; c < jb.k; c += 256) {
-        double w[4];
-        load4<T>(v + c, w);
-        // the row is zero-padded to a multiple of 4; guard x beyond k
-        for (int q = 0; q < 4; ++q)
-            if (c + q < jb.k) acc = fma(w[q], xs[c + q], acc);
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) t[j] = acc;
+// Output of one row (main_H :241-243 for the shear loading form:
+// (H_load * (0 - x)) ./ (-K_Self) with -K_Self = K_ii).
+__device__ inline void hmat_out(int kind, int64_t i, double s, const double* __restrict__ x,
+                                const double* __restrict__ kdiag, int32_t form, double* __restrict__ t,
+                                double* __restrict__ yS, double* __restrict__ yN) {
+    if (kind == 0)
+        t[i] = s;
+    else if (kind == 1)
+        yS[i] = form ? (-s) / kdiag[i] : s + kdiag[i] * x[i];
+    else
+        yN[i] = s;
 }
 
-// phase 2: one workgroup per panel
-template <class T>
-__global__ __launch_bounds__(kPhase2Threads) void k_hphase2(const HPanel* __restrict__ panels,
-                                                            const T* __restrict__ pool,
-                                                            const int32_t* __restrict__ idx,
-                                                            const double* __restrict__ x0,
-                                                            const double* __restrict__ t, int64_t n,
-                                                            const double* __restrict__ kdiag, int32_t form,
-                                                            double* __restrict__ yS, double* __restrict__ yN,
-                                                            CouplingGate g) {
+// One workgroup per tile (columns [c0, c1) of a row block).  The right-hand
+// side is staged in LDS in chunks of kChunk columns, double-buffered: the
+// gather of chunk i + 1 (from x for V panels, through the index list from
+// [x; t] for row-group panels) is issued before chunk i is streamed.  Each row
+// is read by LPR lanes (16-byte loads), LPR the smallest power of two >= the
+// chunk width / 4 (at most 64), so a wavefront load covers 64 / LPR rows.
+// A row block with one tile writes its outputs; a split one writes partial
+// row sums for k_hfinish.
+template <class T, bool GATHER>
+__global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict__ tiles,
+                                                         const HRowBlock* __restrict__ rbs,
+                                                         double* __restrict__ part,
+                                                         const T* __restrict__ pool,
+                                                         const int32_t* __restrict__ idx,
+                                                         const double* __restrict__ x0, double* __restrict__ t,
+                                                         int64_t n, const double* __restrict__ kdiag, int32_t form,
+                                                         double* __restrict__ yS, double* __restrict__ yN,
+                                                         int32_t mark, CouplingGate g) {
     if (g.gate && *g.gate == 0) return;
-    __shared__ double z[kZChunk];
+    __shared__ double zbuf[2][kChunk];
     const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
-    const HPanel P = panels[blockIdx.x];
+    const HTile tl = tiles[blockIdx.x];
+    const HRowBlock R = rbs[tl.rb];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int32_t K = tl.c1, K4 = (K + 3) / 4 * 4;  // columns [tl.c0, K) of the row block
+    const int32_t cw = min(K4 - tl.c0, (int32_t)kChunk);
+    int lpr = 1;
+    while (lpr < 64 && 4 * lpr < cw) lpr <<= 1;
+    const int rpw = 64 / lpr, step_rows = 4 * rpw, sub = lane & (lpr - 1), rl = lane / lpr;
+    const T* base = pool + R.data_off;
+    constexpr int kPer = kChunk / kPhase2Threads;
+    double gv[kPer];
+    auto gather = [&](int32_t c0) {
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int32_t c = c0 + (int32_t)threadIdx.x + q * kPhase2Threads;
+            double val = 0.0;
+            if (c < K) {
+                if (GATHER) {
+                    const int32_t j = idx[R.zoff + c];
+                    val = j < n ? x[j] : t[j - n];
+                } else {
+                    val = x[R.zoff + c];
+                }
+            }
+            gv[q] = val;
+        }
+    };
+    gather(tl.c0);
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) zbuf[0][threadIdx.x + q * kPhase2Threads] = gv[q];
+    __syncthreads();
     double acc[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0;
-    const T* base = pool + P.off;
-    for (int64_t c0 = 0; c0 < P.K; c0 += kZChunk) {
-        const int64_t kc = min((int64_t)kZChunk, (int64_t)P.K - c0);
-        const int64_t kc4 = (kc + 3) / 4 * 4;
-        __syncthreads();
-        for (int64_t c = threadIdx.x; c < kc4; c += kPhase2Threads) {
-            double v = 0.0;
-            if (c < kc) {
-                const int32_t j = idx[P.idx_off + c0 + c];
-                v = j < n ? x[j] : t[j - n];
-            }
-            z[c] = v;
-        }
-        __syncthreads();
-        for (int64_t c = lane * 4; c < kc4; c += 256) {
-            const double z0 = z[c], z1 = z[c + 1], z2 = z[c + 2], z3 = z[c + 3];
+    int buf = 0;
+    for (int32_t c0 = tl.c0; c0 < K4; c0 += kChunk) {
+        const bool more = c0 + kChunk < K4;
+        if (more) gather(c0 + kChunk);
+        const double* z = zbuf[buf];
+        const int32_t lim = min((int32_t)kChunk, K4 - c0);
+        for (int32_t cb = 0; cb < lim; cb += 4 * lpr) {
+            const int32_t c = cb + sub * 4;
+            if (c < lim) {
+                const double z0 = z[c], z1 = z[c + 1], z2 = z[c + 2], z3 = z[c + 3];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = wave + 4 * r;
-                if (row < P.m) {
-                    double w[4];
-                    load4<T>(base + (int64_t)row * P.ld + c0 + c, w);
-                    acc[r] = fma(w[0], z0, acc[r]);
-                    acc[r] = fma(w[1], z1, acc[r]);
-                    acc[r] = fma(w[2], z2, acc[r]);
-                    acc[r] = fma(w[3], z3, acc[r]);
+                for (int r = 0; r < 16; ++r) {
+                    const int row = r * step_rows + wave * rpw + rl;
+                    if (row < R.m) {
+                        double w[4];
+                        load4<T>(base + (int64_t)row * R.ld + c0 + c, w);
+                        acc[r] = fma(w[0], z0, acc[r]);
+                        acc[r] = fma(w[1], z1, acc[r]);
+                        acc[r] = fma(w[2], z2, acc[r]);
+                        acc[r] = fma(w[3], z3, acc[r]);
+                    }
                 }
             }
         }
+        if (more) {
+#pragma unroll
+            for (int q = 0; q < kPer; ++q) zbuf[buf ^ 1][threadIdx.x + q * kPhase2Threads] = gv[q];
+        }
+        __syncthreads();
+        buf ^= 1;
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        const int row = wave + 4 * r;
-        const double s = wave_sum(acc[r]);
-        if (lane == 0 && row < P.m) {
-            const int64_t i = P.row0 + row;
-            if (P.which == 0) {
-                // main_H :241-243: (H_load * (0 - x)) ./ (-K_Self), -K_Self = K_ii
-                yS[i] = form ? (-s) / kdiag[i] : s + kdiag[i] * x[i];
-            } else {
-                yN[i] = s;
-            }
+        double v = acc[r];
+        for (int o = lpr >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        const int row = r * step_rows + wave * rpw + rl;
+        if (sub == 0 && row < R.m) {
+            if (R.ntiles == 1)
+                hmat_out(R.kind, R.out + row, v, x, kdiag, form, t, yS, yN);
+            else
+                part[R.part_base + (int64_t)tl.q * R.m + row] = v;
         }
     }
-    if (g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
+    if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
+}
+
+// finish of the split row blocks: one 64-lane workgroup each, one lane per row,
+// tiles added in column order
+__global__ __launch_bounds__(64) void k_hfinish(const int32_t* __restrict__ multi, const HRowBlock* __restrict__ rbs,
+                                                const double* __restrict__ part, const double* __restrict__ x0,
+                                                const double* __restrict__ kdiag, int32_t form,
+                                                double* __restrict__ t, double* __restrict__ yS,
+                                                double* __restrict__ yN, int32_t mark, CouplingGate g) {
+    if (g.gate && *g.gate == 0) return;
+    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const HRowBlock R = rbs[multi[blockIdx.x]];
+    const int row = threadIdx.x;
+    if (row < R.m) {
+        double s = 0.0;
+        for (int32_t q = 0; q < R.ntiles; ++q) s += part[R.part_base + (int64_t)q * R.m + row];
+        hmat_out(R.kind, R.out + row, s, x, kdiag, form, t, yS, yN);
+    }
+    if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
 }
 
 __global__ void k_mark_skipped(CouplingGate g) {
@@ -168,6 +239,7 @@ __global__ void k_mark_skipped(CouplingGate g) {
 // Adaptive cross approximation with full pivoting of one block (one workgroup).
 struct AcaJob {
     const double* src;  // block in the full matrix (column-major, leading dimension ld)
+    const double* scale;  // max |K_S,b| of the block
     int64_t ld;
     double* R;          // m x k residual workspace
     double* U;          // m x rmax
@@ -210,6 +282,31 @@ __device__ void aca_reduce(double& v, int32_t& id, double& sv, double* sh_v, int
     sv = sh_s[0];
 }
 
+// max |entry| of blocks (one workgroup per block): the scale of the ACA threshold
+struct MaxJob {
+    const double* src;
+    int64_t ld;
+    int32_t m, k;
+};
+
+__global__ __launch_bounds__(kAcaThreads) void k_blockmax(const MaxJob* __restrict__ jobs, double* __restrict__ out) {
+    __shared__ double sh[kAcaThreads / 64];
+    const MaxJob J = jobs[blockIdx.x];
+    double v = 0.0;
+    const int64_t mk = (int64_t)J.m * J.k;
+    for (int64_t e = threadIdx.x; e < mk; e += kAcaThreads) {
+        const int64_t i = e % J.m, j = e / J.m;
+        v = fmax(v, fabs(J.src[i + j * J.ld]));
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kAcaThreads / 64; ++w) v = fmax(v, sh[w]);
+        out[blockIdx.x] = v;
+    }
+}
+
 __global__ __launch_bounds__(kAcaThreads) void k_aca(const AcaJob* __restrict__ jobs, int32_t* __restrict__ rank,
                                                      int32_t* __restrict__ converged) {
     __shared__ double sh_v[kAcaThreads / 64], sh_s[kAcaThreads / 64];
@@ -225,7 +322,7 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(const AcaJob* __restrict__ 
         better(amax, aid, piv, fabs(v), e, v);
     }
     aca_reduce(amax, aid, piv, sh_v, sh_i, sh_s);
-    const double thr = fmax(J.atol, J.rtol * amax);
+    const double thr = fmax(J.atol, J.rtol * *J.scale);
     int32_t r = 0;
     while (amax > thr && r < J.rmax) {
         const int32_t is = aid % m, js = aid / m;
@@ -271,7 +368,93 @@ static int hmat_layout(qdfn_hmat* h) {
             for (double e : *v)
                 if (e != 0.0) nz = 0;
     h->normal_zero = nz;
-    // row groups
+    std::vector<double> pool;
+    std::vector<int32_t> idx;
+    std::vector<HRowBlock> rbs;
+    if (const char* e = getenv("QDFN_HMAT_TILE")) {
+        const int v = atoi(e);
+        if (v >= kChunk && v % kChunk == 0) h->tile_w = v;
+    }
+    std::vector<HTile> tiles;
+    std::vector<int32_t> multi;
+    int64_t part_total = 0;
+    auto add_rowblock = [&](HRowBlock R) {
+        const int32_t t0 = (int32_t)tiles.size();
+        int32_t c = 0;
+        do {
+            HTile T;
+            T.rb = (int32_t)rbs.size();
+            T.c0 = c;
+            T.c1 = std::min<int32_t>(R.K, c + h->tile_w);
+            T.q = (int32_t)tiles.size() - t0;
+            tiles.push_back(T);
+            c = T.c1;
+        } while (c < R.K);
+        R.ntiles = (int32_t)tiles.size() - t0;
+        R.pad = 0;
+        R.part_base = -1;
+        if (R.ntiles > 1) {
+            R.part_base = part_total;
+            part_total += (int64_t)R.ntiles * R.m;
+            multi.push_back((int32_t)rbs.size());
+        }
+        rbs.push_back(R);
+    };
+    // phase 1: V panels, one per source range, in order of first appearance
+    std::vector<int64_t> toff(2 * B.size(), -1);
+    {
+        std::vector<std::pair<int64_t, int64_t>> keys;
+        std::vector<std::vector<std::pair<size_t, int>>> members;
+        for (size_t bi = 0; bi < B.size(); ++bi)
+            for (int w = 0; w < 2; ++w) {
+                const HBlock& b = B[bi];
+                if (b.kind[w] != QDFN_HBLK_LOWRANK || b.rank[w] == 0) continue;
+                size_t gk = 0;
+                while (gk < keys.size() && keys[gk] != std::make_pair(b.s0, b.s1)) ++gk;
+                if (gk == keys.size()) {
+                    keys.push_back({b.s0, b.s1});
+                    members.emplace_back();
+                }
+                members[gk].push_back({bi, w});
+            }
+        int64_t tbase = 0;
+        for (size_t gk = 0; gk < keys.size(); ++gk) {
+            const int64_t s0 = keys[gk].first, k = keys[gk].second - s0 + 1, ld = round4(k);
+            int64_t rows = 0;
+            for (auto& f : members[gk]) {
+                toff[2 * f.first + f.second] = tbase + rows;
+                rows += B[f.first].rank[f.second];
+            }
+            const int64_t off = (int64_t)pool.size();
+            pool.resize(off + rows * ld, 0.0);
+            int64_t row = 0;
+            for (auto& f : members[gk]) {
+                const HBlock& b = B[f.first];
+                const int64_t r = b.rank[f.second];
+                const std::vector<double>& V = b.b[f.second];
+                for (int64_t q = 0; q < r; ++q, ++row)
+                    for (int64_t c = 0; c < k; ++c) pool[off + row * ld + c] = V[q + c * r];
+            }
+            for (int64_t r0 = 0; r0 < rows; r0 += kRowsPerGroup) {
+                HRowBlock R;
+                R.data_off = off + r0 * ld;
+                R.zoff = s0;
+                R.out = tbase + r0;
+                R.m = (int32_t)std::min<int64_t>(kRowsPerGroup, rows - r0);
+                R.K = (int32_t)k;
+                R.ld = (int32_t)ld;
+                R.kind = 0;
+                add_rowblock(R);
+            }
+            tbase += rows;
+        }
+        h->n_t = tbase;
+    }
+    h->n_rb1 = (int64_t)rbs.size();
+    h->n_tiles1 = (int64_t)tiles.size();
+    h->n_multi1 = (int64_t)multi.size();
+    if (n + h->n_t > 0x7fffffffLL) return fail(QDFN_ERR_ARG, "H-matrix: index space exceeds int32");
+    // phase 2: row groups of at most 64 rows whose boundaries include every receiver boundary
     std::vector<int64_t> cut = {0, n};
     for (const HBlock& b : B) {
         cut.push_back(b.r0);
@@ -290,60 +473,10 @@ static int hmat_layout(qdfn_hmat* h) {
         int64_t gi = std::lower_bound(g0.begin(), g0.begin() + G, b.r0) - g0.begin();
         for (; gi < G && g0[gi + 1] - 1 <= b.r1; ++gi) cover[gi].push_back(bi);
     }
-    // phase-1 jobs and t offsets
-    std::vector<HVJob> jobs;
-    std::vector<int64_t> vrow_off;
-    int64_t vtotal = 0;
-    std::vector<int64_t> toff(2 * B.size(), -1);
-    for (size_t bi = 0; bi < B.size(); ++bi)
-        for (int w = 0; w < 2; ++w) {
-            const HBlock& b = B[bi];
-            if (b.kind[w] != QDFN_HBLK_LOWRANK) continue;
-            const int64_t k = b.s1 - b.s0 + 1;
-            toff[2 * bi + w] = (int64_t)jobs.size();
-            for (int64_t r = 0; r < b.rank[w]; ++r) {
-                HVJob j;
-                j.off = vtotal;
-                j.src = b.s0;
-                j.k = (int32_t)k;
-                j.pad = 0;
-                jobs.push_back(j);
-                vtotal += round4(k);
-            }
-        }
-    h->n_jobs = (int64_t)jobs.size();
-    std::vector<double> vh(std::max<int64_t>(vtotal, 4), 0.0);
-    for (size_t bi = 0; bi < B.size(); ++bi)
-        for (int w = 0; w < 2; ++w) {
-            const HBlock& b = B[bi];
-            if (b.kind[w] != QDFN_HBLK_LOWRANK) continue;
-            const int64_t k = b.s1 - b.s0 + 1, r = b.rank[w];
-            for (int64_t q = 0; q < r; ++q) {
-                const HVJob& j = jobs[toff[2 * bi + w] + q];
-                for (int64_t c = 0; c < k; ++c) vh[j.off + c] = b.b[w][q + c * r];  // V is r x k column-major
-            }
-        }
-    // panels
-    std::vector<HPanel> panels;
-    std::vector<int32_t> idx;
-    int64_t ptotal = 0;
     for (int64_t gi = 0; gi < G; ++gi)
         for (int w = 0; w < 2; ++w) {
-            int64_t K = 0;
-            for (int32_t bi : cover[gi]) {
-                const HBlock& b = B[bi];
-                K += b.kind[w] == QDFN_HBLK_DENSE ? (b.s1 - b.s0 + 1) : b.rank[w];
-            }
-            HPanel P;
-            P.off = ptotal;
-            P.idx_off = (int64_t)idx.size();
-            P.row0 = g0[gi];
-            P.m = (int32_t)(g0[gi + 1] - g0[gi]);
-            P.K = (int32_t)K;
-            P.ld = (int32_t)round4(K);
-            P.which = w;
-            panels.push_back(P);
-            ptotal += (int64_t)P.m * P.ld;
+            const int64_t m = g0[gi + 1] - g0[gi], row0 = g0[gi];
+            const int64_t i0 = (int64_t)idx.size();
             for (int32_t bi : cover[gi]) {
                 const HBlock& b = B[bi];
                 if (b.kind[w] == QDFN_HBLK_DENSE)
@@ -351,90 +484,114 @@ static int hmat_layout(qdfn_hmat* h) {
                 else
                     for (int64_t c = 0; c < b.rank[w]; ++c) idx.push_back((int32_t)(n + toff[2 * bi + w] + c));
             }
+            const int64_t K = (int64_t)idx.size() - i0, ld = round4(K);
+            const int64_t off = (int64_t)pool.size();
+            pool.resize(off + m * ld, 0.0);
+            int64_t col = 0;
+            for (int32_t bi : cover[gi]) {
+                const HBlock& b = B[bi];
+                const int64_t mb = b.r1 - b.r0 + 1, roff = row0 - b.r0;
+                const bool diag = (w == 0 && b.s0 == b.r0 && b.s1 == b.r1 && b.kind[0] == QDFN_HBLK_DENSE);
+                const int64_t nc = b.kind[w] == QDFN_HBLK_DENSE ? (b.s1 - b.s0 + 1) : b.rank[w];
+                const std::vector<double>& A = b.a[w];
+                for (int64_t i = 0; i < m; ++i)
+                    for (int64_t c = 0; c < nc; ++c) {
+                        double v = A[(roff + i) + c * mb];
+                        if (diag && row0 + i == b.s0 + c) v = 0.0;  // loading form: zero diagonal
+                        pool[off + i * ld + col + c] = v;
+                    }
+                col += nc;
+            }
+            HRowBlock R;
+            R.data_off = off;
+            R.zoff = i0;
+            R.out = row0;
+            R.m = (int32_t)m;
+            R.K = (int32_t)K;
+            R.ld = (int32_t)ld;
+            R.kind = 1 + w;
+            add_rowblock(R);
         }
-    if (n + (int64_t)jobs.size() > 0x7fffffffLL) return fail(QDFN_ERR_ARG, "H-matrix: index space exceeds int32");
-    std::vector<double> ph(std::max<int64_t>(ptotal, 4), 0.0);
-    for (size_t pi = 0; pi < panels.size(); ++pi) {
-        const HPanel& P = panels[pi];
-        const int w = P.which;
-        int64_t col = 0;
-        for (int32_t bi : cover[(int64_t)pi / 2]) {
-            const HBlock& b = B[bi];
-            const int64_t mb = b.r1 - b.r0 + 1, roff = P.row0 - b.r0;
-            const bool diag = (w == 0 && b.s0 == b.r0 && b.s1 == b.r1);
-            const int64_t nc = b.kind[w] == QDFN_HBLK_DENSE ? (b.s1 - b.s0 + 1) : b.rank[w];
-            const std::vector<double>& A = b.a[w];
-            for (int64_t i = 0; i < P.m; ++i)
-                for (int64_t c = 0; c < nc; ++c) {
-                    double v = A[(roff + i) + c * mb];
-                    if (diag && b.kind[w] == QDFN_HBLK_DENSE && P.row0 + i == b.s0 + c) v = 0.0;
-                    ph[P.off + i * P.ld + col + c] = v;
-                }
-            col += nc;
-        }
-    }
-    h->n_panels = (int64_t)panels.size();
+    h->n_rb2 = (int64_t)rbs.size() - h->n_rb1;
+    h->n_tiles2 = (int64_t)tiles.size() - h->n_tiles1;
+    h->n_multi2 = (int64_t)multi.size() - h->n_multi1;
     h->stored_entries = 0;
     for (const HBlock& b : B)
         for (int w = 0; w < 2; ++w) h->stored_entries += (int64_t)(b.a[w].size() + b.b[w].size());
     // upload
     const size_t es = h->prec == QDFN_PREC_FP32 ? sizeof(float) : sizeof(double);
-    auto upload_vals = [&](DevBuf& d, const std::vector<double>& v) -> int {
-        QDFN_HIP(d.alloc(es * v.size()));
-        if (h->prec == QDFN_PREC_FP32) {
-            std::vector<float> f(v.begin(), v.end());
-            QDFN_HIP(hipMemcpy(d.p, f.data(), es * f.size(), hipMemcpyHostToDevice));
-        } else {
-            QDFN_HIP(hipMemcpy(d.p, v.data(), es * v.size(), hipMemcpyHostToDevice));
-        }
-        return QDFN_OK;
-    };
-    int rc = upload_vals(h->vpool, vh);
-    if (rc) return rc;
-    rc = upload_vals(h->ppool, ph);
-    if (rc) return rc;
-    QDFN_HIP(h->jobs.alloc(sizeof(HVJob) * std::max<size_t>(jobs.size(), 1)));
-    if (!jobs.empty()) QDFN_HIP(hipMemcpy(h->jobs.p, jobs.data(), sizeof(HVJob) * jobs.size(), hipMemcpyHostToDevice));
-    QDFN_HIP(h->panels.alloc(sizeof(HPanel) * std::max<size_t>(panels.size(), 1)));
-    QDFN_HIP(hipMemcpy(h->panels.p, panels.data(), sizeof(HPanel) * panels.size(), hipMemcpyHostToDevice));
+    if (pool.size() < 4) pool.resize(4, 0.0);
+    QDFN_HIP(h->pool.alloc(es * pool.size()));
+    if (h->prec == QDFN_PREC_FP32) {
+        std::vector<float> f(pool.begin(), pool.end());
+        QDFN_HIP(hipMemcpy(h->pool.p, f.data(), es * f.size(), hipMemcpyHostToDevice));
+    } else {
+        QDFN_HIP(hipMemcpy(h->pool.p, pool.data(), es * pool.size(), hipMemcpyHostToDevice));
+    }
+    QDFN_HIP(h->rbs.alloc(sizeof(HRowBlock) * rbs.size()));
+    QDFN_HIP(hipMemcpy(h->rbs.p, rbs.data(), sizeof(HRowBlock) * rbs.size(), hipMemcpyHostToDevice));
+    QDFN_HIP(h->tiles.alloc(sizeof(HTile) * tiles.size()));
+    QDFN_HIP(hipMemcpy(h->tiles.p, tiles.data(), sizeof(HTile) * tiles.size(), hipMemcpyHostToDevice));
+    QDFN_HIP(h->part.alloc(sizeof(double) * std::max<int64_t>(part_total, 1)));
+    QDFN_HIP(h->multi.alloc(sizeof(int32_t) * std::max<size_t>(multi.size(), 1)));
+    if (!multi.empty())
+        QDFN_HIP(hipMemcpy(h->multi.p, multi.data(), sizeof(int32_t) * multi.size(), hipMemcpyHostToDevice));
     QDFN_HIP(h->idx.alloc(sizeof(int32_t) * std::max<size_t>(idx.size(), 1)));
     if (!idx.empty()) QDFN_HIP(hipMemcpy(h->idx.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
-    QDFN_HIP(h->t.alloc(sizeof(double) * std::max<size_t>(jobs.size(), 1)));
+    QDFN_HIP(h->t.alloc(sizeof(double) * std::max<int64_t>(h->n_t, 1)));
     QDFN_HIP(h->kdiag.alloc(sizeof(double) * n));
     QDFN_HIP(hipMemcpy(h->kdiag.p, h->kdiag_h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
     QDFN_HIP(h->x.alloc(sizeof(double) * n));
     QDFN_HIP(h->yS.alloc(sizeof(double) * n));
     QDFN_HIP(h->yN.alloc(sizeof(double) * n));
-    h->device_bytes = (int64_t)(es * (vh.size() + ph.size()) + sizeof(int32_t) * idx.size() +
-                                sizeof(HVJob) * jobs.size() + sizeof(HPanel) * panels.size());
-    h->phase_bytes = (int64_t)(es * (vtotal + ptotal) + sizeof(int32_t) * idx.size() + sizeof(HVJob) * jobs.size() +
-                               sizeof(HPanel) * panels.size() + 8 * (2 * n + 2 * (int64_t)jobs.size()));
+    // HBM bytes of one product: panel entries, indices, descriptors, t (write + read), x, yS, yN
+    h->phase_bytes = (int64_t)(es * pool.size() + sizeof(int32_t) * idx.size() + sizeof(HRowBlock) * rbs.size() +
+                               sizeof(HTile) * tiles.size() + 16 * part_total + 16 * h->n_t + 8 * 3 * n);
+    h->device_bytes = h->phase_bytes;
     return QDFN_OK;
 }
 
 int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, int32_t form, double* d_yS,
                 double* d_yN, hipStream_t s) {
-    if (h->n_jobs > 0) {
-        const unsigned nb = (unsigned)((h->n_jobs + 3) / 4);
-        if (h->prec == QDFN_PREC_FP32)
-            hipLaunchKernelGGL(k_hphase1<float>, dim3(nb), dim3(256), 0, s, h->jobs.as<HVJob>(), h->n_jobs,
-                               h->vpool.as<float>(), d_x, h->t.as<double>(), g);
+    const HRowBlock* rbs = h->rbs.as<HRowBlock>();
+    const HTile* tiles = h->tiles.as<HTile>();
+    const int32_t* multi = h->multi.as<int32_t>();
+    double* part = h->part.as<double>();
+    const double* kd = h->kdiag.as<double>();
+    double* t = h->t.as<double>();
+    const bool f32 = h->prec == QDFN_PREC_FP32;
+    if (h->n_tiles1 > 0) {
+        if (f32)
+            hipLaunchKernelGGL((k_hrow<float, false>), dim3((unsigned)h->n_tiles1), dim3(kPhase2Threads), 0, s, tiles,
+                               rbs, part, h->pool.as<float>(), h->idx.as<int32_t>(), d_x, t, h->n, kd, form, d_yS,
+                               d_yN, 0, g);
         else
-            hipLaunchKernelGGL(k_hphase1<double>, dim3(nb), dim3(256), 0, s, h->jobs.as<HVJob>(), h->n_jobs,
-                               h->vpool.as<double>(), d_x, h->t.as<double>(), g);
+            hipLaunchKernelGGL((k_hrow<double, false>), dim3((unsigned)h->n_tiles1), dim3(kPhase2Threads), 0, s,
+                               tiles, rbs, part, h->pool.as<double>(), h->idx.as<int32_t>(), d_x, t, h->n, kd, form,
+                               d_yS, d_yN, 0, g);
         QDFN_HIP(hipGetLastError());
+        if (h->n_multi1 > 0) {
+            hipLaunchKernelGGL(k_hfinish, dim3((unsigned)h->n_multi1), dim3(64), 0, s, multi, rbs, part, d_x, kd, form,
+                               t, d_yS, d_yN, 0, g);
+            QDFN_HIP(hipGetLastError());
+        }
     }
     if (g.ran_log) hipLaunchKernelGGL(k_mark_skipped, dim3(1), dim3(1), 0, s, g);
-    const unsigned np = (unsigned)h->n_panels;
-    if (h->prec == QDFN_PREC_FP32)
-        hipLaunchKernelGGL(k_hphase2<float>, dim3(np), dim3(kPhase2Threads), 0, s, h->panels.as<HPanel>(),
-                           h->ppool.as<float>(), h->idx.as<int32_t>(), d_x, h->t.as<double>(), h->n,
-                           h->kdiag.as<double>(), form, d_yS, d_yN, g);
+    const int32_t mark_tiles = h->n_multi2 > 0 ? 0 : 1;
+    if (f32)
+        hipLaunchKernelGGL((k_hrow<float, true>), dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
+                           tiles + h->n_tiles1, rbs, part, h->pool.as<float>(), h->idx.as<int32_t>(), d_x, t, h->n,
+                           kd, form, d_yS, d_yN, mark_tiles, g);
     else
-        hipLaunchKernelGGL(k_hphase2<double>, dim3(np), dim3(kPhase2Threads), 0, s, h->panels.as<HPanel>(),
-                           h->ppool.as<double>(), h->idx.as<int32_t>(), d_x, h->t.as<double>(), h->n,
-                           h->kdiag.as<double>(), form, d_yS, d_yN, g);
+        hipLaunchKernelGGL((k_hrow<double, true>), dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
+                           tiles + h->n_tiles1, rbs, part, h->pool.as<double>(), h->idx.as<int32_t>(), d_x, t, h->n,
+                           kd, form, d_yS, d_yN, mark_tiles, g);
     QDFN_HIP(hipGetLastError());
+    if (h->n_multi2 > 0) {
+        hipLaunchKernelGGL(k_hfinish, dim3((unsigned)h->n_multi2), dim3(64), 0, s, multi + h->n_multi1, rbs, part,
+                           d_x, kd, form, t, d_yS, d_yN, 1, g);
+        QDFN_HIP(hipGetLastError());
+    }
     return QDFN_OK;
 }
 
@@ -504,6 +661,25 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
                 lr.push_back({q, w});
             }
         }
+    // threshold scale: max |K_S,b| of every block (device reduction)
+    DevBuf dmax, dmj;
+    {
+        std::vector<MaxJob> mj(nblk);
+        for (int64_t q = 0; q < nblk; ++q) {
+            const HBlock& b = h->blocks[q];
+            mj[q].src = dKS + b.r0 + b.s0 * n;
+            mj[q].ld = n;
+            mj[q].m = (int32_t)(b.r1 - b.r0 + 1);
+            mj[q].k = (int32_t)(b.s1 - b.s0 + 1);
+        }
+        QDFN_HIP(dmax.alloc(sizeof(double) * nblk));
+        QDFN_HIP(dmj.alloc(sizeof(MaxJob) * nblk));
+        QDFN_HIP(hipMemcpy(dmj.p, mj.data(), sizeof(MaxJob) * nblk, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_blockmax, dim3((unsigned)nblk), dim3(kAcaThreads), 0, h->stream, dmj.as<MaxJob>(),
+                           dmax.as<double>());
+        QDFN_HIP(hipGetLastError());
+        QDFN_HIP(hipStreamSynchronize(h->stream));
+    }
     // admissible blocks in batches bounded by a workspace budget
     size_t free_b = 0, total_b = 0;
     QDFN_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -533,6 +709,7 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
             const int64_t rmax = std::min<int64_t>(256, m * k / (m + k));
             AcaJob J;
             J.src = full[w] + b.r0 + b.s0 * n;
+            J.scale = dmax.as<double>() + lr[q].first;
             J.ld = n;
             J.R = ws.as<double>() + off[q - pos];
             J.U = J.R + m * k;

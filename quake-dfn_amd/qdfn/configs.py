@@ -261,6 +261,33 @@ def rough_fault(nx=100, nz=100, h=50.0, top=100.0, seed=20240612, ratio=1e-2, n_
                 dc=np.full(n, 1e-3), mu=mu, nu=nu, rho=mu / vs ** 2, min_normal_stress=2e6, h=h)
 
 
+def problem_from_triangles(m, kself, ks_row_sums, kn_row_sums, rate=1e-9, order=None, **overrides):
+    """Element arrays and run parameters for a triangle workload (rows in
+    `order` when given): FaultMass from the smallest triangle length scale
+    (FaultLengthStrike = FaultLengthDip of ReadSegmentInput,
+    Functions_BuildInputFile.jl:320-321), friction initialised at steady state
+    with V_i = 1e-9, back-slip loading."""
+    idx = np.arange(m["a"].size) if order is None else np.asarray(order)
+    n = idx.size
+    d = dict(RUN3_DEFAULTS)
+    d.update(overrides)
+    lengths = triangle_lengths(m["P1"][idx], m["P2"][idx], m["P3"][idx])
+    vs = d["vertical_length_scale"] if d["vertical_length_scale"] != 0 else lengths.min()
+    mass = np.ones(n) * vs * m["rho"] / 2
+    rec = np.sqrt(mass / np.abs(kself)).min() / 10
+    a, b = m["a"][idx].copy(), m["b"][idx].copy()
+    dc = m["dc"][idx].copy()
+    v_i = np.full(n, 1e-9)
+    prob = dict(n=n, n_loading=0, mass=mass, a=a, b=b, dc=dc, theta_i=dc / v_i, v_i=v_i,
+                friction_i=_friction_init(a, b), sigma0=m["sigma"][idx].copy(), loading_rate=np.zeros(n),
+                mu=m["mu"], rho=m["rho"], switch_v=d["switch_v"], dt_cut=d["dt_cut"],
+                rupture_dt=rec * d["rupture_multiple"], maximum_dt=d["maximum_dt"],
+                min_normal_stress=m["min_normal_stress"], alpha_evo=0.0, evolution_dr=1, planar=0,
+                ext_at_told=0, rec_time_step=rec)
+    prob["ext"] = backslip_tables(ks_row_sums, kn_row_sums, rate)
+    return prob
+
+
 def triangle_lengths(P1, P2, P3):
     """FaultLengthStrike of a triangle as ReadSegmentInput defines it
     (Functions_BuildInputFile.jl:320): the largest |component| of the three

@@ -2,8 +2,9 @@
 
 * Build (device assembly + full-pivot ACA): dense blocks equal the oracle's
   stiffness to 1e-10 of max |K|; every low-rank block satisfies the
-  approximation bound max |U V - K_b| <= rtol max |K_b| (+ the 1e-10 entry
-  tolerance) against the oracle's entries.
+  approximation bound max |U V - K_b| <= rtol max |K_S,b| (+ the 1e-10
+  entry tolerance) against the oracle's entries, for the shear and normal
+  parts of the block.
 * Product: the device H product equals the oracle's HmatSolver on the same
   stored blocks to 1e-12 (FP64 storage), in both the loading form of main_H
   and the plain form, and is within the approximation bound of the dense
@@ -53,6 +54,7 @@ def _check_blocks(blocks, KS, KN, rs, ad):
         B = blocks[w]
         for q, r in enumerate(rs):
             Kb = K[r[2] - 1:r[3], r[0] - 1:r[1]]
+            sb = np.abs(KS[r[2] - 1:r[3], r[0] - 1:r[1]]).max()
             if B["kind"][q] == 0:
                 assert np.abs(B["a"][q] - Kb).max() <= 1e-10 * scale
             else:
@@ -60,7 +62,7 @@ def _check_blocks(blocks, KS, KN, rs, ad):
                 nlr += 1
                 approx = B["a"][q] @ B["b"][q] if B["rank"][q] > 0 else np.zeros_like(Kb)
                 err = np.abs(approx - Kb).max()
-                assert err <= RTOL * np.abs(Kb).max() * (1 + 1e-6) + 1e-10 * scale, (q, w, err)
+                assert err <= RTOL * sb * (1 + 1e-6) + 1e-10 * scale, (q, w, err)
                 assert B["rank"][q] < min(Kb.shape)
     return nlr
 
@@ -90,7 +92,7 @@ def test_rect_product_matches_oracle(net, orc):
     hks, hdg = H.kself()
     np.testing.assert_array_equal(hks, ks)
     # approximation bound against the dense product
-    bound = RTOL * max(np.abs(net["KS"]).max(), np.abs(net["KN"]).max()) * np.abs(x).sum() + 1e-9
+    bound = RTOL * np.abs(net["KS"]).max() * np.abs(x).sum() * 1.000001 + 1e-9
     assert np.abs(yS - net["KS"] @ x).max() <= bound
     assert np.abs(yN - net["KN"] @ x).max() <= bound
 
