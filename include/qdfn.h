@@ -152,10 +152,10 @@ typedef struct qdfn_hmat qdfn_hmat_t;
  * shear/normal stiffness of segment rows (already in sorted order, n x >=8,
  * column-major, leading dimension ld); admissible blocks are compressed to
  * U V (m x r, r x k) by adaptive cross approximation with full pivoting,
- * stopping when max |residual| <= max(aca_atol, aca_rtol * max |K_S,b|), K_S,b
- * the block's shear entries, for the shear and the normal part alike
- * (aca_rtol = 0, aca_atol = Tolerance is the reference's pqrfact(block,
- * atol = Tolerance)).  The whole matrix is
+ * stopping when ||residual||_F <= aca_rtol ||K_S,b||_F or max |residual| <=
+ * aca_atol, K_S,b the block's shear entries, for the shear and the normal part
+ * alike (aca_rtol = 0, aca_atol = Tolerance mirrors the reference's
+ * pqrfact(block, atol = Tolerance)).  The whole matrix is
  * assembled in device memory first, so n is bounded by HBM (about 130k). */
 qdfn_hmat_t* qdfn_hmat_build_rect(const double* seg, int64_t n, int64_t ld, double mu, double nu, int64_t nblk,
                                   const int64_t* range_SR, const int64_t* admissible, double aca_rtol,

@@ -10,11 +10,12 @@
 // Build on the device: the full K_S and K_N are assembled in HBM (FP64), dense
 // blocks are read out of them and admissible blocks are compressed by an
 // adaptive cross approximation with full pivoting, one workgroup per block and
-// matrix, which stops when max |residual| <= max(atol, rtol * max |K_S,b|),
-// K_S,b the block's shear entries (for the normal part too: both are stresses
-// per unit slip, and the normal entries of a near-planar fault are far smaller,
-// so a tolerance relative to their own size would spend rank on noise).  atol
-// alone gives the reference's pqrfact(block, atol = Tolerance).
+// matrix.  The residual is held explicitly, so the stopping test is exact:
+// ||residual||_F <= rtol ||K_S,b||_F or max |residual| <= atol, K_S,b the
+// block's shear entries (for the normal part too: both are stresses per unit
+// slip, and the normal entries of a near-planar fault are far smaller, so a
+// tolerance relative to their own size would spend rank on noise).  atol alone
+// gives the reference's pqrfact(block, atol = Tolerance).
 //
 // Product layout (HBM): the rows are cut into row groups of at most 64 rows
 // whose boundaries include every block's receiver boundaries.  Each row group
@@ -34,6 +35,7 @@
 // Panels and V are stored in FP64 or FP32 (mixed precision).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -47,7 +49,8 @@ namespace qdfn {
 namespace {
 
 constexpr int kRowsPerGroup = 64;
-constexpr int kPhase2Threads = 256;
+constexpr int kPhase2Threads = 1024;
+constexpr int kSlots = 64 / (kPhase2Threads / 64);  // row slots per lane (64 rows over 16 waves)
 constexpr int kChunk = 1024;   // right-hand-side columns staged in LDS per pass (double-buffered)
 
 int64_t round4(int64_t v) { return (v + 3) / 4 * 4; }
@@ -140,55 +143,77 @@ __global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict
     const int32_t cw = min(K4 - tl.c0, (int32_t)kChunk);
     int lpr = 1;
     while (lpr < 64 && 4 * lpr < cw) lpr <<= 1;
-    const int rpw = 64 / lpr, step_rows = 4 * rpw, sub = lane & (lpr - 1), rl = lane / lpr;
+    constexpr int kWaves = kPhase2Threads / 64;
+    const int rpw = 64 / lpr, step_rows = kWaves * rpw, sub = lane & (lpr - 1), rl = lane / lpr;
     const T* base = pool + R.data_off;
     constexpr int kPer = kChunk / kPhase2Threads;
+    // right-hand side: index loads run one chunk ahead of the value loads, so a
+    // chunk's gather never waits on its own index loads
+    int32_t iv[kPer];
     double gv[kPer];
-    auto gather = [&](int32_t c0) {
+    // loads are branch-free (indices clamped, values masked) so that they issue
+    // back to back instead of one at a time
+    auto load_idx = [&](int32_t c0) {
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int32_t c = c0 + (int32_t)threadIdx.x + q * kPhase2Threads;
-            double val = 0.0;
-            if (c < K) {
-                if (GATHER) {
-                    const int32_t j = idx[R.zoff + c];
-                    val = j < n ? x[j] : t[j - n];
-                } else {
-                    val = x[R.zoff + c];
-                }
-            }
-            gv[q] = val;
+            iv[q] = GATHER ? idx[R.zoff + min(c, K - 1)] : 0;
         }
     };
-    gather(tl.c0);
+    auto load_vals = [&](int32_t c0) {
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int32_t c = c0 + (int32_t)threadIdx.x + q * kPhase2Threads;
+            double val;
+            if (GATHER) {
+                const int32_t j = iv[q];
+                val = j < n ? x[j] : t[j - n];
+            } else {
+                val = x[R.zoff + min(c, K - 1)];
+            }
+            gv[q] = c < K ? val : 0.0;
+        }
+    };
+    if (K > 0) {
+        if (GATHER) load_idx(tl.c0);
+        load_vals(tl.c0);
+    } else {
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) gv[q] = 0.0;
+    }
 #pragma unroll
     for (int q = 0; q < kPer; ++q) zbuf[0][threadIdx.x + q * kPhase2Threads] = gv[q];
+    if (GATHER && tl.c0 + kChunk < K4) load_idx(tl.c0 + kChunk);
     __syncthreads();
-    double acc[16];
+    double acc[kSlots];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+    for (int r = 0; r < kSlots; ++r) acc[r] = 0.0;
     int buf = 0;
     for (int32_t c0 = tl.c0; c0 < K4; c0 += kChunk) {
         const bool more = c0 + kChunk < K4;
-        if (more) gather(c0 + kChunk);
+        if (more) {
+            load_vals(c0 + kChunk);
+            if (GATHER && c0 + 2 * kChunk < K4) load_idx(c0 + 2 * kChunk);
+        }
         const double* z = zbuf[buf];
         const int32_t lim = min((int32_t)kChunk, K4 - c0);
         for (int32_t cb = 0; cb < lim; cb += 4 * lpr) {
-            const int32_t c = cb + sub * 4;
-            if (c < lim) {
-                const double z0 = z[c], z1 = z[c + 1], z2 = z[c + 2], z3 = z[c + 3];
+            // lanes past the chunk read its first columns again and add zero
+            const int32_t c = cb + sub * 4 < lim ? cb + sub * 4 : 0;
+            const double zm = cb + sub * 4 < lim ? 1.0 : 0.0;
+            const double z0 = zm * z[c], z1 = zm * z[c + 1], z2 = zm * z[c + 2], z3 = zm * z[c + 3];
+            double w[kSlots][4];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int row = r * step_rows + wave * rpw + rl;
-                    if (row < R.m) {
-                        double w[4];
-                        load4<T>(base + (int64_t)row * R.ld + c0 + c, w);
-                        acc[r] = fma(w[0], z0, acc[r]);
-                        acc[r] = fma(w[1], z1, acc[r]);
-                        acc[r] = fma(w[2], z2, acc[r]);
-                        acc[r] = fma(w[3], z3, acc[r]);
-                    }
-                }
+            for (int r = 0; r < kSlots; ++r) {
+                const int row = min(r * step_rows + wave * rpw + rl, R.m - 1);
+                load4<T>(base + (int64_t)row * R.ld + c0 + c, w[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < kSlots; ++r) {
+                acc[r] = fma(w[r][0], z0, acc[r]);
+                acc[r] = fma(w[r][1], z1, acc[r]);
+                acc[r] = fma(w[r][2], z2, acc[r]);
+                acc[r] = fma(w[r][3], z3, acc[r]);
             }
         }
         if (more) {
@@ -199,7 +224,7 @@ __global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict
         buf ^= 1;
     }
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
+    for (int r = 0; r < kSlots; ++r) {
         double v = acc[r];
         for (int o = lpr >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
         const int row = r * step_rows + wave * rpw + rl;
@@ -239,7 +264,7 @@ __global__ void k_mark_skipped(CouplingGate g) {
 // Adaptive cross approximation with full pivoting of one block (one workgroup).
 struct AcaJob {
     const double* src;  // block in the full matrix (column-major, leading dimension ld)
-    const double* scale;  // max |K_S,b| of the block
+    const double* scale;  // ||K_S,b||_F of the block
     int64_t ld;
     double* R;          // m x k residual workspace
     double* U;          // m x rmax
@@ -258,12 +283,16 @@ __device__ inline void better(double& v, int32_t& id, double& sv, double v2, int
     }
 }
 
-__device__ void aca_reduce(double& v, int32_t& id, double& sv, double* sh_v, int32_t* sh_i, double* sh_s) {
+// workgroup reduction: argmax of |R| (smallest index on ties, with the signed
+// value) and the sum of squares, in a fixed order
+__device__ void aca_reduce(double& v, int32_t& id, double& sv, double& ss, double* sh_v, int32_t* sh_i,
+                           double* sh_s, double* sh_q) {
     for (int o = 32; o > 0; o >>= 1) {
         const double v2 = __shfl_xor(v, o, 64);
         const int32_t i2 = __shfl_xor(id, o, 64);
         const double s2 = __shfl_xor(sv, o, 64);
         better(v, id, sv, v2, i2, s2);
+        ss += __shfl_xor(ss, o, 64);
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __syncthreads();
@@ -271,60 +300,69 @@ __device__ void aca_reduce(double& v, int32_t& id, double& sv, double* sh_v, int
         sh_v[wave] = v;
         sh_i[wave] = id;
         sh_s[wave] = sv;
+        sh_q[wave] = ss;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kAcaThreads / 64; ++w) better(sh_v[0], sh_i[0], sh_s[0], sh_v[w], sh_i[w], sh_s[w]);
+        for (int w = 1; w < kAcaThreads / 64; ++w) {
+            better(sh_v[0], sh_i[0], sh_s[0], sh_v[w], sh_i[w], sh_s[w]);
+            sh_q[0] += sh_q[w];
+        }
     }
     __syncthreads();
     v = sh_v[0];
     id = sh_i[0];
     sv = sh_s[0];
+    ss = sh_q[0];
 }
 
-// max |entry| of blocks (one workgroup per block): the scale of the ACA threshold
+// Frobenius norm of blocks (one workgroup per block, fixed-order sums): the
+// scale of the ACA threshold
 struct MaxJob {
     const double* src;
     int64_t ld;
     int32_t m, k;
 };
 
-__global__ __launch_bounds__(kAcaThreads) void k_blockmax(const MaxJob* __restrict__ jobs, double* __restrict__ out) {
+__global__ __launch_bounds__(kAcaThreads) void k_blocknorm(const MaxJob* __restrict__ jobs, double* __restrict__ out) {
     __shared__ double sh[kAcaThreads / 64];
     const MaxJob J = jobs[blockIdx.x];
     double v = 0.0;
     const int64_t mk = (int64_t)J.m * J.k;
     for (int64_t e = threadIdx.x; e < mk; e += kAcaThreads) {
         const int64_t i = e % J.m, j = e / J.m;
-        v = fmax(v, fabs(J.src[i + j * J.ld]));
+        const double a = J.src[i + j * J.ld];
+        v += a * a;
     }
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kAcaThreads / 64; ++w) v = fmax(v, sh[w]);
-        out[blockIdx.x] = v;
+        for (int w = 1; w < kAcaThreads / 64; ++w) v += sh[w];
+        out[blockIdx.x] = sqrt(v);
     }
 }
 
 __global__ __launch_bounds__(kAcaThreads) void k_aca(const AcaJob* __restrict__ jobs, int32_t* __restrict__ rank,
                                                      int32_t* __restrict__ converged) {
-    __shared__ double sh_v[kAcaThreads / 64], sh_s[kAcaThreads / 64];
+    __shared__ double sh_v[kAcaThreads / 64], sh_s[kAcaThreads / 64], sh_q[kAcaThreads / 64];
     __shared__ int32_t sh_i[kAcaThreads / 64];
     const AcaJob J = jobs[blockIdx.x];
     const int32_t m = J.m, k = J.k, mk = m * k;
-    double amax = -1.0, piv = 0.0;
+    double amax = -1.0, piv = 0.0, ss = 0.0;
     int32_t aid = 0x7fffffff;
     for (int32_t e = threadIdx.x; e < mk; e += kAcaThreads) {
         const int32_t i = e % m, j = e / m;
         const double v = J.src[i + (int64_t)j * J.ld];
         J.R[e] = v;
         better(amax, aid, piv, fabs(v), e, v);
+        ss += v * v;
     }
-    aca_reduce(amax, aid, piv, sh_v, sh_i, sh_s);
-    const double thr = fmax(J.atol, J.rtol * *J.scale);
+    aca_reduce(amax, aid, piv, ss, sh_v, sh_i, sh_s, sh_q);
+    const double fthr = J.rtol * *J.scale;  // Frobenius threshold
+    auto done = [&]() { return amax <= J.atol || sqrt(ss) <= fthr || amax == 0.0; };
     int32_t r = 0;
-    while (amax > thr && r < J.rmax) {
+    while (!done() && r < J.rmax) {
         const int32_t is = aid % m, js = aid / m;
         double* u = J.U + (int64_t)r * m;
         double* v = J.V + (int64_t)r * k;
@@ -334,18 +372,20 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(const AcaJob* __restrict__ 
         amax = -1.0;
         aid = 0x7fffffff;
         piv = 0.0;
+        ss = 0.0;
         for (int32_t e = threadIdx.x; e < mk; e += kAcaThreads) {
             const int32_t i = e % m, j = e / m;
             const double nv = J.R[e] - u[i] * v[j];
             J.R[e] = nv;
             better(amax, aid, piv, fabs(nv), e, nv);
+            ss += nv * nv;
         }
-        aca_reduce(amax, aid, piv, sh_v, sh_i, sh_s);
+        aca_reduce(amax, aid, piv, ss, sh_v, sh_i, sh_s, sh_q);
         ++r;
     }
     if (threadIdx.x == 0) {
         rank[blockIdx.x] = r;
-        converged[blockIdx.x] = amax <= thr ? 1 : 0;
+        converged[blockIdx.x] = done() ? 1 : 0;
     }
 }
 
@@ -435,12 +475,17 @@ static int hmat_layout(qdfn_hmat* h) {
                 for (int64_t q = 0; q < r; ++q, ++row)
                     for (int64_t c = 0; c < k; ++c) pool[off + row * ld + c] = V[q + c * r];
             }
-            for (int64_t r0 = 0; r0 < rows; r0 += kRowsPerGroup) {
+            // narrow panels: several rows per wavefront load, so taller row blocks
+            // (the kernel covers 64 * (64 / LPR) rows, LPR from the first chunk)
+            int lpr = 1;
+            while (lpr < 64 && 4 * lpr < std::min<int64_t>(ld, kChunk)) lpr <<= 1;
+            const int64_t rb_rows = (int64_t)kRowsPerGroup * (64 / lpr);
+            for (int64_t r0 = 0; r0 < rows; r0 += rb_rows) {
                 HRowBlock R;
                 R.data_off = off + r0 * ld;
                 R.zoff = s0;
                 R.out = tbase + r0;
-                R.m = (int32_t)std::min<int64_t>(kRowsPerGroup, rows - r0);
+                R.m = (int32_t)std::min<int64_t>(rb_rows, rows - r0);
                 R.K = (int32_t)k;
                 R.ld = (int32_t)ld;
                 R.kind = 0;
@@ -515,6 +560,29 @@ static int hmat_layout(qdfn_hmat* h) {
     h->n_rb2 = (int64_t)rbs.size() - h->n_rb1;
     h->n_tiles2 = (int64_t)tiles.size() - h->n_tiles1;
     h->n_multi2 = (int64_t)multi.size() - h->n_multi1;
+    // largest tiles first within each phase (the dispatcher hands out workgroups
+    // in order, so the tail of a launch is made of small tiles)
+    auto tile_bytes = [&](const HTile& T) { return (int64_t)rbs[T.rb].m * (T.c1 - T.c0); };
+    auto by_size = [&](const HTile& a, const HTile& b) { return tile_bytes(a) > tile_bytes(b); };
+    std::stable_sort(tiles.begin(), tiles.begin() + h->n_tiles1, by_size);
+    std::stable_sort(tiles.begin() + h->n_tiles1, tiles.end(), by_size);
+    if (getenv("QDFN_HMAT_VERBOSE")) {
+        for (int ph = 0; ph < 2; ++ph) {
+            const size_t b0 = ph ? h->n_tiles1 : 0, b1 = ph ? tiles.size() : h->n_tiles1;
+            if (b1 == b0) continue;
+            std::vector<int64_t> sz;
+            int64_t tot = 0;
+            for (size_t q = b0; q < b1; ++q) {
+                sz.push_back(tile_bytes(tiles[q]));
+                tot += sz.back();
+            }
+            std::sort(sz.begin(), sz.end());
+            fprintf(stderr, "[hmat] phase %d: %zu tiles, %lld entries, per tile min %lld median %lld p90 %lld max %lld, "
+                    "row blocks %lld (split %lld)\n", ph + 1, sz.size(), (long long)tot, (long long)sz.front(),
+                    (long long)sz[sz.size() / 2], (long long)sz[sz.size() * 9 / 10], (long long)sz.back(),
+                    (long long)(ph ? h->n_rb2 : h->n_rb1), (long long)(ph ? h->n_multi2 : h->n_multi1));
+        }
+    }
     h->stored_entries = 0;
     for (const HBlock& b : B)
         for (int w = 0; w < 2; ++w) h->stored_entries += (int64_t)(b.a[w].size() + b.b[w].size());
@@ -661,7 +729,7 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
                 lr.push_back({q, w});
             }
         }
-    // threshold scale: max |K_S,b| of every block (device reduction)
+    // threshold scale: ||K_S,b||_F of every block (device reduction)
     DevBuf dmax, dmj;
     {
         std::vector<MaxJob> mj(nblk);
@@ -675,7 +743,7 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
         QDFN_HIP(dmax.alloc(sizeof(double) * nblk));
         QDFN_HIP(dmj.alloc(sizeof(MaxJob) * nblk));
         QDFN_HIP(hipMemcpy(dmj.p, mj.data(), sizeof(MaxJob) * nblk, hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_blockmax, dim3((unsigned)nblk), dim3(kAcaThreads), 0, h->stream, dmj.as<MaxJob>(),
+        hipLaunchKernelGGL(k_blocknorm, dim3((unsigned)nblk), dim3(kAcaThreads), 0, h->stream, dmj.as<MaxJob>(),
                            dmax.as<double>());
         QDFN_HIP(hipGetLastError());
         QDFN_HIP(hipStreamSynchronize(h->stream));

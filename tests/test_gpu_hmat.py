@@ -2,9 +2,9 @@
 
 * Build (device assembly + full-pivot ACA): dense blocks equal the oracle's
   stiffness to 1e-10 of max |K|; every low-rank block satisfies the
-  approximation bound max |U V - K_b| <= rtol max |K_S,b| (+ the 1e-10
-  entry tolerance) against the oracle's entries, for the shear and normal
-  parts of the block.
+  approximation bound ||U V - K_b||_F <= rtol ||K_S,b||_F (+ the 1e-10 entry
+  tolerance) against the oracle's entries, for the shear and normal parts of
+  the block.
 * Product: the device H product equals the oracle's HmatSolver on the same
   stored blocks to 1e-12 (FP64 storage), in both the loading form of main_H
   and the plain form, and is within the approximation bound of the dense
@@ -54,15 +54,15 @@ def _check_blocks(blocks, KS, KN, rs, ad):
         B = blocks[w]
         for q, r in enumerate(rs):
             Kb = K[r[2] - 1:r[3], r[0] - 1:r[1]]
-            sb = np.abs(KS[r[2] - 1:r[3], r[0] - 1:r[1]]).max()
+            sb = np.linalg.norm(KS[r[2] - 1:r[3], r[0] - 1:r[1]])
             if B["kind"][q] == 0:
                 assert np.abs(B["a"][q] - Kb).max() <= 1e-10 * scale
             else:
                 assert ad[q] == 1
                 nlr += 1
                 approx = B["a"][q] @ B["b"][q] if B["rank"][q] > 0 else np.zeros_like(Kb)
-                err = np.abs(approx - Kb).max()
-                assert err <= RTOL * sb * (1 + 1e-6) + 1e-10 * scale, (q, w, err)
+                err = np.linalg.norm(approx - Kb)
+                assert err <= RTOL * sb * (1 + 1e-6) + 1e-10 * scale * np.sqrt(Kb.size), (q, w, err)
                 assert B["rank"][q] < min(Kb.shape)
     return nlr
 
@@ -92,7 +92,8 @@ def test_rect_product_matches_oracle(net, orc):
     hks, hdg = H.kself()
     np.testing.assert_array_equal(hks, ks)
     # approximation bound against the dense product
-    bound = RTOL * np.abs(net["KS"]).max() * np.abs(x).sum() * 1.000001 + 1e-9
+    # |(H - K) x|_i <= ||H - K||_F ||x||_2 with ||H - K||_F^2 <= sum_b (rtol ||K_S,b||_F)^2 <= rtol^2 ||K_S||_F^2
+    bound = RTOL * np.linalg.norm(net["KS"]) * np.linalg.norm(x) * 1.000001 + 1e-9
     assert np.abs(yS - net["KS"] @ x).max() <= bound
     assert np.abs(yN - net["KN"] @ x).max() <= bound
 
