@@ -38,6 +38,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include "assembly.h"
@@ -344,7 +346,7 @@ __global__ __launch_bounds__(kAcaThreads) void k_blocknorm(const MaxJob* __restr
 }
 
 __global__ __launch_bounds__(kAcaThreads) void k_aca(const AcaJob* __restrict__ jobs, int32_t* __restrict__ rank,
-                                                     int32_t* __restrict__ converged) {
+                                                     int32_t* __restrict__ converged, double* __restrict__ resid) {
     __shared__ double sh_v[kAcaThreads / 64], sh_s[kAcaThreads / 64], sh_q[kAcaThreads / 64];
     __shared__ int32_t sh_i[kAcaThreads / 64];
     const AcaJob J = jobs[blockIdx.x];
@@ -386,6 +388,7 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(const AcaJob* __restrict__ 
     if (threadIdx.x == 0) {
         rank[blockIdx.x] = r;
         converged[blockIdx.x] = done() ? 1 : 0;
+        resid[blockIdx.x] = sqrt(ss);
     }
 }
 
@@ -691,6 +694,147 @@ static int check_blocks(int64_t n, int64_t nblk, const int64_t* range, const int
     return QDFN_OK;
 }
 
+// ---- recompression of ACA factors (host): U V = Q_u (R_u R_v^T) Q_v^T, SVD of
+// the r x r core, truncation to the remaining Frobenius budget.
+
+// thin Householder QR of A (m x r, column-major, m >= r): Q (m x r), R (r x r)
+static void qr_thin(int64_t m, int64_t r, const double* A, std::vector<double>& Q, std::vector<double>& R) {
+    std::vector<double> W(A, A + m * r), tau(r, 0.0), diag(r, 0.0);
+    for (int64_t j = 0; j < r; ++j) {
+        double* a = W.data() + j * m;
+        double nrm = 0.0;
+        for (int64_t i = j; i < m; ++i) nrm += a[i] * a[i];
+        nrm = std::sqrt(nrm);
+        if (nrm == 0.0) {
+            diag[j] = 0.0;
+            continue;
+        }
+        const double alpha = a[j] >= 0 ? -nrm : nrm;
+        a[j] -= alpha;  // v = x - alpha e_j, stored in place
+        double vn = 0.0;
+        for (int64_t i = j; i < m; ++i) vn += a[i] * a[i];
+        tau[j] = vn > 0 ? 2.0 / vn : 0.0;
+        diag[j] = alpha;
+        for (int64_t c = j + 1; c < r; ++c) {
+            double* b = W.data() + c * m;
+            double sdot = 0.0;
+            for (int64_t i = j; i < m; ++i) sdot += a[i] * b[i];
+            sdot *= tau[j];
+            for (int64_t i = j; i < m; ++i) b[i] -= sdot * a[i];
+        }
+    }
+    R.assign(r * r, 0.0);
+    for (int64_t j = 0; j < r; ++j) {
+        for (int64_t i = 0; i < j; ++i) R[i + j * r] = W[i + j * m];
+        R[j + j * r] = diag[j];
+    }
+    Q.assign(m * r, 0.0);
+    for (int64_t j = 0; j < r; ++j) Q[j + j * m] = 1.0;
+    for (int64_t j = r - 1; j >= 0; --j) {
+        const double* a = W.data() + j * m;
+        if (tau[j] == 0.0) continue;
+        for (int64_t c = 0; c < r; ++c) {
+            double* q = Q.data() + c * m;
+            double sdot = 0.0;
+            for (int64_t i = j; i < m; ++i) sdot += a[i] * q[i];
+            sdot *= tau[j];
+            for (int64_t i = j; i < m; ++i) q[i] -= sdot * a[i];
+        }
+    }
+}
+
+// one-sided Jacobi SVD of C (r x r): C = W diag(sig) Z^T
+static void svd_jacobi(int64_t r, std::vector<double> C, std::vector<double>& W, std::vector<double>& sig,
+                       std::vector<double>& Z) {
+    Z.assign(r * r, 0.0);
+    for (int64_t j = 0; j < r; ++j) Z[j + j * r] = 1.0;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        bool rotated = false;
+        for (int64_t p = 0; p < r; ++p)
+            for (int64_t q = p + 1; q < r; ++q) {
+                double al = 0, be = 0, ga = 0;
+                for (int64_t i = 0; i < r; ++i) {
+                    const double x = C[i + p * r], y = C[i + q * r];
+                    al += x * x;
+                    be += y * y;
+                    ga += x * y;
+                }
+                if (std::fabs(ga) <= 1e-15 * std::sqrt(al * be) || ga == 0.0) continue;
+                rotated = true;
+                const double zeta = (be - al) / (2.0 * ga);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), sn = c * t;
+                for (int64_t i = 0; i < r; ++i) {
+                    const double x = C[i + p * r], y = C[i + q * r];
+                    C[i + p * r] = c * x - sn * y;
+                    C[i + q * r] = sn * x + c * y;
+                    const double zx = Z[i + p * r], zy = Z[i + q * r];
+                    Z[i + p * r] = c * zx - sn * zy;
+                    Z[i + q * r] = sn * zx + c * zy;
+                }
+            }
+        if (!rotated) break;
+    }
+    sig.assign(r, 0.0);
+    W.assign(r * r, 0.0);
+    for (int64_t j = 0; j < r; ++j) {
+        double nn = 0.0;
+        for (int64_t i = 0; i < r; ++i) nn += C[i + j * r] * C[i + j * r];
+        sig[j] = std::sqrt(nn);
+        if (sig[j] > 0)
+            for (int64_t i = 0; i < r; ++i) W[i + j * r] = C[i + j * r] / sig[j];
+    }
+}
+
+// U (m x r), V (r x k) -> the shortest factors whose product differs from U V
+// by at most `budget` in the Frobenius norm.
+static void recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V,
+                       double budget) {
+    if (r < 2 || budget <= 0) return;
+    std::vector<double> Vt(k * r);
+    for (int64_t i = 0; i < r; ++i)
+        for (int64_t c = 0; c < k; ++c) Vt[c + i * k] = V[i + c * r];
+    std::vector<double> Qu, Ru, Qv, Rv;
+    qr_thin(m, r, U.data(), Qu, Ru);
+    qr_thin(k, r, Vt.data(), Qv, Rv);
+    std::vector<double> C(r * r, 0.0);  // Ru Rv^T
+    for (int64_t j = 0; j < r; ++j)
+        for (int64_t l = 0; l < r; ++l) {
+            const double rv = Rv[j + l * r];
+            if (rv == 0.0) continue;
+            for (int64_t i = 0; i < r; ++i) C[i + j * r] += Ru[i + l * r] * rv;
+        }
+    std::vector<double> W, sig, Z;
+    svd_jacobi(r, C, W, sig, Z);
+    std::vector<int64_t> ord(r);
+    for (int64_t i = 0; i < r; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) { return sig[a] > sig[b]; });
+    int64_t keep = r;
+    double tail = 0.0;
+    while (keep > 0) {
+        const double s2 = sig[ord[keep - 1]] * sig[ord[keep - 1]];
+        if (std::sqrt(tail + s2) > budget) break;
+        tail += s2;
+        --keep;
+    }
+    if (keep >= r) return;
+    std::vector<double> U2(m * keep, 0.0), V2(keep * k, 0.0);
+    for (int64_t a = 0; a < keep; ++a) {
+        const int64_t j = ord[a];
+        for (int64_t l = 0; l < r; ++l) {
+            const double wl = W[l + j * r] * sig[j];
+            if (wl != 0.0)
+                for (int64_t i = 0; i < m; ++i) U2[i + a * m] += Qu[i + l * m] * wl;
+            const double zl = Z[l + j * r];
+            if (zl != 0.0)
+                for (int64_t c = 0; c < k; ++c) V2[a + c * keep] += Qv[c + l * k] * zl;
+        }
+    }
+    U.swap(U2);
+    V.swap(V2);
+    r = keep;
+}
+
 // Compress / read out every block from full device matrices KS, KN (n x n, ld n).
 static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, int64_t nblk, const int64_t* range,
                             const int64_t* adm, double rtol, double atol) {
@@ -748,6 +892,8 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
         QDFN_HIP(hipGetLastError());
         QDFN_HIP(hipStreamSynchronize(h->stream));
     }
+    std::vector<double> bnorm(nblk);
+    QDFN_HIP(hipMemcpy(bnorm.data(), dmax.p, sizeof(double) * nblk, hipMemcpyDeviceToHost));
     // admissible blocks in batches bounded by a workspace budget
     size_t free_b = 0, total_b = 0;
     QDFN_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -768,7 +914,7 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
             used += need;
             ++end;
         }
-        DevBuf ws, djobs, drank, dconv;
+        DevBuf ws, djobs, drank, dconv, dres;
         QDFN_HIP(ws.alloc(sizeof(double) * used));
         for (size_t q = pos; q < end; ++q) {
             const HBlock& b = h->blocks[lr[q].first];
@@ -786,7 +932,8 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
             J.k = (int32_t)k;
             J.rmax = (int32_t)rmax;
             J.pad = 0;
-            J.rtol = rtol;
+            // half of the budget for the cross approximation, the rest for recompression
+            J.rtol = 0.5 * rtol;
             J.atol = atol;
             jobs.push_back(J);
         }
@@ -794,14 +941,18 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
         QDFN_HIP(djobs.alloc(sizeof(AcaJob) * nj));
         QDFN_HIP(drank.alloc(sizeof(int32_t) * nj));
         QDFN_HIP(dconv.alloc(sizeof(int32_t) * nj));
+        QDFN_HIP(dres.alloc(sizeof(double) * nj));
         QDFN_HIP(hipMemcpy(djobs.p, jobs.data(), sizeof(AcaJob) * nj, hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_aca, dim3((unsigned)nj), dim3(kAcaThreads), 0, h->stream, djobs.as<AcaJob>(),
-                           drank.as<int32_t>(), dconv.as<int32_t>());
+                           drank.as<int32_t>(), dconv.as<int32_t>(), dres.as<double>());
         QDFN_HIP(hipGetLastError());
         QDFN_HIP(hipStreamSynchronize(h->stream));
         std::vector<int32_t> rk(nj), cv(nj);
+        std::vector<double> res(nj);
         QDFN_HIP(hipMemcpy(rk.data(), drank.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost));
         QDFN_HIP(hipMemcpy(cv.data(), dconv.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost));
+        QDFN_HIP(hipMemcpy(res.data(), dres.p, sizeof(double) * nj, hipMemcpyDeviceToHost));
+        std::vector<size_t> to_recompress;
         for (size_t q = pos; q < end; ++q) {
             HBlock& b = h->blocks[lr[q].first];
             const int w = lr[q].second;
@@ -822,8 +973,26 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
                 QDFN_HIP(hipMemcpy(vr.data(), J.V, sizeof(double) * r * k, hipMemcpyDeviceToHost));
                 for (int64_t i = 0; i < r; ++i)
                     for (int64_t c = 0; c < k; ++c) b.b[w][i + c * r] = vr[i * k + c];
+                if (rtol > 0 && r > 1) to_recompress.push_back(q);
             }
         }
+        // recompression on the host threads (blocks are independent)
+        std::atomic<size_t> next(0);
+        auto worker = [&]() {
+            for (size_t i = next++; i < to_recompress.size(); i = next++) {
+                const size_t q = to_recompress[i];
+                HBlock& b = h->blocks[lr[q].first];
+                const int w = lr[q].second;
+                const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+                const double budget = rtol * bnorm[lr[q].first] - res[q - pos];
+                recompress(m, k, b.rank[w], b.a[w], b.b[w], budget);
+            }
+        };
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned i = 1; i < nt; ++i) pool.emplace_back(worker);
+        worker();
+        for (auto& th : pool) th.join();
         pos = end;
     }
     return QDFN_OK;
