@@ -1,15 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark: RHS evaluations per second of the device-resident earthquake-cycle
-loop on the 8-fault, 8,192-element dense network, mixed precision (FP32
-stiffness storage, FP64 accumulation / state / solves) against the FP64 plain
-form, with the event-time error between the two.
+loop, mixed precision against the FP64 plain form, at a stated event-time error.
 
-A step is one accepted time step of main() (scripts/Functions_RSFDFN3DMain_D.jl:
-146-322): one coupling product over all elements plus one or more local-solve
-passes.  `value` = RHS evaluations / s of the mixed-precision run over the
-timed steps, inputs resident in HBM.
+Headline workload: the rough strike-slip fault of 20,000 triangles (100 x 100
+cells of 50 m, sinusoidal roughness), H-matrix operator (cluster leaves of 64,
+ACA tolerance 1e-6) stored in FP32 with FP64 accumulation and FP64 state,
+against the dense FP64 operator ("plain" form, main() of
+scripts/Functions_RSFDFN3DMain_D.jl).  Both loops are main_H/main semantics on
+the device; each run is loaded by back-slip tables built from its own operator
+(UniformBackslip.jl).  Secondary workload: the 8-fault, 8,192-element dense
+network, FP32 storage against FP64.
 
-Prints one JSON line on stdout; progress goes to stderr.
+A step is one accepted time step (one coupling product plus one or more
+local-solve passes).  `value` = RHS evaluations / s of the mixed-precision H run
+over the timed steps (replayed from a captured HIP graph), inputs resident in
+HBM.  Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
@@ -25,15 +30,15 @@ import numpy as np  # noqa: E402
 
 METRIC = "RHS evals/sec + sim-years/hour, mixed vs FP64 on GPU at stated event-time error"
 HBM_PEAK_GBS = 8000.0
-PROFILE = os.path.join(ROOT, "profiles", "coupling_fault_network_8192.md")
+PROFILE = os.path.join(ROOT, "profiles", "hmat_phase2_rough_fault_20000.md")
 
 
 def measured_traffic():
-    """HBM bytes per FP32 coupling launch from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated); None if absent."""
+    """HBM bytes per launch of the phase-2 H kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated); None if absent."""
     try:
         for line in open(PROFILE):
-            if line.startswith("traffic_bytes_per_launch_fp32:"):
+            if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -44,25 +49,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_problem(qd, cfg):
-    from qdfn import configs
-    seg = cfg["seg"]
-    ops = {}
-    for name, prec in (("fp64", qd.PREC_FP64), ("fp32", qd.PREC_FP32)):
-        t0 = time.perf_counter()
-        ops[name] = qd.DenseOperator.from_rect_segments(seg, cfg["mu"], cfg["nu"], prec)
-        log(f"[bench] assembled + built {name} operator n={seg.shape[0]} in {time.perf_counter() - t0:.2f}s")
-    ks, _ = ops["fp64"].kself()
-    ksum, knsum = ops["fp64"].matvec(np.ones(seg.shape[0]), loading_form=False)
-    prob = configs.problem_from_segments(cfg, ks, ksum, knsum)
-    return ops, prob
-
-
 def run_timed(qd, op, prob, steps, warmup, graph_attempts, kernel_steps):
     """Warm up, then the timed region: `steps` accepted steps replayed from the
     captured attempt graph (wall clock around a synchronous call), followed by
-    `kernel_steps` steps launched directly with HIP events around every
-    coupling kernel on the run's stream (per-kernel durations)."""
+    `kernel_steps` steps launched directly with HIP events around the coupling
+    kernel (the phase-2 kernel for an H operator)."""
     sim = qd.Simulation(op, prob, record_step=0, record_capacity=0,
                         trace_capacity=steps + warmup + kernel_steps)
     if warmup:
@@ -72,9 +63,8 @@ def run_timed(qd, op, prob, steps, warmup, graph_attempts, kernel_steps):
     sim.run(steps, graph_attempts=graph_attempts)
     el = time.perf_counter() - t0
     s1 = sim.stats()
-    t1 = time.perf_counter()
-    sim.run(kernel_steps, timed=True)
-    el_k = time.perf_counter() - t1
+    if kernel_steps:
+        sim.run(kernel_steps, timed=True)
     s2 = sim.stats()
     tr = sim.trace()
     sim.close()
@@ -82,38 +72,122 @@ def run_timed(qd, op, prob, steps, warmup, graph_attempts, kernel_steps):
     return dict(elapsed=el, rhs=rhs, retries=s1["retry_passes"] - s0["retry_passes"],
                 sim_seconds=s1["t"] - s0["t"], coupling_ms=s2["coupling_ms_avg"],
                 coupling_timed=s2["coupling_timed"], trace=tr, status=s2["status"],
-                direct_evals_per_s=(s2["rhs_evals"] - s1["rhs_evals"]) / el_k)
+                evals_per_s=rhs / el)
 
 
-def cpu_baseline(ops, prob, n_steps):
-    """The oracle's FP64 main() loop (C restatement, OpenMP rows) on the same
-    stiffness, for a bounded number of steps."""
-    from oracle import oracle as orc
-    n = ops["fp64"].n
-    # the matrices come from the GPU assembly (entries agree with the oracle's
-    # own assembly to 1e-10; assembling 8192^2 pairs on the CPU is not the
-    # loop being timed)
-    from qdfn import configs  # noqa: F401
-    import qdfn
-    seg = prob["_seg"]
-    KS, KN = qdfn.StiffnessMatrix_ByParts_Calculation_Rec(seg, seg, prob["mu"], prob["_nu"])
-    p = dict(prob)
+def summarize(name, r, sec_per_year):
+    r["sim_years_per_hour"] = r["sim_seconds"] / sec_per_year / (r["elapsed"] / 3600.0)
+    log(f"[bench] {name}: {r['rhs']} RHS evals in {r['elapsed']:.3f}s = {r['evals_per_s']:.1f}/s, "
+        f"coupling {r['coupling_ms']*1e3:.1f} us over {r['coupling_timed']}, "
+        f"{r['sim_years_per_hour']:.3f} sim-yr/h, retries {r['retries']}")
+
+
+def rough_fault(qd, args):
+    from qdfn import configs
+    m = configs.rough_fault()
+    n = m["P1"].shape[0]
+    P1, P2, P3, nrm, _, _, slip = qd.RotVerts_UnitVectors(np.hstack([m["P1"], m["P2"], m["P3"]]), n, m["rake"])
+    m = dict(m, P1=P1, P2=P2, P3=P3)
+    ctr = (P1 + P2 + P3) / 3
+    lam = 2 * m["mu"] * m["nu"] / (1 - 2 * m["nu"])
+    st = qd.hmat_structure(ctr, P1, min_elements_to_cut=64, max_levels=40, dist_diam_ratio=args.crit,
+                           arrange_point=(0.0, 0.0, 0.0))
+    p = st["perm"]
+    geo = [a[p] for a in (P1, P2, P3, m["rake"], ctr, nrm, slip)]
     t0 = time.perf_counter()
-    st, res = orc.sim_dense(p, KS, KN, total_step=n_steps, record_step=0, trace=False)
+    d64 = qd.DenseOperator.from_triangles(*geo, m["mu"], lam, qd.PREC_FP64)
+    log(f"[bench] rough fault n={n}: dense FP64 operator in {time.perf_counter() - t0:.2f}s")
+    t0 = time.perf_counter()
+    h32 = qd.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=args.aca_tol,
+                               prec=qd.PREC_FP32)
+    info = h32.info()
+    rk = h32.ranks()
+    log(f"[bench] H FP32 operator in {time.perf_counter() - t0:.2f}s: {info['blocks']} blocks, "
+        f"{int(st['admissible'].sum())} admissible, {info['stored_entries'] / 1e6:.1f} M entries, "
+        f"{info['bytes_per_product'] / 1e6:.1f} MB per product")
+    d32 = qd.DenseOperator.from_triangles(*geo, m["mu"], lam, qd.PREC_FP32)
+    ks, _ = d64.kself()
+    ksum, knsum = d64.matvec(np.ones(n), loading_form=False)
+    prob = configs.problem_from_triangles(m, ks, ksum, knsum, order=p)
+    prob["ext_at_told"] = 1  # main_H reads the external tables at TOld; the plain runs use the same timing
+    hsum, hnsum = h32.matvec(np.ones(n), loading_form=False)
+    prob_h = dict(prob, ext=configs.backslip_tables(hsum, hnsum))
+    # operator accuracy against the dense FP64 product
+    rng = np.random.default_rng(5)
+    acc = {}
+    for label, x in (("random", rng.standard_normal(n)), ("uniform", np.ones(n))):
+        a = d64.matvec(x, loading_form=True)
+        b = h32.matvec(x, loading_form=True)
+        acc[label] = {"tl_rel": float(np.abs(a[0] - b[0]).max() / np.abs(a[0]).max()),
+                      "sigma_rel": float(np.abs(a[1] - b[1]).max() / np.abs(a[1]).max())}
+    ph1, ph2, tot = h32.time_phases(20)
+    res = {}
+    for name, op, pr in (("fp64_dense", d64, prob), ("mixed_h", h32, prob_h), ("fp32_dense", d32, prob)):
+        log(f"[bench] {name}: warmup {args.warmup}, timed {args.steps} steps")
+        r = run_timed(qd, op, pr, args.steps, args.warmup, args.graph_attempts, args.kernel_steps)
+        if r["status"] != 0:
+            raise SystemExit(f"{name} run failed with status {r['status']}")
+        summarize(name, r, configs.SEC_PER_YEAR)
+        r["first_event"] = configs.first_event_time(r["trace"]["told"], r["trace"]["dt"], r["trace"]["vmax"])
+        res[name] = r
+    out = dict(n=n, info=info, st=st, rk=rk, acc=acc, phases=(ph1, ph2, tot), res=res, h32=h32, prob_h=prob_h)
+    d64.close()
+    d32.close()
+    return out
+
+
+def network(qd, args):
+    from qdfn import configs
+    cfg = configs.fault_network()
+    seg = cfg["seg"]
+    ops = {name: qd.DenseOperator.from_rect_segments(seg, cfg["mu"], cfg["nu"], prec)
+           for name, prec in (("fp64", qd.PREC_FP64), ("fp32", qd.PREC_FP32))}
+    ks, _ = ops["fp64"].kself()
+    ksum, knsum = ops["fp64"].matvec(np.ones(seg.shape[0]), loading_form=False)
+    prob = configs.problem_from_segments(cfg, ks, ksum, knsum)
+    res = {}
+    for name in ("fp64", "fp32"):
+        r = run_timed(qd, ops[name], prob, args.network_steps, args.warmup, args.graph_attempts, 0)
+        if r["status"] != 0:
+            raise SystemExit(f"network {name} run failed with status {r['status']}")
+        summarize("network " + name, r, configs.SEC_PER_YEAR)
+        res[name] = r
+    for o in ops.values():
+        o.close()
+    t = [configs.first_event_time(res[k]["trace"]["told"], res[k]["trace"]["dt"], res[k]["trace"]["vmax"])
+         for k in ("fp64", "fp32")]
+    err = abs(t[1] - t[0]) / t[0] if (t[0] and t[1]) else None
+    return {"workload": "fault_network_8192", "fp64_rhs_evals_per_s": round(res["fp64"]["evals_per_s"], 2),
+            "mixed_rhs_evals_per_s": round(res["fp32"]["evals_per_s"], 2),
+            "speedup_mixed_vs_fp64": round(res["fp32"]["evals_per_s"] / res["fp64"]["evals_per_s"], 3),
+            "event_time_rel_err": err, "steps_timed": args.network_steps}
+
+
+def cpu_baseline(h32, st, prob_h, n_steps):
+    """The oracle's FP64 main_H (C restatement of HmatSolver_Pararllel with one
+    thread and of the loop) on the same stored blocks, for a bounded number of
+    steps."""
+    from oracle import oracle as orc
+    HS, HN = h32.blocks(st["range_SR"])
+    t0 = time.perf_counter()
+    status, r = orc.sim_hmat(prob_h, HS, HN, total_step=n_steps, record_step=0, trace=False)
     el = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=res["rhs_evals"] / el, elapsed=el, steps=n_steps, status=st, cores=threads)
+    return dict(value=r["rhs_evals"] / el, elapsed=el, steps=n_steps, status=status)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=8000, help="timed accepted steps per run (rough fault)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--graph-attempts", type=int, default=32)
-    ap.add_argument("--kernel-steps", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--kernel-steps", type=int, default=300, help="direct-launch steps with per-kernel events")
+    ap.add_argument("--crit", type=float, default=1.0, help="DistDiamRatioCrit of the cluster tree")
+    ap.add_argument("--aca-tol", type=float, default=1e-6)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--network-steps", type=int, default=6000)
+    ap.add_argument("--no-network", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU layout is "
@@ -121,52 +195,27 @@ def main():
         sys.exit(2)
 
     import qdfn
-    from qdfn import configs
-    info = qdfn.device_info()
-    log(f"[bench] device {info}")
-    cfg = configs.fault_network()
-    ops, prob = build_problem(qdfn, cfg)
-    prob["_seg"], prob["_nu"] = cfg["seg"], cfg["nu"]
-    n = cfg["seg"].shape[0]
-
-    res = {}
-    for name in ("fp64", "fp32"):
-        log(f"[bench] {name}: warmup {args.warmup} steps, timed {args.steps} steps")
-        r = run_timed(qdfn, ops[name], prob, args.steps, args.warmup, args.graph_attempts, args.kernel_steps)
-        if r["status"] != 0:
-            raise SystemExit(f"{name} run failed with status {r['status']}")
-        r["evals_per_s"] = r["rhs"] / r["elapsed"]
-        r["sim_years_per_hour"] = r["sim_seconds"] / configs.SEC_PER_YEAR / (r["elapsed"] / 3600.0)
-        log(f"[bench] {name}: {r['rhs']} RHS evals in {r['elapsed']:.3f}s = {r['evals_per_s']:.1f}/s, "
-            f"coupling {r['coupling_ms']*1e3:.1f} us avg over {r['coupling_timed']}, "
-            f"sim {r['sim_seconds']/configs.SEC_PER_YEAR:.4f} yr, retries {r['retries']}, "
-            f"direct-launch {r['direct_evals_per_s']:.1f}/s")
-        res[name] = r
-
-    # event-time error between mixed and FP64 over the whole run
-    def onset(r):
-        t = r["trace"]
-        return configs.first_event_time(t["told"], t["dt"], t["vmax"])
-    t64, t32 = onset(res["fp64"]), onset(res["fp32"])
-    ev_err = abs(t32 - t64) / t64 if (t64 and t32) else None
-    # divergence of the step sequences (simulated time reached) as a second measure
-    tt64 = res["fp64"]["trace"]["told"]
-    tt32 = res["fp32"]["trace"]["told"]
-    m = min(tt64.size, tt32.size)
-    time_dev = float(np.max(np.abs(tt32[:m] - tt64[:m]) / np.maximum(tt64[:m], 1.0))) if m else None
-
-    mixed = res["fp32"]
-    bytes_launch = ops["fp32"].bytes_per_product()
-    achieved = bytes_launch / (mixed["coupling_ms"] * 1e-3) / 1e9
+    log(f"[bench] device {qdfn.device_info()}")
+    rf = rough_fault(qdfn, args)
+    res = rf["res"]
+    mixed, plain = res["mixed_h"], res["fp64_dense"]
+    t64, th, t32 = plain["first_event"], mixed["first_event"], res["fp32_dense"]["first_event"]
+    ev_err = abs(th - t64) / t64 if (th and t64) else None
+    ev_ref = abs(t32 - t64) / t64 if (t32 and t64) else None
+    ph1, ph2, tot = rf["phases"]
+    bytes2 = rf["info"]["phase2_bytes"]
+    achieved = bytes2 / (mixed["coupling_ms"] * 1e-3) / 1e9
+    net = None if args.no_network else network(qdfn, args)
     cpu = None
     if not args.no_cpu:
-        log(f"[bench] CPU baseline: oracle FP64 loop, {args.cpu_steps} steps")
-        c = cpu_baseline(ops, prob, args.cpu_steps)
-        cpu = {"value": round(c["value"], 3), "unit": "RHS evals/s", "cores": c["cores"], "kind": "port",
-               "sample": f"{c['steps']} accepted steps of the same 8,192-element network, FP64 C restatement "
-                         f"of main() with OpenMP over rows ({c['elapsed']:.1f}s)"}
-        log(f"[bench] CPU baseline {c['value']:.2f} RHS evals/s on {c['cores']} threads")
-
+        log(f"[bench] CPU baseline: oracle FP64 main_H, {args.cpu_steps} steps")
+        c = cpu_baseline(rf["h32"], rf["st"], rf["prob_h"], args.cpu_steps)
+        cpu = {"value": round(c["value"], 4), "unit": "RHS evals/s", "cores": 1, "kind": "port",
+               "sample": f"{c['steps']} accepted steps of the 20,000-triangle H-matrix run, FP64 C restatement "
+                         f"of main_H / HmatSolver on the same blocks, one thread ({c['elapsed']:.1f}s)"}
+        log(f"[bench] CPU baseline {c['value']:.3f} RHS evals/s")
+    rf["h32"].close()
+    rk = rf["rk"]
     out = {
         "metric": METRIC,
         "value": round(mixed["evals_per_s"], 2),
@@ -179,18 +228,29 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded 8-fault network, back-slip loading)",
-        "config": {"workload": "fault_network_8192", "n_elements": n, "storage": "fp32 stiffness, fp64 state",
-                   "steps_timed": args.steps},
-        "fp64": {"rhs_evals_per_s": round(res["fp64"]["evals_per_s"], 2),
-                 "sim_years_per_hour": round(res["fp64"]["sim_years_per_hour"], 4),
-                 "coupling_us": round(res["fp64"]["coupling_ms"] * 1e3, 2)},
+        "data": "synthetic (seeded rough fault, back-slip loading)",
+        "config": {"workload": "rough_fault_20000", "n_elements": rf["n"], "operator": "H-matrix, FP32 storage",
+                   "leaf": 64, "dist_diam_ratio": args.crit, "aca_tol": args.aca_tol,
+                   "plain": "dense FP64", "steps_timed": args.steps},
+        "fp64_plain": {"rhs_evals_per_s": round(plain["evals_per_s"], 2),
+                       "sim_years_per_hour": round(plain["sim_years_per_hour"], 4),
+                       "coupling_us": round(plain["coupling_ms"] * 1e3, 2)},
         "mixed": {"rhs_evals_per_s": round(mixed["evals_per_s"], 2),
                   "sim_years_per_hour": round(mixed["sim_years_per_hour"], 4),
-                  "coupling_us": round(mixed["coupling_ms"] * 1e3, 2)},
-        "speedup_mixed_vs_fp64": round(mixed["evals_per_s"] / res["fp64"]["evals_per_s"], 3),
+                  "phase2_kernel_us": round(mixed["coupling_ms"] * 1e3, 2),
+                  "h_product_us": {"phase1": round(ph1 * 1e3, 2), "phase2": round(ph2 * 1e3, 2),
+                                   "total": round(tot * 1e3, 2)},
+                  "stored_entries": rf["info"]["stored_entries"],
+                  "bytes_per_product": rf["info"]["bytes_per_product"],
+                  "mean_rank_shear": round(float(rk["rank_S"][rk["kind_S"] == 1].mean()), 2),
+                  "mean_rank_normal": round(float(rk["rank_N"][rk["kind_N"] == 1].mean()), 2)},
+        "fp32_dense": {"rhs_evals_per_s": round(res["fp32_dense"]["evals_per_s"], 2)},
+        "speedup_mixed_vs_fp64": round(mixed["evals_per_s"] / plain["evals_per_s"], 3),
         "event_time_rel_err": ev_err,
-        "sim_time_rel_dev": time_dev,
+        "event_time_rel_err_fp32_dense": ev_ref,
+        "first_event_s": {"fp64_plain": t64, "mixed_h": th, "fp32_dense": t32},
+        "operator_rel_err": rf["acc"],
+        "network_8192": net,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic()},
         "cpu_baseline": cpu,

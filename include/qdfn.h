@@ -175,8 +175,11 @@ qdfn_hmat_t* qdfn_hmat_create(int64_t n, int64_t nblk, const int64_t* range_SR, 
                               const double* const* S_b, const int32_t* kind_N, const int64_t* rank_N,
                               const double* const* N_a, const double* const* N_b, int32_t prec);
 void qdfn_hmat_destroy(qdfn_hmat_t* h);
-/* n, block count, device bytes of the stored operator, stored entries (both matrices). */
-int qdfn_hmat_info(qdfn_hmat_t* h, int64_t* n, int64_t* nblk, int64_t* device_bytes, int64_t* stored_entries);
+/* n, block count, HBM bytes one product moves, stored entries (both matrices),
+ * and phase_bytes[2]: algorithmic bytes of the phase-1 (t = V x) and phase-2
+ * (y = panels [x; t]) kernels. */
+int qdfn_hmat_info(qdfn_hmat_t* h, int64_t* n, int64_t* nblk, int64_t* device_bytes, int64_t* stored_entries,
+                   int64_t* phase_bytes);
 /* Per block: kind and rank of the shear and normal parts (Ranks_Shear/Ranks_Normal). */
 int qdfn_hmat_get_ranks(qdfn_hmat_t* h, int32_t* kind_S, int64_t* rank_S, int32_t* kind_N, int64_t* rank_N);
 /* One block's stored factors (which = 0 shear, 1 normal): dense m x k into a,
@@ -192,6 +195,9 @@ int qdfn_hmat_normal_zero(qdfn_hmat_t* h);
 int qdfn_hmat_matvec(qdfn_hmat_t* h, const double* x, double* yS, double* yN, int32_t loading_form);
 /* Average duration (ms, HIP events) of `reps` H products (both phases). */
 int qdfn_hmat_time_matvec(qdfn_hmat_t* h, int32_t reps, double* avg_ms);
+/* Average durations (ms, HIP events on the operator's stream) of phase 1
+ * (with its finish), the phase-2 kernel, and the whole product. */
+int qdfn_hmat_time_phases(qdfn_hmat_t* h, int32_t reps, double* ms_phase1, double* ms_phase2, double* ms_total);
 
 /* ---- rate-and-state local solve ---------------------------------------- */
 
@@ -270,7 +276,8 @@ int qdfn_sim_set_external(qdfn_sim_t* sim, int64_t nt, const double* times, cons
 /* Advance n_steps accepted steps (the reference's TotalStep loop body).
  * mode 0: attempts replayed from a captured HIP graph (graph_attempts per
  * replay); mode 1: direct launches with HIP events around every coupling
- * kernel (fills coupling_ms_avg).  Returns QDFN_OK or the run's status. */
+ * kernel (fills coupling_ms_avg; for an H operator the phase-2 kernel).
+ * Returns QDFN_OK or the run's status. */
 int qdfn_sim_run(qdfn_sim_t* sim, int64_t n_steps, int32_t mode, int32_t graph_attempts);
 int qdfn_sim_get_stats(qdfn_sim_t* sim, qdfn_sim_stats* st);
 /* Current state (after the last accepted step): V, Disp, Theta, Friction,

@@ -54,6 +54,7 @@ struct qdfn_hmat {
     int64_t stored_entries = 0;
     int64_t device_bytes = 0;
     int64_t phase_bytes = 0;              // HBM bytes one product reads and writes
+    int64_t alg_bytes1 = 0, alg_bytes2 = 0; // algorithmic bytes of the phase-1 / phase-2 main kernels
     hipStream_t stream = nullptr;
 };
 
@@ -61,6 +62,8 @@ namespace qdfn {
 // Both phases of one product on stream s.  form 1: yS = (H_load (0 - x)) ./ K_ii,
 // form 0: yS = H_S x; yN = H_N x.  The gate skips the product as for the dense
 // coupling.
+// Events (optional): ev0 before phase 1, ev1 between the phases, ev2 after the
+// phase-2 main kernel (before its finish).
 int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, int32_t form, double* d_yS,
-                double* d_yN, hipStream_t s);
+                double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2);
 }  // namespace qdfn

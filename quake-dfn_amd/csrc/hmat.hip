@@ -619,11 +619,28 @@ static int hmat_layout(qdfn_hmat* h) {
     h->phase_bytes = (int64_t)(es * pool.size() + sizeof(int32_t) * idx.size() + sizeof(HRowBlock) * rbs.size() +
                                sizeof(HTile) * tiles.size() + 16 * part_total + 16 * h->n_t + 8 * 3 * n);
     h->device_bytes = h->phase_bytes;
+    // algorithmic bytes of each phase's main kernel: stored entries (unpadded),
+    // index list (phase 2), gathered right-hand side, outputs
+    int64_t e1 = 0, e2 = 0, k1 = 0, k2 = 0, o1 = 0, o2 = 0;
+    for (size_t q = 0; q < rbs.size(); ++q) {
+        const HRowBlock& R = rbs[q];
+        if ((int64_t)q < h->n_rb1) {
+            e1 += (int64_t)R.m * R.K;
+            k1 += R.K;
+            o1 += R.m;
+        } else {
+            e2 += (int64_t)R.m * R.K;
+            k2 += R.K;
+            o2 += R.m;
+        }
+    }
+    h->alg_bytes1 = (int64_t)es * e1 + 8 * k1 + 8 * o1;
+    h->alg_bytes2 = (int64_t)es * e2 + 12 * k2 + 8 * o2;
     return QDFN_OK;
 }
 
 int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, int32_t form, double* d_yS,
-                double* d_yN, hipStream_t s) {
+                double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
     const HRowBlock* rbs = h->rbs.as<HRowBlock>();
     const HTile* tiles = h->tiles.as<HTile>();
     const int32_t* multi = h->multi.as<int32_t>();
@@ -631,6 +648,7 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
     const double* kd = h->kdiag.as<double>();
     double* t = h->t.as<double>();
     const bool f32 = h->prec == QDFN_PREC_FP32;
+    if (ev0) QDFN_HIP(hipEventRecord(ev0, s));
     if (h->n_tiles1 > 0) {
         if (f32)
             hipLaunchKernelGGL((k_hrow<float, false>), dim3((unsigned)h->n_tiles1), dim3(kPhase2Threads), 0, s, tiles,
@@ -648,6 +666,7 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
         }
     }
     if (g.ran_log) hipLaunchKernelGGL(k_mark_skipped, dim3(1), dim3(1), 0, s, g);
+    if (ev1) QDFN_HIP(hipEventRecord(ev1, s));
     const int32_t mark_tiles = h->n_multi2 > 0 ? 0 : 1;
     if (f32)
         hipLaunchKernelGGL((k_hrow<float, true>), dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
@@ -658,6 +677,7 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
                            tiles + h->n_tiles1, rbs, part, h->pool.as<double>(), h->idx.as<int32_t>(), d_x, t, h->n,
                            kd, form, d_yS, d_yN, mark_tiles, g);
     QDFN_HIP(hipGetLastError());
+    if (ev2) QDFN_HIP(hipEventRecord(ev2, s));
     if (h->n_multi2 > 0) {
         hipLaunchKernelGGL(k_hfinish, dim3((unsigned)h->n_multi2), dim3(64), 0, s, multi + h->n_multi1, rbs, part,
                            d_x, kd, form, t, d_yS, d_yN, 1, g);
@@ -1150,12 +1170,16 @@ extern "C" void qdfn_hmat_destroy(qdfn_hmat_t* h) {
 }
 
 extern "C" int qdfn_hmat_info(qdfn_hmat_t* h, int64_t* n, int64_t* nblk, int64_t* device_bytes,
-                              int64_t* stored_entries) {
+                              int64_t* stored_entries, int64_t* phase_bytes) {
     if (!h) return fail(QDFN_ERR_ARG, "qdfn_hmat_info: null handle");
     if (n) *n = h->n;
     if (nblk) *nblk = (int64_t)h->blocks.size();
     if (device_bytes) *device_bytes = h->phase_bytes;
     if (stored_entries) *stored_entries = h->stored_entries;
+    if (phase_bytes) {
+        phase_bytes[0] = h->alg_bytes1;
+        phase_bytes[1] = h->alg_bytes2;
+    }
     return QDFN_OK;
 }
 
@@ -1198,7 +1222,7 @@ extern "C" int qdfn_hmat_matvec(qdfn_hmat_t* h, const double* x, double* yS, dou
     const size_t b = sizeof(double) * h->n;
     QDFN_HIP(hipMemcpyAsync(h->x.p, x, b, hipMemcpyHostToDevice, s));
     int rc = launch_hmat(h, h->x.as<double>(), CouplingGate{}, loading_form ? 1 : 0, h->yS.as<double>(),
-                         h->yN.as<double>(), s);
+                         h->yN.as<double>(), s, nullptr, nullptr, nullptr);
     if (rc) return rc;
     if (yS) QDFN_HIP(hipMemcpyAsync(yS, h->yS.p, b, hipMemcpyDeviceToHost, s));
     if (yN) QDFN_HIP(hipMemcpyAsync(yN, h->yN.p, b, hipMemcpyDeviceToHost, s));
@@ -1206,18 +1230,51 @@ extern "C" int qdfn_hmat_matvec(qdfn_hmat_t* h, const double* x, double* yS, dou
     return QDFN_OK;
 }
 
+extern "C" int qdfn_hmat_time_phases(qdfn_hmat_t* h, int32_t reps, double* ms_phase1, double* ms_phase2,
+                                     double* ms_total) {
+    if (!h || reps <= 0) return fail(QDFN_ERR_ARG, "qdfn_hmat_time_phases: bad arguments");
+    hipStream_t s = h->stream;
+    QDFN_HIP(hipMemsetAsync(h->x.p, 0, sizeof(double) * h->n, s));
+    hipEvent_t e[4];
+    for (auto& x : e) QDFN_HIP(hipEventCreate(&x));
+    double a = 0, b = 0, c = 0;
+    int rc = QDFN_OK;
+    for (int r = 0; r <= reps && rc == QDFN_OK; ++r) {
+        rc = launch_hmat(h, h->x.as<double>(), CouplingGate{}, 1, h->yS.as<double>(), h->yN.as<double>(), s, e[0], e[1],
+                         e[2]);
+        if (rc) break;
+        QDFN_HIP(hipEventRecord(e[3], s));
+        QDFN_HIP(hipEventSynchronize(e[3]));
+        if (r == 0) continue;  // warm-up
+        float t01 = 0, t12 = 0, t03 = 0;
+        QDFN_HIP(hipEventElapsedTime(&t01, e[0], e[1]));
+        QDFN_HIP(hipEventElapsedTime(&t12, e[1], e[2]));
+        QDFN_HIP(hipEventElapsedTime(&t03, e[0], e[3]));
+        a += t01;
+        b += t12;
+        c += t03;
+    }
+    for (auto& x : e) (void)hipEventDestroy(x);
+    if (ms_phase1) *ms_phase1 = a / reps;
+    if (ms_phase2) *ms_phase2 = b / reps;
+    if (ms_total) *ms_total = c / reps;
+    return rc;
+}
+
 extern "C" int qdfn_hmat_time_matvec(qdfn_hmat_t* h, int32_t reps, double* avg_ms) {
     if (!h || reps <= 0 || !avg_ms) return fail(QDFN_ERR_ARG, "qdfn_hmat_time_matvec: bad arguments");
     hipStream_t s = h->stream;
     QDFN_HIP(hipMemsetAsync(h->x.p, 0, sizeof(double) * h->n, s));
-    int rc = launch_hmat(h, h->x.as<double>(), CouplingGate{}, 1, h->yS.as<double>(), h->yN.as<double>(), s);
+    int rc = launch_hmat(h, h->x.as<double>(), CouplingGate{}, 1, h->yS.as<double>(), h->yN.as<double>(), s, nullptr,
+                         nullptr, nullptr);
     if (rc) return rc;
     hipEvent_t e0, e1;
     QDFN_HIP(hipEventCreate(&e0));
     QDFN_HIP(hipEventCreate(&e1));
     QDFN_HIP(hipEventRecord(e0, s));
     for (int r = 0; r < reps && rc == QDFN_OK; ++r)
-        rc = launch_hmat(h, h->x.as<double>(), CouplingGate{}, 1, h->yS.as<double>(), h->yN.as<double>(), s);
+        rc = launch_hmat(h, h->x.as<double>(), CouplingGate{}, 1, h->yS.as<double>(), h->yN.as<double>(), s, nullptr,
+                         nullptr, nullptr);
     QDFN_HIP(hipEventRecord(e1, s));
     QDFN_HIP(hipEventSynchronize(e1));
     float ms = 0;

@@ -359,13 +359,13 @@ static int enqueue_attempt(qdfn_sim* s, int slot, hipEvent_t ev0, hipEvent_t ev1
     g.parity_stride = n;
     g.ran_log = slot >= 0 ? s->A.ran_log : nullptr;
     g.slot = slot;
-    if (ev0) QDFN_HIP(hipEventRecord(ev0, s->stream));
     int rc;
     if (s->hop) {
-        rc = launch_hmat(s->hop, s->A.D, g, 1, s->A.tl, s->A.sigprod, s->stream);
+        // timed mode: events around the phase-2 kernel (the dominant one)
+        rc = launch_hmat(s->hop, s->A.D, g, 1, s->A.tl, s->A.sigprod, s->stream, nullptr, ev0, ev1);
         if (rc) return rc;
-        if (ev1) QDFN_HIP(hipEventRecord(ev1, s->stream));
     } else {
+        if (ev0) QDFN_HIP(hipEventRecord(ev0, s->stream));
         rc = launch_coupling(s->op, s->A.D, g, s->stream);
         if (rc) return rc;
         if (ev1) QDFN_HIP(hipEventRecord(ev1, s->stream));

@@ -87,7 +87,8 @@ SIGNATURES = {
     "qdfn_hmat_build_tri": (VP, [I64, PD, PD, PD, PD, PD, PD, PD, D, D, I64, PI64, PI64, D, D, I32]),
     "qdfn_hmat_create": (VP, [I64, I64, PI64, PI64, PI32, PI64, PPD, PPD, PI32, PI64, PPD, PPD, I32]),
     "qdfn_hmat_destroy": (None, [VP]),
-    "qdfn_hmat_info": (C.c_int, [VP, PI64, PI64, PI64, PI64]),
+    "qdfn_hmat_info": (C.c_int, [VP, PI64, PI64, PI64, PI64, PI64]),
+    "qdfn_hmat_time_phases": (C.c_int, [VP, I32, PD, PD, PD]),
     "qdfn_hmat_get_ranks": (C.c_int, [VP, PI32, PI64, PI32, PI64]),
     "qdfn_hmat_get_block": (C.c_int, [VP, I64, I32, PD, PD]),
     "qdfn_hmat_kself": (C.c_int, [VP, PD, PD]),

@@ -139,8 +139,16 @@ class HMatrix:
 
     def info(self):
         n, nb, db, se = _lib.I64(), _lib.I64(), _lib.I64(), _lib.I64()
-        check(lib().qdfn_hmat_info(self.handle, C.byref(n), C.byref(nb), C.byref(db), C.byref(se)))
-        return {"n": n.value, "blocks": nb.value, "bytes_per_product": db.value, "stored_entries": se.value}
+        pb = (_lib.I64 * 2)()
+        check(lib().qdfn_hmat_info(self.handle, C.byref(n), C.byref(nb), C.byref(db), C.byref(se), pb))
+        return {"n": n.value, "blocks": nb.value, "bytes_per_product": db.value, "stored_entries": se.value,
+                "phase1_bytes": pb[0], "phase2_bytes": pb[1]}
+
+    def time_phases(self, reps=20):
+        """(phase-1 ms, phase-2 kernel ms, whole product ms), HIP events."""
+        a, b, c = _lib.D(), _lib.D(), _lib.D()
+        check(lib().qdfn_hmat_time_phases(self.handle, reps, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def ranks(self):
         nb = self.info()["blocks"]

@@ -1,0 +1,55 @@
+"""Launch the H-matrix product alone, for kernel traces and counter collection.
+
+    python -m qdfn.profile_hmat [--reps 50] [--crit 1.0] [--aca-tol 1e-6] [--no-calibration]
+
+Builds the 20,000-triangle rough-fault H operator (FP32 storage) on the device
+and runs `reps` products (phase 1: t = V x, phase 2: y = panels [x; t]).
+Unless --no-calibration is given it first builds the 8,192-element
+fault-network dense FP32 operator, whose build launches k_build, a plain
+streaming kernel of known traffic (reads K_S and K_N in FP64: 1 GiB, writes E
+and K_N in FP32: 512 MiB), against which the HBM counters (FETCH_SIZE /
+WRITE_SIZE) are calibrated.
+"""
+import argparse
+import sys
+
+import numpy as np
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--crit", type=float, default=1.0)
+    ap.add_argument("--aca-tol", type=float, default=1e-6)
+    ap.add_argument("--no-calibration", action="store_true")
+    a = ap.parse_args()
+    import qdfn
+    from qdfn import configs
+    if not a.no_calibration:
+        cfg = configs.fault_network()
+        op = qdfn.DenseOperator.from_rect_segments(cfg["seg"], cfg["mu"], cfg["nu"], qdfn.PREC_FP32)
+        op.close()
+        print("calibration operator built (k_build<float>: reads 1048576 KiB, writes 524288 KiB)", flush=True)
+    m = configs.rough_fault()
+    n = m["P1"].shape[0]
+    P1, P2, P3, nrm, _, _, slip = qdfn.RotVerts_UnitVectors(np.hstack([m["P1"], m["P2"], m["P3"]]), n, m["rake"])
+    ctr = (P1 + P2 + P3) / 3
+    lam = 2 * m["mu"] * m["nu"] / (1 - 2 * m["nu"])
+    st = qdfn.hmat_structure(ctr, P1, min_elements_to_cut=64, max_levels=40, dist_diam_ratio=a.crit,
+                             arrange_point=(0.0, 0.0, 0.0))
+    p = st["perm"]
+    geo = [x[p] for x in (P1, P2, P3, m["rake"], ctr, nrm, slip)]
+    h = qdfn.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=a.aca_tol,
+                               prec=qdfn.PREC_FP32)
+    info = h.info()
+    t1, t2, tot = h.time_phases(a.reps)
+    print(f"H product FP32 n={n}: phase 1 {t1 * 1e3:.1f} us ({info['phase1_bytes'] / 1e6:.1f} MB), "
+          f"phase 2 kernel {t2 * 1e3:.1f} us ({info['phase2_bytes'] / 1e6:.1f} MB, "
+          f"{info['phase2_bytes'] / (t2 * 1e-3) / 1e9:.0f} GB/s), total {tot * 1e3:.1f} us "
+          f"({info['bytes_per_product'] / 1e6:.1f} MB)", flush=True)
+    h.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
