@@ -82,6 +82,18 @@ def summarize(name, r, sec_per_year):
         f"{r['sim_years_per_hour']:.3f} sim-yr/h, retries {r['retries']}")
 
 
+def divergence(tr, ref, rel=1e-3):
+    """First step at which the simulated time of a run departs from the FP64
+    plain run by more than `rel` (relative), with the plain run's time there."""
+    a, b = np.asarray(tr["told"]), np.asarray(ref["told"])
+    m = min(a.size, b.size)
+    d = np.abs(a[:m] - b[:m]) > rel * np.maximum(np.abs(b[:m]), 1.0)
+    if not d.any():
+        return None
+    k = int(np.argmax(d))
+    return {"step": k, "sim_time_s": float(b[k])}
+
+
 def rough_fault(qd, args):
     from qdfn import configs
     m = configs.rough_fault()
@@ -249,6 +261,8 @@ def main():
         "event_time_rel_err": ev_err,
         "event_time_rel_err_fp32_dense": ev_ref,
         "first_event_s": {"fp64_plain": t64, "mixed_h": th, "fp32_dense": t32},
+        "divergence_from_fp64": {"mixed_h": divergence(mixed["trace"], plain["trace"]),
+                                 "fp32_dense": divergence(res["fp32_dense"]["trace"], plain["trace"])},
         "operator_rel_err": rf["acc"],
         "network_8192": net,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
