@@ -106,3 +106,45 @@ def test_edge_points_are_nan(orc):
     assert np.all(np.isnan(S))
     S, _ = orc.tdstress_hs([0.0], [50.0], [-500.0], P1, P2, P3, 1.0, 0.0, 0.0, MU, LAM)
     assert np.all(np.isfinite(S))
+
+
+def test_rotverts_frames(orc):
+    """RotVerts_UnitVectors (Functions_BuildInputFile.jl:338-377): vertices
+    reordered so the normal has n_y > 0; unit, orthogonal normal/strike/dip;
+    horizontal strike; slip = strike cos(rake) + dip sin(rake)."""
+    from qdfn import configs
+    m = configs.rough_fault(nx=6, nz=5, h=60.0, ratio=2e-2)
+    rake = np.linspace(-170, 170, m["P1"].shape[0])
+    P1, P2, P3, nrm, st, dp, slip = orc.rotverts_unitvectors(m["P1"], m["P2"], m["P3"], rake)
+    c = np.cross(P2 - P1, P3 - P1)
+    np.testing.assert_allclose(c / np.linalg.norm(c, axis=1)[:, None], nrm, atol=1e-14)
+    assert (nrm[:, 1] > 0).all()
+    for v in (nrm, st, dp, slip):
+        np.testing.assert_allclose(np.linalg.norm(v, axis=1), 1.0, atol=1e-14)
+    assert np.abs((nrm * st).sum(1)).max() < 1e-14 and np.abs((nrm * dp).sum(1)).max() < 1e-14
+    assert np.abs(st[:, 2]).max() < 1e-14
+    r = np.deg2rad(rake)[:, None]
+    np.testing.assert_allclose(slip, st * np.cos(r) + dp * np.sin(r), atol=1e-14)
+
+
+def test_triangle_stiffness_is_traction_projection(orc):
+    """StiffnessMatrix_ByParts_Calculation_Tri (:551-594): column s equals
+    TDstressHS of source s (Ss = -cos rake, Ds = sin rake) at the receiver
+    centroids, traction t = sigma n, projected on n (K_N) and on the slip
+    vector (K_S); self-stiffness negative."""
+    from qdfn import configs
+    m = configs.rough_fault(nx=4, nz=3, h=60.0, ratio=2e-2)
+    rake = np.full(m["P1"].shape[0], 30.0)
+    P1, P2, P3, nrm, st, dp, slip = orc.rotverts_unitvectors(m["P1"], m["P2"], m["P3"], rake)
+    ctr = (P1 + P2 + P3) / 3
+    KS, KN = orc.stiffness_tri(P1, P2, P3, rake, ctr, nrm, slip, MU, LAM)
+    assert (np.diag(KS) < 0).all()
+    for s in (0, 5, 17):
+        S, _ = orc.tdstress_hs(*ctr.T, P1[s], P2[s], P3[s], -np.cos(np.deg2rad(30.0)), np.sin(np.deg2rad(30.0)),
+                               0.0, MU, LAM)
+        sig = np.zeros((ctr.shape[0], 3, 3))
+        for (i, j), k in {(0, 0): 0, (1, 1): 1, (2, 2): 2, (0, 1): 3, (0, 2): 4, (1, 2): 5}.items():
+            sig[:, i, j] = sig[:, j, i] = S[:, k]
+        t = np.einsum("rij,rj->ri", sig, nrm)
+        np.testing.assert_allclose(KN[:, s], (t * nrm).sum(1), rtol=1e-12, atol=1e-12 * np.abs(KN).max())
+        np.testing.assert_allclose(KS[:, s], (t * slip).sum(1), rtol=1e-12, atol=1e-12 * np.abs(KS).max())
