@@ -130,4 +130,63 @@ function SolveOneTimeStep(ConvergenceCrit, DispOld, FrictionOld, ThetaOld, VOld,
     return V, Friction, Disp, Theta, EffNormalStress, Dt_All, InstabilityThistime, Float64.(flag)
 end
 
+# ---- H-matrix (Hmatrix = true) --------------------------------------------
+
+# H operator from the reference's ShearStiffness_H / NormalStiffness_H (RUN2_BUILD_INPUT.jl:362):
+# dense blocks pass as they are; a LowRankApprox PartialQR F (A[:, F[:p]] ~ F[:Q] F[:R]) passes as
+# U = F[:Q] and V with V[:, F[:p]] = F[:R].  prec: 0 FP64, 1 FP32 storage.
+function HmatCreate(FaultCount, ElementRange_SR, Admissible, ShearStiffness_H, NormalStiffness_H; prec=0)
+    nb = size(ElementRange_SR, 1)
+    rng = Matrix{Int64}(permutedims(ElementRange_SR))          # 4 x nb = nb x 4 row-major
+    adm = Vector{Int64}(vec(Admissible))
+    keep = Any[]
+    parts = map((ShearStiffness_H, NormalStiffness_H)) do H
+        kind = zeros(Int32, nb); rank = zeros(Int64, nb)
+        A = Vector{Ptr{Float64}}(undef, nb); B = fill(Ptr{Float64}(0), nb)
+        for q in 1:nb
+            blk = H[q]
+            if blk isa AbstractMatrix
+                M = Matrix{Float64}(blk); push!(keep, M); A[q] = pointer(M)
+            else
+                k = size(blk[:R], 2); r = size(blk[:Q], 2)
+                U = Matrix{Float64}(blk[:Q]); V = zeros(r, k); V[:, blk[:p]] = blk[:R]
+                push!(keep, U); push!(keep, V)
+                kind[q] = 1; rank[q] = r; A[q] = pointer(U); B[q] = pointer(V)
+            end
+        end
+        push!(keep, kind, rank, A, B)
+        (kind, rank, A, B)
+    end
+    h = GC.@preserve rng adm keep ccall((:qdfn_hmat_create, LIB), Ptr{Cvoid},
+        (Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int32}, Ptr{Int64}, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}},
+         Ptr{Int32}, Ptr{Int64}, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Int32),
+        FaultCount, nb, rng, adm, parts[1]..., parts[2]..., Int32(prec))
+    h == C_NULL && check(-1)
+    return h
+end
+
+# HmatBuild_R (Functions_BuildInputFile.jl:725) on the device: returns an operator handle
+# (ACA with full pivoting + SVD recompression, Frobenius tolerance relative to each block's
+# shear norm; Tolerance in the reference's absolute sense can be passed as atol with rtol = 0).
+function HmatBuild_R_Device(ShearModulus, PoissonRatio, ElementRange_SR, Input_Segment, Admissible;
+                            rtol=1e-6, atol=0.0, prec=1)
+    seg = Matrix{Float64}(Input_Segment); n = size(seg, 1)
+    rng = Matrix{Int64}(permutedims(ElementRange_SR)); adm = Vector{Int64}(vec(Admissible))
+    h = GC.@preserve seg rng adm ccall((:qdfn_hmat_build_rect, LIB), Ptr{Cvoid},
+        (Ptr{Float64}, Int64, Int64, Float64, Float64, Int64, Ptr{Int64}, Ptr{Int64}, Float64, Float64, Int32),
+        seg, n, n, ShearModulus, PoissonRatio, length(adm), rng, adm, rtol, atol, Int32(prec))
+    h == C_NULL && check(-1)
+    return h
+end
+
+HmatDestroy(h) = ccall((:qdfn_hmat_destroy, LIB), Cvoid, (Ptr{Cvoid},), h)
+
+# HmatSolver_Pararllel (Functions_Hmatrix.jl:631) for both matrices: (shear, normal) products.
+function HmatSolver(h, x, n; loading_form=false)
+    xv = Vector{Float64}(vec(x)); yS = zeros(n); yN = zeros(n)
+    GC.@preserve xv yS yN check(ccall((:qdfn_hmat_matvec, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int32), h, xv, yS, yN, Int32(loading_form)))
+    return yS, yN
+end
+
 end # module
