@@ -11,10 +11,12 @@ from .assembly import (BuildMatrixByParts_Even_Rec, BuildMatrixByParts_Even_Tri,
                        RotVerts_UnitVectors, StiffnessMatrix_ByParts_Calculation_Rec,
                        StiffnessMatrix_ByParts_Calculation_Tri, TDstressHS)
 from .hmat import HMatrix, hmat_structure, main_H
+from .inputs import BulkToSegment, LRtoRake, ReadBulkInput, ReadSegmentInput
 from .sim import DenseOperator, Simulation, main
 from .solvers import SolveOneTimeStep
 
 __all__ = ["PREC_FP32", "PREC_FP64", "QdfnError", "device_info", "lib", "Okada_DC3D_Vector",
            "StiffnessMatrix_ByParts_Calculation_Rec", "BuildMatrixByParts_Even_Rec", "TDstressHS",
            "RotVerts_UnitVectors", "StiffnessMatrix_ByParts_Calculation_Tri", "BuildMatrixByParts_Even_Tri",
-           "DenseOperator", "Simulation", "main", "SolveOneTimeStep", "HMatrix", "hmat_structure", "main_H"]
+           "DenseOperator", "Simulation", "main", "SolveOneTimeStep", "HMatrix", "hmat_structure", "main_H",
+           "BulkToSegment", "LRtoRake", "ReadBulkInput", "ReadSegmentInput"]

@@ -1,8 +1,7 @@
-"""Input-format helpers and the seeded synthetic workloads.
+"""Run set-up helpers and the seeded synthetic workloads.
 
-Format layer (reference file:line):
-  BulkToSegment      scripts/Functions_BuildInputFile.jl:82-179 (rectangle subdivision)
-  LRtoRake           scripts/Functions_BuildInputFile.jl:182-213
+The bulk-geometry subdivision (BulkToSegment, LRtoRake) lives in qdfn.inputs.
+Set-up (reference file:line):
   run_setup          scripts/QUAKEDFN_RunARG.jl:159-201 (mass, RecTimeStep, RuptureDt)
   backslip_tables    Tools/ExternalStress/UniformBackslip.jl:75-94 (uniform back-slip loading)
 
@@ -20,67 +19,8 @@ SEC_PER_YEAR = 3.15576e7
 RUN3_DEFAULTS = dict(dt_cut=5.0, switch_v=1e-2, rupture_multiple=3.0, maximum_dt=1e7, vertical_length_scale=0.0)
 
 
-def _cosd(x):
-    return np.cos(np.deg2rad(x))
-
-
-def _sind(x):
-    return np.sin(np.deg2rad(x))
-
-
-def LRtoRake(Switch_StrikeSlip_or_ReverseNormal, Input_Bulk):
-    """Column 8 (LR/RN flag, -1 or +1) -> rake angle in degrees."""
-    B = np.array(Input_Bulk, dtype=np.float64, copy=True)
-    if Switch_StrikeSlip_or_ReverseNormal == 1:
-        B[:, 7] = np.where(B[:, 7] == -1.0, 0.0, 180.0)
-    elif Switch_StrikeSlip_or_ReverseNormal == 2:
-        shallow = B[:, 6] < 90.0
-        ll = B[:, 7] == -1.0
-        B[:, 7] = np.where(shallow, np.where(ll, 90.0, 270.0), np.where(ll, 270.0, 90.0))
-    return B
-
-
-def BulkToSegment(Input_Bulk, counts=None):
-    """Subdivide bulk rectangles (18 columns) into Input_Segment rows (19 columns):
-    1-3 centre (z = depth), 4 strike length, 5 dip length, 6 strike, 7 dip, 8 rake,
-    9 a, 10 b, 11 Dc, 12 Theta_i, 13 V_i, 14 Friction_i, 15 normal stress,
-    16 V_Const, 17 bulk index, 18-19 bulk lengths.  Element counts along strike
-    and dip are ceil(length / column 18) as in the reference, unless `counts`
-    gives them explicitly per bulk fault."""
-    B = np.asarray(Input_Bulk, dtype=np.float64)
-    if counts is None:
-        nl = np.ceil(B[:, 3] / B[:, 17]).astype(int)
-        nw = np.ceil(B[:, 4] / B[:, 17]).astype(int)
-    else:
-        nl = np.array([c[0] for c in counts], dtype=int)
-        nw = np.array([c[1] for c in counts], dtype=int)
-    rows = []
-    for k in range(B.shape[0]):
-        L, W = B[k, 3], B[k, 4]
-        st, dp = B[k, 5], B[k, 6]
-        Rs = np.array([[_cosd(st), -_sind(st), 0.0], [_sind(st), _cosd(st), 0.0], [0.0, 0.0, 1.0]])
-        Rd = np.array([[1.0, 0.0, 0.0], [0.0, _cosd(dp), -_sind(dp)], [0.0, _sind(dp), _cosd(dp)]])
-        R = Rs @ Rd
-        for li in range(1, nl[k] + 1):
-            for wi in range(1, nw[k] + 1):
-                if nl[k] * nw[k] == 1:
-                    c = B[k, 0:3].copy()
-                else:
-                    loc = np.array([-L / 2 + L * (2 * li - 1) / nl[k] / 2, -W / 2 + W * (2 * wi - 1) / nw[k] / 2, 0.0])
-                    c = R @ loc + np.array([B[k, 0], B[k, 1], -B[k, 2]])
-                    c[2] = -c[2]
-                row = np.zeros(19)
-                row[0:3] = c
-                row[3] = L / nl[k]
-                row[4] = W / nw[k]
-                row[5:14] = B[k, 5:14]
-                row[14] = B[k, 14] + B[k, 15] * c[2]
-                row[15] = B[k, 16]
-                row[16] = k + 1
-                row[17] = L
-                row[18] = W
-                rows.append(row)
-    return np.array(rows)
+from .inputs import BulkToSegment, LRtoRake  # noqa: E402,F401
+from .inputs import sind as _sind  # noqa: E402
 
 
 def run_setup(seg, rho, kself, rupture_multiple=3.0, vertical_length_scale=0.0):
