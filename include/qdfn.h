@@ -292,6 +292,37 @@ int qdfn_sim_get_history(qdfn_sim_t* sim, int64_t count, double* time, double* d
 int qdfn_sim_get_trace(qdfn_sim_t* sim, int64_t count, double* told, double* dt, double* vmax);
 void qdfn_sim_destroy(qdfn_sim_t* sim);
 
+/* ---- event catalog (scripts/Functions_Plot.jl:430-579) -------------------
+ * get_event_fragments (:430-510): per element, a fragment opens at every
+ * record where the slip rate (D[t]-D[t-1])/(T[t]-T[t-1]) rises above
+ * DispRateCrits and sums dD * FaultArea * ShearModulus while it stays above;
+ * fragments are returned sorted by time (stable).  merge_fragments (:513-579)
+ * joins fragment i to the event of the nearest earlier fragment within
+ * MergeTimeCriteria and MergeDistanceCriteria and gives the events' time and
+ * location (those of their first fragment) and Mw = (log10 M0 - 9.05) / 1.5.
+ * Locations are count x 3 column-major (FaultCenter rows); fault_center is
+ * n x 3 column-major; element / fragment_event indices are 0-based. */
+typedef struct qdfn_catalog qdfn_catalog_t;
+/* Fragments from a run's device-resident History_Time / History_Disp. */
+qdfn_catalog_t* qdfn_catalog_from_sim(qdfn_sim_t* sim, const double* fault_center, const double* fault_area,
+                                      double shear_modulus, double disp_rate_crit);
+/* Fragments from host histories: time[n_rec], disp n_rec x n column-major (leading dimension ld). */
+qdfn_catalog_t* qdfn_catalog_from_history(int64_t n_rec, int64_t n, const double* time, const double* disp,
+                                          int64_t ld, const double* fault_center, const double* fault_area,
+                                          double shear_modulus, double disp_rate_crit);
+/* A catalog holding given (time-sorted) fragments, for qdfn_catalog_merge alone. */
+qdfn_catalog_t* qdfn_catalog_from_fragments(int64_t count, const double* time, const double* location,
+                                            const double* moment);
+/* Fragment count, then (any pointer may be NULL) the arrays, sized by count. */
+int qdfn_catalog_fragments(qdfn_catalog_t* c, int64_t* count, double* time, double* location, double* moment,
+                           int32_t* element);
+/* merge_fragments; sets the event count. */
+int qdfn_catalog_merge(qdfn_catalog_t* c, double merge_time, double merge_distance, int64_t* count);
+/* Merged events (count), and each fragment's event (fragment count entries). */
+int qdfn_catalog_events(qdfn_catalog_t* c, int64_t* count, double* time, double* magnitude, double* location,
+                        double* moment, int32_t* fragment_event);
+void qdfn_catalog_destroy(qdfn_catalog_t* c);
+
 #ifdef __cplusplus
 }
 #endif

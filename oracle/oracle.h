@@ -110,6 +110,13 @@ int64_t orc_build_hierarchy(int64_t n_nodes, const int64_t *node_begin, const in
                             int64_t total_levels, double crit, int64_t loading_begin, int64_t loading_end,
                             int64_t cap, int64_t *range_SR, int64_t *admissible);
 
+/* catalog.c */
+int64_t orc_event_fragments(int64_t n_rec, int64_t n, const double *time, const double *disp, int64_t ld,
+                            const double *center, const double *area, double mu, double crit, int64_t cap,
+                            double *ftime, double *floc, double *fmom, int32_t *felem);
+int64_t orc_merge_fragments(int64_t F, const double *ftime, const double *floc, const double *fmom, double mt,
+                            double md, double *mtime, double *mloc, double *mmom, double *mw, int32_t *fid);
+
 #ifdef __cplusplus
 }
 #endif

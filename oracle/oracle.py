@@ -15,6 +15,8 @@ Function names follow the reference functions they restate:
                         Functions_Solvers.jl:2-380
   interp_external       InterpolateFromStressChange Functions_Solvers.jl:465-540
   sim_dense / sim_hmat  main / main_H              Functions_RSFDFN3DMain_{D,H}.jl
+  event_fragments / merge_fragments
+                        get_event_fragments / merge_fragments  scripts/Functions_Plot.jl:430-579
 """
 import ctypes as C
 import os
@@ -126,6 +128,11 @@ def _declare(L):
         L.orc_sim_hmat.restype = C.c_int
         L.orc_build_hierarchy.argtypes = [I64, PI64, PI64, PI64, PD, PD, I64, D, I64, I64, I64, PI64, PI64]
         L.orc_build_hierarchy.restype = I64
+    if hasattr(L, "orc_event_fragments"):
+        L.orc_event_fragments.argtypes = [I64, I64, PD, PD, I64, PD, PD, D, D, I64, PD, PD, PD, PI32]
+        L.orc_event_fragments.restype = I64
+        L.orc_merge_fragments.argtypes = [I64, PD, PD, PD, D, D, PD, PD, PD, PD, PI32]
+        L.orc_merge_fragments.restype = I64
 
 
 def _f64(a):
@@ -422,3 +429,34 @@ def build_hierarchy(node_info, node_geo, total_levels, crit, loading_range=None,
     if nb < 0:
         raise RuntimeError("build_hierarchy: capacity exceeded")
     return rng[:nb].copy(), adm[:nb].copy()
+
+
+def event_fragments(time, disp, center, area, mu, crit):
+    """get_event_fragments on arrays: time[n_rec], disp n_rec x n, center n x 3,
+    area[n].  Returns (time, location count x 3, moment, element 0-based)."""
+    time = _f64(time)
+    disp = np.asfortranarray(disp, dtype=np.float64)
+    center = np.asfortranarray(center, dtype=np.float64)
+    area = _f64(area)
+    n_rec, n = disp.shape
+    cap = max(1, n * ((n_rec + 1) // 2))
+    ft, fl, fm = np.zeros(cap), np.zeros(3 * cap), np.zeros(cap)
+    fe = np.zeros(cap, dtype=np.int32)
+    F = lib().orc_event_fragments(n_rec, n, _p(time), _p(disp), n_rec, _p(center), _p(area), mu, crit, cap,
+                                  _p(ft), _p(fl), _p(fm), _pi32(fe))
+    assert F >= 0
+    return ft[:F].copy(), fl[:3 * F].reshape(3, F).T.copy(), fm[:F].copy(), fe[:F].copy()
+
+
+def merge_fragments(ftime, floc, fmom, mt, md):
+    """merge_fragments: returns (time, Mw, location E x 3, moment, fragment_event 0-based)."""
+    ftime = _f64(ftime)
+    F = ftime.shape[0]
+    floc = np.asfortranarray(np.asarray(floc, dtype=np.float64).reshape(F, 3))
+    fmom = _f64(fmom)
+    cap = max(F, 1)
+    mt_, ml, mm, mw = np.zeros(cap), np.zeros(3 * cap), np.zeros(cap), np.zeros(cap)
+    fid = np.zeros(cap, dtype=np.int32)
+    E = lib().orc_merge_fragments(F, _p(ftime), _p(floc), _p(fmom), mt, md, _p(mt_), _p(ml), _p(mm), _p(mw),
+                                  _pi32(fid))
+    return mt_[:E].copy(), mw[:E].copy(), ml.reshape(3, cap)[:, :E].T.copy(), mm[:E].copy(), fid[:F].copy()

@@ -72,4 +72,15 @@ __host__ __device__ inline double cos_deg(double x) {
 
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+// Device-resident histories of a run (History_Time / History_Disp): time[n_rec],
+// disp column-major n_rec x n with leading dimension ld.  Valid until the run
+// is destroyed; work on them is ordered after the run's stream.
+struct SimHistoryView {
+    const double* time = nullptr;
+    const double* disp = nullptr;
+    int64_t n_rec = 0, ld = 0, n = 0;
+    hipStream_t stream = nullptr;
+};
+int sim_history_view(qdfn_sim_t* s, SimHistoryView* v);
+
 }  // namespace qdfn

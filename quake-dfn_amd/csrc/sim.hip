@@ -680,6 +680,19 @@ extern "C" int qdfn_sim_get_history(qdfn_sim_t* s, int64_t count, double* time, 
     return QDFN_OK;
 }
 
+int qdfn::sim_history_view(qdfn_sim_t* s, SimHistoryView* v) {
+    if (!s || !v) return fail(QDFN_ERR_ARG, "sim_history_view: null argument");
+    QDFN_HIP(hipStreamSynchronize(s->stream));
+    QDFN_HIP(hipMemcpy(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost));
+    v->time = s->A.h_time;
+    v->disp = s->A.h_d;
+    v->n_rec = s->h_st.rec;
+    v->ld = s->P.record_capacity;
+    v->n = s->P.n;
+    v->stream = s->stream;
+    return QDFN_OK;
+}
+
 extern "C" int qdfn_sim_get_trace(qdfn_sim_t* s, int64_t count, double* told, double* dt, double* vmax) {
     if (!s || count < 0 || count > s->P.trace_capacity) return fail(QDFN_ERR_ARG, "qdfn_sim_get_trace: bad count");
     if (count == 0) return QDFN_OK;

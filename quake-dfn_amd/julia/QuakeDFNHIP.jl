@@ -189,4 +189,50 @@ function HmatSolver(h, x, n; loading_form=false)
     return yS, yN
 end
 
+# ---- event catalog (Functions_Plot.jl:430-579) ------------------------------
+
+function _catalog_fragments(c)
+    cnt = Ref{Int64}(0)
+    check(ccall((:qdfn_catalog_fragments, LIB), Cint, (Ptr{Cvoid}, Ref{Int64}, Ptr{Float64}, Ptr{Float64},
+          Ptr{Float64}, Ptr{Int32}), c, cnt, C_NULL, C_NULL, C_NULL, C_NULL))
+    F = cnt[]; t = zeros(F); loc = zeros(F, 3); m = zeros(F)
+    F > 0 && check(ccall((:qdfn_catalog_fragments, LIB), Cint, (Ptr{Cvoid}, Ref{Int64}, Ptr{Float64},
+                         Ptr{Float64}, Ptr{Float64}, Ptr{Int32}), c, cnt, t, loc, m, C_NULL))
+    return t, loc, m, F
+end
+
+# get_event_fragments with the arrays the reference loads from Result.jld2 / Result_Input.jld2
+# (History_Time, History_Disp, FaultCenter, FaultArea as :453-465 computes it, ShearModulus).
+function get_event_fragments(ResultTime, ResultDisp, FaultCenter, FaultArea, ShearModulus, DispRateCrits)
+    t = Vector{Float64}(vec(ResultTime)); D = Matrix{Float64}(ResultDisp)
+    ctr = Matrix{Float64}(FaultCenter); A = Vector{Float64}(vec(FaultArea))
+    c = GC.@preserve t D ctr A ccall((:qdfn_catalog_from_history, LIB), Ptr{Cvoid},
+        (Int64, Int64, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Float64, Float64),
+        length(t), size(D, 2), t, D, max(size(D, 1), 1), ctr, A, ShearModulus, DispRateCrits)
+    c == C_NULL && check(-1)
+    out = _catalog_fragments(c)
+    ccall((:qdfn_catalog_destroy, LIB), Cvoid, (Ptr{Cvoid},), c)
+    return out
+end
+
+# merge_fragments: same arguments and returns; with no fragments, the reference's placeholders.
+function merge_fragments(EventTime_Fragment, EventLocation_Fragment, EventMoment_Fragment, EventCount_Fragment,
+                         MergeTimeCriteria, MergeDistanceCriteria)
+    F = EventCount_Fragment
+    F == 0 && return [0;], zeros(0), [0 0 0]
+    t = Vector{Float64}(EventTime_Fragment[1:F]); loc = Matrix{Float64}(EventLocation_Fragment[1:F, :])
+    m = Vector{Float64}(EventMoment_Fragment[1:F])
+    c = GC.@preserve t loc m ccall((:qdfn_catalog_from_fragments, LIB), Ptr{Cvoid},
+        (Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), F, t, loc, m)
+    c == C_NULL && check(-1)
+    E = Ref{Int64}(0)
+    check(ccall((:qdfn_catalog_merge, LIB), Cint, (Ptr{Cvoid}, Float64, Float64, Ref{Int64}), c,
+                MergeTimeCriteria, MergeDistanceCriteria, E))
+    tm = zeros(E[]); mw = zeros(E[]); lm = zeros(E[], 3)
+    check(ccall((:qdfn_catalog_events, LIB), Cint, (Ptr{Cvoid}, Ref{Int64}, Ptr{Float64}, Ptr{Float64},
+          Ptr{Float64}, Ptr{Float64}, Ptr{Int32}), c, E, tm, mw, lm, C_NULL, C_NULL))
+    ccall((:qdfn_catalog_destroy, LIB), Cvoid, (Ptr{Cvoid},), c)
+    return tm, mw, lm
+end
+
 end # module

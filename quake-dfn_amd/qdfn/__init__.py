@@ -10,6 +10,7 @@ from ._lib import PREC_FP32, PREC_FP64, QdfnError, device_info, lib
 from .assembly import (BuildMatrixByParts_Even_Rec, BuildMatrixByParts_Even_Tri, Okada_DC3D_Vector,
                        RotVerts_UnitVectors, StiffnessMatrix_ByParts_Calculation_Rec,
                        StiffnessMatrix_ByParts_Calculation_Tri, TDstressHS)
+from .catalog import EventCatalog, fault_area, get_event_fragments, merge_fragments
 from .hmat import HMatrix, hmat_structure, main_H
 from .inputs import BulkToSegment, LRtoRake, ReadBulkInput, ReadSegmentInput
 from .sim import DenseOperator, Simulation, main
@@ -19,4 +20,5 @@ __all__ = ["PREC_FP32", "PREC_FP64", "QdfnError", "device_info", "lib", "Okada_D
            "StiffnessMatrix_ByParts_Calculation_Rec", "BuildMatrixByParts_Even_Rec", "TDstressHS",
            "RotVerts_UnitVectors", "StiffnessMatrix_ByParts_Calculation_Tri", "BuildMatrixByParts_Even_Tri",
            "DenseOperator", "Simulation", "main", "SolveOneTimeStep", "HMatrix", "hmat_structure", "main_H",
-           "BulkToSegment", "LRtoRake", "ReadBulkInput", "ReadSegmentInput"]
+           "BulkToSegment", "LRtoRake", "ReadBulkInput", "ReadSegmentInput",
+           "EventCatalog", "fault_area", "get_event_fragments", "merge_fragments"]

@@ -104,6 +104,13 @@ SIGNATURES = {
     "qdfn_sim_get_history": (C.c_int, [VP, I64, PD, PD, PD, PD, PD, PD]),
     "qdfn_sim_get_trace": (C.c_int, [VP, I64, PD, PD, PD]),
     "qdfn_sim_destroy": (None, [VP]),
+    "qdfn_catalog_from_sim": (VP, [VP, PD, PD, D, D]),
+    "qdfn_catalog_from_history": (VP, [I64, I64, PD, PD, I64, PD, PD, D, D]),
+    "qdfn_catalog_from_fragments": (VP, [I64, PD, PD, PD]),
+    "qdfn_catalog_fragments": (C.c_int, [VP, C.POINTER(I64), PD, PD, PD, PI32]),
+    "qdfn_catalog_merge": (C.c_int, [VP, D, D, C.POINTER(I64)]),
+    "qdfn_catalog_events": (C.c_int, [VP, C.POINTER(I64), PD, PD, PD, PD, PI32]),
+    "qdfn_catalog_destroy": (None, [VP]),
 }
 
 

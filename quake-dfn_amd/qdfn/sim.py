@@ -165,6 +165,12 @@ class Simulation:
         return {"History_Time": t, "History_Dt": dt, "History_V": mats[0], "History_Disp": mats[1],
                 "History_Theta": mats[2], "History_NormalStress": mats[3]}
 
+    def event_catalog(self, FaultCenter, FaultArea, ShearModulus, DispRateCrits=1e-2):
+        """Fragments of get_event_fragments (scripts/Functions_Plot.jl:430-510)
+        from this run's device-resident histories; see qdfn.catalog."""
+        from .catalog import EventCatalog
+        return EventCatalog.from_sim(self, FaultCenter, FaultArea, ShearModulus, DispRateCrits)
+
     def trace(self):
         cnt = self.stats()["n_traced"]
         a, b, c = np.zeros(cnt), np.zeros(cnt), np.zeros(cnt)
