@@ -36,6 +36,22 @@ struct HTile {
     int32_t rb, c0, c1, q;
 };
 
+// phase-1 wavefront task: rows [0, m) of a row-major V panel slice at
+// data_off (leading dimension ld, K columns multiplying x[xoff + c]); row q
+// writes t[out + q], or vpart[part + q] when the panel is split into column
+// segments (part >= 0); lpr lanes per row
+struct VTask {
+    int64_t data_off;
+    int32_t xoff, out, m, K, ld, lpr, part, pad;
+};
+
+// phase-1 finish of one row of a column-split panel:
+// t[out] = sum over q < nseg of vpart[part0 + q * stride]
+struct VFin {
+    int64_t part0;
+    int32_t stride, nseg, out, pad;
+};
+
 }  // namespace qdfn
 
 struct qdfn_hmat {
@@ -46,11 +62,12 @@ struct qdfn_hmat {
     std::vector<double> kdiag_h;          // K_S,ii (0 where no diagonal block)
     qdfn::DevBuf kdiag;
     qdfn::DevBuf pool, idx, t;            // panel entries (V panels then row-group panels), gather indices, t
-    qdfn::DevBuf rbs, tiles, part, multi; // row blocks, tiles, partial sums, split row blocks
+    qdfn::DevBuf rbs, tiles, part, multi; // row blocks, tiles, partial sums, split row blocks (phase 2)
+    qdfn::DevBuf vtasks, vfins, vpart;    // phase-1 wavefront tasks, split-panel finish, partial sums
     qdfn::DevBuf x, yS, yN;               // vectors for qdfn_hmat_matvec
     int64_t n_rb1 = 0, n_rb2 = 0, n_t = 0;
-    int64_t n_tiles1 = 0, n_tiles2 = 0, n_multi1 = 0, n_multi2 = 0;
-    int32_t tile_w = 1024;                // columns per tile
+    int64_t n_tiles1 = 0, n_tiles2 = 0, n_multi1 = 0, n_multi2 = 0, n_vtasks = 0, n_vfins = 0;
+    int32_t tile_w = 4096;                // columns per tile (phase 2)
     int64_t stored_entries = 0;
     int64_t device_bytes = 0;
     int64_t phase_bytes = 0;              // HBM bytes one product reads and writes

@@ -54,6 +54,10 @@ constexpr int kRowsPerGroup = 64;
 constexpr int kPhase2Threads = 1024;
 constexpr int kSlots = 64 / (kPhase2Threads / 64);  // row slots per lane (64 rows over 16 waves)
 constexpr int kChunk = 1024;   // right-hand-side columns staged in LDS per pass (double-buffered)
+constexpr int kVThreads = 256; // phase 1: four independent wavefronts per workgroup
+constexpr int kVSlots = 4;     // phase 1: row slots per lane and step
+constexpr int kVSeg = 1024;    // phase 1: widest column segment of one task
+constexpr int kVTaskEntries = 4096;   // phase 1: target entries per task
 
 int64_t round4(int64_t v) { return (v + 3) / 4 * 4; }
 
@@ -204,18 +208,18 @@ __global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict
             const int32_t c = cb + sub * 4 < lim ? cb + sub * 4 : 0;
             const double zm = cb + sub * 4 < lim ? 1.0 : 0.0;
             const double z0 = zm * z[c], z1 = zm * z[c + 1], z2 = zm * z[c + 2], z3 = zm * z[c + 3];
-            double w[kSlots][4];
+            T w[kSlots][4];
 #pragma unroll
             for (int r = 0; r < kSlots; ++r) {
                 const int row = min(r * step_rows + wave * rpw + rl, R.m - 1);
-                load4<T>(base + (int64_t)row * R.ld + c0 + c, w[r]);
+                load4raw<T>(base + (uint32_t)(row * R.ld + c0 + c), w[r]);
             }
 #pragma unroll
             for (int r = 0; r < kSlots; ++r) {
-                acc[r] = fma(w[r][0], z0, acc[r]);
-                acc[r] = fma(w[r][1], z1, acc[r]);
-                acc[r] = fma(w[r][2], z2, acc[r]);
-                acc[r] = fma(w[r][3], z3, acc[r]);
+                acc[r] = fma((double)w[r][0], z0, acc[r]);
+                acc[r] = fma((double)w[r][1], z1, acc[r]);
+                acc[r] = fma((double)w[r][2], z2, acc[r]);
+                acc[r] = fma((double)w[r][3], z3, acc[r]);
             }
         }
         if (more) {
@@ -238,6 +242,113 @@ __global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict
         }
     }
     if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
+}
+
+// Phase 1 (t = V x over the V panels): one wavefront per task, a task being
+// rows [0, m) of one panel over a column segment of at most kVSeg columns.
+// The task is walked as a flat sequence of steps (row group, column step),
+// software-pipelined two deep: the loads of step i + 1 (panel entries and the
+// x values they multiply) are issued before step i is accumulated, so a
+// wavefront always has a step's loads in flight.  A row group is kVSlots rows
+// per lane; each row is read by lpr lanes with 16-byte loads; loads are
+// branch-free (clamped, masked).  Wavefronts never synchronise; x is read
+// straight from global memory (L2-resident, shared by the panel's tasks).
+// Tasks of panels wider than kVSeg write partial sums (vpart) that
+// k_vfinish adds in segment order.
+template <class T>
+struct VStep {
+    T w[kVSlots][4];
+    double z[4];
+};
+
+template <class T>
+__global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ tasks, int32_t ntasks,
+                                                      const T* __restrict__ pool, const double* __restrict__ x0,
+                                                      double* __restrict__ t, double* __restrict__ vpart,
+                                                      CouplingGate g) {
+    if (g.gate && *g.gate == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
+    if (ti >= ntasks) return;
+    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const VTask tk = tasks[ti];
+    const int lpr = tk.lpr, rpw = 64 / lpr, sub = lane & (lpr - 1), rl = lane / lpr;
+    const int32_t K = tk.K, K4 = (K + 3) / 4 * 4;
+    const int32_t cstep = 4 * lpr, nc = (K4 + cstep - 1) / cstep;  // column steps per row group
+    const int32_t gr = kVSlots * rpw, ng = (tk.m + gr - 1) / gr;     // rows per group, groups
+    const int32_t total = nc * ng;
+    const T* base = pool + tk.data_off;
+    const double* xs = x + tk.xoff;
+    auto load = [&](int32_t it, VStep<T>& S) {
+        const int32_t gi = it / nc, ci = it - gi * nc;
+        const int32_t c = ci * cstep + 4 * sub;
+        const int32_t cc = c < K4 ? c : 0;
+        // masked in the integer domain so that the load stays unconditional (a
+        // select lets the compiler move the load into a branch that waits for
+        // every load in flight)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long long bits = __double_as_longlong(xs[min(cc + q, K - 1)]);
+            S.z[q] = __longlong_as_double((c < K4 && cc + q < K) ? bits : 0LL);
+        }
+#pragma unroll
+        for (int r = 0; r < kVSlots; ++r) {
+            const int32_t row = min(gi * gr + rl + r * rpw, tk.m - 1);
+            load4raw<T>(base + (uint32_t)(row * tk.ld + cc), S.w[r]);
+        }
+    };
+    double acc[kVSlots];
+#pragma unroll
+    for (int r = 0; r < kVSlots; ++r) acc[r] = 0.0;
+    // accumulate step it; at a group's last column step, reduce over the row's
+    // lanes, write and restart
+    auto consume = [&](int32_t it, const VStep<T>& S) {
+        const int32_t gi = it / nc, ci = it - gi * nc;
+#pragma unroll
+        for (int r = 0; r < kVSlots; ++r) {
+            acc[r] = fma((double)S.w[r][0], S.z[0], acc[r]);
+            acc[r] = fma((double)S.w[r][1], S.z[1], acc[r]);
+            acc[r] = fma((double)S.w[r][2], S.z[2], acc[r]);
+            acc[r] = fma((double)S.w[r][3], S.z[3], acc[r]);
+        }
+        if (ci == nc - 1) {
+#pragma unroll
+            for (int r = 0; r < kVSlots; ++r) {
+                double v = acc[r];
+                for (int o = lpr >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                const int32_t row = gi * gr + rl + r * rpw;
+                if (sub == 0 && row < tk.m) {
+                    if (tk.part < 0)
+                        t[tk.out + row] = v;
+                    else
+                        vpart[tk.part + row] = v;
+                }
+                acc[r] = 0.0;
+            }
+        }
+    };
+    VStep<T> A, B;
+    load(0, A);
+    for (int32_t it = 0; it < total; it += 2) {
+        load(min(it + 1, total - 1), B);
+        consume(it, A);
+        load(min(it + 2, total - 1), A);
+        if (it + 1 < total) consume(it + 1, B);
+    }
+}
+
+// Phase-1 finish: t rows of column-split panels, segments added in order.
+__global__ __launch_bounds__(256) void k_vfinish(const VFin* __restrict__ fins, int32_t nfin,
+                                                 const double* __restrict__ vpart, double* __restrict__ t,
+                                                 CouplingGate g) {
+    if (g.gate && *g.gate == 0) return;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    // fins are per row: i indexes a row of some split row chunk
+    if (i >= nfin) return;
+    const VFin f = fins[i];
+    double s = 0.0;
+    for (int32_t q = 0; q < f.nseg; ++q) s += vpart[f.part0 + (int64_t)q * f.stride];
+    t[f.out] = s;
 }
 
 // finish of the split row blocks: one 64-lane workgroup each, one lane per row,
@@ -420,6 +531,9 @@ static int hmat_layout(qdfn_hmat* h) {
     }
     std::vector<HTile> tiles;
     std::vector<int32_t> multi;
+    std::vector<VTask> vtasks;
+    std::vector<VFin> vfins;
+    int64_t v_entries = 0, v_cols = 0, vpart_total = 0;
     int64_t part_total = 0;
     auto add_rowblock = [&](HRowBlock R) {
         const int32_t t0 = (int32_t)tiles.size();
@@ -478,22 +592,45 @@ static int hmat_layout(qdfn_hmat* h) {
                 for (int64_t q = 0; q < r; ++q, ++row)
                     for (int64_t c = 0; c < k; ++c) pool[off + row * ld + c] = V[q + c * r];
             }
-            // narrow panels: several rows per wavefront load, so taller row blocks
-            // (the kernel covers 64 * (64 / LPR) rows, LPR from the first chunk)
+            // wavefront tasks: column segments of at most kVSeg columns, row chunks of
+            // about kVTaskEntries entries (whole row groups); narrow panels read
+            // several rows per wavefront load
+            const int64_t nseg = (k + kVSeg - 1) / kVSeg;
+            const int64_t segw = round4((k + nseg - 1) / nseg);  // 16-byte aligned segment starts
             int lpr = 1;
-            while (lpr < 64 && 4 * lpr < std::min<int64_t>(ld, kChunk)) lpr <<= 1;
-            const int64_t rb_rows = (int64_t)kRowsPerGroup * (64 / lpr);
-            for (int64_t r0 = 0; r0 < rows; r0 += rb_rows) {
-                HRowBlock R;
-                R.data_off = off + r0 * ld;
-                R.zoff = s0;
-                R.out = tbase + r0;
-                R.m = (int32_t)std::min<int64_t>(rb_rows, rows - r0);
-                R.K = (int32_t)k;
-                R.ld = (int32_t)ld;
-                R.kind = 0;
-                add_rowblock(R);
+            while (lpr < 64 && 4 * lpr < round4(segw)) lpr <<= 1;
+            const int64_t gr = (int64_t)kVSlots * (64 / lpr);
+            const int64_t chunk = std::max<int64_t>(gr, kVTaskEntries / std::max<int64_t>(segw, 1) / gr * gr);
+            for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+                const int64_t mr = std::min<int64_t>(chunk, rows - r0);
+                for (int64_t sg = 0; sg < nseg; ++sg) {
+                    const int64_t cs = sg * segw, ce = std::min<int64_t>(k, cs + segw);
+                    if (cs >= ce) return fail(QDFN_ERR_STATE, "H-matrix: empty phase-1 segment");
+                    VTask T;
+                    T.data_off = off + r0 * ld + cs;
+                    T.xoff = (int32_t)(s0 + cs);
+                    T.out = (int32_t)(tbase + r0);
+                    T.m = (int32_t)mr;
+                    T.K = (int32_t)(ce - cs);
+                    T.ld = (int32_t)ld;
+                    T.lpr = lpr;
+                    T.part = nseg > 1 ? (int32_t)(vpart_total + sg * mr) : -1;
+                    vtasks.push_back(T);
+                }
+                if (nseg > 1) {
+                    for (int64_t q = 0; q < mr; ++q) {
+                        VFin F;
+                        F.part0 = vpart_total + q;
+                        F.stride = (int32_t)mr;
+                        F.nseg = (int32_t)nseg;
+                        F.out = (int32_t)(tbase + r0 + q);
+                        vfins.push_back(F);
+                    }
+                    vpart_total += nseg * mr;
+                }
             }
+            v_entries += rows * k;
+            v_cols += k;
             tbase += rows;
         }
         h->n_t = tbase;
@@ -569,6 +706,20 @@ static int hmat_layout(qdfn_hmat* h) {
     auto by_size = [&](const HTile& a, const HTile& b) { return tile_bytes(a) > tile_bytes(b); };
     std::stable_sort(tiles.begin(), tiles.begin() + h->n_tiles1, by_size);
     std::stable_sort(tiles.begin() + h->n_tiles1, tiles.end(), by_size);
+    std::stable_sort(vtasks.begin(), vtasks.end(), [](const VTask& a, const VTask& b) {
+        return (int64_t)a.m * a.ld > (int64_t)b.m * b.ld;
+    });
+    h->n_vtasks = (int64_t)vtasks.size();
+    h->n_vfins = (int64_t)vfins.size();
+    if (vpart_total > 0x7fffffffLL) return fail(QDFN_ERR_ARG, "H-matrix: phase-1 partial space exceeds int32");
+    if (getenv("QDFN_HMAT_VERBOSE") && !vtasks.empty()) {
+        std::vector<int64_t> sz;
+        for (const VTask& T : vtasks) sz.push_back((int64_t)T.m * T.K);
+        std::sort(sz.begin(), sz.end());
+        fprintf(stderr, "[hmat] phase 1: %zu wavefront tasks, %lld entries, per task min %lld median %lld p90 %lld "
+                "max %lld\n", sz.size(), (long long)v_entries, (long long)sz.front(), (long long)sz[sz.size() / 2],
+                (long long)sz[sz.size() * 9 / 10], (long long)sz.back());
+    }
     if (getenv("QDFN_HMAT_VERBOSE")) {
         for (int ph = 0; ph < 2; ++ph) {
             const size_t b0 = ph ? h->n_tiles1 : 0, b1 = ph ? tiles.size() : h->n_tiles1;
@@ -603,6 +754,13 @@ static int hmat_layout(qdfn_hmat* h) {
     QDFN_HIP(hipMemcpy(h->rbs.p, rbs.data(), sizeof(HRowBlock) * rbs.size(), hipMemcpyHostToDevice));
     QDFN_HIP(h->tiles.alloc(sizeof(HTile) * tiles.size()));
     QDFN_HIP(hipMemcpy(h->tiles.p, tiles.data(), sizeof(HTile) * tiles.size(), hipMemcpyHostToDevice));
+    QDFN_HIP(h->vtasks.alloc(sizeof(VTask) * std::max<size_t>(vtasks.size(), 1)));
+    if (!vtasks.empty())
+        QDFN_HIP(hipMemcpy(h->vtasks.p, vtasks.data(), sizeof(VTask) * vtasks.size(), hipMemcpyHostToDevice));
+    QDFN_HIP(h->vfins.alloc(sizeof(VFin) * std::max<size_t>(vfins.size(), 1)));
+    if (!vfins.empty())
+        QDFN_HIP(hipMemcpy(h->vfins.p, vfins.data(), sizeof(VFin) * vfins.size(), hipMemcpyHostToDevice));
+    QDFN_HIP(h->vpart.alloc(sizeof(double) * std::max<int64_t>(vpart_total, 1)));
     QDFN_HIP(h->part.alloc(sizeof(double) * std::max<int64_t>(part_total, 1)));
     QDFN_HIP(h->multi.alloc(sizeof(int32_t) * std::max<size_t>(multi.size(), 1)));
     if (!multi.empty())
@@ -617,7 +775,9 @@ static int hmat_layout(qdfn_hmat* h) {
     QDFN_HIP(h->yN.alloc(sizeof(double) * n));
     // HBM bytes of one product: panel entries, indices, descriptors, t (write + read), x, yS, yN
     h->phase_bytes = (int64_t)(es * pool.size() + sizeof(int32_t) * idx.size() + sizeof(HRowBlock) * rbs.size() +
-                               sizeof(HTile) * tiles.size() + 16 * part_total + 16 * h->n_t + 8 * 3 * n);
+                               sizeof(HTile) * tiles.size() + sizeof(VTask) * vtasks.size() +
+                               sizeof(VFin) * vfins.size() + 16 * part_total + 16 * vpart_total + 16 * h->n_t +
+                               8 * 3 * n);
     h->device_bytes = h->phase_bytes;
     // algorithmic bytes of each phase's main kernel: stored entries (unpadded),
     // index list (phase 2), gathered right-hand side, outputs
@@ -634,6 +794,10 @@ static int hmat_layout(qdfn_hmat* h) {
             o2 += R.m;
         }
     }
+    // phase 1: V entries, each panel's x slice once, t
+    e1 = v_entries;
+    k1 = v_cols;
+    o1 = h->n_t;
     h->alg_bytes1 = (int64_t)es * e1 + 8 * k1 + 8 * o1;
     h->alg_bytes2 = (int64_t)es * e2 + 12 * k2 + 8 * o2;
     return QDFN_OK;
@@ -649,19 +813,20 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
     double* t = h->t.as<double>();
     const bool f32 = h->prec == QDFN_PREC_FP32;
     if (ev0) QDFN_HIP(hipEventRecord(ev0, s));
-    if (h->n_tiles1 > 0) {
+    if (h->n_vtasks > 0) {
+        const unsigned grid = (unsigned)((h->n_vtasks + kVThreads / 64 - 1) / (kVThreads / 64));
+        const VTask* vt = h->vtasks.as<VTask>();
+        double* vp = h->vpart.as<double>();
         if (f32)
-            hipLaunchKernelGGL((k_hrow<float, false>), dim3((unsigned)h->n_tiles1), dim3(kPhase2Threads), 0, s, tiles,
-                               rbs, part, h->pool.as<float>(), h->idx.as<int32_t>(), d_x, t, h->n, kd, form, d_yS,
-                               d_yN, 0, g);
+            hipLaunchKernelGGL((k_vpanel<float>), dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
+                               h->pool.as<float>(), d_x, t, vp, g);
         else
-            hipLaunchKernelGGL((k_hrow<double, false>), dim3((unsigned)h->n_tiles1), dim3(kPhase2Threads), 0, s,
-                               tiles, rbs, part, h->pool.as<double>(), h->idx.as<int32_t>(), d_x, t, h->n, kd, form,
-                               d_yS, d_yN, 0, g);
+            hipLaunchKernelGGL((k_vpanel<double>), dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
+                               h->pool.as<double>(), d_x, t, vp, g);
         QDFN_HIP(hipGetLastError());
-        if (h->n_multi1 > 0) {
-            hipLaunchKernelGGL(k_hfinish, dim3((unsigned)h->n_multi1), dim3(64), 0, s, multi, rbs, part, d_x, kd, form,
-                               t, d_yS, d_yN, 0, g);
+        if (h->n_vfins > 0) {
+            hipLaunchKernelGGL(k_vfinish, dim3((unsigned)((h->n_vfins + 255) / 256)), dim3(256), 0, s,
+                               h->vfins.as<VFin>(), (int32_t)h->n_vfins, vp, t, g);
             QDFN_HIP(hipGetLastError());
         }
     }
