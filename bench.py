@@ -175,6 +175,41 @@ def network(qd, args):
             "event_time_rel_err": err, "steps_timed": args.network_steps}
 
 
+def ensemble(qd, args):
+    """256 independent planar faults of 500-4,000 elements (FP32 operators, one
+    per distinct geometry) advanced together by qdfn_ensemble_run for up to
+    20 simulated years each (step cap args.ensemble_steps per member)."""
+    from qdfn import configs
+    from qdfn.ensemble import problem_for_member
+    t0 = time.perf_counter()
+    members = configs.ensemble_members()
+    e = qd.Ensemble.from_members(members, problem_for_member, max_rhs=0, record_step=10)
+    info = e.info()
+    log(f"[bench] ensemble: {len(members)} members, {sum(m['seg'].shape[0] for m in members)} elements, "
+        f"{info['groups']} operator groups (max_rhs {info['max_rhs']}, autotuned), built in "
+        f"{time.perf_counter() - t0:.1f}s")
+    t_end = 20 * configs.SEC_PER_YEAR
+    s0 = e.stats()
+    t0 = time.perf_counter()
+    e.run(args.ensemble_steps, t_end=t_end, graph_attempts=16)
+    el = time.perf_counter() - t0
+    s1 = e.stats()
+    rhs = sum(b["rhs_evals"] - a["rhs_evals"] for a, b in zip(s0, s1))
+    steps = sum(b["steps"] - a["steps"] for a, b in zip(s0, s1))
+    years = sum(b["t"] - a["t"] for a, b in zip(s0, s1)) / configs.SEC_PER_YEAR
+    done = sum(1 for b in s1 if b["t"] >= t_end)
+    log(f"[bench] ensemble: {rhs} RHS evals ({steps} steps) in {el:.2f}s = {rhs / el:.0f}/s; "
+        f"{done}/{len(members)} members reached 20 years")
+    out = {"workload": "ensemble_256", "members": len(members),
+           "elements": int(sum(m["seg"].shape[0] for m in members)), "operator_groups": info["groups"],
+           "max_rhs": info["max_rhs"], "matrix_bytes_per_sweep": info["bytes_per_sweep"],
+           "rhs_evals_per_s": round(rhs / el, 1), "steps": steps, "elapsed_s": round(el, 3),
+           "sim_years_per_hour": round(years / (el / 3600.0), 1),
+           "members_at_20_years": done, "step_cap_per_member": args.ensemble_steps, "dtype": "f64 state, fp32 operator"}
+    e.close()
+    return out
+
+
 def cpu_baseline(h32, st, prob_h, n_steps):
     """The oracle's FP64 main_H (C restatement of HmatSolver_Pararllel with one
     thread and of the loop) on the same stored blocks, for a bounded number of
@@ -200,6 +235,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--network-steps", type=int, default=6000)
     ap.add_argument("--no-network", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=20000, help="accepted-step cap per ensemble member")
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU layout is "
@@ -218,6 +255,7 @@ def main():
     bytes2 = rf["info"]["phase2_bytes"]
     achieved = bytes2 / (mixed["coupling_ms"] * 1e-3) / 1e9
     net = None if args.no_network else network(qdfn, args)
+    ens = None if args.no_ensemble else ensemble(qdfn, args)
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: oracle FP64 main_H, {args.cpu_steps} steps")
@@ -265,6 +303,7 @@ def main():
                                  "fp32_dense": divergence(res["fp32_dense"]["trace"], plain["trace"])},
         "operator_rel_err": rf["acc"],
         "network_8192": net,
+        "ensemble_256": ens,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic()},
         "cpu_baseline": cpu,

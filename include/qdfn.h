@@ -292,6 +292,26 @@ int qdfn_sim_get_history(qdfn_sim_t* sim, int64_t count, double* time, double* d
 int qdfn_sim_get_trace(qdfn_sim_t* sim, int64_t count, double* told, double* dt, double* vmax);
 void qdfn_sim_destroy(qdfn_sim_t* sim);
 
+/* ---- batched ensemble -----------------------------------------------------
+ * Many independent runs of main() (scripts/Functions_RSFDFN3DMain_D.jl:2-324,
+ * the loop QUAKEDFN_RunARG.jl:247-262 starts once per input file), advanced
+ * together: one attempt of every run per batched launch sequence, attempts
+ * replayed from one captured HIP graph.  Runs must be dense runs with FP32
+ * operators; runs created on the same operator handle (same geometry) are
+ * read as one multi-right-hand-side product.  Each run's trajectory is
+ * bitwise the one it has when run alone.  The runs are borrowed (they must
+ * outlive the ensemble) and keep their getters (stats, state, history, trace,
+ * catalog); do not advance them with qdfn_sim_run while in an ensemble.
+ * max_rhs: runs per operator pass (1..4), 0 = pick by timing (autotune). */
+typedef struct qdfn_ensemble qdfn_ensemble_t;
+qdfn_ensemble_t* qdfn_ensemble_create(int64_t m, qdfn_sim_t* const* sims, int32_t max_rhs);
+/* Advance every run by up to n_steps accepted steps, stopping a run early
+ * once its simulated time reaches t_end (t_end = 0: no time limit). */
+int qdfn_ensemble_run(qdfn_ensemble_t* e, int64_t n_steps, double t_end, int32_t graph_attempts);
+/* Operator groups, runs per pass, matrix bytes of a sweep in which every run needs its coupling. */
+int qdfn_ensemble_info(qdfn_ensemble_t* e, int64_t* groups, int32_t* max_rhs, int64_t* bytes_per_sweep);
+void qdfn_ensemble_destroy(qdfn_ensemble_t* e);
+
 /* ---- event catalog (scripts/Functions_Plot.jl:430-579) -------------------
  * get_event_fragments (:430-510): per element, a fragment opens at every
  * record where the slip rate (D[t]-D[t-1])/(T[t]-T[t-1]) rises above

@@ -1,0 +1,40 @@
+// Host side of a device-resident run (qdfn_sim_t), shared by sim.hip and the
+// batched ensemble (ensemble.hip).
+#pragma once
+#include "dense.h"
+#include "hmat.h"
+#include "sim_device.h"
+
+// A captured sequence of attempts and its executable graph.
+struct GraphSlot {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int attempts = 0;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        attempts = 0;
+    }
+};
+
+// One run: its operator (borrowed), parameters, stream, device arrays and
+// state block.  A run taken into an ensemble is advanced by the ensemble's
+// launches; its getters keep working.
+struct qdfn_sim {
+    qdfn_dense* op = nullptr;   // dense coupling (main) ...
+    qdfn_hmat* hop = nullptr;   // ... or H-matrix coupling (main_H)
+    qdfn_sim_params P{};
+    hipStream_t stream = nullptr;
+    qdfn::DevBuf consts, state2, single, ints, cand, ext, hist, trace, dstate, ranlog, parts;
+    qdfn::SimArrays A{};
+    qdfn::SimState* d_st = nullptr;
+    qdfn::SimState h_st{};
+    bool started = false;
+    GraphSlot graph;  // captured attempt sequence
+    // timing of coupling kernels (mode 1)
+    double coup_ms_sum = 0.0;
+    int64_t coup_count = 0;
+};
+

@@ -1,0 +1,304 @@
+// Device side of the earthquake-cycle loop (main() of
+// scripts/Functions_RSFDFN3DMain_D.jl:2-324, main_H's identical control logic
+// in scripts/Functions_RSFDFN3DMain_H.jl:180-383): run state, per-element
+// attempt and the retry-or-accept decision, shared by the single-run kernels
+// (sim.hip) and the batched ensemble kernels (ensemble.hip).
+#pragma once
+#include <math.h>
+
+#include "friction_device.h"
+#include "qdfn_internal.h"
+
+namespace qdfn {
+
+struct SimState {
+    double dt, told;
+    double t_end;           // > 0: the run also stops once TOld reaches it
+    int64_t step, target, rhs_evals, retry_passes, stalls, rec, ntrace;
+    int32_t need_coupling;  // gate for the coupling kernels
+    int32_t iteration;      // attempt number within the current step (1-based)
+    int32_t status;         // QDFN_OK or error code
+    int32_t done;           // 1: requested steps reached or error
+    int32_t parity;         // buffer holding the "Old" state
+    int32_t pad;
+};
+
+// Per-workgroup partial results of one attempt.
+struct BlockPart {
+    int32_t ok, any4, found, pad;
+    double mindt, vmax, vmax_next, dtmin_next;
+};
+
+// Device arrays of a run.  Double-buffered state arrays are [2][n].
+struct SimArrays {
+    int64_t n, n_loading;
+    // constants
+    const double *a, *b, *dc, *f0, *kself, *omega, *mass, *tau0, *sigma0, *lrate;
+    const int8_t* cand;  // Dt-control candidate set (unstable patch or all non-loading)
+    // double-buffered state
+    double *V, *D, *TH, *F, *SIG;
+    // single
+    double *accel, *tl, *sigprod;
+    int32_t* sw;  // SolverSwitch to use if the next attempt retries this step
+    int32_t* ran_log;  // per-attempt "coupling ran" marks of a timed batch
+    BlockPart* parts;
+    unsigned int* arrive;
+    // external stress (nt == 0: none)
+    int64_t nt;
+    const double *ext_t, *ext_n, *ext_s;
+    // histories
+    int64_t rec_cap, trace_cap, record_step;
+    double *h_time, *h_dt, *h_v, *h_d, *h_th, *h_sig;
+    double *t_told, *t_dt, *t_vmax;
+    // parameters
+    double mu, rho, switch_v, dt_cut, rupture_dt, maximum_dt, min_sig, alpha_evo, conv, v0;
+    int32_t dr, planar, ext_at_told, pad;
+};
+
+// InterpolateFromStressChange (Functions_Solvers.jl:465-540) for one element.
+__device__ inline void ext_interp(const SimArrays& A, double time, int64_t i, double& dsn, double& dtau) {
+    int64_t idx = 0;
+    for (int64_t k = 1; k <= A.nt; ++k)
+        if (time >= A.ext_t[k - 1]) idx = k;
+    const int64_t nt = A.nt;
+    if (idx == 0) {
+        const double td = time / A.ext_t[0];
+        dsn = td * (A.ext_n[0 + i * nt] - 0.0) + 0.0;
+        dtau = td * (A.ext_s[0 + i * nt] - 0.0) + 0.0;
+    } else if (idx == nt) {
+        dsn = A.ext_n[(idx - 1) + i * nt];
+        dtau = A.ext_s[(idx - 1) + i * nt];
+    } else {
+        const double td = time - A.ext_t[idx - 1];
+        const double span = A.ext_t[idx] - A.ext_t[idx - 1];
+        const double ln = A.ext_n[(idx - 1) + i * nt], un = A.ext_n[idx + i * nt];
+        const double ls = A.ext_s[(idx - 1) + i * nt], us = A.ext_s[idx + i * nt];
+        dsn = td / span * (un - ln) + ln;
+        dtau = td / span * (us - ls) + ls;
+    }
+}
+
+constexpr int kStepBlock = 256;
+constexpr int kSetupBlock = 1024;
+constexpr int kTimedBatch = 64;
+
+template <int NT, typename T, typename Op>
+__device__ inline T block_reduce(T v, Op op, T* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off, 64));
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    T r = sh[0];
+    for (int k = 1; k < NT / 64; ++k) r = op(r, sh[k]);
+    __syncthreads();
+    return r;
+}
+
+struct MinOp { __device__ double operator()(double a, double b) const { return b < a ? b : a; } };
+struct MaxOp { __device__ double operator()(double a, double b) const { return b > a ? b : a; } };
+struct AndOp { __device__ int operator()(int a, int b) const { return a & b; } };
+struct OrOp { __device__ int operator()(int a, int b) const { return a | b; } };
+
+// DtRef / Dt selection of main() (:171-187); returns the new Dt.
+__device__ inline double select_dt(const SimArrays& A, double dt_prev, double vmax, double dtmin) {
+    double dtref;
+    if (vmax > A.switch_v) dtref = A.rupture_dt;
+    else if (dtmin > A.maximum_dt) dtref = A.maximum_dt;
+    else dtref = dtmin;
+    if (dtref <= A.rupture_dt) return dtref;
+    if (dt_prev < dtref / 1.2) return dt_prev * 1.2;
+    return dtref;
+}
+
+// Run-start step setup (:148-187) from the stored Old state and Accel; one
+// workgroup of kSetupBlock threads.
+__device__ __forceinline__ void setup_body(const SimArrays& A, SimState* __restrict__ st) {
+    __shared__ double shd[kSetupBlock / 64];
+    __shared__ int shi[kSetupBlock / 64];
+    if (st->done) return;
+    const int64_t n = A.n;
+    const int64_t o = (int64_t)st->parity * n;
+    double vmax = -INFINITY, dmin = INFINITY;
+    int found = 0;
+    for (int64_t i = threadIdx.x; i < n; i += kSetupBlock) {
+        const double v = A.V[o + i];
+        if (A.cand[i]) {
+            vmax = v > vmax ? v : vmax;
+            const double r = fabs(v / A.accel[i]);
+            if (!(isnan(r) || r == 0.0)) {
+                dmin = r < dmin ? r : dmin;
+                found = 1;
+            }
+        }
+    }
+    vmax = block_reduce<kSetupBlock>(vmax, MaxOp(), shd);
+    dmin = block_reduce<kSetupBlock>(dmin, MinOp(), shd);
+    found = block_reduce<kSetupBlock>(found, OrOp(), shi);
+    if (threadIdx.x == 0) {
+        if (!found) {
+            st->status = QDFN_ERR_EMPTY_DTMIN;
+            st->done = 1;
+            st->need_coupling = 0;
+        } else {
+            st->dt = select_dt(A, st->dt, vmax, dmin / A.dt_cut);
+            st->iteration = 1;
+            st->need_coupling = 1;
+            st->rhs_evals += 1;
+        }
+    }
+}
+
+// One attempt of SolveOneTimeStep (Functions_Solvers.jl:334-380) with the
+// effective normal stress of main() (:212-226), plus the step decision, for
+// workgroup blk of the nblk (kStepBlock threads each) that cover the run.
+__device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restrict__ st, unsigned blk, unsigned nblk) {
+    __shared__ double shd[kStepBlock / 64];
+    __shared__ int shi[kStepBlock / 64];
+    __shared__ int s_last;
+    if (st->done) return;
+    const int64_t n = A.n;
+    const int64_t i = (int64_t)blk * kStepBlock + threadIdx.x;
+    const int p = st->parity;
+    const int64_t o = (int64_t)p * n, w = (int64_t)(1 - p) * n;
+    const double dt = st->dt, told = st->told;
+    const int fresh = st->iteration == 1;
+    const int64_t step = st->step;
+    const bool record = A.record_step > 0 && (step + 1) % A.record_step == 0 && st->rec < A.rec_cap;
+    const int64_t rec = st->rec;
+
+    int ok = 1, any4 = 0, found = 0;
+    double mindt = INFINITY, vmax = -INFINITY, vmax_next = -INFINITY, dtmin_next = INFINITY;
+    if (i < n) {
+        const double v_old = A.V[o + i];
+        double dsn = 0.0, dtau = 0.0;
+        if (A.nt > 0) ext_interp(A, A.ext_at_told ? told : told + dt, i, dsn, dtau);
+        double sig = (A.planar ? 0.0 : A.sigprod[i]) + A.sigma0[i] + dsn;
+        if (sig < A.min_sig) sig = A.min_sig;
+        double vn;
+        if (i < n - A.n_loading) {
+            const int sw = fresh ? (v_old > A.switch_v ? 1 : 0) : A.sw[i];
+            ElemIn e;
+            e.conv = A.conv; e.d_old = A.D[o + i]; e.f_old = A.F[o + i]; e.th_old = A.TH[o + i];
+            e.v_old = v_old; e.f0 = A.f0[i]; e.a = A.a[i]; e.b = A.b[i]; e.dc = A.dc[i]; e.v0 = A.v0;
+            e.k = A.kself[i]; e.omega = A.omega[i]; e.mass = A.mass[i]; e.mu = A.mu; e.sig = sig;
+            e.tl = 0.0 + A.tl[i]; e.rho = A.rho; e.dtau = dtau; e.switch_v = A.switch_v; e.alpha_evo = A.alpha_evo;
+            e.sig_old = A.SIG[o + i]; e.tau0 = A.tau0[i]; e.dr = A.dr;
+            const ElemOut r = (sw == 1) ? solve_high_v(e, dt) : solve_low_v(e, dt);
+            A.V[w + i] = r.v; A.F[w + i] = r.f; A.D[w + i] = r.d; A.TH[w + i] = r.th; A.SIG[w + i] = r.sig;
+            ok = r.ok;
+            any4 = r.instab == 4;
+            mindt = r.dt;
+            // SolverSwitch if the step is retried (:242-249)
+            A.sw[i] = r.instab == 3 ? 1 : (r.instab == 4 ? 0 : sw);
+            if (r.stall) atomicAdd((unsigned long long*)&st->stalls, (unsigned long long)r.stall);
+            vn = r.v;
+            vmax = vn;
+            if (record) {
+                A.h_v[rec + i * A.rec_cap] = vn;
+                A.h_d[rec + i * A.rec_cap] = r.d;
+                A.h_th[rec + i * A.rec_cap] = r.th;
+                A.h_sig[rec + i * A.rec_cap] = r.sig;
+            }
+        } else {
+            vn = A.lrate[i];
+            const double d = A.D[o + i] + vn * dt;
+            A.V[w + i] = vn; A.F[w + i] = A.F[o + i]; A.D[w + i] = d; A.TH[w + i] = A.TH[o + i];
+            A.SIG[w + i] = 1e10;
+            mindt = dt;
+            if (record) {
+                A.h_v[rec + i * A.rec_cap] = vn;
+                A.h_d[rec + i * A.rec_cap] = d;
+                A.h_th[rec + i * A.rec_cap] = A.TH[o + i];
+                A.h_sig[rec + i * A.rec_cap] = 1e10;
+            }
+        }
+        // quantities of the next step's setup, valid if this attempt is accepted
+        const double acc = (vn - v_old) / dt;
+        A.accel[i] = acc;
+        if (A.cand[i]) {
+            vmax_next = vn;
+            const double r = fabs(vn / acc);
+            if (!(isnan(r) || r == 0.0)) {
+                dtmin_next = r;
+                found = 1;
+            }
+        }
+    }
+    ok = block_reduce<kStepBlock>(ok, AndOp(), shi);
+    any4 = block_reduce<kStepBlock>(any4, OrOp(), shi);
+    found = block_reduce<kStepBlock>(found, OrOp(), shi);
+    mindt = block_reduce<kStepBlock>(mindt, MinOp(), shd);
+    vmax = block_reduce<kStepBlock>(vmax, MaxOp(), shd);
+    vmax_next = block_reduce<kStepBlock>(vmax_next, MaxOp(), shd);
+    dtmin_next = block_reduce<kStepBlock>(dtmin_next, MinOp(), shd);
+    if (threadIdx.x == 0) {
+        BlockPart& bp = A.parts[blk];
+        bp.ok = ok; bp.any4 = any4; bp.found = found;
+        bp.mindt = mindt; bp.vmax = vmax; bp.vmax_next = vmax_next; bp.dtmin_next = dtmin_next;
+        __threadfence();
+        const unsigned prev = atomicAdd(A.arrive, 1u);
+        s_last = prev == nblk - 1;
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    // ---- last workgroup: the decision of main() ----
+    __threadfence();
+    *A.arrive = 0u;
+    int OK = 1, ANY4 = 0, FOUND = 0;
+    double MINDT = INFINITY, VMAX = -INFINITY, VNEXT = -INFINITY, DNEXT = INFINITY;
+    for (unsigned k = 0; k < nblk; ++k) {
+        const BlockPart bp = A.parts[k];
+        OK &= bp.ok; ANY4 |= bp.any4; FOUND |= bp.found;
+        MINDT = bp.mindt < MINDT ? bp.mindt : MINDT;
+        VMAX = bp.vmax > VMAX ? bp.vmax : VMAX;
+        VNEXT = bp.vmax_next > VNEXT ? bp.vmax_next : VNEXT;
+        DNEXT = bp.dtmin_next < DNEXT ? bp.dtmin_next : DNEXT;
+    }
+    if (!OK) {  // retry the step with the reduced Dt (:240-258)
+        st->dt = ANY4 ? 1e-3 : MINDT;
+        st->need_coupling = 0;
+        st->retry_passes += 1;
+        if (st->iteration > 100) {
+            st->status = QDFN_ERR_RETRIES;
+            st->done = 1;
+        }
+        st->iteration += 1;
+        return;
+    }
+    // accept (:262-319); Dt = min(Dt_All) == dt when every element converged
+    const double dta = MINDT;
+    if (st->ntrace < A.trace_cap) {
+        A.t_told[st->ntrace] = told;
+        A.t_dt[st->ntrace] = dta;
+        A.t_vmax[st->ntrace] = VMAX;
+        st->ntrace += 1;
+    }
+    if (record) {
+        A.h_time[rec] = told;
+        A.h_dt[rec] = dta;
+        st->rec = rec + 1;
+    }
+    st->told = told + dta;
+    st->parity = 1 - p;
+    st->step = step + 1;
+    if (step + 1 >= st->target || (st->t_end > 0.0 && told + dta >= st->t_end)) {
+        st->dt = dta;
+        st->done = 1;
+        st->need_coupling = 0;
+        return;
+    }
+    if (!FOUND) {
+        st->dt = dta;
+        st->status = QDFN_ERR_EMPTY_DTMIN;
+        st->done = 1;
+        st->need_coupling = 0;
+        return;
+    }
+    st->dt = select_dt(A, dta, VNEXT, DNEXT / A.dt_cut);
+    st->iteration = 1;
+    st->need_coupling = 1;
+    st->rhs_evals += 1;
+}
+
+}  // namespace qdfn

@@ -11,6 +11,7 @@ from .assembly import (BuildMatrixByParts_Even_Rec, BuildMatrixByParts_Even_Tri,
                        RotVerts_UnitVectors, StiffnessMatrix_ByParts_Calculation_Rec,
                        StiffnessMatrix_ByParts_Calculation_Tri, TDstressHS)
 from .catalog import EventCatalog, fault_area, get_event_fragments, merge_fragments
+from .ensemble import Ensemble
 from .hmat import HMatrix, hmat_structure, main_H
 from .inputs import BulkToSegment, LRtoRake, ReadBulkInput, ReadSegmentInput
 from .sim import DenseOperator, Simulation, main
@@ -21,4 +22,4 @@ __all__ = ["PREC_FP32", "PREC_FP64", "QdfnError", "device_info", "lib", "Okada_D
            "RotVerts_UnitVectors", "StiffnessMatrix_ByParts_Calculation_Tri", "BuildMatrixByParts_Even_Tri",
            "DenseOperator", "Simulation", "main", "SolveOneTimeStep", "HMatrix", "hmat_structure", "main_H",
            "BulkToSegment", "LRtoRake", "ReadBulkInput", "ReadSegmentInput",
-           "EventCatalog", "fault_area", "get_event_fragments", "merge_fragments"]
+           "EventCatalog", "fault_area", "get_event_fragments", "merge_fragments", "Ensemble"]

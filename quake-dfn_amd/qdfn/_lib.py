@@ -104,6 +104,10 @@ SIGNATURES = {
     "qdfn_sim_get_history": (C.c_int, [VP, I64, PD, PD, PD, PD, PD, PD]),
     "qdfn_sim_get_trace": (C.c_int, [VP, I64, PD, PD, PD]),
     "qdfn_sim_destroy": (None, [VP]),
+    "qdfn_ensemble_create": (VP, [I64, C.POINTER(VP), I32]),
+    "qdfn_ensemble_run": (C.c_int, [VP, I64, D, I32]),
+    "qdfn_ensemble_info": (C.c_int, [VP, C.POINTER(I64), C.POINTER(I32), C.POINTER(I64)]),
+    "qdfn_ensemble_destroy": (None, [VP]),
     "qdfn_catalog_from_sim": (VP, [VP, PD, PD, D, D]),
     "qdfn_catalog_from_history": (VP, [I64, I64, PD, PD, I64, PD, PD, D, D]),
     "qdfn_catalog_from_fragments": (VP, [I64, PD, PD, PD]),

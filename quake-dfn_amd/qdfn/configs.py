@@ -10,6 +10,8 @@ Workloads (all arrays seeded with numpy PCG64):
   fault_network  8 rectangular faults of 32 x 32 elements (8,192 elements)
   rough_fault    a non-planar strike-slip surface with sinusoidal roughness meshed
                  into triangles (100 x 100 cells of 50 m -> 20,000 triangles)
+  ensemble_members  256 planar faults of 500-4,000 elements of 50 m, seeded
+                 per-element friction (the batched-ensemble workload)
 """
 import numpy as np
 
@@ -120,6 +122,38 @@ def fault_network(seed=20240611, n_faults=8, nx=32, nz=32, mu=32e9, nu=0.25, vs=
     seg[:, 14] = sigma
     rho = mu / vs ** 2
     return dict(name="fault_network_8192", seg=seg, mu=mu, nu=nu, rho=rho, min_normal_stress=2e6)
+
+
+def ensemble_members(m=256, seed=20240613, n_min=500, n_max=4000, elem=50.0, mu=32e9, nu=0.25, vs=3464.0):
+    """Config: m independent planar faults, N_k ~ U{n_min..n_max} (seeded), each
+    a vertical strike-slip rectangle of elem-sized elements on a grid
+    round(sqrt(2 N)) x round(sqrt(N / 2)) with its top at the surface;
+    per-element a ~ U(0.008, 0.012), b - a ~ U(-0.004, 0.006),
+    Dc ~ U(5e-4, 5e-3) m, sigma_n ~ U(30, 80) MPa.  Members with the same grid
+    have the same geometry (key "grid") and can share one operator."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    rho = mu / vs ** 2
+    for k in range(m):
+        N = int(rng.integers(n_min, n_max + 1))
+        nx, nz = max(2, int(round(np.sqrt(2 * N)))), max(1, int(round(np.sqrt(N / 2))))
+        L, W = nx * elem, nz * elem
+        bulk = np.zeros((1, 18))
+        bulk[0, :8] = [0.0, 0.0, W / 2, L, W, 90.0, 90.0, 1.0]
+        bulk[0, 8:18] = [0.01, 0.015, 1e-3, 1e6, 1e-9, 0.6, 50e6, 0.0, 0.0, elem]
+        seg = BulkToSegment(LRtoRake(1, bulk), counts=[(nx, nz)])
+        n = seg.shape[0]
+        a = rng.uniform(0.008, 0.012, n)
+        b = a + rng.uniform(-0.004, 0.006, n)
+        seg[:, 8], seg[:, 9] = a, b
+        seg[:, 10] = rng.uniform(5e-4, 5e-3, n)
+        seg[:, 12] = 1e-9
+        seg[:, 11] = seg[:, 10] / seg[:, 12]
+        seg[:, 13] = _friction_init(a, b)
+        seg[:, 14] = rng.uniform(30e6, 80e6, n)
+        out.append(dict(name=f"ensemble_member_{k}", seg=seg, mu=mu, nu=nu, rho=rho, min_normal_stress=2e6,
+                        grid=(nx, nz)))
+    return out
 
 
 def problem_from_segments(cfg, kself, ks_row_sums, kn_row_sums, rate=1e-9, **overrides):

@@ -1,0 +1,82 @@
+"""Batched ensemble of independent runs (qdfn_ensemble_*, include/qdfn.h).
+
+Each member is a run of main() (scripts/Functions_RSFDFN3DMain_D.jl:2-324)
+with its own dense FP32 operator or one shared with members of the same
+geometry, its own friction, Dt control and retries.  All members advance
+together: one attempt of every member per batched launch sequence, replayed
+from one captured HIP graph; members sharing an operator are read as one
+multi-right-hand-side product.  A member's trajectory is bitwise the one it
+has when run alone.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, check_handle, lib
+from .sim import DenseOperator, Simulation
+
+
+class Ensemble:
+    def __init__(self, sims, max_rhs=0):
+        """sims: Simulation objects on FP32 DenseOperators (borrowed; kept alive
+        by this object).  max_rhs: members per operator pass (1-4), 0 = autotune."""
+        self.sims = list(sims)
+        arr = (_lib.VP * len(self.sims))(*[s._h for s in self.sims])
+        self._h = check_handle(lib().qdfn_ensemble_create(len(self.sims), arr, int(max_rhs)),
+                               "qdfn_ensemble_create")
+        self.ops = []
+
+    @classmethod
+    def from_members(cls, members, problem_fn, max_rhs=0, record_step=10, record_capacity=0, trace_capacity=0):
+        """Build one FP32 operator per distinct geometry (member["grid"]), one run
+        per member with problem_fn(member, op) as its problem, and the ensemble."""
+        ops, sims = {}, []
+        for mem in members:
+            key = mem["grid"]
+            if key not in ops:
+                ops[key] = DenseOperator.from_rect_segments(mem["seg"], mem["mu"], mem["nu"], _lib.PREC_FP32)
+            op = ops[key]
+            sims.append(Simulation(op, problem_fn(mem, op), record_step=record_step,
+                                   record_capacity=record_capacity, trace_capacity=trace_capacity))
+        e = cls(sims, max_rhs)
+        e.ops = list(ops.values())
+        return e
+
+    def run(self, n_steps, t_end=0.0, graph_attempts=16):
+        """Advance every member by up to n_steps accepted steps, a member stopping
+        early once its simulated time reaches t_end (0: no time limit)."""
+        check(lib().qdfn_ensemble_run(self._h, int(n_steps), float(t_end), int(graph_attempts)))
+
+    def info(self):
+        g, r, b = _lib.I64(), _lib.I32(), _lib.I64()
+        check(lib().qdfn_ensemble_info(self._h, C.byref(g), C.byref(r), C.byref(b)))
+        return {"groups": g.value, "max_rhs": r.value, "bytes_per_sweep": b.value}
+
+    def stats(self):
+        return [s.stats() for s in self.sims]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qdfn_ensemble_destroy(self._h)
+            self._h = None
+        for s in self.sims:
+            s.close()
+        for o in self.ops:
+            o.close()
+        self.sims, self.ops = [], []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def problem_for_member(mem, op):
+    """The member's problem on its operator: RUN3 defaults, back-slip loading from
+    the operator's own row sums (UniformBackslip.jl)."""
+    from . import configs
+    ks, _ = op.kself()
+    ksum, knsum = op.matvec(np.ones(op.n), loading_form=False)
+    return configs.problem_from_segments(mem, ks, ksum, knsum)
