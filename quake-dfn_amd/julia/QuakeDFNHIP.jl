@@ -235,4 +235,74 @@ function merge_fragments(EventTime_Fragment, EventLocation_Fragment, EventMoment
     return tm, mw, lm
 end
 
+# ---- device-resident runs (main, Functions_RSFDFN3DMain_D.jl:2) and ensembles ----
+
+# qdfn_sim_params, field for field (include/qdfn.h)
+struct SimParams
+    n::Int64; n_loading::Int64
+    mu::Float64; rho::Float64; switch_v::Float64; dt_cut::Float64; rupture_dt::Float64; maximum_dt::Float64
+    min_normal_stress::Float64; alpha_evo::Float64
+    evolution_dr::Int32; planar::Int32; ext_at_told::Int32; pad::Int32
+    record_step::Int64; record_capacity::Int64; trace_capacity::Int64
+end
+
+# Dense operator from an Input_Segment table (assembled on the device); prec 0 FP64, 1 FP32.
+function DenseCreateRect(Input_Segment, ShearModulus, PoissonRatio; prec=1)
+    seg = Matrix{Float64}(Input_Segment); n = size(seg, 1)
+    h = GC.@preserve seg ccall((:qdfn_dense_create_rect, LIB), Ptr{Cvoid},
+        (Ptr{Float64}, Int64, Int64, Float64, Float64, Int32), seg, n, n, ShearModulus, PoissonRatio, Int32(prec))
+    h == C_NULL && check(-1)
+    return h
+end
+
+# main()'s set-up on the device: element arrays as QUAKEDFN_RunARG.jl passes them.
+function SimCreateDense(op, p::SimParams, FaultMass, Fault_a, Fault_b, Fault_Dc, Fault_Theta_i, Fault_V_i,
+                        Fault_Friction_i, Fault_NormalStress, Fault_V_Const)
+    arrs = map(a -> Vector{Float64}(vec(a)), (FaultMass, Fault_a, Fault_b, Fault_Dc, Fault_Theta_i, Fault_V_i,
+                                              Fault_Friction_i, Fault_NormalStress, Fault_V_Const))
+    pr = Ref(p)
+    h = GC.@preserve arrs pr ccall((:qdfn_sim_create_dense, LIB), Ptr{Cvoid},
+        (Ptr{Cvoid}, Ref{SimParams}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), op, pr, arrs...)
+    h == C_NULL && check(-1)
+    return h
+end
+
+# Input_ExternalStressChange tables: times (nt), ext_n / ext_s nt x n.
+function SimSetExternal(sim, times, ext_n, ext_s)
+    t = Vector{Float64}(vec(times)); en = Matrix{Float64}(ext_n); es = Matrix{Float64}(ext_s)
+    GC.@preserve t en es check(ccall((:qdfn_sim_set_external, LIB), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), sim, length(t), t, en, es))
+end
+
+SimRun(sim, n_steps; mode=0, graph_attempts=32) =
+    check(ccall((:qdfn_sim_run, LIB), Cint, (Ptr{Cvoid}, Int64, Int32, Int32), sim, n_steps, Int32(mode),
+                Int32(graph_attempts)))
+
+# History_Time, History_Dt, History_V, History_Disp, History_Theta, History_NormalStress (count x n).
+function SimHistory(sim, count, n)
+    t = zeros(count); dt = zeros(count); V = zeros(count, n); D = zeros(count, n); Th = zeros(count, n)
+    S = zeros(count, n)
+    GC.@preserve t dt V D Th S check(ccall((:qdfn_sim_get_history, LIB), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        sim, count, t, dt, V, D, Th, S))
+    return t, dt, V, D, Th, S
+end
+
+SimDestroy(sim) = ccall((:qdfn_sim_destroy, LIB), Cvoid, (Ptr{Cvoid},), sim)
+
+# Many runs (one per input file) advanced together; runs on the same operator share its reads.
+function EnsembleCreate(sims::Vector{Ptr{Cvoid}}; max_rhs=0)
+    e = GC.@preserve sims ccall((:qdfn_ensemble_create, LIB), Ptr{Cvoid}, (Int64, Ptr{Ptr{Cvoid}}, Int32),
+                                length(sims), sims, Int32(max_rhs))
+    e == C_NULL && check(-1)
+    return e
+end
+
+EnsembleRun(e, n_steps; t_end=0.0, graph_attempts=16) =
+    check(ccall((:qdfn_ensemble_run, LIB), Cint, (Ptr{Cvoid}, Int64, Float64, Int32), e, n_steps, t_end,
+                Int32(graph_attempts)))
+
+EnsembleDestroy(e) = ccall((:qdfn_ensemble_destroy, LIB), Cvoid, (Ptr{Cvoid},), e)
+
 end # module
