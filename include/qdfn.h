@@ -288,9 +288,36 @@ int qdfn_sim_get_state(qdfn_sim_t* sim, double* V, double* Disp, double* Theta, 
  * column-major (History_* of the reference, rows = records). */
 int qdfn_sim_get_history(qdfn_sim_t* sim, int64_t count, double* time, double* dt, double* V, double* Disp,
                          double* Theta, double* Sigma);
+/* Solver used by each element in each recorded step's accepted attempt
+ * (count x n column-major, 1 = Solver_HighV_D, 0 = Solver_LowV_D; the
+ * per-element SolverSwitch of main(), Functions_RSFDFN3DMain_D.jl:152-158, 242-249). */
+int qdfn_sim_get_solver_history(qdfn_sim_t* sim, int64_t count, int8_t* solver);
 /* Per-step trace: TOld, Dt and max V over non-loading elements. */
 int qdfn_sim_get_trace(qdfn_sim_t* sim, int64_t count, double* told, double* dt, double* vmax);
 void qdfn_sim_destroy(qdfn_sim_t* sim);
+
+/* ---- adjoint of the step loop ---------------------------------------------
+ * Gradient with respect to the per-element a, b and Dc of
+ *   L = sum_o (Disp_K[obs_o] - d_target_o)^2 + w_time (t_first - t_target)^2
+ * for the K accepted steps a dense run has taken since its creation (it must
+ * record every step: record_step = 1, record_capacity >= K; aging law with
+ * alpha = 0).  t_first: the first step whose max V over the
+ * non-loading elements reaches v_onset, interpolated in log10 V between the
+ * end times of that step and the one before (the term is dropped if
+ * w_time = 0 or no onset occurs).  v_i / theta_i: the run's initial V and
+ * Theta (as given to qdfn_sim_create_dense).  A discrete adjoint of the
+ * forward's own arithmetic (Functions_RSFDFN3DMain_D.jl:146-319,
+ * Functions_Solvers.jl:26-327): the implicit-function derivative of each
+ * converged local solve, transposed coupling products, the Dt controller's
+ * argmin and growth rules, with the forward's discrete choices held fixed.
+ * It has no counterpart in the reference (which has no gradients). */
+typedef struct {
+    int64_t steps, output_mismatches, select_mismatches, onset_step, retried_steps;
+    double t_first, ms_backward;
+} qdfn_adjoint_info;
+int qdfn_sim_adjoint(qdfn_sim_t* sim, const double* v_i, const double* theta_i, int64_t n_obs, const int64_t* obs,
+                     const double* d_target, double w_time, double t_target, double v_onset, double* loss,
+                     double* grad_a, double* grad_b, double* grad_dc, qdfn_adjoint_info* info);
 
 /* ---- batched ensemble -----------------------------------------------------
  * Many independent runs of main() (scripts/Functions_RSFDFN3DMain_D.jl:2-324,

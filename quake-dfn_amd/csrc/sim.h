@@ -27,7 +27,7 @@ struct qdfn_sim {
     qdfn_hmat* hop = nullptr;   // ... or H-matrix coupling (main_H)
     qdfn_sim_params P{};
     hipStream_t stream = nullptr;
-    qdfn::DevBuf consts, state2, single, ints, cand, ext, hist, trace, dstate, ranlog, parts;
+    qdfn::DevBuf consts, state2, single, ints, cand, ext, hist, histsw, trace, dstate, ranlog, parts;
     qdfn::SimArrays A{};
     qdfn::SimState* d_st = nullptr;
     qdfn::SimState h_st{};

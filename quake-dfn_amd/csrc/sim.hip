@@ -149,7 +149,8 @@ static qdfn_sim* sim_create(const char* who, int64_t n_op, const double* kself_h
     if (s->parts.alloc(sizeof(BlockPart) * nblk)) return bail("partials alloc");
     if (s->cand.alloc(n) || hipMemcpy(s->cand.p, cand.data(), n, hipMemcpyHostToDevice)) return bail("cand");
     const int64_t rc_cap = p->record_capacity, tc = p->trace_capacity;
-    if (s->hist.alloc(sizeof(double) * (2 * rc_cap + 4 * rc_cap * n + 1))) return bail("history alloc");
+    if (s->hist.alloc(sizeof(double) * (3 * rc_cap + 4 * rc_cap * n + 1))) return bail("history alloc");
+    if (s->histsw.alloc(rc_cap * n + 1)) return bail("history alloc");
     if (s->trace.alloc(sizeof(double) * (3 * tc + 1))) return bail("trace alloc");
     if (s->dstate.alloc(sizeof(SimState))) return bail("state alloc");
     s->d_st = s->dstate.as<SimState>();
@@ -179,7 +180,8 @@ static qdfn_sim* sim_create(const char* who, int64_t n_op, const double* kself_h
     A.rec_cap = rc_cap; A.trace_cap = tc; A.record_step = p->record_step;
     double* hh = s->hist.as<double>();
     A.h_time = hh; A.h_dt = hh + rc_cap; A.h_v = hh + 2 * rc_cap; A.h_d = A.h_v + rc_cap * n;
-    A.h_th = A.h_d + rc_cap * n; A.h_sig = A.h_th + rc_cap * n;
+    A.h_th = A.h_d + rc_cap * n; A.h_sig = A.h_th + rc_cap * n; A.h_dtsel = A.h_sig + rc_cap * n;
+    A.h_sw = s->histsw.as<int8_t>();
     double* tt = s->trace.as<double>();
     A.t_told = tt; A.t_dt = tt + tc; A.t_vmax = tt + 2 * tc;
     A.mu = p->mu; A.rho = p->rho; A.switch_v = p->switch_v; A.dt_cut = p->dt_cut; A.rupture_dt = p->rupture_dt;
@@ -384,6 +386,16 @@ int qdfn::sim_history_view(qdfn_sim_t* s, SimHistoryView* v) {
     v->ld = s->P.record_capacity;
     v->n = s->P.n;
     v->stream = s->stream;
+    return QDFN_OK;
+}
+
+extern "C" int qdfn_sim_get_solver_history(qdfn_sim_t* s, int64_t count, int8_t* solver) {
+    if (!s || !solver || count < 0 || count > s->P.record_capacity)
+        return fail(QDFN_ERR_ARG, "qdfn_sim_get_solver_history: bad arguments");
+    if (count == 0) return QDFN_OK;
+    QDFN_HIP(hipStreamSynchronize(s->stream));
+    const int64_t cap = s->P.record_capacity;
+    QDFN_HIP(hipMemcpy2D(solver, count, s->A.h_sw, cap, count, s->P.n, hipMemcpyDeviceToHost));
     return QDFN_OK;
 }
 

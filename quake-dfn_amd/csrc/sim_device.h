@@ -14,6 +14,7 @@ namespace qdfn {
 struct SimState {
     double dt, told;
     double t_end;           // > 0: the run also stops once TOld reaches it
+    double dt_first;        // Dt of the current step's first attempt (the selected Dt)
     int64_t step, target, rhs_evals, retry_passes, stalls, rec, ntrace;
     int32_t need_coupling;  // gate for the coupling kernels
     int32_t iteration;      // attempt number within the current step (1-based)
@@ -49,6 +50,8 @@ struct SimArrays {
     // histories
     int64_t rec_cap, trace_cap, record_step;
     double *h_time, *h_dt, *h_v, *h_d, *h_th, *h_sig;
+    double* h_dtsel;  // selected Dt of each recorded step (before any retry)
+    int8_t* h_sw;     // solver of each element in the recorded step's accepted attempt (1: HighV)
     double *t_told, *t_dt, *t_vmax;
     // parameters
     double mu, rho, switch_v, dt_cut, rupture_dt, maximum_dt, min_sig, alpha_evo, conv, v0;
@@ -142,6 +145,7 @@ __device__ __forceinline__ void setup_body(const SimArrays& A, SimState* __restr
             st->need_coupling = 0;
         } else {
             st->dt = select_dt(A, st->dt, vmax, dmin / A.dt_cut);
+            st->dt_first = st->dt;
             st->iteration = 1;
             st->need_coupling = 1;
             st->rhs_evals += 1;
@@ -199,6 +203,7 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
                 A.h_d[rec + i * A.rec_cap] = r.d;
                 A.h_th[rec + i * A.rec_cap] = r.th;
                 A.h_sig[rec + i * A.rec_cap] = r.sig;
+                A.h_sw[rec + i * A.rec_cap] = (int8_t)sw;
             }
         } else {
             vn = A.lrate[i];
@@ -211,6 +216,7 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
                 A.h_d[rec + i * A.rec_cap] = d;
                 A.h_th[rec + i * A.rec_cap] = A.TH[o + i];
                 A.h_sig[rec + i * A.rec_cap] = 1e10;
+                A.h_sw[rec + i * A.rec_cap] = 0;
             }
         }
         // quantities of the next step's setup, valid if this attempt is accepted
@@ -277,6 +283,7 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
     if (record) {
         A.h_time[rec] = told;
         A.h_dt[rec] = dta;
+        A.h_dtsel[rec] = st->dt_first;
         st->rec = rec + 1;
     }
     st->told = told + dta;
@@ -296,6 +303,7 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
         return;
     }
     st->dt = select_dt(A, dta, VNEXT, DNEXT / A.dt_cut);
+    st->dt_first = st->dt;
     st->iteration = 1;
     st->need_coupling = 1;
     st->rhs_evals += 1;

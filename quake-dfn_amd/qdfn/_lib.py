@@ -54,6 +54,11 @@ class SimStats(C.Structure):
                 ("coupling_ms_avg", D), ("coupling_timed", I64)]
 
 
+class AdjointInfo(C.Structure):
+    _fields_ = [("steps", I64), ("output_mismatches", I64), ("select_mismatches", I64), ("onset_step", I64),
+                ("retried_steps", I64), ("t_first", D), ("ms_backward", D)]
+
+
 class HmatParams(C.Structure):
     _fields_ = [("min_elements_to_cut", I64), ("max_levels", I64), ("dist_diam_ratio", D),
                 ("arrange_point", D * 3), ("n_loading", I64), ("kmeans_max_iter", I64)]
@@ -103,7 +108,10 @@ SIGNATURES = {
     "qdfn_sim_get_state": (C.c_int, [VP, PD, PD, PD, PD, PD, PD]),
     "qdfn_sim_get_history": (C.c_int, [VP, I64, PD, PD, PD, PD, PD, PD]),
     "qdfn_sim_get_trace": (C.c_int, [VP, I64, PD, PD, PD]),
+    "qdfn_sim_get_solver_history": (C.c_int, [VP, I64, C.POINTER(C.c_int8)]),
     "qdfn_sim_destroy": (None, [VP]),
+    "qdfn_sim_adjoint": (C.c_int, [VP, PD, PD, I64, C.POINTER(I64), PD, D, D, D, PD, PD, PD, PD,
+                                   C.POINTER(AdjointInfo)]),
     "qdfn_ensemble_create": (VP, [I64, C.POINTER(VP), I32]),
     "qdfn_ensemble_run": (C.c_int, [VP, I64, D, I32]),
     "qdfn_ensemble_info": (C.c_int, [VP, C.POINTER(I64), C.POINTER(I32), C.POINTER(I64)]),

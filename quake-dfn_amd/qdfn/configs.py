@@ -156,6 +156,30 @@ def ensemble_members(m=256, seed=20240613, n_min=500, n_max=4000, elem=50.0, mu=
     return out
 
 
+def adjoint_fault(nx=64, nz=64, elem=50.0, seed=20240614, mu=32e9, nu=0.25, vs=3464.0):
+    """Config: one vertical strike-slip fault of nx x nz elements of elem m (64 x 64
+    of 50 m = 4,096 elements), top at the surface, seeded per-element
+    a ~ U(0.008, 0.012), b - a ~ U(-0.004, 0.006), Dc ~ U(5e-4, 5e-3) m,
+    sigma_n ~ U(30, 80) MPa (the parameters of the adjoint workload)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L, W = nx * elem, nz * elem
+    bulk = np.zeros((1, 18))
+    bulk[0, :8] = [0.0, 0.0, W / 2, L, W, 90.0, 90.0, 1.0]
+    bulk[0, 8:18] = [0.01, 0.015, 1e-3, 1e6, 1e-9, 0.6, 50e6, 0.0, 0.0, elem]
+    seg = BulkToSegment(LRtoRake(1, bulk), counts=[(nx, nz)])
+    n = seg.shape[0]
+    a = rng.uniform(0.008, 0.012, n)
+    b = a + rng.uniform(-0.004, 0.006, n)
+    seg[:, 8], seg[:, 9] = a, b
+    seg[:, 10] = rng.uniform(5e-4, 5e-3, n)
+    seg[:, 12] = 1e-9
+    seg[:, 11] = seg[:, 10] / seg[:, 12]
+    seg[:, 13] = _friction_init(a, b)
+    seg[:, 14] = rng.uniform(30e6, 80e6, n)
+    rho = mu / vs ** 2
+    return dict(name=f"adjoint_fault_{n}", seg=seg, mu=mu, nu=nu, rho=rho, min_normal_stress=2e6)
+
+
 def problem_from_segments(cfg, kself, ks_row_sums, kn_row_sums, rate=1e-9, **overrides):
     """Element arrays and run parameters for a rectangle workload: the
     Input_Segment columns plus RUN3 defaults and back-slip loading."""

@@ -165,11 +165,37 @@ class Simulation:
         return {"History_Time": t, "History_Dt": dt, "History_V": mats[0], "History_Disp": mats[1],
                 "History_Theta": mats[2], "History_NormalStress": mats[3]}
 
+    def adjoint(self, v_i, theta_i, obs, d_target, w_time=0.0, t_target=0.0, v_onset=1e-3):
+        """Loss and gradients (a, b, Dc) of
+        L = sum_o (Disp[obs_o] - d_target_o)^2 + w_time (t_first - t_target)^2
+        over every step this run has taken (see qdfn_sim_adjoint); the run must
+        have been created with record_step=1 and record_capacity >= its steps.
+        Returns (loss, {"a": ..., "b": ..., "dc": ...}, info dict)."""
+        obs = np.ascontiguousarray(obs, dtype=np.int64).ravel()
+        tgt = _f64(d_target)
+        if tgt.size != obs.size:
+            raise ValueError("one target per observation element")
+        vi, th = _f64(v_i), _f64(theta_i)
+        ga, gb, gd = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        loss = _lib.D()
+        info = _lib.AdjointInfo()
+        check(lib().qdfn_sim_adjoint(self._h, ptr(vi), ptr(th), obs.size, obs.ctypes.data_as(_lib.C.POINTER(_lib.I64)),
+                                     ptr(tgt), float(w_time), float(t_target), float(v_onset), _lib.C.byref(loss),
+                                     ptr(ga), ptr(gb), ptr(gd), _lib.C.byref(info)))
+        return loss.value, {"a": ga, "b": gb, "dc": gd}, {k: getattr(info, k) for k, _ in info._fields_}
+
     def event_catalog(self, FaultCenter, FaultArea, ShearModulus, DispRateCrits=1e-2):
         """Fragments of get_event_fragments (scripts/Functions_Plot.jl:430-510)
         from this run's device-resident histories; see qdfn.catalog."""
         from .catalog import EventCatalog
         return EventCatalog.from_sim(self, FaultCenter, FaultArea, ShearModulus, DispRateCrits)
+
+    def solver_history(self):
+        """Solver of each element in each recorded step (1 HighV, 0 LowV), records x n."""
+        cnt = self.stats()["n_recorded"]
+        out = np.zeros((cnt, self.n), dtype=np.int8, order="F")
+        check(lib().qdfn_sim_get_solver_history(self._h, cnt, out.ctypes.data_as(_lib.C.POINTER(_lib.C.c_int8))))
+        return out
 
     def trace(self):
         cnt = self.stats()["n_traced"]
