@@ -210,6 +210,70 @@ def ensemble(qd, args):
     return out
 
 
+def adjoint_case(qd, args):
+    """64 x 64 elements of 50 m (4,096), per-element a, b, Dc; loss = squared
+    slip misfit at 32 seeded elements (+ squared first-event-time misfit when
+    the window holds the onset) over a window of accepted steps; gradient by
+    qdfn_sim_adjoint, checked against central differences on 16 random
+    parameters.  Windows: args.adjoint_steps steps (through the first event;
+    the rupture's sensitivity grows exponentially, so differences are taken on
+    4 parameters only, to show it), the onset plus 38 steps (the first HighV
+    solves) and the interseismic steps before the onset."""
+    from qdfn import adjoint, configs
+    m = configs.adjoint_fault()
+    seg = m["seg"]
+    op = qd.DenseOperator.from_rect_segments(seg, m["mu"], m["nu"], qd.PREC_FP64)
+    ks, _ = op.kself()
+    ksum, knsum = op.matvec(np.ones(op.n), loading_form=False)
+    prob = configs.problem_from_segments(m, ks, ksum, knsum)
+    rng = np.random.default_rng(41)
+    obs = np.sort(rng.choice(op.n, 32, replace=False))
+    out = {"workload": "adjoint_fault_4096", "observations": 32}
+    K = args.adjoint_steps
+    windows = [("full_window", K, 4)]
+    while windows:
+        name, K, n_fd = windows.pop(0)
+        sim = qd.Simulation(op, prob, record_step=1, record_capacity=K)
+        t0 = time.perf_counter()
+        sim.run(K)
+        fwd_s = time.perf_counter() - t0
+        h = sim.history()
+        dK = h["History_Disp"][-1]
+        target = dK[obs] * 1.1 + 1e-3
+        _, t_first = adjoint.misfit(h, 0, obs, target)
+        if t_first is not None:
+            t_target = t_first * 1.05
+            w_time = float(np.sum((dK[obs] - target) ** 2)) / (t_first - t_target) ** 2
+        else:
+            t_target, w_time = 0.0, 0.0
+        t0 = time.perf_counter()
+        L, g, info = sim.adjoint(prob["v_i"], prob["theta_i"], obs, target, w_time, t_target)
+        bwd_s = time.perf_counter() - t0
+        sim.close()
+        rng2 = np.random.default_rng(43)
+        params = [(("a", "b", "dc")[q % 3], int(rng2.integers(op.n))) for q in range(16)][:n_fd]
+        chk = adjoint.fd_check(op, prob, K, obs, target, g, params, w_time, t_target)
+        smooth = [c for c in chk if c["smoothness"] < 1e-3]
+        worst = max((c["rel_err"] for c in smooth), default=None)
+        log(f"[bench] adjoint ({name}): {K} steps forward {fwd_s * 1e3:.1f} ms, backward "
+            f"{info['ms_backward']:.1f} ms (GPU), onset step {info['onset_step']}, retried steps "
+            f"{info['retried_steps']}, max |grad| {max(np.abs(v).max() for v in g.values()):.3e}; central "
+            f"differences: {len(smooth)}/{n_fd} parameters with a smooth scale, max rel. error there {worst}")
+        out[name] = {"steps": K, "loss": L, "time_term": w_time > 0.0, "forward_ms": round(fwd_s * 1e3, 2),
+                     "backward_ms": round(info["ms_backward"], 2), "backward_wall_ms": round(bwd_s * 1e3, 2),
+                     "onset_step": info["onset_step"], "retried_steps": info["retried_steps"],
+                     "grad_max_abs": float(max(np.abs(v).max() for v in g.values())),
+                     "fd_params": n_fd, "fd_smooth_params": len(smooth), "grad_max_rel_err_fd": worst,
+                     "fd": [{k: (float(f"{v:.9g}") if isinstance(v, float) else v) for k, v in c.items()}
+                            for c in chk]}
+        if name == "full_window" and info["onset_step"] > 2:
+            # through the onset and the first 30-40 steps of the event (HighV solves)
+            windows.append(("onset_window", min(K, int(info["onset_step"]) + 38), 16))
+            windows.append(("interseismic", int(info["onset_step"]) - 2, 16))
+    op.close()
+    return out
+
+
 def cpu_baseline(h32, st, prob_h, n_steps):
     """The oracle's FP64 main_H (C restatement of HmatSolver_Pararllel with one
     thread and of the loop) on the same stored blocks, for a bounded number of
@@ -237,6 +301,8 @@ def main():
     ap.add_argument("--no-network", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=20000, help="accepted-step cap per ensemble member")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU layout is "
@@ -256,6 +322,7 @@ def main():
     achieved = bytes2 / (mixed["coupling_ms"] * 1e-3) / 1e9
     net = None if args.no_network else network(qdfn, args)
     ens = None if args.no_ensemble else ensemble(qdfn, args)
+    adj = None if args.no_adjoint else adjoint_case(qdfn, args)
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: oracle FP64 main_H, {args.cpu_steps} steps")
@@ -304,6 +371,7 @@ def main():
         "operator_rel_err": rf["acc"],
         "network_8192": net,
         "ensemble_256": ens,
+        "adjoint_4096": adj,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic()},
         "cpu_baseline": cpu,

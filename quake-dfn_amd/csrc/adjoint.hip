@@ -170,7 +170,9 @@ __global__ __launch_bounds__(1024) void k_adj_select(AdjArgs A, double dt_acc, d
             arg = si[q];
         }
     }
-    AdjScal& S = *A.sc;
+    // the scalar block is produced by the previous kernel: accessed through
+    // volatile so that it is read with vector loads, never from the scalar cache
+    volatile AdjScal& S = *A.sc;
     const double lam = S.lam_dt_next;
     // select_dt (sim_device.h) with its branch recorded
     const double dtmin = dmin / A.dt_cut;
@@ -355,12 +357,16 @@ __global__ __launch_bounds__(256) void k_adj_transpose(const T* __restrict__ E, 
 // time and Dt adjoints of the step; hands lam_dt / lam_t to the step before
 __global__ void k_adj_scalar(AdjArgs A, int64_t nblk, double retry_factor) {
     if (threadIdx.x != 0) return;
+    // block sums and the scalar block come from earlier kernels: volatile
+    // (vector) loads, never the scalar cache
+    const volatile double* pdt = A.part_dt;
+    const volatile double* pt = A.part_t;
     double sdt = 0.0, st = 0.0;
     for (int64_t q = 0; q < nblk; ++q) {
-        sdt += A.part_dt[q];
-        st += A.part_t[q];
+        sdt += pdt[q];
+        st += pt[q];
     }
-    AdjScal& S = *A.sc;
+    volatile AdjScal& S = *A.sc;
     const double lam_t_old = S.lam_t_new + st;
     double lam_dt = S.lam_dt_acc + S.lam_t_new + sdt + (A.ext_at_told ? 0.0 : st);
     // the accepted Dt is the selected one times 2^-j (j halvings over the
@@ -372,8 +378,8 @@ __global__ void k_adj_scalar(AdjArgs A, int64_t nblk, double retry_factor) {
 
 __global__ void k_adj_add(double* p, int64_t i, double v, AdjScal* sc, double lam_t) {
     if (threadIdx.x != 0) return;
-    if (p) p[i] += v;
-    if (sc) sc->lam_t_new += lam_t;
+    if (p) *(volatile double*)(p + i) += v;
+    if (sc) *(volatile double*)&sc->lam_t_new += lam_t;
 }
 
 // gradient of f0 = Friction_i - a ln(V_i / V0) - b ln(Theta_i V0 / Dc) into a, b, Dc

@@ -217,7 +217,9 @@ __global__ __launch_bounds__(256) void k_ens_count(SimState* const* __restrict__
                                                    int32_t* __restrict__ out) {
     __shared__ int sh[4];
     int c = 0;
-    for (int k = threadIdx.x; k < m; k += 256) c += states[k]->done ? 0 : 1;
+    // the flags were written by earlier kernels: volatile (vector) loads, never
+    // the scalar cache, which a one-workgroup kernel can find holding an old line
+    for (int k = threadIdx.x; k < m; k += 256) c += *(const volatile int32_t*)&states[k]->done ? 0 : 1;
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
     __syncthreads();

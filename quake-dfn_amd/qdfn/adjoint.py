@@ -76,3 +76,38 @@ def with_params(prob, a=None, b=None, dc=None):
         if v is not None:
             q[k] = np.asarray(v, dtype=np.float64).copy()
     return q
+
+
+def fd_check(op, prob, steps, obs, d_target, grads, params, w_time=0.0, t_target=0.0, v_onset=1e-3,
+             rel_steps=(1e-3, 1e-4, 1e-5, 1e-6)):
+    """Central differences of the loss for each (kind, index) in params at
+    several relative steps.  The step loop's discrete choices (the Dt
+    controller's argmin and growth rule, solver switches) make the loss
+    piecewise smooth, and its secant solves make it noisy below ~1e-6
+    relative, so for each parameter the step whose forward and backward
+    one-sided differences agree best is taken (the scale on which the loss
+    is smooth).  Returns one dict per parameter: adjoint, difference, step,
+    smoothness (|forward - backward| / |central|) and relative error."""
+    L0 = forward_loss(op, prob, steps, obs, d_target, w_time, t_target, v_onset)[0]
+    out = []
+    for kind, i in params:
+        p = prob[kind]
+        best = None
+        for hr in rel_steps:
+            h = hr * abs(p[i])
+            pp, pm = p.copy(), p.copy()
+            pp[i] += h
+            pm[i] -= h
+            Lp = forward_loss(op, with_params(prob, **{kind: pp}), steps, obs, d_target, w_time, t_target,
+                              v_onset)[0]
+            Lm = forward_loss(op, with_params(prob, **{kind: pm}), steps, obs, d_target, w_time, t_target,
+                              v_onset)[0]
+            c = (Lp - Lm) / (2 * h)
+            fw, bw = (Lp - L0) / h, (L0 - Lm) / h
+            smooth = abs(fw - bw) / max(abs(c), 1e-300)
+            if best is None or smooth < best["smoothness"]:
+                best = {"kind": kind, "index": int(i), "adjoint": float(grads[kind][i]), "fd": float(c),
+                        "rel_step": hr, "smoothness": float(smooth)}
+        best["rel_err"] = abs(best["adjoint"] - best["fd"]) / max(abs(best["fd"]), 1e-300)
+        out.append(best)
+    return out
