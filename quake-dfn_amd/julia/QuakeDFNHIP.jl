@@ -305,4 +305,24 @@ EnsembleRun(e, n_steps; t_end=0.0, graph_attempts=16) =
 
 EnsembleDestroy(e) = ccall((:qdfn_ensemble_destroy, LIB), Cvoid, (Ptr{Cvoid},), e)
 
+# Gradient of sum_o (Disp[obs_o] - target_o)^2 + w_time (t_first - t_target)^2 in a, b, Dc
+# for every step the run (recorded every step) has taken; obs are 1-based element indices.
+struct AdjointInfo
+    steps::Int64; output_mismatches::Int64; select_mismatches::Int64; onset_step::Int64; retried_steps::Int64
+    t_first::Float64; ms_backward::Float64
+end
+
+function SimAdjoint(sim, V_i, Theta_i, obs, target; w_time=0.0, t_target=0.0, v_onset=1e-3)
+    n = length(V_i)
+    vi = Vector{Float64}(vec(V_i)); th = Vector{Float64}(vec(Theta_i))
+    o = Vector{Int64}(vec(obs) .- 1); tg = Vector{Float64}(vec(target))
+    loss = Ref{Float64}(0.0); ga = zeros(n); gb = zeros(n); gd = zeros(n)
+    info = Ref{AdjointInfo}()
+    GC.@preserve vi th o tg ga gb gd check(ccall((:qdfn_sim_adjoint, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Float64}, Float64, Float64, Float64,
+         Ref{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ref{AdjointInfo}),
+        sim, vi, th, length(o), o, tg, w_time, t_target, v_onset, loss, ga, gb, gd, info))
+    return loss[], ga, gb, gd, info[]
+end
+
 end # module
