@@ -165,6 +165,39 @@ qdfn_hmat_t* qdfn_hmat_build_tri(int64_t n, const double* P1, const double* P2, 
                                  const double* rake, const double* ctr, const double* nrm, const double* slip,
                                  double mu, double lambda, int64_t nblk, const int64_t* range_SR,
                                  const int64_t* admissible, double aca_rtol, double aca_atol, int32_t prec);
+/* Sampled construction: the same operator as qdfn_hmat_build_rect / _tri
+ * (same arguments and block structure) without assembling any admissible
+ * block.  Replaces the full-block assembly + pqrfact of HmatBuild_T / HmatBuild_R
+ * (scripts/Functions_BuildInputFile.jl:674-716, :754-797): each admissible
+ * part is approximated by partially pivoted cross approximation, which
+ * evaluates only the rows and columns it pivots on, and stops when the
+ * estimate |u_r| |v_r| <= (aca_rtol / 2) ||S||_F holds on two consecutive
+ * steps (||S||_F the block's shear-norm estimate; with aca_atol > 0 also when
+ * max|u_r| max|v_r| <= aca_atol), then is recompressed to the other half of
+ * aca_rtol.  Parts that reach the rank cap are evaluated in full and take the
+ * full build's path; inadmissible blocks are evaluated in full.  Memory and
+ * entry evaluations grow like the H-matrix, so n is not bounded by n^2
+ * device memory. */
+qdfn_hmat_t* qdfn_hmat_build_rect_sampled(const double* seg, int64_t n, int64_t ld, double mu, double nu,
+                                          int64_t nblk, const int64_t* range_SR, const int64_t* admissible,
+                                          double aca_rtol, double aca_atol, int32_t prec);
+qdfn_hmat_t* qdfn_hmat_build_tri_sampled(int64_t n, const double* P1, const double* P2, const double* P3,
+                                         const double* rake, const double* ctr, const double* nrm,
+                                         const double* slip, double mu, double lambda, int64_t nblk,
+                                         const int64_t* range_SR, const int64_t* admissible, double aca_rtol,
+                                         double aca_atol, int32_t prec);
+typedef struct {
+    int64_t entries_evaluated; /* stiffness entries computed by the build (n*n for the full builds) */
+    int64_t steps;             /* line-evaluation / approximation launch pairs (sampled builds) */
+    int64_t parts_sampled;     /* admissible parts finished by the sampled approximation */
+    int64_t blocks_fallback;   /* parts evaluated in full after reaching the rank cap */
+    double ms_build;           /* wall time of the build (ms) */
+    double ms_phase[5];        /* of which: [0] entry evaluation of dense and fully evaluated blocks (the
+                                  full builds: the assembly), [1] sampled approximation, [2] full-pivot
+                                  approximation, [3] host recompression, [4] product layout */
+} qdfn_hmat_build_stats;
+/* Statistics of the build that made h (zeros for qdfn_hmat_create). */
+int qdfn_hmat_build_info(qdfn_hmat_t* h, qdfn_hmat_build_stats* out);
 /* Operator from given blocks (ShearStiffness_H / NormalStiffness_H as written
  * by RUN2_BUILD_INPUT.jl:362): per block kind_S[b], rank_S[b], S_a[b] (dense
  * m x k, or U m x r), S_b[b] (NULL, or V r x k), all column-major; the same for

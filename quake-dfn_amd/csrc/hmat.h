@@ -1,5 +1,6 @@
 // H-matrix operator: host block store and device layout (see hmat.hip).
 #pragma once
+#include <chrono>
 #include <vector>
 
 #include "dense.h"
@@ -73,9 +74,21 @@ struct qdfn_hmat {
     int64_t phase_bytes = 0;              // HBM bytes one product reads and writes
     int64_t alg_bytes1 = 0, alg_bytes2 = 0; // algorithmic bytes of the phase-1 / phase-2 main kernels
     hipStream_t stream = nullptr;
+    // construction statistics
+    int64_t build_entries = 0, build_steps = 0, build_sampled = 0, build_fallback = 0;
+    double build_ms = 0.0;
+    double build_phase_ms[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // see qdfn_hmat_build_stats
 };
 
 namespace qdfn {
+// Adds the wall time of its scope to *acc (milliseconds).
+struct ScopeMs {
+    double* acc;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit ScopeMs(double* a) : acc(a) {}
+    ~ScopeMs() { *acc += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
 // Both phases of one product on stream s.  form 1: yS = (H_load (0 - x)) ./ K_ii,
 // form 0: yS = H_S x; yN = H_N x.  The gate skips the product as for the dense
 // coupling.
@@ -83,4 +96,36 @@ namespace qdfn {
 // phase-2 main kernel (before its finish).
 int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, int32_t form, double* d_yS,
                 double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2);
+
+// ---- construction helpers shared by the full and the sampled builds ----
+qdfn_hmat* new_hmat(int64_t n, int32_t prec);
+int check_blocks(int64_t n, int64_t nblk, const int64_t* range, const int64_t* adm);
+// blocks from ElementRange_SR (1-based) and Admissible
+void init_blocks(qdfn_hmat* h, int64_t nblk, const int64_t* range, const int64_t* adm);
+// device panels and product tasks from h->blocks
+int hmat_layout(qdfn_hmat* h);
+// rank cap of the cross approximation of an m x k block
+int64_t aca_rank_cap(int64_t m, int64_t k);
+// U V (m x r, r x k) -> shortest factors within `budget` (Frobenius)
+void recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V, double budget);
+
+// part w of block q, held in device memory (column-major, leading dimension ld)
+struct DeviceBlock {
+    int64_t q;
+    int w;
+    const double* src;
+    int64_t ld;
+};
+// Frobenius norms of device blocks (fixed-order sums)
+int block_norms(const std::vector<const double*>& src, const std::vector<int64_t>& ld,
+                const std::vector<int32_t>& m, const std::vector<int32_t>& k, std::vector<double>& out,
+                hipStream_t stream);
+// full-pivot cross approximation + recompression of the listed parts;
+// bnorm[q] is block q's threshold scale ||K_S,b||_F.  Parts that reach the
+// rank cap are stored dense.
+int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, const std::vector<double>& bnorm,
+                           double rtol, double atol);
+// recompress(items[i]) with budgets[i] on the host threads
+void recompress_parallel(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, const std::vector<size_t>& items,
+                         const std::vector<double>& budgets);
 }  // namespace qdfn

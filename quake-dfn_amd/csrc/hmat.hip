@@ -34,6 +34,7 @@
 // small finish kernel (no atomics: results are deterministic run to run).
 // Panels and V are stored in FP64 or FP32 (mixed precision).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -505,7 +506,8 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(const AcaJob* __restrict__ 
 
 // ------------------------------------------------------ host-side layout ----
 
-static int hmat_layout(qdfn_hmat* h) {
+int hmat_layout(qdfn_hmat* h) {
+    ScopeMs timer(&h->build_phase_ms[4]);
     const int64_t n = h->n;
     auto& B = h->blocks;
     // K_S,ii from the diagonal blocks (src range == rcv range)
@@ -853,7 +855,7 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
 
 // ---------------------------------------------------------------- build ----
 
-static qdfn_hmat* new_hmat(int64_t n, int32_t prec) {
+qdfn_hmat* new_hmat(int64_t n, int32_t prec) {
     if (n <= 0 || (prec != QDFN_PREC_FP64 && prec != QDFN_PREC_FP32)) {
         fail(QDFN_ERR_ARG, "H-matrix: bad size or precision");
         return nullptr;
@@ -869,7 +871,7 @@ static qdfn_hmat* new_hmat(int64_t n, int32_t prec) {
     return h;
 }
 
-static int check_blocks(int64_t n, int64_t nblk, const int64_t* range, const int64_t* adm) {
+int check_blocks(int64_t n, int64_t nblk, const int64_t* range, const int64_t* adm) {
     if (nblk <= 0 || !range || !adm) return fail(QDFN_ERR_ARG, "H-matrix: empty block list");
     for (int64_t q = 0; q < nblk; ++q) {
         const int64_t* r = range + 4 * q;
@@ -973,7 +975,7 @@ static void svd_jacobi(int64_t r, std::vector<double> C, std::vector<double>& W,
 
 // U (m x r), V (r x k) -> the shortest factors whose product differs from U V
 // by at most `budget` in the Frobenius norm.
-static void recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V,
+void recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V,
                        double budget) {
     if (r < 2 || budget <= 0) return;
     std::vector<double> Vt(k * r);
@@ -1020,65 +1022,56 @@ static void recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U,
     r = keep;
 }
 
-// Compress / read out every block from full device matrices KS, KN (n x n, ld n).
-static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, int64_t nblk, const int64_t* range,
-                            const int64_t* adm, double rtol, double atol) {
-    const int64_t n = h->n;
-    h->blocks.resize(nblk);
-    for (int64_t q = 0; q < nblk; ++q) {
-        HBlock& b = h->blocks[q];
-        b.s0 = range[4 * q] - 1;
-        b.s1 = range[4 * q + 1] - 1;
-        b.r0 = range[4 * q + 2] - 1;
-        b.r1 = range[4 * q + 3] - 1;
-        b.adm = adm[q] > 0 ? 1 : 0;
+// Copy a device block (column-major, leading dimension ld) into part w of b as dense.
+static int read_dense(HBlock& b, int w, const double* src, int64_t ld) {
+    const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+    b.kind[w] = QDFN_HBLK_DENSE;
+    b.rank[w] = 0;
+    b.a[w].resize(m * k);
+    b.b[w].clear();
+    QDFN_HIP(hipMemcpy2D(b.a[w].data(), sizeof(double) * m, src, sizeof(double) * ld, sizeof(double) * m, k,
+                         hipMemcpyDeviceToHost));
+    return QDFN_OK;
+}
+
+int64_t aca_rank_cap(int64_t m, int64_t k) { return std::min<int64_t>(256, m * k / (m + k)); }
+
+int block_norms(const std::vector<const double*>& src, const std::vector<int64_t>& ld,
+                const std::vector<int32_t>& m, const std::vector<int32_t>& k, std::vector<double>& out,
+                hipStream_t stream) {
+    const size_t nb = src.size();
+    out.assign(nb, 0.0);
+    if (nb == 0) return QDFN_OK;
+    std::vector<MaxJob> mj(nb);
+    for (size_t q = 0; q < nb; ++q) {
+        mj[q].src = src[q];
+        mj[q].ld = ld[q];
+        mj[q].m = m[q];
+        mj[q].k = k[q];
     }
-    const double* full[2] = {dKS, dKN};
-    auto read_dense = [&](HBlock& b, int w) -> int {
-        const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
-        b.kind[w] = QDFN_HBLK_DENSE;
-        b.rank[w] = 0;
-        b.a[w].resize(m * k);
-        b.b[w].clear();
-        QDFN_HIP(hipMemcpy2D(b.a[w].data(), sizeof(double) * m, full[w] + b.r0 + b.s0 * n, sizeof(double) * n,
-                             sizeof(double) * m, k, hipMemcpyDeviceToHost));
-        return QDFN_OK;
-    };
-    // dense blocks
-    std::vector<std::pair<int64_t, int>> lr;  // (block, which) to compress
-    for (int64_t q = 0; q < nblk; ++q)
-        for (int w = 0; w < 2; ++w) {
-            HBlock& b = h->blocks[q];
-            const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
-            const int64_t rmax = std::min<int64_t>(256, m * k / (m + k));
-            if (!b.adm || rmax < 1 || m * k > 0x7fffffffLL) {
-                int rc = read_dense(b, w);
-                if (rc) return rc;
-            } else {
-                lr.push_back({q, w});
-            }
-        }
-    // threshold scale: ||K_S,b||_F of every block (device reduction)
     DevBuf dmax, dmj;
+    QDFN_HIP(dmax.alloc(sizeof(double) * nb));
+    QDFN_HIP(dmj.alloc(sizeof(MaxJob) * nb));
+    QDFN_HIP(hipMemcpy(dmj.p, mj.data(), sizeof(MaxJob) * nb, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_blocknorm, dim3((unsigned)nb), dim3(kAcaThreads), 0, stream, dmj.as<MaxJob>(),
+                       dmax.as<double>());
+    QDFN_HIP(hipGetLastError());
+    QDFN_HIP(hipStreamSynchronize(stream));
+    QDFN_HIP(hipMemcpy(out.data(), dmax.p, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    return QDFN_OK;
+}
+
+int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, const std::vector<double>& bnorm,
+                           double rtol, double atol) {
+    const double rc0 = h->build_phase_ms[3];
+    double total = 0.0;  // full-pivot time: this scope minus the recompression inside it
     {
-        std::vector<MaxJob> mj(nblk);
-        for (int64_t q = 0; q < nblk; ++q) {
-            const HBlock& b = h->blocks[q];
-            mj[q].src = dKS + b.r0 + b.s0 * n;
-            mj[q].ld = n;
-            mj[q].m = (int32_t)(b.r1 - b.r0 + 1);
-            mj[q].k = (int32_t)(b.s1 - b.s0 + 1);
-        }
-        QDFN_HIP(dmax.alloc(sizeof(double) * nblk));
-        QDFN_HIP(dmj.alloc(sizeof(MaxJob) * nblk));
-        QDFN_HIP(hipMemcpy(dmj.p, mj.data(), sizeof(MaxJob) * nblk, hipMemcpyHostToDevice));
-        hipLaunchKernelGGL(k_blocknorm, dim3((unsigned)nblk), dim3(kAcaThreads), 0, h->stream, dmj.as<MaxJob>(),
-                           dmax.as<double>());
-        QDFN_HIP(hipGetLastError());
-        QDFN_HIP(hipStreamSynchronize(h->stream));
-    }
-    std::vector<double> bnorm(nblk);
-    QDFN_HIP(hipMemcpy(bnorm.data(), dmax.p, sizeof(double) * nblk, hipMemcpyDeviceToHost));
+        ScopeMs whole(&total);
+    // threshold scales on the device, indexed by block
+    DevBuf dscale;
+    QDFN_HIP(dscale.alloc(sizeof(double) * std::max<size_t>(1, bnorm.size())));
+    if (!bnorm.empty())
+        QDFN_HIP(hipMemcpy(dscale.p, bnorm.data(), sizeof(double) * bnorm.size(), hipMemcpyHostToDevice));
     // admissible blocks in batches bounded by a workspace budget
     size_t free_b = 0, total_b = 0;
     QDFN_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -1090,9 +1083,9 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
         int64_t used = 0;
         size_t end = pos;
         while (end < lr.size()) {
-            const HBlock& b = h->blocks[lr[end].first];
+            const HBlock& b = h->blocks[lr[end].q];
             const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
-            const int64_t rmax = std::min<int64_t>(256, m * k / (m + k));
+            const int64_t rmax = aca_rank_cap(m, k);
             const int64_t need = m * k + m * rmax + rmax * k;
             if (end > pos && (used + need) * 8 > budget) break;
             off.push_back(used);
@@ -1102,14 +1095,13 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
         DevBuf ws, djobs, drank, dconv, dres;
         QDFN_HIP(ws.alloc(sizeof(double) * used));
         for (size_t q = pos; q < end; ++q) {
-            const HBlock& b = h->blocks[lr[q].first];
-            const int w = lr[q].second;
+            const HBlock& b = h->blocks[lr[q].q];
             const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
-            const int64_t rmax = std::min<int64_t>(256, m * k / (m + k));
+            const int64_t rmax = aca_rank_cap(m, k);
             AcaJob J;
-            J.src = full[w] + b.r0 + b.s0 * n;
-            J.scale = dmax.as<double>() + lr[q].first;
-            J.ld = n;
+            J.src = lr[q].src;
+            J.scale = dscale.as<double>() + lr[q].q;
+            J.ld = lr[q].ld;
             J.R = ws.as<double>() + off[q - pos];
             J.U = J.R + m * k;
             J.V = J.U + m * rmax;
@@ -1139,12 +1131,12 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
         QDFN_HIP(hipMemcpy(res.data(), dres.p, sizeof(double) * nj, hipMemcpyDeviceToHost));
         std::vector<size_t> to_recompress;
         for (size_t q = pos; q < end; ++q) {
-            HBlock& b = h->blocks[lr[q].first];
-            const int w = lr[q].second;
+            HBlock& b = h->blocks[lr[q].q];
+            const int w = lr[q].w;
             const AcaJob& J = jobs[q - pos];
             const int64_t m = J.m, k = J.k, r = rk[q - pos];
             if (!cv[q - pos]) {  // rank cap reached: keep the block dense
-                int rc = read_dense(b, w);
+                int rc = read_dense(b, w, lr[q].src, lr[q].ld);
                 if (rc) return rc;
                 continue;
             }
@@ -1162,25 +1154,84 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
             }
         }
         // recompression on the host threads (blocks are independent)
-        std::atomic<size_t> next(0);
-        auto worker = [&]() {
-            for (size_t i = next++; i < to_recompress.size(); i = next++) {
-                const size_t q = to_recompress[i];
-                HBlock& b = h->blocks[lr[q].first];
-                const int w = lr[q].second;
-                const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
-                const double budget = rtol * bnorm[lr[q].first] - res[q - pos];
-                recompress(m, k, b.rank[w], b.a[w], b.b[w], budget);
-            }
-        };
-        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        std::vector<std::thread> pool;
-        for (unsigned i = 1; i < nt; ++i) pool.emplace_back(worker);
-        worker();
-        for (auto& th : pool) th.join();
+        std::vector<double> budgets(to_recompress.size());
+        for (size_t i = 0; i < to_recompress.size(); ++i) {
+            const size_t q = to_recompress[i];
+            budgets[i] = rtol * bnorm[lr[q].q] - res[q - pos];
+        }
+        recompress_parallel(h, lr, to_recompress, budgets);
         pos = end;
     }
+    }
+    h->build_phase_ms[2] += total - (h->build_phase_ms[3] - rc0);
     return QDFN_OK;
+}
+
+void recompress_parallel(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, const std::vector<size_t>& items,
+                         const std::vector<double>& budgets) {
+    ScopeMs timer(&h->build_phase_ms[3]);
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+        for (size_t i = next++; i < items.size(); i = next++) {
+            HBlock& b = h->blocks[lr[items[i]].q];
+            const int w = lr[items[i]].w;
+            const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+            recompress(m, k, b.rank[w], b.a[w], b.b[w], budgets[i]);
+        }
+    };
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned i = 1; i < nt; ++i) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+}
+
+void init_blocks(qdfn_hmat* h, int64_t nblk, const int64_t* range, const int64_t* adm) {
+    h->blocks.resize(nblk);
+    for (int64_t q = 0; q < nblk; ++q) {
+        HBlock& b = h->blocks[q];
+        b.s0 = range[4 * q] - 1;
+        b.s1 = range[4 * q + 1] - 1;
+        b.r0 = range[4 * q + 2] - 1;
+        b.r1 = range[4 * q + 3] - 1;
+        b.adm = adm[q] > 0 ? 1 : 0;
+    }
+}
+
+// Compress / read out every block from full device matrices KS, KN (n x n, ld n).
+static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, int64_t nblk, const int64_t* range,
+                            const int64_t* adm, double rtol, double atol) {
+    const int64_t n = h->n;
+    init_blocks(h, nblk, range, adm);
+    const double* full[2] = {dKS, dKN};
+    // dense blocks
+    std::vector<DeviceBlock> lr;  // parts to compress
+    for (int64_t q = 0; q < nblk; ++q)
+        for (int w = 0; w < 2; ++w) {
+            HBlock& b = h->blocks[q];
+            const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+            const double* src = full[w] + b.r0 + b.s0 * n;
+            if (!b.adm || aca_rank_cap(m, k) < 1 || m * k > 0x7fffffffLL) {
+                int rc = read_dense(b, w, src, n);
+                if (rc) return rc;
+            } else {
+                lr.push_back({q, w, src, n});
+            }
+        }
+    // threshold scale: ||K_S,b||_F of every block (device reduction)
+    std::vector<const double*> ns(nblk);
+    std::vector<int64_t> nld(nblk, n);
+    std::vector<int32_t> nm(nblk), nk(nblk);
+    for (int64_t q = 0; q < nblk; ++q) {
+        const HBlock& b = h->blocks[q];
+        ns[q] = dKS + b.r0 + b.s0 * n;
+        nm[q] = (int32_t)(b.r1 - b.r0 + 1);
+        nk[q] = (int32_t)(b.s1 - b.s0 + 1);
+    }
+    std::vector<double> bnorm;
+    int rc = block_norms(ns, nld, nm, nk, bnorm, h->stream);
+    if (rc) return rc;
+    return compress_device_blocks(h, lr, bnorm, rtol, atol);
 }
 
 }  // namespace qdfn
@@ -1195,6 +1246,7 @@ extern "C" qdfn_hmat_t* qdfn_hmat_build_rect(const double* seg, int64_t n, int64
         return nullptr;
     }
     if (check_blocks(n, nblk, range_SR, admissible)) return nullptr;
+    const auto t_build = std::chrono::steady_clock::now();
     qdfn_hmat* h = new_hmat(n, prec);
     if (!h) return nullptr;
     DevBuf drcv, dks, dkn;
@@ -1206,14 +1258,18 @@ extern "C" qdfn_hmat_t* qdfn_hmat_build_rect(const double* seg, int64_t n, int64
         rc = fail(QDFN_ERR_HIP, "qdfn_hmat_build_rect: upload failed");
     if (rc == QDFN_OK && (hipMemsetAsync(dks.p, 0, b, h->stream) || hipMemsetAsync(dkn.p, 0, b, h->stream)))
         rc = fail(QDFN_ERR_HIP, "memset");
-    if (rc == QDFN_OK)
+    if (rc == QDFN_OK) {
+        ScopeMs timer(&h->build_phase_ms[0]);
         rc = assemble_rect_device(seg, n, ld, drcv.as<double>(), n, ld, mu, nu, dks.as<double>(), dkn.as<double>(), n,
                                   h->stream);
+    }
     if (rc == QDFN_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = fail(QDFN_ERR_HIP, "assembly");
     if (rc == QDFN_OK)
         rc = blocks_from_full(h, dks.as<double>(), dkn.as<double>(), nblk, range_SR, admissible, aca_rtol, aca_atol);
     dks.release();
     dkn.release();
+    h->build_entries = n * n;
+    h->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
     if (rc == QDFN_OK) rc = hmat_layout(h);
     if (rc != QDFN_OK) {
         qdfn_hmat_destroy(h);
@@ -1237,6 +1293,7 @@ extern "C" qdfn_hmat_t* qdfn_hmat_build_tri(int64_t n, const double* P1, const d
             fail(QDFN_ERR_ARG, "Half-space solution: Z coordinates must be negative!");
             return nullptr;
         }
+    const auto t_build = std::chrono::steady_clock::now();
     qdfn_hmat* h = new_hmat(n, prec);
     if (!h) return nullptr;
     DevBuf d1, d2, d3, dr, dc, dn, dsl, dks, dkn;
@@ -1250,14 +1307,18 @@ extern "C" qdfn_hmat_t* qdfn_hmat_build_tri(int64_t n, const double* P1, const d
     for (int k = 0; rc == QDFN_OK && k < 7; ++k)
         if (hipMemcpy(dst[k]->p, src[k], k == 3 ? sizeof(double) * n : v3, hipMemcpyHostToDevice) != hipSuccess)
             rc = fail(QDFN_ERR_HIP, "qdfn_hmat_build_tri: upload failed");
-    if (rc == QDFN_OK)
+    if (rc == QDFN_OK) {
+        ScopeMs timer(&h->build_phase_ms[0]);
         rc = assemble_tri_device(n, d1.as<double>(), d2.as<double>(), d3.as<double>(), dr.as<double>(), n,
                                  dc.as<double>(), dn.as<double>(), dsl.as<double>(), mu, lambda, dks.as<double>(),
                                  dkn.as<double>(), n, h->stream);
+    }
     if (rc == QDFN_OK)
         rc = blocks_from_full(h, dks.as<double>(), dkn.as<double>(), nblk, range_SR, admissible, aca_rtol, aca_atol);
     dks.release();
     dkn.release();
+    h->build_entries = n * n;
+    h->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
     if (rc == QDFN_OK) rc = hmat_layout(h);
     if (rc != QDFN_OK) {
         qdfn_hmat_destroy(h);

@@ -7,6 +7,8 @@
 //                       rotation -> shear/normal traction, one thread per
 //                       (receiver, source) pair, nothing staged through HBM
 //                       but the two output entries.
+//   k_rect_lines      - the same entries along sample lines (rows / columns of
+//                       H-matrix blocks chosen by the cross approximation).
 // Bound: FP64 VALU (transcendental-heavy); output is 16 B per pair.
 #include "okada_device.h"
 #include "qdfn_internal.h"
@@ -34,15 +36,10 @@ struct RectSource {
     SlipW s;
 };
 
-__global__ __launch_bounds__(256) void k_stiffness_rect(const RectSource* __restrict__ srcs, int64_t ns,
-                                                        const double* __restrict__ rcv, int64_t nr, int64_t ldr,
-                                                        double mu, double lam, double* __restrict__ KS,
-                                                        double* __restrict__ KN, int64_t ldk, int64_t rblocks) {
-    const int64_t bid = blockIdx.x;
-    const int64_t s = bid / rblocks;
-    const int64_t r = (bid - s * rblocks) * 256 + threadIdx.x;
-    if (s >= ns || r >= nr) return;
-    const RectSource src = srcs[s];
+// One stiffness entry: shear / normal traction at receiver r (row r of the
+// receiver table, leading dimension ldr) from unit slip on source `src`.
+__device__ __forceinline__ void rect_entry(const RectSource& src, const double* __restrict__ rcv, int64_t r,
+                                           int64_t ldr, double mu, double lam, double& ks, double& kn) {
     const double xd = rcv[r] - src.x, yd = rcv[r + ldr] - src.y;
     const double X = xd * src.cs - yd * src.sn;
     const double Y = xd * src.sn + yd * src.cs;
@@ -79,8 +76,43 @@ __global__ __launch_bounds__(256) void k_stiffness_rect(const RectSource* __rest
     const double s23 = RS[1][0] * R[2][0] + RS[1][1] * R[2][1] + RS[1][2] * R[2][2];
     const double s33 = RS[2][0] * R[2][0] + RS[2][1] * R[2][1] + RS[2][2] * R[2][2];
     const double rrake = rcv[r + 7 * ldr];
-    KN[r + s * ldk] = -s33;
-    KS[r + s * ldk] = cos_deg(rrake) * s13 + sin_deg(rrake) * s23;
+    kn = -s33;
+    ks = cos_deg(rrake) * s13 + sin_deg(rrake) * s23;
+}
+
+__global__ __launch_bounds__(256) void k_stiffness_rect(const RectSource* __restrict__ srcs, int64_t ns,
+                                                        const double* __restrict__ rcv, int64_t nr, int64_t ldr,
+                                                        double mu, double lam, double* __restrict__ KS,
+                                                        double* __restrict__ KN, int64_t ldk, int64_t rblocks) {
+    const int64_t bid = blockIdx.x;
+    const int64_t s = bid / rblocks;
+    const int64_t r = (bid - s * rblocks) * 256 + threadIdx.x;
+    if (s >= ns || r >= nr) return;
+    const RectSource src = srcs[s];
+    double ks, kn;
+    rect_entry(src, rcv, r, ldr, mu, lam, ks, kn);
+    KN[r + s * ldk] = kn;
+    KS[r + s * ldk] = ks;
+}
+
+// Entries along sample lines (sampled H-matrix construction): wavefront w of
+// workgroup b takes tile 4 b + w, 64 consecutive entries of one line.
+__global__ __launch_bounds__(256) void k_rect_lines(const RectSource* __restrict__ srcs,
+                                                    const double* __restrict__ rcv, int64_t ldr, double mu,
+                                                    double lam, const SampleLine* lines,
+                                                    const SampleTile* __restrict__ tiles, int64_t ntiles,
+                                                    double* __restrict__ outS, double* __restrict__ outN) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ntiles) return;
+    const SampleTile tl = tiles[t];
+    SampleLine L;
+    if (!read_line(lines, tl, L)) return;
+    const int32_t e = tl.first + (int32_t)(threadIdx.x & 63);
+    if (e >= L.len) return;
+    const int64_t r = L.rcv + (int64_t)e * L.drcv, s = L.src + (int64_t)e * L.dsrc;
+    double ks, kn;
+    rect_entry(srcs[s], rcv, r, ldr, mu, lam, ks, kn);
+    write_entry(L, e, ks, kn, outS, outN);
 }
 
 static SlipW make_slip(double d1, double d2, double d3) {
@@ -93,14 +125,8 @@ static SlipW make_slip(double d1, double d2, double d3) {
     return s;
 }
 
-// Device-side assembly into caller-provided device buffers (used by the
-// public entry points and by the dense operator's on-device build).
-int assemble_rect_device(const double* src_host, int64_t ns, int64_t lds, const double* d_rcv, int64_t nr,
-                         int64_t ldr, double mu, double nu, double* d_KS, double* d_KN, int64_t ldk,
-                         hipStream_t stream) {
-    if (ns <= 0 || nr <= 0) return QDFN_OK;
-    const double lam = 2.0 * mu * nu / (1.0 - 2.0 * nu);
-    const double alpha = (lam + mu) / (lam + 2.0 * mu);
+// Per-source constants of a HOST segment table (n x >= 8, leading dimension lds).
+static RectSource* rect_sources(const double* src_host, int64_t ns, int64_t lds, double alpha) {
     RectSource* h = new RectSource[ns];
     for (int64_t s = 0; s < ns; ++s) {
         RectSource& q = h[s];
@@ -116,9 +142,22 @@ int assemble_rect_device(const double* src_host, int64_t ns, int64_t lds, const 
         const double rake = src_host[s + 7 * lds];
         q.s = make_slip(cos_deg(rake), sin_deg(rake), 0.0);
     }
+    return h;
+}
+
+// Device-side assembly into caller-provided device buffers (used by the
+// public entry points and by the dense operator's on-device build).
+int assemble_rect_device(const double* src_host, int64_t ns, int64_t lds, const double* d_rcv, int64_t nr,
+                         int64_t ldr, double mu, double nu, double* d_KS, double* d_KN, int64_t ldk,
+                         hipStream_t stream) {
+    if (ns <= 0 || nr <= 0) return QDFN_OK;
+    const double lam = 2.0 * mu * nu / (1.0 - 2.0 * nu);
+    const double alpha = (lam + mu) / (lam + 2.0 * mu);
+    RectSource* h = rect_sources(src_host, ns, lds, alpha);
     DevBuf d_src;
     hipError_t e = d_src.alloc(sizeof(RectSource) * ns);
     if (e == hipSuccess) e = hipMemcpyAsync(d_src.p, h, sizeof(RectSource) * ns, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
     delete[] h;
     if (e != hipSuccess) return fail(QDFN_ERR_HIP, std::string("stiffness_rect upload: ") + hipGetErrorString(e));
     const int64_t rblocks = (nr + 255) / 256;
@@ -128,6 +167,39 @@ int assemble_rect_device(const double* src_host, int64_t ns, int64_t lds, const 
                        d_rcv, nr, ldr, mu, lam, d_KS, d_KN, ldk, rblocks);
     QDFN_HIP(hipGetLastError());
     QDFN_HIP(hipStreamSynchronize(stream));  // d_src is released on return
+    return QDFN_OK;
+}
+
+int make_rect_entries(EntrySource& es, const double* seg, int64_t n, int64_t ld, double mu, double nu,
+                      hipStream_t stream) {
+    if (n <= 0 || ld < n || !seg) return fail(QDFN_ERR_ARG, "rectangle entries: bad table");
+    es.kind = 0;
+    es.n = n;
+    es.mu = mu;
+    es.lam = 2.0 * mu * nu / (1.0 - 2.0 * nu);
+    const double alpha = (es.lam + mu) / (es.lam + 2.0 * mu);
+    RectSource* h = rect_sources(seg, n, ld, alpha);
+    hipError_t e = es.srcs.alloc(sizeof(RectSource) * n);
+    if (e == hipSuccess) e = es.tab.alloc(sizeof(double) * n * 8);
+    if (e == hipSuccess) e = hipMemcpyAsync(es.srcs.p, h, sizeof(RectSource) * n, hipMemcpyHostToDevice, stream);
+    // receiver table with leading dimension n
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(es.tab.p, sizeof(double) * n, seg, sizeof(double) * ld, sizeof(double) * n, 8,
+                             hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    delete[] h;
+    if (e != hipSuccess) return fail(QDFN_ERR_HIP, std::string("rectangle entries upload: ") + hipGetErrorString(e));
+    return QDFN_OK;
+}
+
+int launch_rect_lines(const EntrySource& es, const SampleLine* d_lines, const SampleTile* d_tiles, int64_t ntiles,
+                      double* d_outS, double* d_outN, hipStream_t stream) {
+    if (ntiles <= 0) return QDFN_OK;
+    const int64_t blocks = (ntiles + 3) / 4;
+    if (blocks > 0x7fffffffLL) return fail(QDFN_ERR_ARG, "rectangle lines: too many tiles for one launch");
+    hipLaunchKernelGGL(k_rect_lines, dim3((unsigned)blocks), dim3(256), 0, stream, es.srcs.as<RectSource>(),
+                       es.tab.as<double>(), es.n, es.mu, es.lam, d_lines, d_tiles, ntiles, d_outS, d_outN);
+    QDFN_HIP(hipGetLastError());
     return QDFN_OK;
 }
 

@@ -4,6 +4,8 @@
 //   k_tde_points       TDstressHS at n points for one triangle
 //   k_tde_stiffness    fused TDstressHS -> traction -> normal / slip projection,
 //                      one lane per (receiver, source) pair
+//   k_tde_lines        the same entries along sample lines (rows / columns of
+//                      H-matrix blocks chosen by the cross approximation)
 //   k_rotverts         RotVerts_UnitVectors, one lane per triangle
 // Bound: FP64 VALU (six angular-dislocation strains and six free-surface
 // corrections per pair); output 16 B per pair.
@@ -328,7 +330,21 @@ __global__ __launch_bounds__(256) void k_tde_points(int64_t n, const double* __r
 }
 
 // StiffnessMatrix_ByParts_Calculation_Tri (:551-594): t = sigma n_r,
-// K_N = t . n_r (tension positive), K_S = t . slip_r.
+// K_N = t . n_r (tension positive), K_S = t . slip_r.  One entry: receiver r
+// (ctr / nrm / slip n x 3, leading dimension nr) and source triangle src.
+__device__ __forceinline__ void tri_entry(const TriSource& src, const double* __restrict__ ctr,
+                                          const double* __restrict__ nrm, const double* __restrict__ slip,
+                                          int64_t r, int64_t nr, double mu, double lam, double& ks, double& kn) {
+    double S[6], E[6];
+    tde_point(ctr[r], ctr[r + nr], ctr[r + 2 * nr], src, mu, lam, S, E);
+    const double n0 = nrm[r], n1 = nrm[r + nr], n2 = nrm[r + 2 * nr];
+    const double t0 = S[0] * n0 + S[3] * n1 + S[4] * n2;
+    const double t1 = S[3] * n0 + S[1] * n1 + S[5] * n2;
+    const double t2 = S[4] * n0 + S[5] * n1 + S[2] * n2;
+    kn = t0 * n0 + t1 * n1 + t2 * n2;
+    ks = t0 * slip[r] + t1 * slip[r + nr] + t2 * slip[r + 2 * nr];
+}
+
 __global__ __launch_bounds__(256) void k_tde_stiffness(const TriSource* __restrict__ src, int64_t ns,
                                                        const double* __restrict__ ctr, const double* __restrict__ nrm,
                                                        const double* __restrict__ slip, int64_t nr, double mu,
@@ -338,14 +354,30 @@ __global__ __launch_bounds__(256) void k_tde_stiffness(const TriSource* __restri
     const int64_t s = bid / rblocks;
     const int64_t r = (bid - s * rblocks) * 256 + threadIdx.x;
     if (s >= ns || r >= nr) return;
-    double S[6], E[6];
-    tde_point(ctr[r], ctr[r + nr], ctr[r + 2 * nr], src[s], mu, lam, S, E);
-    const double n0 = nrm[r], n1 = nrm[r + nr], n2 = nrm[r + 2 * nr];
-    const double t0 = S[0] * n0 + S[3] * n1 + S[4] * n2;
-    const double t1 = S[3] * n0 + S[1] * n1 + S[5] * n2;
-    const double t2 = S[4] * n0 + S[5] * n1 + S[2] * n2;
-    KN[r + s * ldk] = t0 * n0 + t1 * n1 + t2 * n2;
-    KS[r + s * ldk] = t0 * slip[r] + t1 * slip[r + nr] + t2 * slip[r + 2 * nr];
+    double ks, kn;
+    tri_entry(src[s], ctr, nrm, slip, r, nr, mu, lam, ks, kn);
+    KN[r + s * ldk] = kn;
+    KS[r + s * ldk] = ks;
+}
+
+// Entries along sample lines (sampled H-matrix construction): wavefront w of
+// workgroup b takes tile 4 b + w, 64 consecutive entries of one line.
+__global__ __launch_bounds__(256) void k_tde_lines(const TriSource* __restrict__ src, const double* __restrict__ ctr,
+                                                   const double* __restrict__ nrm, const double* __restrict__ slip,
+                                                   int64_t n, double mu, double lam, const SampleLine* lines,
+                                                   const SampleTile* __restrict__ tiles, int64_t ntiles,
+                                                   double* __restrict__ outS, double* __restrict__ outN) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ntiles) return;
+    const SampleTile tl = tiles[t];
+    SampleLine L;
+    if (!read_line(lines, tl, L)) return;
+    const int32_t e = tl.first + (int32_t)(threadIdx.x & 63);
+    if (e >= L.len) return;
+    const int64_t r = L.rcv + (int64_t)e * L.drcv, s = L.src + (int64_t)e * L.dsrc;
+    double ks, kn;
+    tri_entry(src[s], ctr, nrm, slip, r, n, mu, lam, ks, kn);
+    write_entry(L, e, ks, kn, outS, outN);
 }
 
 // RotVerts_UnitVectors (:338-377), P1..P3 and outputs n x 3 column-major.
@@ -425,6 +457,50 @@ int assemble_tri_device(int64_t ns, const double* d_P1, const double* d_P2, cons
                        d_nrm, d_slip, nr, mu, lambda, d_KS, d_KN, ldk, rblocks);
     QDFN_HIP(hipGetLastError());
     QDFN_HIP(hipStreamSynchronize(stream));
+    return QDFN_OK;
+}
+
+int make_tri_entries(EntrySource& es, int64_t n, const double* P1, const double* P2, const double* P3,
+                     const double* rake, const double* ctr, const double* nrm, const double* slip, double mu,
+                     double lambda, hipStream_t stream) {
+    if (n <= 0 || !P1 || !P2 || !P3 || !rake || !ctr || !nrm || !slip)
+        return fail(QDFN_ERR_ARG, "triangle entries: bad inputs");
+    es.kind = 1;
+    es.n = n;
+    es.mu = mu;
+    es.lam = lambda;
+    const double nu = 1 / (1 + lambda / mu) / 2;
+    DevBuf d1, d2, d3, dr;
+    const size_t v3 = sizeof(double) * 3 * n;
+    QDFN_HIP(d1.alloc(v3));
+    QDFN_HIP(d2.alloc(v3));
+    QDFN_HIP(d3.alloc(v3));
+    QDFN_HIP(dr.alloc(sizeof(double) * n));
+    QDFN_HIP(es.tab.alloc(3 * v3));
+    QDFN_HIP(hipMemcpyAsync(d1.p, P1, v3, hipMemcpyHostToDevice, stream));
+    QDFN_HIP(hipMemcpyAsync(d2.p, P2, v3, hipMemcpyHostToDevice, stream));
+    QDFN_HIP(hipMemcpyAsync(d3.p, P3, v3, hipMemcpyHostToDevice, stream));
+    QDFN_HIP(hipMemcpyAsync(dr.p, rake, sizeof(double) * n, hipMemcpyHostToDevice, stream));
+    QDFN_HIP(hipMemcpyAsync(es.tab.as<double>(), ctr, v3, hipMemcpyHostToDevice, stream));
+    QDFN_HIP(hipMemcpyAsync(es.tab.as<double>() + 3 * n, nrm, v3, hipMemcpyHostToDevice, stream));
+    QDFN_HIP(hipMemcpyAsync(es.tab.as<double>() + 6 * n, slip, v3, hipMemcpyHostToDevice, stream));
+    int rc = prepare_sources(n, d1.as<double>(), d2.as<double>(), d3.as<double>(), dr.as<double>(), nullptr, nu,
+                             es.srcs, stream);
+    if (rc) return rc;
+    QDFN_HIP(hipStreamSynchronize(stream));  // d1..dr are released on return
+    return QDFN_OK;
+}
+
+int launch_tri_lines(const EntrySource& es, const SampleLine* d_lines, const SampleTile* d_tiles, int64_t ntiles,
+                     double* d_outS, double* d_outN, hipStream_t stream) {
+    if (ntiles <= 0) return QDFN_OK;
+    const int64_t blocks = (ntiles + 3) / 4;
+    if (blocks > 0x7fffffffLL) return fail(QDFN_ERR_ARG, "triangle lines: too many tiles for one launch");
+    const double* tab = es.tab.as<double>();
+    hipLaunchKernelGGL(k_tde_lines, dim3((unsigned)blocks), dim3(256), 0, stream, es.srcs.as<TriSource>(), tab,
+                       tab + 3 * es.n, tab + 6 * es.n, es.n, es.mu, es.lam, d_lines, d_tiles, ntiles, d_outS,
+                       d_outN);
+    QDFN_HIP(hipGetLastError());
     return QDFN_OK;
 }
 

@@ -59,6 +59,11 @@ class AdjointInfo(C.Structure):
                 ("retried_steps", I64), ("t_first", D), ("ms_backward", D)]
 
 
+class HmatBuildStats(C.Structure):
+    _fields_ = [("entries_evaluated", I64), ("steps", I64), ("parts_sampled", I64), ("blocks_fallback", I64),
+                ("ms_build", D), ("ms_phase", D * 5)]
+
+
 class HmatParams(C.Structure):
     _fields_ = [("min_elements_to_cut", I64), ("max_levels", I64), ("dist_diam_ratio", D),
                 ("arrange_point", D * 3), ("n_loading", I64), ("kmeans_max_iter", I64)]
@@ -90,6 +95,9 @@ SIGNATURES = {
                                       PI64, PI64]),
     "qdfn_hmat_build_rect": (VP, [PD, I64, I64, D, D, I64, PI64, PI64, D, D, I32]),
     "qdfn_hmat_build_tri": (VP, [I64, PD, PD, PD, PD, PD, PD, PD, D, D, I64, PI64, PI64, D, D, I32]),
+    "qdfn_hmat_build_rect_sampled": (VP, [PD, I64, I64, D, D, I64, PI64, PI64, D, D, I32]),
+    "qdfn_hmat_build_tri_sampled": (VP, [I64, PD, PD, PD, PD, PD, PD, PD, D, D, I64, PI64, PI64, D, D, I32]),
+    "qdfn_hmat_build_info": (C.c_int, [VP, C.c_void_p]),
     "qdfn_hmat_create": (VP, [I64, I64, PI64, PI64, PI32, PI64, PPD, PPD, PI32, PI64, PPD, PPD, I32]),
     "qdfn_hmat_destroy": (None, [VP]),
     "qdfn_hmat_info": (C.c_int, [VP, PI64, PI64, PI64, PI64, PI64]),

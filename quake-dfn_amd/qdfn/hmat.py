@@ -76,28 +76,35 @@ class HMatrix:
         self.prec = prec
 
     @classmethod
-    def build_rect(cls, seg, mu, nu, range_SR, admissible, aca_rtol=1e-6, aca_atol=0.0, prec=_lib.PREC_FP64):
-        """HmatBuild_R on a segment table already in sorted order (n x >= 8)."""
+    def build_rect(cls, seg, mu, nu, range_SR, admissible, aca_rtol=1e-6, aca_atol=0.0, prec=_lib.PREC_FP64,
+                   sampled=False):
+        """HmatBuild_R on a segment table already in sorted order (n x >= 8).
+        sampled=True: admissible blocks from sampled rows and columns only
+        (qdfn_hmat_build_rect_sampled), never assembled in full."""
         seg = np.asfortranarray(seg, dtype=np.float64)
         n = seg.shape[0]
         rs, ad = _i64(range_SR), _i64(admissible)
-        h = check_handle(lib().qdfn_hmat_build_rect(ptr(seg), n, n, float(mu), float(nu), ad.size,
-                                                    rs.ctypes.data_as(PI64), ad.ctypes.data_as(PI64),
-                                                    float(aca_rtol), float(aca_atol), prec), "qdfn_hmat_build_rect")
+        name = "qdfn_hmat_build_rect_sampled" if sampled else "qdfn_hmat_build_rect"
+        h = check_handle(getattr(lib(), name)(ptr(seg), n, n, float(mu), float(nu), ad.size,
+                                              rs.ctypes.data_as(PI64), ad.ctypes.data_as(PI64),
+                                              float(aca_rtol), float(aca_atol), prec), name)
         return cls(h, n, prec)
 
     @classmethod
     def build_tri(cls, P1, P2, P3, Rake, FaultCenter, UnitVector_Normal, UnitVector_Slip, ShearModulus, lam,
-                  range_SR, admissible, aca_rtol=1e-6, aca_atol=0.0, prec=_lib.PREC_FP64):
-        """HmatBuild_T on triangles already in sorted order."""
+                  range_SR, admissible, aca_rtol=1e-6, aca_atol=0.0, prec=_lib.PREC_FP64, sampled=False):
+        """HmatBuild_T on triangles already in sorted order.  sampled=True:
+        admissible blocks from sampled rows and columns only
+        (qdfn_hmat_build_tri_sampled), never assembled in full."""
         f = lambda a: np.asfortranarray(a, dtype=np.float64)
         P1, P2, P3, ctr, nrm, slip = map(f, (P1, P2, P3, FaultCenter, UnitVector_Normal, UnitVector_Slip))
         n = P1.shape[0]
         rs, ad = _i64(range_SR), _i64(admissible)
-        h = check_handle(lib().qdfn_hmat_build_tri(n, ptr(P1), ptr(P2), ptr(P3), ptr(_f64(Rake)), ptr(ctr),
-                                                   ptr(nrm), ptr(slip), float(ShearModulus), float(lam), ad.size,
-                                                   rs.ctypes.data_as(PI64), ad.ctypes.data_as(PI64),
-                                                   float(aca_rtol), float(aca_atol), prec), "qdfn_hmat_build_tri")
+        name = "qdfn_hmat_build_tri_sampled" if sampled else "qdfn_hmat_build_tri"
+        h = check_handle(getattr(lib(), name)(n, ptr(P1), ptr(P2), ptr(P3), ptr(_f64(Rake)), ptr(ctr),
+                                              ptr(nrm), ptr(slip), float(ShearModulus), float(lam), ad.size,
+                                              rs.ctypes.data_as(PI64), ad.ctypes.data_as(PI64),
+                                              float(aca_rtol), float(aca_atol), prec), name)
         return cls(h, n, prec)
 
     @classmethod
@@ -143,6 +150,16 @@ class HMatrix:
         check(lib().qdfn_hmat_info(self.handle, C.byref(n), C.byref(nb), C.byref(db), C.byref(se), pb))
         return {"n": n.value, "blocks": nb.value, "bytes_per_product": db.value, "stored_entries": se.value,
                 "phase1_bytes": pb[0], "phase2_bytes": pb[1]}
+
+    def build_info(self):
+        """Entries evaluated, sampled-approximation steps, parts finished by
+        sampling / evaluated in full, build wall time (ms)."""
+        st = _lib.HmatBuildStats()
+        check(lib().qdfn_hmat_build_info(self.handle, C.byref(st)))
+        out = {f: getattr(st, f) for f, _ in st._fields_ if f != "ms_phase"}
+        names = ("ms_evaluation", "ms_sampling", "ms_full_pivot", "ms_recompression", "ms_layout")
+        out.update({k: st.ms_phase[i] for i, k in enumerate(names)})
+        return out
 
     def time_phases(self, reps=20):
         """(phase-1 ms, phase-2 kernel ms, whole product ms), HIP events."""
