@@ -94,6 +94,15 @@ def divergence(tr, ref, rel=1e-3):
     return {"step": k, "sim_time_s": float(b[k])}
 
 
+def _build_summary(h):
+    b = h.build_info()
+    keys = ("entries_evaluated", "parts_sampled", "blocks_fallback", "steps", "ms_build", "ms_evaluation",
+            "ms_sampling", "ms_full_pivot", "ms_recompression", "ms_layout")
+    out = {k: (round(b[k], 1) if isinstance(b[k], float) else b[k]) for k in keys}
+    out["stored_entries"] = h.info()["stored_entries"]
+    return out
+
+
 def rough_fault(qd, args):
     from qdfn import configs
     m = configs.rough_fault()
@@ -117,6 +126,21 @@ def rough_fault(qd, args):
     log(f"[bench] H FP32 operator in {time.perf_counter() - t0:.2f}s: {info['blocks']} blocks, "
         f"{int(st['admissible'].sum())} admissible, {info['stored_entries'] / 1e6:.1f} M entries, "
         f"{info['bytes_per_product'] / 1e6:.1f} MB per product")
+    builds = {"full": _build_summary(h32)}
+    if not args.no_sampled_build:
+        # the same operator from sampled rows and columns (no full assembly)
+        hs = qd.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=args.aca_tol,
+                                  prec=qd.PREC_FP32, sampled=True)
+        builds["sampled"] = _build_summary(hs)
+        xr = np.random.default_rng(6).standard_normal(n)
+        a, b, c = h32.matvec(xr, loading_form=False), hs.matvec(xr, loading_form=False), d64.matvec(xr, False)
+        builds["product_rel_err_vs_dense_fp64"] = {
+            k: [float(np.linalg.norm(y[w] - c[w]) / np.linalg.norm(c[w])) for w in (0, 1)]
+            for k, y in (("full", a), ("sampled", b))}
+        hs.close()
+        log(f"[bench] H build full {builds['full']['ms_build']:.0f} ms ({builds['full']['entries_evaluated']} "
+            f"pairs), sampled {builds['sampled']['ms_build']:.0f} ms ({builds['sampled']['entries_evaluated']} "
+            f"pairs); product errors {builds['product_rel_err_vs_dense_fp64']}")
     d32 = qd.DenseOperator.from_triangles(*geo, m["mu"], lam, qd.PREC_FP32)
     ks, _ = d64.kself()
     ksum, knsum = d64.matvec(np.ones(n), loading_form=False)
@@ -142,7 +166,8 @@ def rough_fault(qd, args):
         summarize(name, r, configs.SEC_PER_YEAR)
         r["first_event"] = configs.first_event_time(r["trace"]["told"], r["trace"]["dt"], r["trace"]["vmax"])
         res[name] = r
-    out = dict(n=n, info=info, st=st, rk=rk, acc=acc, phases=(ph1, ph2, tot), res=res, h32=h32, prob_h=prob_h)
+    out = dict(n=n, info=info, st=st, rk=rk, acc=acc, phases=(ph1, ph2, tot), res=res, h32=h32, prob_h=prob_h,
+               builds=builds)
     d64.close()
     d32.close()
     return out
@@ -305,6 +330,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-sampled-build", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU layout is "
@@ -371,6 +397,7 @@ def main():
         "divergence_from_fp64": {"mixed_h": divergence(mixed["trace"], plain["trace"]),
                                  "fp32_dense": divergence(res["fp32_dense"]["trace"], plain["trace"])},
         "operator_rel_err": rf["acc"],
+        "h_build": rf["builds"],
         "network_8192": net,
         "ensemble_256": ens,
         "adjoint_4096": adj,

@@ -116,6 +116,17 @@ struct DeviceBlock {
     const double* src;
     int64_t ld;
 };
+// a rows x cols column-major device matrix (leading dimension ld) to pack at dst
+struct PackItem {
+    const double* src;
+    int64_t ld;
+    int32_t rows, cols;
+    int64_t dst;
+};
+// Copy the items to the host in one transfer: item i lands at host[items[i].dst]
+// (offsets assigned here, in order).
+int pack_to_host(std::vector<PackItem>& items, std::vector<double>& host, hipStream_t stream);
+
 // Frobenius norms of device blocks (fixed-order sums)
 int block_norms(const std::vector<const double*>& src, const std::vector<int64_t>& ld,
                 const std::vector<int32_t>& m, const std::vector<int32_t>& k, std::vector<double>& out,

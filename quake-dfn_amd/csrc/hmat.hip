@@ -1022,16 +1022,44 @@ void recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::v
     r = keep;
 }
 
-// Copy a device block (column-major, leading dimension ld) into part w of b as dense.
-static int read_dense(HBlock& b, int w, const double* src, int64_t ld) {
+__global__ __launch_bounds__(256) void k_pack(const PackItem* __restrict__ items, double* __restrict__ dst) {
+    const PackItem it = items[blockIdx.x];
+    const int64_t n = (int64_t)it.rows * it.cols;
+    for (int64_t e = threadIdx.x; e < n; e += 256) {
+        const int64_t i = e % it.rows, j = e / it.rows;
+        dst[it.dst + e] = it.src[i + j * it.ld];
+    }
+}
+
+int pack_to_host(std::vector<PackItem>& items, std::vector<double>& host, hipStream_t stream) {
+    int64_t total = 0;
+    for (PackItem& it : items) {
+        it.dst = total;
+        total += (int64_t)it.rows * it.cols;
+    }
+    host.resize(total);
+    if (items.empty() || total == 0) return QDFN_OK;
+    DevBuf d, di;
+    QDFN_HIP(d.alloc(sizeof(double) * total));
+    QDFN_HIP(di.alloc(sizeof(PackItem) * items.size()));
+    QDFN_HIP(hipMemcpyAsync(di.p, items.data(), sizeof(PackItem) * items.size(), hipMemcpyHostToDevice, stream));
+    for (size_t b = 0; b < items.size(); b += 0x7fffffff) {
+        const size_t nb = std::min<size_t>(items.size() - b, 0x7fffffff);
+        hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, stream, di.as<PackItem>() + b, d.as<double>());
+        QDFN_HIP(hipGetLastError());
+    }
+    QDFN_HIP(hipMemcpyAsync(host.data(), d.p, sizeof(double) * total, hipMemcpyDeviceToHost, stream));
+    QDFN_HIP(hipStreamSynchronize(stream));
+    return QDFN_OK;
+}
+
+// part w of b as dense from m x k packed entries
+static void set_dense(HBlock& b, int w, const double* a) {
     const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
     b.kind[w] = QDFN_HBLK_DENSE;
     b.rank[w] = 0;
-    b.a[w].resize(m * k);
+    b.a[w].assign(a, a + m * k);
     b.b[w].clear();
-    QDFN_HIP(hipMemcpy2D(b.a[w].data(), sizeof(double) * m, src, sizeof(double) * ld, sizeof(double) * m, k,
-                         hipMemcpyDeviceToHost));
-    return QDFN_OK;
 }
 
 int64_t aca_rank_cap(int64_t m, int64_t k) { return std::min<int64_t>(256, m * k / (m + k)); }
@@ -1129,15 +1157,30 @@ int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, con
         QDFN_HIP(hipMemcpy(rk.data(), drank.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost));
         QDFN_HIP(hipMemcpy(cv.data(), dconv.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost));
         QDFN_HIP(hipMemcpy(res.data(), dres.p, sizeof(double) * nj, hipMemcpyDeviceToHost));
+        // one packed copy: capped parts dense (from the source), the others U and V
+        std::vector<PackItem> items;
+        for (size_t q = pos; q < end; ++q) {
+            const AcaJob& J = jobs[q - pos];
+            const int32_t r = rk[q - pos];
+            if (!cv[q - pos]) {
+                items.push_back({lr[q].src, lr[q].ld, J.m, J.k, 0});
+            } else if (r > 0) {
+                items.push_back({J.U, J.m, J.m, r, 0});
+                items.push_back({J.V, J.k, J.k, r, 0});  // V row-major r x k, read as k x r
+            }
+        }
+        std::vector<double> hp;
+        int rc = pack_to_host(items, hp, h->stream);
+        if (rc) return rc;
         std::vector<size_t> to_recompress;
+        size_t ii = 0;
         for (size_t q = pos; q < end; ++q) {
             HBlock& b = h->blocks[lr[q].q];
             const int w = lr[q].w;
             const AcaJob& J = jobs[q - pos];
             const int64_t m = J.m, k = J.k, r = rk[q - pos];
             if (!cv[q - pos]) {  // rank cap reached: keep the block dense
-                int rc = read_dense(b, w, lr[q].src, lr[q].ld);
-                if (rc) return rc;
+                set_dense(b, w, hp.data() + items[ii++].dst);
                 continue;
             }
             b.kind[w] = QDFN_HBLK_LOWRANK;
@@ -1145,9 +1188,9 @@ int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, con
             b.a[w].resize(m * r);
             b.b[w].resize(r * k);
             if (r > 0) {
-                QDFN_HIP(hipMemcpy(b.a[w].data(), J.U, sizeof(double) * m * r, hipMemcpyDeviceToHost));
-                std::vector<double> vr(r * k);
-                QDFN_HIP(hipMemcpy(vr.data(), J.V, sizeof(double) * r * k, hipMemcpyDeviceToHost));
+                const double* u = hp.data() + items[ii++].dst;
+                const double* vr = hp.data() + items[ii++].dst;
+                std::copy(u, u + m * r, b.a[w].begin());
                 for (int64_t i = 0; i < r; ++i)
                     for (int64_t c = 0; c < k; ++c) b.b[w][i + c * r] = vr[i * k + c];
                 if (rtol > 0 && r > 1) to_recompress.push_back(q);
@@ -1204,20 +1247,29 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
     const int64_t n = h->n;
     init_blocks(h, nblk, range, adm);
     const double* full[2] = {dKS, dKN};
-    // dense blocks
+    // dense blocks (one packed copy)
     std::vector<DeviceBlock> lr;  // parts to compress
+    std::vector<PackItem> items;
+    std::vector<std::pair<int64_t, int>> dense_parts;
     for (int64_t q = 0; q < nblk; ++q)
         for (int w = 0; w < 2; ++w) {
             HBlock& b = h->blocks[q];
             const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
             const double* src = full[w] + b.r0 + b.s0 * n;
             if (!b.adm || aca_rank_cap(m, k) < 1 || m * k > 0x7fffffffLL) {
-                int rc = read_dense(b, w, src, n);
-                if (rc) return rc;
+                items.push_back({src, n, (int32_t)m, (int32_t)k, 0});
+                dense_parts.push_back({q, w});
             } else {
                 lr.push_back({q, w, src, n});
             }
         }
+    {
+        std::vector<double> hp;
+        int rc = pack_to_host(items, hp, h->stream);
+        if (rc) return rc;
+        for (size_t i = 0; i < items.size(); ++i)
+            set_dense(h->blocks[dense_parts[i].first], dense_parts[i].second, hp.data() + items[i].dst);
+    }
     // threshold scale: ||K_S,b||_F of every block (device reduction)
     std::vector<const double*> ns(nblk);
     std::vector<int64_t> nld(nblk, n);

@@ -434,16 +434,18 @@ int dense_blocks(const EntrySource& es, qdfn_hmat* h, const std::vector<int64_t>
         QDFN_HIP(dN.alloc(sizeof(double) * used));
         int rc = eval_full_blocks(es, h, sel, 2, off, dS.as<double>(), dN.as<double>(), bs, h->stream);
         if (rc) return rc;
+        std::vector<double> hS(used), hN(used);
+        QDFN_HIP(hipMemcpy(hS.data(), dS.p, sizeof(double) * used, hipMemcpyDeviceToHost));
+        QDFN_HIP(hipMemcpy(hN.data(), dN.p, sizeof(double) * used, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < sel.size(); ++i) {
             HBlock& b = h->blocks[sel[i]];
             const int64_t mk = (b.r1 - b.r0 + 1) * (b.s1 - b.s0 + 1);
             for (int w = 0; w < 2; ++w) {
+                const double* a = (w ? hN : hS).data() + off[i];
                 b.kind[w] = QDFN_HBLK_DENSE;
                 b.rank[w] = 0;
-                b.a[w].resize(mk);
+                b.a[w].assign(a, a + mk);
                 b.b[w].clear();
-                QDFN_HIP(hipMemcpy(b.a[w].data(), (w ? dN : dS).as<double>() + off[i], sizeof(double) * mk,
-                                   hipMemcpyDeviceToHost));
             }
         }
         bs.dense += (int64_t)sel.size();
@@ -601,9 +603,18 @@ int sampled_parts(const EntrySource& es, qdfn_hmat* h, const std::vector<int64_t
         QDFN_HIP(hipStreamSynchronize(h->stream));
         std::vector<SState> s(nj);
         QDFN_HIP(hipMemcpy(s.data(), dst.p, sizeof(SState) * nj, hipMemcpyDeviceToHost));
-        std::vector<double> U(nu), V(nv);
-        QDFN_HIP(hipMemcpy(U.data(), dU.p, sizeof(double) * nu, hipMemcpyDeviceToHost));
-        QDFN_HIP(hipMemcpy(V.data(), dV.p, sizeof(double) * nv, hipMemcpyDeviceToHost));
+        // the factors of the converged parts, in one packed copy
+        std::vector<PackItem> items;
+        for (int32_t i = 0; i < nj; ++i)
+            if (s[i].status == kConverged && s[i].rank > 0) {
+                const SJob& J = jobs[i];
+                items.push_back({dU.as<double>() + J.u_off, J.m, J.m, s[i].rank, 0});
+                items.push_back({dV.as<double>() + J.v_off, J.k, J.k, s[i].rank, 0});  // row-major r x k
+            }
+        std::vector<double> hp;
+        int rc = pack_to_host(items, hp, h->stream);
+        if (rc) return rc;
+        size_t ii = 0;
         for (int32_t i = 0; i < nj; ++i) {
             const SJob& J = jobs[i];
             const int64_t q = sel[i];
@@ -618,10 +629,16 @@ int sampled_parts(const EntrySource& es, qdfn_hmat* h, const std::vector<int64_t
             const int64_t m = J.m, k = J.k, r = s[i].rank;
             b.kind[w] = QDFN_HBLK_LOWRANK;
             b.rank[w] = r;
-            b.a[w].assign(U.begin() + J.u_off, U.begin() + J.u_off + m * r);
             b.b[w].resize(r * k);
-            for (int64_t l = 0; l < r; ++l)
-                for (int64_t c = 0; c < k; ++c) b.b[w][l + c * r] = V[J.v_off + l * k + c];
+            if (r > 0) {
+                const double* u = hp.data() + items[ii++].dst;
+                const double* v = hp.data() + items[ii++].dst;
+                b.a[w].assign(u, u + m * r);
+                for (int64_t l = 0; l < r; ++l)
+                    for (int64_t c = 0; c < k; ++c) b.b[w][l + c * r] = v[l * k + c];
+            } else {
+                b.a[w].clear();
+            }
             ++bs.sampled;
         }
     }
