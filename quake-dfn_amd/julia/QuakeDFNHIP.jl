@@ -168,11 +168,14 @@ end
 # HmatBuild_R (Functions_BuildInputFile.jl:725) on the device: returns an operator handle
 # (ACA with full pivoting + SVD recompression, Frobenius tolerance relative to each block's
 # shear norm; Tolerance in the reference's absolute sense can be passed as atol with rtol = 0).
+# sampled = true: admissible blocks from sampled rows and columns, never assembled in full
+# (qdfn_hmat_build_rect_sampled; n is then not bounded by n^2 device memory).
 function HmatBuild_R_Device(ShearModulus, PoissonRatio, ElementRange_SR, Input_Segment, Admissible;
-                            rtol=1e-6, atol=0.0, prec=1)
+                            rtol=1e-6, atol=0.0, prec=1, sampled=false)
     seg = Matrix{Float64}(Input_Segment); n = size(seg, 1)
     rng = Matrix{Int64}(permutedims(ElementRange_SR)); adm = Vector{Int64}(vec(Admissible))
-    h = GC.@preserve seg rng adm ccall((:qdfn_hmat_build_rect, LIB), Ptr{Cvoid},
+    fn = sampled ? :qdfn_hmat_build_rect_sampled : :qdfn_hmat_build_rect
+    h = GC.@preserve seg rng adm ccall((fn, LIB), Ptr{Cvoid},
         (Ptr{Float64}, Int64, Int64, Float64, Float64, Int64, Ptr{Int64}, Ptr{Int64}, Float64, Float64, Int32),
         seg, n, n, ShearModulus, PoissonRatio, length(adm), rng, adm, rtol, atol, Int32(prec))
     h == C_NULL && check(-1)
