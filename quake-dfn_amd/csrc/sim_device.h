@@ -103,6 +103,43 @@ struct MaxOp { __device__ double operator()(double a, double b) const { return b
 struct AndOp { __device__ int operator()(int a, int b) const { return a & b; } };
 struct OrOp { __device__ int operator()(int a, int b) const { return a | b; } };
 
+// b folded into a (the seven quantities of a step's decision)
+__device__ inline void fold_part(BlockPart& a, const BlockPart& b) {
+    a.ok = AndOp()(a.ok, b.ok);
+    a.any4 = OrOp()(a.any4, b.any4);
+    a.found = OrOp()(a.found, b.found);
+    a.mindt = MinOp()(a.mindt, b.mindt);
+    a.vmax = MaxOp()(a.vmax, b.vmax);
+    a.vmax_next = MaxOp()(a.vmax_next, b.vmax_next);
+    a.dtmin_next = MinOp()(a.dtmin_next, b.dtmin_next);
+}
+
+// Workgroup reduction of all seven quantities at once (one barrier pair); per
+// quantity the order of block_reduce: butterfly within the wavefront, then
+// wavefronts in order.
+template <int NT>
+__device__ inline BlockPart block_reduce_part(BlockPart v, BlockPart* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) {
+        BlockPart o;
+        o.ok = __shfl_xor(v.ok, off, 64);
+        o.any4 = __shfl_xor(v.any4, off, 64);
+        o.found = __shfl_xor(v.found, off, 64);
+        o.mindt = __shfl_xor(v.mindt, off, 64);
+        o.vmax = __shfl_xor(v.vmax, off, 64);
+        o.vmax_next = __shfl_xor(v.vmax_next, off, 64);
+        o.dtmin_next = __shfl_xor(v.dtmin_next, off, 64);
+        fold_part(v, o);
+    }
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    BlockPart r = sh[0];
+    for (int k = 1; k < NT / 64; ++k) fold_part(r, sh[k]);
+    __syncthreads();
+    return r;
+}
+
 // DtRef / Dt selection of main() (:171-187); returns the new Dt.
 __device__ inline double select_dt(const SimArrays& A, double dt_prev, double vmax, double dtmin) {
     double dtref;
@@ -157,8 +194,7 @@ __device__ __forceinline__ void setup_body(const SimArrays& A, SimState* __restr
 // effective normal stress of main() (:212-226), plus the step decision, for
 // workgroup blk of the nblk (kStepBlock threads each) that cover the run.
 __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restrict__ st, unsigned blk, unsigned nblk) {
-    __shared__ double shd[kStepBlock / 64];
-    __shared__ int shi[kStepBlock / 64];
+    __shared__ BlockPart shp[kStepBlock / 64];
     __shared__ int s_last;
     if (st->done) return;
     const int64_t n = A.n;
@@ -231,36 +267,37 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
             }
         }
     }
-    ok = block_reduce<kStepBlock>(ok, AndOp(), shi);
-    any4 = block_reduce<kStepBlock>(any4, OrOp(), shi);
-    found = block_reduce<kStepBlock>(found, OrOp(), shi);
-    mindt = block_reduce<kStepBlock>(mindt, MinOp(), shd);
-    vmax = block_reduce<kStepBlock>(vmax, MaxOp(), shd);
-    vmax_next = block_reduce<kStepBlock>(vmax_next, MaxOp(), shd);
-    dtmin_next = block_reduce<kStepBlock>(dtmin_next, MinOp(), shd);
+    BlockPart mine;
+    mine.ok = ok; mine.any4 = any4; mine.found = found; mine.pad = 0;
+    mine.mindt = mindt; mine.vmax = vmax; mine.vmax_next = vmax_next; mine.dtmin_next = dtmin_next;
+    const BlockPart bpart = block_reduce_part<kStepBlock>(mine, shp);
     if (threadIdx.x == 0) {
-        BlockPart& bp = A.parts[blk];
-        bp.ok = ok; bp.any4 = any4; bp.found = found;
-        bp.mindt = mindt; bp.vmax = vmax; bp.vmax_next = vmax_next; bp.dtmin_next = dtmin_next;
+        A.parts[blk] = bpart;
         __threadfence();
         const unsigned prev = atomicAdd(A.arrive, 1u);
         s_last = prev == nblk - 1;
     }
     __syncthreads();
-    if (!s_last || threadIdx.x != 0) return;
+    if (!s_last) return;
     // ---- last workgroup: the decision of main() ----
+    // every thread folds a strided share of the workgroups' parts (written by
+    // other workgroups: read through vector loads), then one reduction
     __threadfence();
-    *A.arrive = 0u;
-    int OK = 1, ANY4 = 0, FOUND = 0;
-    double MINDT = INFINITY, VMAX = -INFINITY, VNEXT = -INFINITY, DNEXT = INFINITY;
-    for (unsigned k = 0; k < nblk; ++k) {
-        const BlockPart bp = A.parts[k];
-        OK &= bp.ok; ANY4 |= bp.any4; FOUND |= bp.found;
-        MINDT = bp.mindt < MINDT ? bp.mindt : MINDT;
-        VMAX = bp.vmax > VMAX ? bp.vmax : VMAX;
-        VNEXT = bp.vmax_next > VNEXT ? bp.vmax_next : VNEXT;
-        DNEXT = bp.dtmin_next < DNEXT ? bp.dtmin_next : DNEXT;
+    BlockPart all;
+    all.ok = 1; all.any4 = 0; all.found = 0; all.pad = 0;
+    all.mindt = INFINITY; all.vmax = -INFINITY; all.vmax_next = -INFINITY; all.dtmin_next = INFINITY;
+    for (unsigned k = threadIdx.x; k < nblk; k += kStepBlock) {
+        const volatile BlockPart* q = A.parts + k;
+        BlockPart b;
+        b.ok = q->ok; b.any4 = q->any4; b.found = q->found; b.pad = 0;
+        b.mindt = q->mindt; b.vmax = q->vmax; b.vmax_next = q->vmax_next; b.dtmin_next = q->dtmin_next;
+        fold_part(all, b);
     }
+    all = block_reduce_part<kStepBlock>(all, shp);
+    if (threadIdx.x != 0) return;
+    *A.arrive = 0u;
+    const int OK = all.ok, ANY4 = all.any4, FOUND = all.found;
+    const double MINDT = all.mindt, VMAX = all.vmax, VNEXT = all.vmax_next, DNEXT = all.dtmin_next;
     if (!OK) {  // retry the step with the reduced Dt (:240-258)
         st->dt = ANY4 ? 1e-3 : MINDT;
         st->need_coupling = 0;
