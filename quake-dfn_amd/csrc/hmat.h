@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "dense.h"
+#include "sim_device.h"
 #include "qdfn_internal.h"
 
 namespace qdfn {
@@ -66,6 +67,7 @@ struct qdfn_hmat {
     qdfn::DevBuf rbs, tiles, part, multi; // row blocks, tiles, partial sums, split row blocks (phase 2)
     qdfn::DevBuf vtasks, vfins, vpart;    // phase-1 wavefront tasks, split-panel finish, partial sums
     qdfn::DevBuf x, yS, yN;               // vectors for qdfn_hmat_matvec
+    qdfn::DevBuf fin_row, fin_blk;        // rows of split phase-2 row blocks (see RowFinish)
     int64_t n_rb1 = 0, n_rb2 = 0, n_t = 0;
     int64_t n_tiles1 = 0, n_tiles2 = 0, n_multi1 = 0, n_multi2 = 0, n_vtasks = 0, n_vfins = 0;
     int32_t tile_w = 4096;                // columns per tile (phase 2)
@@ -94,8 +96,11 @@ struct ScopeMs {
 // coupling.
 // Events (optional): ev0 before phase 1, ev1 between the phases, ev2 after the
 // phase-2 main kernel (before its finish).
+// finish_in_step: the rows of split row blocks are left as partial sums for the
+// step kernel (fin_row / fin_blk), and the finish kernel is not launched.
 int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, int32_t form, double* d_yS,
-                double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2);
+                double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2,
+                bool finish_in_step = false);
 
 // ---- construction helpers shared by the full and the sampled builds ----
 qdfn_hmat* new_hmat(int64_t n, int32_t prec);

@@ -769,6 +769,27 @@ int hmat_layout(qdfn_hmat* h) {
         QDFN_HIP(hipMemcpy(h->multi.p, multi.data(), sizeof(int32_t) * multi.size(), hipMemcpyHostToDevice));
     QDFN_HIP(h->idx.alloc(sizeof(int32_t) * std::max<size_t>(idx.size(), 1)));
     if (!idx.empty()) QDFN_HIP(hipMemcpy(h->idx.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
+    // rows of split phase-2 row blocks, for a step kernel that forms them itself
+    {
+        std::vector<int32_t> fr(2 * n, -1);
+        std::vector<RowFinish> fb;
+        for (int64_t q = h->n_multi1; q < (int64_t)multi.size(); ++q) {
+            const HRowBlock& R = rbs[multi[q]];
+            RowFinish F;
+            F.part_base = R.part_base;
+            F.m = R.m;
+            F.ntiles = R.ntiles;
+            F.out = (int32_t)R.out;
+            F.pad = 0;
+            for (int32_t r = 0; r < R.m; ++r) fr[(R.kind == 1 ? 0 : n) + R.out + r] = (int32_t)fb.size();
+            fb.push_back(F);
+        }
+        QDFN_HIP(h->fin_row.alloc(sizeof(int32_t) * 2 * n));
+        QDFN_HIP(hipMemcpy(h->fin_row.p, fr.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice));
+        QDFN_HIP(h->fin_blk.alloc(sizeof(RowFinish) * std::max<size_t>(fb.size(), 1)));
+        if (!fb.empty())
+            QDFN_HIP(hipMemcpy(h->fin_blk.p, fb.data(), sizeof(RowFinish) * fb.size(), hipMemcpyHostToDevice));
+    }
     QDFN_HIP(h->t.alloc(sizeof(double) * std::max<int64_t>(h->n_t, 1)));
     QDFN_HIP(h->kdiag.alloc(sizeof(double) * n));
     QDFN_HIP(hipMemcpy(h->kdiag.p, h->kdiag_h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
@@ -806,7 +827,7 @@ int hmat_layout(qdfn_hmat* h) {
 }
 
 int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, int32_t form, double* d_yS,
-                double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2) {
+                double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2, bool finish_in_step) {
     const HRowBlock* rbs = h->rbs.as<HRowBlock>();
     const HTile* tiles = h->tiles.as<HTile>();
     const int32_t* multi = h->multi.as<int32_t>();
@@ -834,7 +855,8 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
     }
     if (g.ran_log) hipLaunchKernelGGL(k_mark_skipped, dim3(1), dim3(1), 0, s, g);
     if (ev1) QDFN_HIP(hipEventRecord(ev1, s));
-    const int32_t mark_tiles = h->n_multi2 > 0 ? 0 : 1;
+    const bool finish = h->n_multi2 > 0 && !finish_in_step;
+    const int32_t mark_tiles = finish ? 0 : 1;
     if (f32)
         hipLaunchKernelGGL((k_hrow<float, true>), dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
                            tiles + h->n_tiles1, rbs, part, h->pool.as<float>(), h->idx.as<int32_t>(), d_x, t, h->n,
@@ -845,7 +867,7 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
                            kd, form, d_yS, d_yN, mark_tiles, g);
     QDFN_HIP(hipGetLastError());
     if (ev2) QDFN_HIP(hipEventRecord(ev2, s));
-    if (h->n_multi2 > 0) {
+    if (finish) {
         hipLaunchKernelGGL(k_hfinish, dim3((unsigned)h->n_multi2), dim3(64), 0, s, multi + h->n_multi1, rbs, part,
                            d_x, kd, form, t, d_yS, d_yN, 1, g);
         QDFN_HIP(hipGetLastError());

@@ -56,7 +56,7 @@ static int enqueue_attempt(qdfn_sim* s, int slot, hipEvent_t ev0, hipEvent_t ev1
     int rc;
     if (s->hop) {
         // timed mode: events around the phase-2 kernel (the dominant one)
-        rc = launch_hmat(s->hop, s->A.D, g, 1, s->A.tl, s->A.sigprod, s->stream, nullptr, ev0, ev1);
+        rc = launch_hmat(s->hop, s->A.D, g, 1, s->A.tl, s->A.sigprod, s->stream, nullptr, ev0, ev1, true);
         if (rc) return rc;
     } else {
         if (ev0) QDFN_HIP(hipEventRecord(ev0, s->stream));
@@ -188,6 +188,10 @@ static qdfn_sim* sim_create(const char* who, int64_t n_op, const double* kself_h
     A.maximum_dt = p->maximum_dt; A.min_sig = p->min_normal_stress; A.alpha_evo = p->alpha_evo;
     A.conv = 1e-9; A.v0 = V0; A.dr = p->evolution_dr; A.planar = p->planar || normal_zero;
     A.ext_at_told = p->ext_at_told;
+    A.fin_row = nullptr;
+    A.fin_blk = nullptr;
+    A.fin_part = nullptr;
+    A.fin_kdiag = nullptr;
     return s;
 }
 
@@ -223,7 +227,13 @@ extern "C" qdfn_sim_t* qdfn_sim_create_hmat(qdfn_hmat_t* h, const qdfn_sim_param
     for (int64_t i = 0; i < h->n; ++i) kself[i] = -h->kdiag_h[i];
     qdfn_sim* s = sim_create("qdfn_sim_create_hmat", h->n, kself.data(), h->normal_zero, p, mass, a, b, dc,
                              theta_i, v_i, friction_i, sigma0, loading_rate);
-    if (s) s->hop = h;
+    if (s) {
+        s->hop = h;
+        s->A.fin_row = h->fin_row.as<int32_t>();
+        s->A.fin_blk = h->fin_blk.as<RowFinish>();
+        s->A.fin_part = h->part.as<double>();
+        s->A.fin_kdiag = h->kdiag.as<double>();
+    }
     return s;
 }
 

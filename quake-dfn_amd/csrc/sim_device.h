@@ -30,6 +30,15 @@ struct BlockPart {
     double mindt, vmax, vmax_next, dtmin_next;
 };
 
+// A row block of an H operator whose rows are summed from tile partials:
+// row i (out <= i < out + m) is the sum over q < ntiles, in order, of
+// part[part_base + q * m + (i - out)].  With an H operator the step kernel
+// forms these rows itself (the product's separate finish is not launched).
+struct RowFinish {
+    int64_t part_base;
+    int32_t m, ntiles, out, pad;
+};
+
 // Device arrays of a run.  Double-buffered state arrays are [2][n].
 struct SimArrays {
     int64_t n, n_loading;
@@ -53,10 +62,23 @@ struct SimArrays {
     double* h_dtsel;  // selected Dt of each recorded step (before any retry)
     int8_t* h_sw;     // solver of each element in the recorded step's accepted attempt (1: HighV)
     double *t_told, *t_dt, *t_vmax;
+    // H operator rows formed here (null otherwise): fin_row[i] / fin_row[n + i]
+    // index fin_blk for the shear / normal row i (-1: the product wrote it)
+    const int32_t* fin_row;
+    const RowFinish* fin_blk;
+    const double* fin_part;
+    const double* fin_kdiag;  // K_S,ii of the operator (loading form: TL = (-sum) / K_S,ii)
     // parameters
     double mu, rho, switch_v, dt_cut, rupture_dt, maximum_dt, min_sig, alpha_evo, conv, v0;
     int32_t dr, planar, ext_at_told, pad;
 };
+
+__device__ inline double finish_row(const SimArrays& A, int32_t b, int64_t i) {
+    const RowFinish F = A.fin_blk[b];
+    double s = 0.0;
+    for (int32_t q = 0; q < F.ntiles; ++q) s += A.fin_part[F.part_base + (int64_t)q * F.m + (i - F.out)];
+    return s;
+}
 
 // InterpolateFromStressChange (Functions_Solvers.jl:465-540) for one element.
 __device__ inline void ext_interp(const SimArrays& A, double time, int64_t i, double& dsn, double& dtau) {
@@ -213,7 +235,20 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
         const double v_old = A.V[o + i];
         double dsn = 0.0, dtau = 0.0;
         if (A.nt > 0) ext_interp(A, A.ext_at_told ? told : told + dt, i, dsn, dtau);
-        double sig = (A.planar ? 0.0 : A.sigprod[i]) + A.sigma0[i] + dsn;
+        double tl_i = A.tl[i], sp_i = A.planar ? 0.0 : A.sigprod[i];
+        if (A.fin_row) {  // rows of split H row blocks (the finish of the product, in the same order)
+            const int32_t bs = A.fin_row[i], bn = A.fin_row[n + i];
+            if (bs >= 0) {
+                tl_i = (-finish_row(A, bs, i)) / A.fin_kdiag[i];
+                A.tl[i] = tl_i;
+            }
+            if (bn >= 0) {
+                const double v = finish_row(A, bn, i);
+                A.sigprod[i] = v;
+                if (!A.planar) sp_i = v;
+            }
+        }
+        double sig = sp_i + A.sigma0[i] + dsn;
         if (sig < A.min_sig) sig = A.min_sig;
         double vn;
         if (i < n - A.n_loading) {
@@ -222,7 +257,7 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
             e.conv = A.conv; e.d_old = A.D[o + i]; e.f_old = A.F[o + i]; e.th_old = A.TH[o + i];
             e.v_old = v_old; e.f0 = A.f0[i]; e.a = A.a[i]; e.b = A.b[i]; e.dc = A.dc[i]; e.v0 = A.v0;
             e.k = A.kself[i]; e.omega = A.omega[i]; e.mass = A.mass[i]; e.mu = A.mu; e.sig = sig;
-            e.tl = 0.0 + A.tl[i]; e.rho = A.rho; e.dtau = dtau; e.switch_v = A.switch_v; e.alpha_evo = A.alpha_evo;
+            e.tl = 0.0 + tl_i; e.rho = A.rho; e.dtau = dtau; e.switch_v = A.switch_v; e.alpha_evo = A.alpha_evo;
             e.sig_old = A.SIG[o + i]; e.tau0 = A.tau0[i]; e.dr = A.dr;
             const ElemOut r = (sw == 1) ? solve_high_v(e, dt) : solve_low_v(e, dt);
             A.V[w + i] = r.v; A.F[w + i] = r.f; A.D[w + i] = r.d; A.TH[w + i] = r.th; A.SIG[w + i] = r.sig;
