@@ -459,6 +459,12 @@ int dense_blocks(const EntrySource& es, qdfn_hmat* h, const std::vector<int64_t>
 // scaled by shear_norm.
 int fallback_blocks(const EntrySource& es, qdfn_hmat* h, const std::vector<int64_t>& blks, bool both,
                     double rtol, double atol, std::vector<double>& shear_norm, BuildStats& bs) {
+    for (int64_t q : blks) {
+        const HBlock& b = h->blocks[q];
+        if ((b.r1 - b.r0 + 1) * (b.s1 - b.s0 + 1) > 0x7fffffffLL)
+            return fail(QDFN_ERR_ARG, "sampled H build: a block of more than 2^31 - 1 entries reached the rank cap "
+                                      "(looser aca_rtol or a finer cluster tree)");
+    }
     const int64_t budget = workspace_budget() / 2;
     size_t pos = 0;
     while (pos < blks.size()) {
@@ -654,8 +660,13 @@ int build_sampled(qdfn_hmat* h, const EntrySource& es, int64_t nblk, const int64
     for (int64_t q = 0; q < nblk; ++q) {
         const HBlock& b = h->blocks[q];
         const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
-        if (!b.adm || aca_rank_cap(m, k) < 1 || m * k > 0x7fffffffLL) dense.push_back(q);
-        else lowr.push_back(q);
+        if (!b.adm || aca_rank_cap(m, k) < 1) {
+            if (m * k > 0x7fffffffLL)
+                return fail(QDFN_ERR_ARG, "sampled H build: an inadmissible block exceeds 2^31 - 1 entries");
+            dense.push_back(q);
+        } else {
+            lowr.push_back(q);  // sampled whatever its size
+        }
     }
     int rc;
     {

@@ -43,3 +43,25 @@ def test_error_path_without_device():
     assert "qdfn_okada_stiffness_rect" in qdfn._lib.last_error()
     assert L.qdfn_dense_create(None, None, 4, 4, 0) is None
     assert "null" in qdfn._lib.last_error()
+
+
+def test_sampled_build_argument_checks_without_device():
+    import numpy as np
+    import qdfn
+    from qdfn import _lib
+    L = qdfn.lib()
+    PD, PI64 = _lib.PD, _lib.PI64
+    assert L.qdfn_hmat_build_tri_sampled(4, None, None, None, None, None, None, None, 1.0, 1.0, 1, None, None,
+                                         1e-6, 0.0, _lib.PREC_FP64) is None
+    assert "null input" in _lib.last_error()
+    assert L.qdfn_hmat_build_rect_sampled(None, 4, 4, 1.0, 0.25, 1, None, None, 1e-6, 0.0,
+                                          _lib.PREC_FP64) is None
+    assert "qdfn_hmat_build_rect_sampled" in _lib.last_error()
+    # a block range outside [1, n] is rejected before the device is touched
+    seg = np.zeros((4, 16), order="F")
+    rng = np.array([[1, 4, 1, 5]], dtype=np.int64)
+    adm = np.array([0], dtype=np.int64)
+    assert L.qdfn_hmat_build_rect_sampled(seg.ctypes.data_as(PD), 4, 4, 1.0, 0.25, 1, rng.ctypes.data_as(PI64),
+                                          adm.ctypes.data_as(PI64), 1e-6, 0.0, _lib.PREC_FP64) is None
+    assert "out of bounds" in _lib.last_error()
+    assert L.qdfn_hmat_build_info(None, None) == _lib.QDFN_ERR_ARG
