@@ -20,16 +20,22 @@
 //                                 estimate ||S_{r+1}||^2 = ||S_r||^2 +
 //                                 2 sum_l (u_l.u_r)(v_l.v_r) + |u_r|^2 |v_r|^2,
 //                                 stop test, next row i* = argmax |u_r| over
-//                                 rows not yet used, request row i*.
-// Stop test: |u_r| |v_r| <= eps ||S||_F (eps = rtol / 2; ||S||_F the block's
-// shear norm estimate, for the normal part as well) or, with atol > 0,
-// max|u_r| max|v_r| <= atol, met on two consecutive steps.  A residual row so
-// small that the whole block would be within tolerance counts as such a step
-// without adding a term.  A part that reaches the rank cap falls back to full
-// evaluation of its block and the full-pivot approximation the full build
-// uses; inadmissible blocks are evaluated in full (both parts in one pass).
-// Approximations are recompressed on the host to the other half of the
-// tolerance, as in the full build.
+//                                 rows not yet used, request row i*;
+//                   probe column: residual column of a probe (below).
+// Stop test: a term with |u_r| |v_r| <= eps ||S||_F (eps = rtol / 2; ||S||_F
+// the block's shear norm estimate, for the normal part as well; with
+// atol > 0 also max|u_r| max|v_r| <= atol), or a negligible residual row or
+// probe column (max |residual| sqrt(m k) <= eps ||S||_F: the whole block would
+// be within tolerance if every entry were that small).  Three such steps in a
+// row stop the part.  After a step that meets the test the next request is a
+// probe rather than the argmax-|u_r| row: alternately the unused row nearest
+// to frac(q phi) m and the column frac(q phi) k (phi the golden ratio), so the
+// test is confirmed away from the rows the pivots favour; a probe column that
+// is not negligible restarts the cross at its largest residual row.  A part
+// that reaches the rank cap falls back to full evaluation of its block and the
+// full-pivot approximation the full build uses; inadmissible blocks are
+// evaluated in full (both parts in one pass).  Approximations are recompressed
+// on the host to the other half of the tolerance, as in the full build.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -373,7 +379,7 @@ int64_t workspace_budget() {
 }
 
 struct BuildStats {
-    int64_t entries = 0;     // kernel entries evaluated
+    int64_t entries = 0;     // receiver-source pairs evaluated
     int64_t steps = 0;       // launch pairs of the sampled approximation
     int64_t sampled = 0, fallback = 0, dense = 0;
 };
