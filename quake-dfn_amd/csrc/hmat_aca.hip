@@ -39,6 +39,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "assembly.h"
@@ -190,14 +193,16 @@ __device__ void finish(volatile SState* vs, SampleLine* lines, int32_t jb, int32
 // One step of every active part (see the file comment).  State, lines and
 // factor entries written by earlier launches are read through vector loads;
 // every decision is computed identically by all threads from reduced values.
-__global__ __launch_bounds__(kSThreads) void k_aca_step(const SJob* __restrict__ jobs, SState* st, SampleLine* lines,
+__global__ __launch_bounds__(kSThreads) void k_aca_step(const int32_t* act,
+                                                        const SJob* __restrict__ jobs, SState* st, SampleLine* lines,
                                                         double* U, double* V, double* smp_pool, uint8_t* used_pool,
                                                         double eps, double atol) {
     __shared__ double sh_l[kRankCap];   // U row i* (row phase) or V column j* (column phases)
     __shared__ double sh_du[kRankCap];  // u_l . u_r
     __shared__ double sh_dv[kRankCap];  // v_l . v_r
     __shared__ int32_t sh_status;
-    const int32_t jb = blockIdx.x;
+    // the parts still active at the last count (rewritten between launches)
+    const int32_t jb = *(const volatile int32_t*)(act + blockIdx.x);
     const SJob J = jobs[jb];
     volatile SState* vs = st + jb;
     if (threadIdx.x == 0) sh_status = vs->status;
@@ -352,18 +357,27 @@ __global__ __launch_bounds__(kSThreads) void k_aca_step(const SJob* __restrict__
     }
 }
 
-// number of active parts
-__global__ __launch_bounds__(256) void k_aca_count(const SState* st, int32_t njobs, int32_t* out) {
-    __shared__ int32_t sh[4];
-    int32_t c = 0;
-    for (int32_t j = threadIdx.x; j < njobs; j += 256) {
-        const volatile SState* s = st + j;
-        c += s->status == kActive ? 1 : 0;
-    }
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) *out = sh[0] + sh[1] + sh[2] + sh[3];
+// status of every part (written by the step kernel: read through vector loads)
+__global__ __launch_bounds__(256) void k_aca_status(const SState* st, int32_t njobs, int8_t* __restrict__ out) {
+    const int32_t j = (int32_t)blockIdx.x * 256 + (int32_t)threadIdx.x;
+    if (j >= njobs) return;
+    const volatile SState* s = st + j;
+    out[j] = (int8_t)s->status;
+}
+
+// progress of a build on stderr (QDFN_HMAT_VERBOSE set), wall time since the first line
+void vlog(const char* fmt, ...) {
+    static const bool on = getenv("QDFN_HMAT_VERBOSE") != nullptr;
+    if (!on) return;
+    static const auto t0 = std::chrono::steady_clock::now();
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    va_list ap;
+    va_start(ap, fmt);
+    fprintf(stderr, "[hmat build %9.1f ms] ", ms);
+    vfprintf(stderr, fmt, ap);
+    fprintf(stderr, "\n");
+    fflush(stderr);
+    va_end(ap);
 }
 
 int launch_lines(const EntrySource& es, const SampleLine* d_lines, const SampleTile* d_tiles, int64_t ntiles,
@@ -577,7 +591,7 @@ int sampled_parts(const EntrySource& es, qdfn_hmat* h, const std::vector<int64_t
             ++pos;
         }
         const int32_t nj = (int32_t)jobs.size();
-        DevBuf dj, dst, dl, dt, dU, dV, dsm, dused, dcnt;
+        DevBuf dj, dst, dl, dt, dU, dV, dsm, dused, dact, dstat;
         QDFN_HIP(dj.alloc(sizeof(SJob) * nj));
         QDFN_HIP(dst.alloc(sizeof(SState) * nj));
         QDFN_HIP(dl.alloc(sizeof(SampleLine) * nj));
@@ -586,33 +600,56 @@ int sampled_parts(const EntrySource& es, qdfn_hmat* h, const std::vector<int64_t
         QDFN_HIP(dV.alloc(sizeof(double) * nv));
         QDFN_HIP(dsm.alloc(sizeof(double) * ns));
         QDFN_HIP(dused.alloc(nb));
-        QDFN_HIP(dcnt.alloc(sizeof(int32_t)));
+        QDFN_HIP(dact.alloc(sizeof(int32_t) * nj));
+        QDFN_HIP(dstat.alloc(nj));
+        std::vector<int32_t> act(nj);
+        for (int32_t i = 0; i < nj; ++i) act[i] = i;
         QDFN_HIP(hipMemcpy(dj.p, jobs.data(), sizeof(SJob) * nj, hipMemcpyHostToDevice));
         QDFN_HIP(hipMemcpy(dt.p, tiles.data(), sizeof(SampleTile) * tiles.size(), hipMemcpyHostToDevice));
+        QDFN_HIP(hipMemcpy(dact.p, act.data(), sizeof(int32_t) * nj, hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_aca_init, dim3(nj), dim3(64), 0, h->stream, dj.as<SJob>(), nj, dst.as<SState>(),
                            dl.as<SampleLine>(), dused.as<uint8_t>());
         QDFN_HIP(hipGetLastError());
-        int32_t active = nj;
-        // each part needs at most max_steps line/step pairs; the count is
-        // checked every 8 steps
-        for (int32_t it = 0; it < max_steps && active > 0; ++it) {
-            int rc = launch_lines(es, dl.as<SampleLine>(), dt.as<SampleTile>(), (int64_t)tiles.size(),
-                                  dsm.as<double>(), dsm.as<double>(), h->stream);
+        int32_t nact = nj;
+        int64_t ntl = (int64_t)tiles.size();
+        std::vector<int8_t> stat(nj);
+        // each part needs at most max_steps line/step pairs; every 8 steps the
+        // launches are narrowed to the parts still active
+        for (int32_t it = 0; it < max_steps && nact > 0; ++it) {
+            int rc = launch_lines(es, dl.as<SampleLine>(), dt.as<SampleTile>(), ntl, dsm.as<double>(),
+                                  dsm.as<double>(), h->stream);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_aca_step, dim3(nj), dim3(kSThreads), 0, h->stream, dj.as<SJob>(), dst.as<SState>(),
-                               dl.as<SampleLine>(), dU.as<double>(), dV.as<double>(), dsm.as<double>(),
-                               dused.as<uint8_t>(), eps, atol);
+            hipLaunchKernelGGL(k_aca_step, dim3(nact), dim3(kSThreads), 0, h->stream, dact.as<int32_t>(),
+                               dj.as<SJob>(), dst.as<SState>(), dl.as<SampleLine>(), dU.as<double>(),
+                               dV.as<double>(), dsm.as<double>(), dused.as<uint8_t>(), eps, atol);
             QDFN_HIP(hipGetLastError());
             ++bs.steps;
             if ((it & 7) == 7) {
-                hipLaunchKernelGGL(k_aca_count, dim3(1), dim3(256), 0, h->stream, dst.as<SState>(), nj,
-                                   dcnt.as<int32_t>());
+                hipLaunchKernelGGL(k_aca_status, dim3((unsigned)((nj + 255) / 256)), dim3(256), 0, h->stream,
+                                   dst.as<SState>(), nj, dstat.as<int8_t>());
                 QDFN_HIP(hipGetLastError());
-                QDFN_HIP(hipMemcpyAsync(&active, dcnt.p, sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+                QDFN_HIP(hipMemcpyAsync(stat.data(), dstat.p, nj, hipMemcpyDeviceToHost, h->stream));
                 QDFN_HIP(hipStreamSynchronize(h->stream));
+                act.clear();
+                tiles.clear();
+                for (int32_t i = 0; i < nj; ++i) {
+                    if (stat[i] != kActive) continue;
+                    const int32_t ln = std::max(jobs[i].m, jobs[i].k);
+                    for (int32_t f = 0; f < ln; f += kLineTile) tiles.push_back({i, f});
+                    act.push_back(i);
+                }
+                nact = (int32_t)act.size();
+                ntl = (int64_t)tiles.size();
+                if (nact > 0) {
+                    QDFN_HIP(hipMemcpyAsync(dact.p, act.data(), sizeof(int32_t) * nact, hipMemcpyHostToDevice,
+                                            h->stream));
+                    QDFN_HIP(hipMemcpyAsync(dt.p, tiles.data(), sizeof(SampleTile) * ntl, hipMemcpyHostToDevice,
+                                            h->stream));
+                }
             }
         }
         QDFN_HIP(hipStreamSynchronize(h->stream));
+        vlog("sampled %s parts: batch of %d, %d steps", w ? "normal" : "shear", nj, (int)bs.steps);
         std::vector<SState> s(nj);
         QDFN_HIP(hipMemcpy(s.data(), dst.p, sizeof(SState) * nj, hipMemcpyDeviceToHost));
         // the factors of the converged parts, in one packed copy
@@ -680,6 +717,7 @@ int build_sampled(qdfn_hmat* h, const EntrySource& es, int64_t nblk, const int64
         rc = dense_blocks(es, h, dense, bs);
     }
     if (rc) return rc;
+    vlog("inadmissible blocks: %zu evaluated in full; %zu admissible", dense.size(), lowr.size());
     // shear parts: the threshold scale is each block's running norm estimate
     std::vector<double> ref(nblk, -1.0), estS(nblk, 0.0), estN(nblk, 0.0), shear_norm(nblk, 0.0);
     std::vector<int64_t> fbS, fbN;
@@ -690,6 +728,8 @@ int build_sampled(qdfn_hmat* h, const EntrySource& es, int64_t nblk, const int64
     // both parts, both through the full-pivot approximation
     rc = fallback_blocks(es, h, fbS, true, rtol, atol, shear_norm, bs);
     if (rc) return rc;
+    vlog("shear parts: %lld sampled, %zu blocks at the rank cap evaluated in full", (long long)bs.sampled,
+         fbS.size());
     // normal parts of the other blocks, scaled by the shear norm of the same block
     std::vector<bool> fellS(nblk, false);
     for (int64_t q : fbS) fellS[q] = true;
@@ -703,6 +743,7 @@ int build_sampled(qdfn_hmat* h, const EntrySource& es, int64_t nblk, const int64
     if (rc) return rc;
     rc = fallback_blocks(es, h, fbN, false, rtol, atol, shear_norm, bs);
     if (rc) return rc;
+    vlog("normal parts done: %zu at the rank cap evaluated in full", fbN.size());
     // recompression of the sampled parts to the other half of the tolerance
     if (rtol > 0) {
         std::vector<bool> fell[2] = {fellS, std::vector<bool>(nblk, false)};
@@ -719,6 +760,7 @@ int build_sampled(qdfn_hmat* h, const EntrySource& es, int64_t nblk, const int64
                 budgets.push_back(0.5 * rtol * shear_norm[q]);
             }
         recompress_parallel(h, lr, items, budgets);
+        vlog("recompressed %zu sampled parts", items.size());
     }
     h->build_entries = bs.entries;
     h->build_steps = bs.steps;
