@@ -25,8 +25,9 @@
 // Stop test: a term with |u_r| |v_r| <= eps ||S||_F (eps = rtol / 2; ||S||_F
 // the block's shear norm estimate, for the normal part as well; with
 // atol > 0 also max|u_r| max|v_r| <= atol), or a negligible residual row or
-// probe column (max |residual| sqrt(m k) <= eps ||S||_F: the whole block would
-// be within tolerance if every entry were that small).  Three such steps in a
+// probe column (sqrt(m) |row| <= eps / 2 ||S||_F, sqrt(k) |column| likewise:
+// the block's residual would be within half the tolerance if every row or
+// column were as large).  Three such steps in a
 // row stop the part.  After a step that meets the test the next request is a
 // probe rather than the argmax-|u_r| row: alternately the unused row nearest
 // to frac(q phi) m and the column frac(q phi) k (phi the golden ratio), so the
@@ -217,26 +218,29 @@ __global__ __launch_bounds__(kSThreads) void k_aca_step(const int32_t* act,
     double* smp = smp_pool + J.smp_off;
     uint8_t* used = used_pool + J.used_off;
     const double ref = J.ref >= 0.0 ? J.ref : sqrt(fmax(norm2, 0.0));
-    const double scale_mk = sqrt((double)m * (double)k);
-    auto negligible = [&](double amax) {
-        return amax <= 0.0 || amax * scale_mk <= eps * ref || (atol > 0.0 && amax <= atol);
+    // a residual row (column) is negligible when the block's residual would be
+    // within half the tolerance if every row (column) were as large:
+    // sqrt(m) |row| <= eps / 2 ||S||_F (sqrt(k) |column| for a column)
+    auto negligible = [&](double amax, double line2, int32_t other) {
+        return amax <= 0.0 || sqrt(line2 * (double)other) <= 0.5 * eps * ref || (atol > 0.0 && amax <= atol);
     };
     const int tid = threadIdx.x;
     if (phase == 0) {
         // ---- row i*: residual, pivot column
         for (int32_t l = tid; l < rank; l += kSThreads) sh_l[l] = Uj[istar + (int64_t)l * m];
         __syncthreads();
-        double amax = -1.0, piv = 0.0, dummy = 0.0;
+        double amax = -1.0, piv = 0.0, rr = 0.0;
         int32_t aid = 0x7fffffff;
         for (int32_t j = tid; j < k; j += kSThreads) {
             double r = smp[j];
             for (int32_t l = 0; l < rank; ++l) r -= sh_l[l] * Vj[(int64_t)l * k + j];
             smp[j] = r;
             better_max(amax, aid, piv, fabs(r), j, r);
+            rr += r * r;
         }
-        wg_argmax_sum(amax, aid, piv, dummy);
+        wg_argmax_sum(amax, aid, piv, rr);
         if (tid == 0) used[istar] = 1;
-        if (!negligible(amax)) {
+        if (!negligible(amax, rr, m)) {
             double vv = 0.0, vmax = -1.0, vsg = 0.0;
             int32_t vid = 0x7fffffff;
             const double inv = 1.0 / piv;
@@ -269,17 +273,18 @@ __global__ __launch_bounds__(kSThreads) void k_aca_step(const int32_t* act,
     for (int32_t l = tid; l < rank; l += kSThreads) sh_l[l] = Vj[(int64_t)l * k + jstar];
     __syncthreads();
     if (phase == 2) {
-        double amax = -1.0, asg = 0.0, dummy = 0.0;
+        double amax = -1.0, asg = 0.0, cc = 0.0;
         int32_t aid = 0x7fffffff;
         for (int32_t i = tid; i < m; i += kSThreads) {
             if (used[i]) continue;  // rows already pivoted on are reproduced exactly
             double u = smp[i];
             for (int32_t l = 0; l < rank; ++l) u -= Uj[i + (int64_t)l * m] * sh_l[l];
             better_max(amax, aid, asg, fabs(u), i, u);
+            cc += u * u;
         }
-        wg_argmax_sum(amax, aid, asg, dummy);
+        wg_argmax_sum(amax, aid, asg, cc);
         if (tid == 0) vs->pcols = vs->pcols + 1;
-        if (aid != 0x7fffffff && !negligible(amax)) {
+        if (aid != 0x7fffffff && !negligible(amax, cc, k)) {
             // restart the cross at the probe column's largest residual row
             if (tid == 0) {
                 vs->hits = 0;
@@ -538,6 +543,8 @@ int fallback_blocks(const EntrySource& es, qdfn_hmat* h, const std::vector<int64
         rc = compress_device_blocks(h, lr, sc, rtol, atol);
         if (rc) return rc;
         bs.fallback += (int64_t)lr.size();
+        vlog("full evaluation and full-pivot approximation: %zu blocks (%lld entries each part)", sel.size(),
+             (long long)used);
     }
     return QDFN_OK;
 }
