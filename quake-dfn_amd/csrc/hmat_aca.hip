@@ -43,6 +43,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "assembly.h"
@@ -54,7 +55,8 @@ namespace {
 
 constexpr int kSThreads = 256;
 constexpr int kSWaves = kSThreads / 64;
-constexpr int kRankCap = 256;
+constexpr int kRankCap = 1024;  // sampled parts: at most this rank (LDS arrays of the step kernel)
+constexpr int64_t kFallbackMaxEntries = int64_t(1) << 28;  // largest block evaluated in full at the cap
 constexpr int kLineTile = 64;
 constexpr int kStopHits = 3;  // the stop test on a term, then on a probe row and a probe column
 
@@ -486,9 +488,11 @@ int fallback_blocks(const EntrySource& es, qdfn_hmat* h, const std::vector<int64
                     double rtol, double atol, std::vector<double>& shear_norm, BuildStats& bs) {
     for (int64_t q : blks) {
         const HBlock& b = h->blocks[q];
-        if ((b.r1 - b.r0 + 1) * (b.s1 - b.s0 + 1) > 0x7fffffffLL)
-            return fail(QDFN_ERR_ARG, "sampled H build: a block of more than 2^31 - 1 entries reached the rank cap "
-                                      "(looser aca_rtol or a finer cluster tree)");
+        const int64_t mk = (b.r1 - b.r0 + 1) * (b.s1 - b.s0 + 1);
+        if (mk > kFallbackMaxEntries)
+            return fail(QDFN_ERR_ARG, "sampled H build: a block of " + std::to_string(mk) +
+                                          " entries reached the rank cap of the cross approximation (a looser "
+                                          "aca_rtol or a finer cluster tree)");
     }
     const int64_t budget = workspace_budget() / 2;
     size_t pos = 0;
