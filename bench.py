@@ -375,7 +375,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded rough fault, back-slip loading)",
         "config": {"workload": "rough_fault_20000", "n_elements": rf["n"], "operator": "H-matrix, FP32 storage",
-                   "leaf": 64, "dist_diam_ratio": args.crit, "aca_tol": args.aca_tol,
+                   "leaf": 64, "dist_diam_ratio": args.crit, "aca_tol": args.aca_tol, "rank_cap": 1024,
                    "plain": "dense FP64", "steps_timed": args.steps},
         "fp64_plain": {"rhs_evals_per_s": round(plain["evals_per_s"], 2),
                        "sim_years_per_hour": round(plain["sim_years_per_hour"], 4),
