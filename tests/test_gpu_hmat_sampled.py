@@ -15,7 +15,8 @@ pqrfact, scripts/Functions_BuildInputFile.jl:674-716, :754-797).
 * Products within 2 rtol ||K_S||_F ||x||_2 of the dense products (the full
   build's bound, doubled for the estimate).
 * Fewer kernel evaluations than the n^2 of the full build on a mesh with
-  leaves of 64 (rough fault, 2,400 triangles).
+  leaves of 64 (rough fault, 2,400 triangles), where the full build's
+  products are also checked against the dense ones.
 * main_H on the sampled operator: first-event time within 1e-6 of the run on
   the full build's operator.
 """
@@ -137,6 +138,15 @@ def test_tri_leaf64_fewer_evaluations(orc, qd):
     Hs = qd.HMatrix.build_tri(*geo, mu, lam, rs, ad, aca_rtol=rtol, sampled=True)
     err = _check_case(qd, Hf, Hs, KS, KN, rs, ad, rtol, 13)
     n = KS.shape[0]
+    # the full build (rank ceiling 1024: large admissible blocks are low rank)
+    # within its bound rtol ||K_S||_F ||x||_2 of the dense products
+    x = np.random.default_rng(14).standard_normal(n)
+    fS, fN = Hf.matvec(x, loading_form=False)
+    bound = rtol * np.linalg.norm(KS) * np.linalg.norm(x) * 1.000001
+    assert np.linalg.norm(fS - KS @ x) <= bound
+    assert np.linalg.norm(fN - KN @ x) <= bound
+    rk = Hf.ranks()
+    assert int((rk["kind_S"] == 1).sum()) > 0
     bi = Hs.build_info()
     print(f"sampled build n={n}: {bi}, parts {err.size}, err/rtol median {np.median(err):.3g} "
           f"p98 {np.percentile(err, 98):.3g} max {err.max():.3g}", flush=True)
