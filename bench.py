@@ -8,8 +8,10 @@ ACA tolerance 1e-6) stored in FP32 with FP64 accumulation and FP64 state,
 against the dense FP64 operator ("plain" form, main() of
 scripts/Functions_RSFDFN3DMain_D.jl).  Both loops are main_H/main semantics on
 the device; each run is loaded by back-slip tables built from its own operator
-(UniformBackslip.jl).  Secondary workload: the 8-fault, 8,192-element dense
-network, FP32 storage against FP64.
+(UniformBackslip.jl).  `bf16_far_h` reports the same H build with its low-rank
+factors stored in BF16 (dense blocks FP32), with its own operator error.
+Secondary workload: the 8-fault, 8,192-element dense network, FP32 storage
+against FP64.
 
 A step is one accepted time step (one coupling product plus one or more
 local-solve passes).  `value` = RHS evaluations / s of the mixed-precision H run
@@ -141,6 +143,14 @@ def rough_fault(qd, args):
         log(f"[bench] H build full {builds['full']['ms_build']:.0f} ms ({builds['full']['entries_evaluated']} "
             f"pairs), sampled {builds['sampled']['ms_build']:.0f} ms ({builds['sampled']['entries_evaluated']} "
             f"pairs); product errors {builds['product_rel_err_vs_dense_fp64']}")
+    hb = None
+    if not args.no_bf16_far:
+        # the same build with the low-rank factors stored in BF16
+        t0 = time.perf_counter()
+        hb = qd.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=args.aca_tol,
+                                  prec=qd.PREC_BF16_FAR)
+        log(f"[bench] H BF16-factor operator in {time.perf_counter() - t0:.2f}s: "
+            f"{hb.info()['bytes_per_product'] / 1e6:.1f} MB per product")
     d32 = qd.DenseOperator.from_triangles(*geo, m["mu"], lam, qd.PREC_FP32)
     ks, _ = d64.kself()
     ksum, knsum = d64.matvec(np.ones(n), loading_form=False)
@@ -149,16 +159,29 @@ def rough_fault(qd, args):
     hsum, hnsum = h32.matvec(np.ones(n), loading_form=False)
     prob_h = dict(prob, ext=configs.backslip_tables(hsum, hnsum))
     # operator accuracy against the dense FP64 product
-    rng = np.random.default_rng(5)
-    acc = {}
-    for label, x in (("random", rng.standard_normal(n)), ("uniform", np.ones(n))):
-        a = d64.matvec(x, loading_form=True)
-        b = h32.matvec(x, loading_form=True)
-        acc[label] = {"tl_rel": float(np.abs(a[0] - b[0]).max() / np.abs(a[0]).max()),
-                      "sigma_rel": float(np.abs(a[1] - b[1]).max() / np.abs(a[1]).max())}
+    def accuracy(h):
+        rng = np.random.default_rng(5)
+        acc = {}
+        for label, x in (("random", rng.standard_normal(n)), ("uniform", np.ones(n))):
+            a = d64.matvec(x, loading_form=True)
+            b = h.matvec(x, loading_form=True)
+            acc[label] = {"tl_rel": float(np.abs(a[0] - b[0]).max() / np.abs(a[0]).max()),
+                          "sigma_rel": float(np.abs(a[1] - b[1]).max() / np.abs(a[1]).max())}
+        return acc
+
+    acc = accuracy(h32)
     ph1, ph2, tot = h32.time_phases(20)
+    runs = [("fp64_dense", d64, prob), ("mixed_h", h32, prob_h), ("fp32_dense", d32, prob)]
+    bf = None
+    if hb is not None:
+        bsum, bnsum = hb.matvec(np.ones(n), loading_form=False)
+        runs.append(("bf16_far_h", hb, dict(prob, ext=configs.backslip_tables(bsum, bnsum))))
+        bi = hb.info()
+        bf = {"operator_rel_err": accuracy(hb), "bytes_per_product": bi["bytes_per_product"],
+              "phase_bytes": [bi["phase1_bytes"], bi["phase2_bytes"]],
+              "h_product_us": [round(v * 1e3, 2) for v in hb.time_phases(20)]}
     res = {}
-    for name, op, pr in (("fp64_dense", d64, prob), ("mixed_h", h32, prob_h), ("fp32_dense", d32, prob)):
+    for name, op, pr in runs:
         log(f"[bench] {name}: warmup {args.warmup}, timed {args.steps} steps")
         r = run_timed(qd, op, pr, args.steps, args.warmup, args.graph_attempts, args.kernel_steps)
         if r["status"] != 0:
@@ -166,8 +189,14 @@ def rough_fault(qd, args):
         summarize(name, r, configs.SEC_PER_YEAR)
         r["first_event"] = configs.first_event_time(r["trace"]["told"], r["trace"]["dt"], r["trace"]["vmax"])
         res[name] = r
+    if hb is not None:
+        r = res["bf16_far_h"]
+        bf.update(rhs_evals_per_s=round(r["evals_per_s"], 2), sim_years_per_hour=round(r["sim_years_per_hour"], 4),
+                  first_event_s=r["first_event"], divergence_from_fp64=divergence(r["trace"], res["fp64_dense"]["trace"]))
+        bf["h_product_us"] = dict(zip(("phase1", "phase2", "total"), bf["h_product_us"]))
+        hb.close()
     out = dict(n=n, info=info, st=st, rk=rk, acc=acc, phases=(ph1, ph2, tot), res=res, h32=h32, prob_h=prob_h,
-               builds=builds)
+               builds=builds, bf16_far=bf)
     d64.close()
     d32.close()
     return out
@@ -331,6 +360,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-sampled-build", action="store_true")
+    ap.add_argument("--no-bf16-far", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU layout is "
@@ -397,6 +427,7 @@ def main():
         "divergence_from_fp64": {"mixed_h": divergence(mixed["trace"], plain["trace"]),
                                  "fp32_dense": divergence(res["fp32_dense"]["trace"], plain["trace"])},
         "operator_rel_err": rf["acc"],
+        "bf16_far_h": rf["bf16_far"],
         "h_build": rf["builds"],
         "network_8192": net,
         "ensemble_256": ens,

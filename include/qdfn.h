@@ -83,6 +83,10 @@ int qdfn_rotverts_unitvectors(int64_t n, double* P1, double* P2, double* P3, con
 
 #define QDFN_PREC_FP64 0 /* FP64 storage, FP64 accumulation */
 #define QDFN_PREC_FP32 1 /* FP32 storage, FP64 accumulation */
+/* H operators only: low-rank factors U, V in BF16, dense blocks in FP32, FP64
+ * accumulation (about 2^-9 relative per factor entry; the error against the
+ * FP64 product is reported per result, not bounded by the ACA tolerance) */
+#define QDFN_PREC_BF16_FAR 2
 
 typedef struct qdfn_dense qdfn_dense_t;
 

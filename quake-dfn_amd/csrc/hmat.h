@@ -20,17 +20,15 @@ struct HBlock {
     std::vector<double> a[2], b[2];
 };
 
-// Row block of the product: up to 64 rows of a row-major panel, one workgroup.
-//   phase 1 (kind 0): rows of V factors sharing one source range; column c
-//                     multiplies x[zoff + c]; row q writes t[out + q];
-//   phase 2 (kind 1 shear, 2 normal): one receiver row group of one matrix;
-//                     column c multiplies w[idx[zoff + c]] with w = [x; t];
-//                     row q writes row out + q of yS / yN.
+// Row block of the phase-2 product: one receiver row group (up to 64 rows) of
+// one matrix (kind 1 shear, 2 normal) as a row-major panel; column c multiplies
+// w[idx[zoff + c]] with w = [x; t]; row q writes row out + q of yS / yN.
 struct HRowBlock {
     int64_t data_off, zoff, out;
     int64_t part_base;      // split row blocks: tile q's row sums at part[part_base + q * m]
     int32_t m, K, ld, kind;
-    int32_t ntiles, pad;
+    int32_t ntiles;
+    int32_t lo;             // entries in the BF16 pool (QDFN_PREC_BF16_FAR: rows of U)
 };
 
 // work item: columns [c0, c1) of one row block (its q-th tile)
@@ -64,6 +62,7 @@ struct qdfn_hmat {
     std::vector<double> kdiag_h;          // K_S,ii (0 where no diagonal block)
     qdfn::DevBuf kdiag;
     qdfn::DevBuf pool, idx, t;            // panel entries (V panels then row-group panels), gather indices, t
+    qdfn::DevBuf pool_lo;                 // QDFN_PREC_BF16_FAR: BF16 V panels, then row groups' U panels
     qdfn::DevBuf rbs, tiles, part, multi; // row blocks, tiles, partial sums, split row blocks (phase 2)
     qdfn::DevBuf vtasks, vfins, vpart;    // phase-1 wavefront tasks, split-panel finish, partial sums
     qdfn::DevBuf x, yS, yN;               // vectors for qdfn_hmat_matvec

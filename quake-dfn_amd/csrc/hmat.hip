@@ -27,12 +27,15 @@
 //            that (contiguous) range of x;
 //   phase 2: y = panel * [x; t] per row group, the right-hand side gathered
 //            through an index list.
-// Both phases run the same kernel: a panel is cut into row blocks of at most
-// 64 rows and those into tiles of at most 1024 columns, one workgroup each,
-// which streams its rows with 16-byte loads, FP64 accumulation and wavefront
-// reductions; split row blocks are added tile by tile in column order by a
-// small finish kernel (no atomics: results are deterministic run to run).
-// Panels and V are stored in FP64 or FP32 (mixed precision).
+// Phase 1 runs one wavefront per task (k_vpanel); phase 2 cuts each row-group
+// panel into tiles of at most tile_w columns, one workgroup each (k_hrow).  Both
+// stream their rows with wide loads, FP64 accumulation and wavefront
+// reductions; split rows are added tile by tile in column order by a small
+// finish kernel (no atomics: results are deterministic run to run).
+// Storage: FP64, FP32, or BF16 low-rank factors with FP32 dense blocks
+// (QDFN_PREC_BF16_FAR); then every row group with both kinds of columns has an
+// FP32 panel (dense blocks) and a BF16 panel (rows of U) whose tiles write
+// partial sums that the finish adds in tile order, FP32 tiles first.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -62,29 +65,31 @@ constexpr int kVTaskEntries = 4096;   // phase 1: target entries per task
 
 int64_t round4(int64_t v) { return (v + 3) / 4 * 4; }
 
+// d rounded once to BF16 (8 significant bits, ties to even); magnitudes below
+// the smallest normal FP32 become signed zeros
+uint16_t to_bf16(double d) {
+    if (!std::isfinite(d) || std::fabs(d) < 0x1p-126) {
+        const float f = std::isfinite(d) ? (d < 0.0 ? -0.0f : 0.0f) : (float)d;
+        uint32_t b;
+        std::memcpy(&b, &f, 4);
+        return (uint16_t)(b >> 16);
+    }
+    int e;
+    std::frexp(d, &e);  // |d| = f 2^e, 0.5 <= f < 1
+    const float r = (float)std::ldexp(std::nearbyint(std::ldexp(d, 8 - e)), e - 8);  // exact in FP32
+    uint32_t b;
+    std::memcpy(&b, &r, 4);
+    return (uint16_t)(b >> 16);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- kernels ----
 
-template <class T>
-__device__ inline void load4(const T* p, double v[4]);
-template <>
-__device__ inline void load4<float>(const float* p, double v[4]) {
-    const float4 q = *reinterpret_cast<const float4*>(p);
-    v[0] = q.x;
-    v[1] = q.y;
-    v[2] = q.z;
-    v[3] = q.w;
-}
-template <>
-__device__ inline void load4<double>(const double* p, double v[4]) {
-    const double2 a = reinterpret_cast<const double2*>(p)[0];
-    const double2 b = reinterpret_cast<const double2*>(p)[1];
-    v[0] = a.x;
-    v[1] = a.y;
-    v[2] = b.x;
-    v[3] = b.y;
-}
+// BF16 entry (the upper half of an FP32 word)
+struct bf16_t {
+    uint16_t u;
+};
 
 template <class T>
 __device__ inline void load4raw(const T* p, T v[4]) {
@@ -103,6 +108,39 @@ __device__ inline void load4raw(const T* p, T v[4]) {
         v[3] = b.y;
     }
 }
+
+// W consecutive entries of one row as loaded (one 16-byte load for FP32 and
+// BF16, two for FP64) and their FP64 values
+template <class T>
+struct RowLoad;
+template <>
+struct RowLoad<float> {
+    static constexpr int W = 4;
+    float4 v;
+    __device__ void load(const float* p) { v = *reinterpret_cast<const float4*>(p); }
+    __device__ double get(int j) const { return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w; }
+};
+template <>
+struct RowLoad<double> {
+    static constexpr int W = 4;
+    double2 a, b;
+    __device__ void load(const double* p) {
+        a = reinterpret_cast<const double2*>(p)[0];
+        b = reinterpret_cast<const double2*>(p)[1];
+    }
+    __device__ double get(int j) const { return j == 0 ? a.x : j == 1 ? a.y : j == 2 ? b.x : b.y; }
+};
+template <>
+struct RowLoad<bf16_t> {
+    static constexpr int W = 8;
+    uint4 v;
+    __device__ void load(const bf16_t* p) { v = *reinterpret_cast<const uint4*>(p); }
+    // entry 2k is the low half of word k, entry 2k + 1 the high half
+    __device__ double get(int j) const {
+        const uint32_t w = (j >> 1) == 0 ? v.x : (j >> 1) == 1 ? v.y : (j >> 1) == 2 ? v.z : v.w;
+        return (double)__uint_as_float((j & 1) ? (w & 0xffff0000u) : (w << 16));
+    }
+};
 
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -124,32 +162,24 @@ __device__ inline void hmat_out(int kind, int64_t i, double s, const double* __r
 
 // One workgroup per tile (columns [c0, c1) of a row block).  The right-hand
 // side is staged in LDS in chunks of kChunk columns, double-buffered: the
-// gather of chunk i + 1 (from x for V panels, through the index list from
-// [x; t] for row-group panels) is issued before chunk i is streamed.  Each row
-// is read by LPR lanes (16-byte loads), LPR the smallest power of two >= the
-// chunk width / 4 (at most 64), so a wavefront load covers 64 / LPR rows.
-// A row block with one tile writes its outputs; a split one writes partial
-// row sums for k_hfinish.
+// gather of chunk i + 1 (through the index list from [x; t]) is issued before
+// chunk i is streamed.  Each row is read by LPR lanes, W entries each (one
+// 16-byte load: W = 4 in FP32, 8 in BF16; two loads in FP64), LPR the smallest
+// power of two >= the chunk width / W (at most 64), so a wavefront load covers
+// 64 / LPR rows.  A row block with one tile writes its outputs; a split one
+// writes partial row sums, added by k_hfinish or the step kernel.
 template <class T, bool GATHER>
-__global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict__ tiles,
-                                                         const HRowBlock* __restrict__ rbs,
-                                                         double* __restrict__ part,
-                                                         const T* __restrict__ pool,
-                                                         const int32_t* __restrict__ idx,
-                                                         const double* __restrict__ x0, double* __restrict__ t,
-                                                         int64_t n, const double* __restrict__ kdiag, int32_t form,
-                                                         double* __restrict__ yS, double* __restrict__ yN,
-                                                         int32_t mark, CouplingGate g) {
-    if (g.gate && *g.gate == 0) return;
-    __shared__ double zbuf[2][kChunk];
-    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
-    const HTile tl = tiles[blockIdx.x];
-    const HRowBlock R = rbs[tl.rb];
+__device__ __forceinline__ void hrow_body(const HTile& tl, const HRowBlock& R, double* __restrict__ part,
+                                          const T* __restrict__ pool, const int32_t* __restrict__ idx,
+                                          const double* __restrict__ x, double* __restrict__ t, int64_t n,
+                                          const double* __restrict__ kdiag, int32_t form, double* __restrict__ yS,
+                                          double* __restrict__ yN, double (*zbuf)[kChunk]) {
+    constexpr int W = RowLoad<T>::W;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int32_t K = tl.c1, K4 = (K + 3) / 4 * 4;  // columns [tl.c0, K) of the row block
-    const int32_t cw = min(K4 - tl.c0, (int32_t)kChunk);
+    const int32_t K = tl.c1, KW = (K + W - 1) / W * W;  // columns [tl.c0, K) of the row block
+    const int32_t cw = min(KW - tl.c0, (int32_t)kChunk);
     int lpr = 1;
-    while (lpr < 64 && 4 * lpr < cw) lpr <<= 1;
+    while (lpr < 64 && W * lpr < cw) lpr <<= 1;
     constexpr int kWaves = kPhase2Threads / 64;
     const int rpw = 64 / lpr, step_rows = kWaves * rpw, sub = lane & (lpr - 1), rl = lane / lpr;
     const T* base = pool + R.data_off;
@@ -190,38 +220,37 @@ __global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict
     }
 #pragma unroll
     for (int q = 0; q < kPer; ++q) zbuf[0][threadIdx.x + q * kPhase2Threads] = gv[q];
-    if (GATHER && tl.c0 + kChunk < K4) load_idx(tl.c0 + kChunk);
+    if (GATHER && tl.c0 + kChunk < KW) load_idx(tl.c0 + kChunk);
     __syncthreads();
     double acc[kSlots];
 #pragma unroll
     for (int r = 0; r < kSlots; ++r) acc[r] = 0.0;
     int buf = 0;
-    for (int32_t c0 = tl.c0; c0 < K4; c0 += kChunk) {
-        const bool more = c0 + kChunk < K4;
+    for (int32_t c0 = tl.c0; c0 < KW; c0 += kChunk) {
+        const bool more = c0 + kChunk < KW;
         if (more) {
             load_vals(c0 + kChunk);
-            if (GATHER && c0 + 2 * kChunk < K4) load_idx(c0 + 2 * kChunk);
+            if (GATHER && c0 + 2 * kChunk < KW) load_idx(c0 + 2 * kChunk);
         }
         const double* z = zbuf[buf];
-        const int32_t lim = min((int32_t)kChunk, K4 - c0);
-        for (int32_t cb = 0; cb < lim; cb += 4 * lpr) {
+        const int32_t lim = min((int32_t)kChunk, KW - c0);
+        for (int32_t cb = 0; cb < lim; cb += W * lpr) {
             // lanes past the chunk read its first columns again and add zero
-            const int32_t c = cb + sub * 4 < lim ? cb + sub * 4 : 0;
-            const double zm = cb + sub * 4 < lim ? 1.0 : 0.0;
-            const double z0 = zm * z[c], z1 = zm * z[c + 1], z2 = zm * z[c + 2], z3 = zm * z[c + 3];
-            T w[kSlots][4];
+            const int32_t c = cb + sub * W < lim ? cb + sub * W : 0;
+            const double zm = cb + sub * W < lim ? 1.0 : 0.0;
+            double zz[W];
+#pragma unroll
+            for (int j = 0; j < W; ++j) zz[j] = zm * z[c + j];
+            RowLoad<T> w[kSlots];
 #pragma unroll
             for (int r = 0; r < kSlots; ++r) {
                 const int row = min(r * step_rows + wave * rpw + rl, R.m - 1);
-                load4raw<T>(base + (uint32_t)(row * R.ld + c0 + c), w[r]);
+                w[r].load(base + (uint32_t)(row * R.ld + c0 + c));
             }
 #pragma unroll
-            for (int r = 0; r < kSlots; ++r) {
-                acc[r] = fma((double)w[r][0], z0, acc[r]);
-                acc[r] = fma((double)w[r][1], z1, acc[r]);
-                acc[r] = fma((double)w[r][2], z2, acc[r]);
-                acc[r] = fma((double)w[r][3], z3, acc[r]);
-            }
+            for (int r = 0; r < kSlots; ++r)
+#pragma unroll
+                for (int j = 0; j < W; ++j) acc[r] = fma(w[r].get(j), zz[j], acc[r]);
         }
         if (more) {
 #pragma unroll
@@ -242,6 +271,50 @@ __global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict
                 part[R.part_base + (int64_t)tl.q * R.m + row] = v;
         }
     }
+}
+
+template <class T, bool GATHER>
+__global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict__ tiles,
+                                                         const HRowBlock* __restrict__ rbs,
+                                                         double* __restrict__ part,
+                                                         const T* __restrict__ pool,
+                                                         const int32_t* __restrict__ idx,
+                                                         const double* __restrict__ x0, double* __restrict__ t,
+                                                         int64_t n, const double* __restrict__ kdiag, int32_t form,
+                                                         double* __restrict__ yS, double* __restrict__ yN,
+                                                         int32_t mark, CouplingGate g) {
+    if (g.gate && *g.gate == 0) return;
+    __shared__ double zbuf[2][kChunk];
+    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const HTile tl = tiles[blockIdx.x];
+    const HRowBlock R = rbs[tl.rb];
+    hrow_body<T, GATHER>(tl, R, part, pool, idx, x, t, n, kdiag, form, yS, yN, zbuf);
+    if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
+}
+
+// Phase 2 with BF16 low-rank factors: FP32 panels (dense blocks) and BF16
+// panels (rows of U) in one launch, tiles in one size-ordered list; the row
+// block's flag picks the body (uniform per workgroup).  At most 64 VGPRs, so
+// that two 1,024-thread workgroups fit a CU as for k_hrow.
+__global__ __launch_bounds__(kPhase2Threads, 8) void k_hrow_mixed(const HTile* __restrict__ tiles,
+                                                               const HRowBlock* __restrict__ rbs,
+                                                               double* __restrict__ part,
+                                                               const float* __restrict__ pool,
+                                                               const bf16_t* __restrict__ pool_lo,
+                                                               const int32_t* __restrict__ idx,
+                                                               const double* __restrict__ x0, double* __restrict__ t,
+                                                               int64_t n, const double* __restrict__ kdiag,
+                                                               int32_t form, double* __restrict__ yS,
+                                                               double* __restrict__ yN, int32_t mark, CouplingGate g) {
+    if (g.gate && *g.gate == 0) return;
+    __shared__ double zbuf[2][kChunk];
+    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const HTile tl = tiles[blockIdx.x];
+    const HRowBlock R = rbs[tl.rb];
+    if (R.lo)
+        hrow_body<bf16_t, true>(tl, R, part, pool_lo, idx, x, t, n, kdiag, form, yS, yN, zbuf);
+    else
+        hrow_body<float, true>(tl, R, part, pool, idx, x, t, n, kdiag, form, yS, yN, zbuf);
     if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
 }
 
@@ -329,6 +402,87 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ 
         }
     };
     VStep<T> A, B;
+    load(0, A);
+    for (int32_t it = 0; it < total; it += 2) {
+        load(min(it + 1, total - 1), B);
+        consume(it, A);
+        load(min(it + 2, total - 1), A);
+        if (it + 1 < total) consume(it + 1, B);
+    }
+}
+
+// Phase 1 over BF16 V panels: k_vpanel's schedule with eight entries of a row
+// per lane and step (one 16-byte load), so a step covers twice the columns.
+// Panel leading dimensions and segment starts are multiples of eight entries.
+struct VStep8 {
+    uint4 w[kVSlots];
+    double z[8];
+};
+
+__global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restrict__ tasks, int32_t ntasks,
+                                                           const uint16_t* __restrict__ pool,
+                                                           const double* __restrict__ x0, double* __restrict__ t,
+                                                           double* __restrict__ vpart, CouplingGate g) {
+    if (g.gate && *g.gate == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
+    if (ti >= ntasks) return;
+    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const VTask tk = tasks[ti];
+    const int lpr = tk.lpr, rpw = 64 / lpr, sub = lane & (lpr - 1), rl = lane / lpr;
+    const int32_t K = tk.K, K8 = (K + 7) / 8 * 8;
+    const int32_t cstep = 8 * lpr, nc = (K8 + cstep - 1) / cstep;
+    const int32_t gr = kVSlots * rpw, ng = (tk.m + gr - 1) / gr;
+    const int32_t total = nc * ng;
+    const uint16_t* base = pool + tk.data_off;
+    const double* xs = x + tk.xoff;
+    auto load = [&](int32_t it, VStep8& S) {
+        const int32_t gi = it / nc, ci = it - gi * nc;
+        const int32_t c = ci * cstep + 8 * sub;
+        const int32_t cc = c < K8 ? c : 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long long bits = __double_as_longlong(xs[min(cc + q, K - 1)]);
+            S.z[q] = __longlong_as_double((c < K8 && cc + q < K) ? bits : 0LL);
+        }
+#pragma unroll
+        for (int r = 0; r < kVSlots; ++r) {
+            const int32_t row = min(gi * gr + rl + r * rpw, tk.m - 1);
+            S.w[r] = *reinterpret_cast<const uint4*>(base + (uint32_t)(row * tk.ld + cc));
+        }
+    };
+    double acc[kVSlots];
+#pragma unroll
+    for (int r = 0; r < kVSlots; ++r) acc[r] = 0.0;
+    // entry 2j of a step is the low half of word j, entry 2j + 1 the high half
+    auto consume = [&](int32_t it, const VStep8& S) {
+        const int32_t gi = it / nc, ci = it - gi * nc;
+#pragma unroll
+        for (int r = 0; r < kVSlots; ++r) {
+            const uint32_t wd[4] = {S.w[r].x, S.w[r].y, S.w[r].z, S.w[r].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[r] = fma((double)__uint_as_float(wd[j] << 16), S.z[2 * j], acc[r]);
+                acc[r] = fma((double)__uint_as_float(wd[j] & 0xffff0000u), S.z[2 * j + 1], acc[r]);
+            }
+        }
+        if (ci == nc - 1) {
+#pragma unroll
+            for (int r = 0; r < kVSlots; ++r) {
+                double v = acc[r];
+                for (int o = lpr >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                const int32_t row = gi * gr + rl + r * rpw;
+                if (sub == 0 && row < tk.m) {
+                    if (tk.part < 0)
+                        t[tk.out + row] = v;
+                    else
+                        vpart[tk.part + row] = v;
+                }
+                acc[r] = 0.0;
+            }
+        }
+    };
+    VStep8 A, B;
     load(0, A);
     for (int32_t it = 0; it < total; it += 2) {
         load(min(it + 1, total - 1), B);
@@ -524,7 +678,11 @@ int hmat_layout(qdfn_hmat* h) {
             for (double e : *v)
                 if (e != 0.0) nz = 0;
     h->normal_zero = nz;
-    std::vector<double> pool;
+    // entries of the FP64 / FP32 panels (pool) and, with BF16 low-rank factors,
+    // of the V panels and the row groups' U panels (lo)
+    const bool lo_far = h->prec == QDFN_PREC_BF16_FAR;
+    std::vector<double> pool, lo;
+    std::vector<double>& vdst = lo_far ? lo : pool;
     std::vector<int32_t> idx;
     std::vector<HRowBlock> rbs;
     if (const char* e = getenv("QDFN_HMAT_TILE")) {
@@ -537,20 +695,23 @@ int hmat_layout(qdfn_hmat* h) {
     std::vector<VFin> vfins;
     int64_t v_entries = 0, v_cols = 0, vpart_total = 0;
     int64_t part_total = 0;
-    auto add_rowblock = [&](HRowBlock R) {
-        const int32_t t0 = (int32_t)tiles.size();
+    // tiles of row block rb (columns [0, K)) numbered from q into tl; returns the next q
+    auto add_tiles = [&](int32_t rb, int32_t K, int32_t q, std::vector<HTile>& tl) {
         int32_t c = 0;
         do {
             HTile T;
-            T.rb = (int32_t)rbs.size();
+            T.rb = rb;
             T.c0 = c;
-            T.c1 = std::min<int32_t>(R.K, c + h->tile_w);
-            T.q = (int32_t)tiles.size() - t0;
-            tiles.push_back(T);
+            T.c1 = std::min<int32_t>(K, c + h->tile_w);
+            T.q = q++;
+            tl.push_back(T);
             c = T.c1;
-        } while (c < R.K);
-        R.ntiles = (int32_t)tiles.size() - t0;
-        R.pad = 0;
+        } while (c < K);
+        return q;
+    };
+    auto add_rowblock = [&](HRowBlock R, bool in_lo) {
+        R.ntiles = add_tiles((int32_t)rbs.size(), R.K, 0, tiles);
+        R.lo = in_lo;
         R.part_base = -1;
         if (R.ntiles > 1) {
             R.part_base = part_total;
@@ -558,6 +719,20 @@ int hmat_layout(qdfn_hmat* h) {
             multi.push_back((int32_t)rbs.size());
         }
         rbs.push_back(R);
+    };
+    // the FP32 and the BF16 panel of one row group: one split row (tiles of Ra
+    // then of Rb, partial sums in one area, finished through Ra)
+    auto add_pair = [&](HRowBlock Ra, HRowBlock Rb) {
+        const int32_t ia = (int32_t)rbs.size();
+        const int32_t q = add_tiles(ia + 1, Rb.K, add_tiles(ia, Ra.K, 0, tiles), tiles);
+        Ra.ntiles = Rb.ntiles = q;
+        Ra.lo = 0;
+        Rb.lo = 1;
+        Ra.part_base = Rb.part_base = part_total;
+        part_total += (int64_t)q * Ra.m;
+        multi.push_back(ia);
+        rbs.push_back(Ra);
+        rbs.push_back(Rb);
     };
     // phase 1: V panels, one per source range, in order of first appearance
     std::vector<int64_t> toff(2 * B.size(), -1);
@@ -576,33 +751,42 @@ int hmat_layout(qdfn_hmat* h) {
                 }
                 members[gk].push_back({bi, w});
             }
+        // BF16 panels are read eight entries per lane (k_vpanel_bf16), the others four
+        const int64_t al = lo_far ? 8 : 4;
+        auto round_al = [&](int64_t v) { return (v + al - 1) / al * al; };
+        int64_t task_entries = lo_far ? 2 * kVTaskEntries : kVTaskEntries;
+        if (const char* e = getenv("QDFN_HMAT_VTASK")) {
+            const int64_t v = atoll(e);
+            if (v >= 256 && v <= (1 << 20)) task_entries = v;
+        }
         int64_t tbase = 0;
         for (size_t gk = 0; gk < keys.size(); ++gk) {
-            const int64_t s0 = keys[gk].first, k = keys[gk].second - s0 + 1, ld = round4(k);
+            const int64_t s0 = keys[gk].first, k = keys[gk].second - s0 + 1, ld = round_al(k);
             int64_t rows = 0;
             for (auto& f : members[gk]) {
                 toff[2 * f.first + f.second] = tbase + rows;
                 rows += B[f.first].rank[f.second];
             }
-            const int64_t off = (int64_t)pool.size();
-            pool.resize(off + rows * ld, 0.0);
+            const int64_t off = (int64_t)vdst.size();
+            vdst.resize(off + rows * ld, 0.0);
             int64_t row = 0;
             for (auto& f : members[gk]) {
                 const HBlock& b = B[f.first];
                 const int64_t r = b.rank[f.second];
                 const std::vector<double>& V = b.b[f.second];
                 for (int64_t q = 0; q < r; ++q, ++row)
-                    for (int64_t c = 0; c < k; ++c) pool[off + row * ld + c] = V[q + c * r];
+                    for (int64_t c = 0; c < k; ++c) vdst[off + row * ld + c] = V[q + c * r];
             }
             // wavefront tasks: column segments of at most kVSeg columns, row chunks of
-            // about kVTaskEntries entries (whole row groups); narrow panels read
-            // several rows per wavefront load
+            // about task_entries entries (whole row groups; twice kVTaskEntries for
+            // BF16 panels, whose steps cover twice the columns; QDFN_HMAT_VTASK sets
+            // it); narrow panels read several rows per wavefront load
             const int64_t nseg = (k + kVSeg - 1) / kVSeg;
-            const int64_t segw = round4((k + nseg - 1) / nseg);  // 16-byte aligned segment starts
+            const int64_t segw = round_al((k + nseg - 1) / nseg);  // 16-byte aligned segment starts
             int lpr = 1;
-            while (lpr < 64 && 4 * lpr < round4(segw)) lpr <<= 1;
+            while (lpr < 64 && al * lpr < segw) lpr <<= 1;
             const int64_t gr = (int64_t)kVSlots * (64 / lpr);
-            const int64_t chunk = std::max<int64_t>(gr, kVTaskEntries / std::max<int64_t>(segw, 1) / gr * gr);
+            const int64_t chunk = std::max<int64_t>(gr, task_entries / std::max<int64_t>(segw, 1) / gr * gr);
             for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
                 const int64_t mr = std::min<int64_t>(chunk, rows - r0);
                 for (int64_t sg = 0; sg < nseg; ++sg) {
@@ -660,51 +844,79 @@ int hmat_layout(qdfn_hmat* h) {
         int64_t gi = std::lower_bound(g0.begin(), g0.begin() + G, b.r0) - g0.begin();
         for (; gi < G && g0[gi + 1] - 1 <= b.r1; ++gi) cover[gi].push_back(bi);
     }
+    int64_t out_rows2 = 0;
     for (int64_t gi = 0; gi < G; ++gi)
         for (int w = 0; w < 2; ++w) {
             const int64_t m = g0[gi + 1] - g0[gi], row0 = g0[gi];
-            const int64_t i0 = (int64_t)idx.size();
-            for (int32_t bi : cover[gi]) {
-                const HBlock& b = B[bi];
-                if (b.kind[w] == QDFN_HBLK_DENSE)
-                    for (int64_t c = 0; c <= b.s1 - b.s0; ++c) idx.push_back((int32_t)(b.s0 + c));
-                else
-                    for (int64_t c = 0; c < b.rank[w]; ++c) idx.push_back((int32_t)(n + toff[2 * bi + w] + c));
+            out_rows2 += m;
+            // the row group's panel in dst over the columns of its dense blocks
+            // (sel 1), of its low-rank factors (sel 2) or of both (sel 0); BF16
+            // panels (in lo) have leading dimensions of multiples of eight
+            auto panel = [&](int sel, std::vector<double>& dst) {
+                const int64_t al = &dst == &lo ? 8 : 4;
+                auto take = [&](const HBlock& b) {
+                    const bool dense = b.kind[w] == QDFN_HBLK_DENSE;
+                    return sel == 0 || (sel == 1) == dense;
+                };
+                const int64_t i0 = (int64_t)idx.size();
+                for (int32_t bi : cover[gi]) {
+                    const HBlock& b = B[bi];
+                    if (!take(b)) continue;
+                    if (b.kind[w] == QDFN_HBLK_DENSE)
+                        for (int64_t c = 0; c <= b.s1 - b.s0; ++c) idx.push_back((int32_t)(b.s0 + c));
+                    else
+                        for (int64_t c = 0; c < b.rank[w]; ++c) idx.push_back((int32_t)(n + toff[2 * bi + w] + c));
+                }
+                const int64_t K = (int64_t)idx.size() - i0, ld = (K + al - 1) / al * al;
+                const int64_t off = (int64_t)dst.size();
+                dst.resize(off + m * ld, 0.0);
+                int64_t col = 0;
+                for (int32_t bi : cover[gi]) {
+                    const HBlock& b = B[bi];
+                    if (!take(b)) continue;
+                    const int64_t mb = b.r1 - b.r0 + 1, roff = row0 - b.r0;
+                    const bool diag = (w == 0 && b.s0 == b.r0 && b.s1 == b.r1 && b.kind[0] == QDFN_HBLK_DENSE);
+                    const int64_t nc = b.kind[w] == QDFN_HBLK_DENSE ? (b.s1 - b.s0 + 1) : b.rank[w];
+                    const std::vector<double>& A = b.a[w];
+                    for (int64_t i = 0; i < m; ++i)
+                        for (int64_t c = 0; c < nc; ++c) {
+                            double v = A[(roff + i) + c * mb];
+                            if (diag && row0 + i == b.s0 + c) v = 0.0;  // loading form: zero diagonal
+                            dst[off + i * ld + col + c] = v;
+                        }
+                    col += nc;
+                }
+                HRowBlock R;
+                R.data_off = off;
+                R.zoff = i0;
+                R.out = row0;
+                R.m = (int32_t)m;
+                R.K = (int32_t)K;
+                R.ld = (int32_t)ld;
+                R.kind = 1 + w;
+                return R;
+            };
+            if (!lo_far) {
+                add_rowblock(panel(0, pool), false);
+                continue;
             }
-            const int64_t K = (int64_t)idx.size() - i0, ld = round4(K);
-            const int64_t off = (int64_t)pool.size();
-            pool.resize(off + m * ld, 0.0);
-            int64_t col = 0;
-            for (int32_t bi : cover[gi]) {
-                const HBlock& b = B[bi];
-                const int64_t mb = b.r1 - b.r0 + 1, roff = row0 - b.r0;
-                const bool diag = (w == 0 && b.s0 == b.r0 && b.s1 == b.r1 && b.kind[0] == QDFN_HBLK_DENSE);
-                const int64_t nc = b.kind[w] == QDFN_HBLK_DENSE ? (b.s1 - b.s0 + 1) : b.rank[w];
-                const std::vector<double>& A = b.a[w];
-                for (int64_t i = 0; i < m; ++i)
-                    for (int64_t c = 0; c < nc; ++c) {
-                        double v = A[(roff + i) + c * mb];
-                        if (diag && row0 + i == b.s0 + c) v = 0.0;  // loading form: zero diagonal
-                        pool[off + i * ld + col + c] = v;
-                    }
-                col += nc;
-            }
-            HRowBlock R;
-            R.data_off = off;
-            R.zoff = i0;
-            R.out = row0;
-            R.m = (int32_t)m;
-            R.K = (int32_t)K;
-            R.ld = (int32_t)ld;
-            R.kind = 1 + w;
-            add_rowblock(R);
+            const HRowBlock Ra = panel(1, pool), Rb = panel(2, lo);
+            if (Rb.K == 0)
+                add_rowblock(Ra, false);
+            else if (Ra.K == 0)
+                add_rowblock(Rb, true);
+            else
+                add_pair(Ra, Rb);
         }
     h->n_rb2 = (int64_t)rbs.size() - h->n_rb1;
     h->n_tiles2 = (int64_t)tiles.size() - h->n_tiles1;
     h->n_multi2 = (int64_t)multi.size() - h->n_multi1;
     // largest tiles first within each phase (the dispatcher hands out workgroups
     // in order, so the tail of a launch is made of small tiles)
-    auto tile_bytes = [&](const HTile& T) { return (int64_t)rbs[T.rb].m * (T.c1 - T.c0); };
+    const int64_t es = h->prec == QDFN_PREC_FP64 ? sizeof(double) : sizeof(float);  // pool entries
+    auto tile_bytes = [&](const HTile& T) {
+        return (int64_t)rbs[T.rb].m * (T.c1 - T.c0) * (rbs[T.rb].lo ? (int64_t)sizeof(uint16_t) : es);
+    };
     auto by_size = [&](const HTile& a, const HTile& b) { return tile_bytes(a) > tile_bytes(b); };
     std::stable_sort(tiles.begin(), tiles.begin() + h->n_tiles1, by_size);
     std::stable_sort(tiles.begin() + h->n_tiles1, tiles.end(), by_size);
@@ -733,7 +945,7 @@ int hmat_layout(qdfn_hmat* h) {
                 tot += sz.back();
             }
             std::sort(sz.begin(), sz.end());
-            fprintf(stderr, "[hmat] phase %d: %zu tiles, %lld entries, per tile min %lld median %lld p90 %lld max %lld, "
+            fprintf(stderr, "[hmat] phase %d: %zu tiles, %lld bytes, per tile min %lld median %lld p90 %lld max %lld, "
                     "row blocks %lld (split %lld)\n", ph + 1, sz.size(), (long long)tot, (long long)sz.front(),
                     (long long)sz[sz.size() / 2], (long long)sz[sz.size() * 9 / 10], (long long)sz.back(),
                     (long long)(ph ? h->n_rb2 : h->n_rb1), (long long)(ph ? h->n_multi2 : h->n_multi1));
@@ -743,10 +955,15 @@ int hmat_layout(qdfn_hmat* h) {
     for (const HBlock& b : B)
         for (int w = 0; w < 2; ++w) h->stored_entries += (int64_t)(b.a[w].size() + b.b[w].size());
     // upload
-    const size_t es = h->prec == QDFN_PREC_FP32 ? sizeof(float) : sizeof(double);
     if (pool.size() < 4) pool.resize(4, 0.0);
     QDFN_HIP(h->pool.alloc(es * pool.size()));
-    if (h->prec == QDFN_PREC_FP32) {
+    if (lo_far) {
+        std::vector<uint16_t> b(std::max<size_t>(lo.size(), 4), 0);
+        for (size_t q = 0; q < lo.size(); ++q) b[q] = to_bf16(lo[q]);
+        QDFN_HIP(h->pool_lo.alloc(sizeof(uint16_t) * b.size()));
+        QDFN_HIP(hipMemcpy(h->pool_lo.p, b.data(), sizeof(uint16_t) * b.size(), hipMemcpyHostToDevice));
+    }
+    if (h->prec != QDFN_PREC_FP64) {
         std::vector<float> f(pool.begin(), pool.end());
         QDFN_HIP(hipMemcpy(h->pool.p, f.data(), es * f.size(), hipMemcpyHostToDevice));
     } else {
@@ -797,32 +1014,22 @@ int hmat_layout(qdfn_hmat* h) {
     QDFN_HIP(h->yS.alloc(sizeof(double) * n));
     QDFN_HIP(h->yN.alloc(sizeof(double) * n));
     // HBM bytes of one product: panel entries, indices, descriptors, t (write + read), x, yS, yN
-    h->phase_bytes = (int64_t)(es * pool.size() + sizeof(int32_t) * idx.size() + sizeof(HRowBlock) * rbs.size() +
-                               sizeof(HTile) * tiles.size() + sizeof(VTask) * vtasks.size() +
+    h->phase_bytes = (int64_t)(es * pool.size() + sizeof(uint16_t) * lo.size() + sizeof(int32_t) * idx.size() +
+                               sizeof(HRowBlock) * rbs.size() + sizeof(HTile) * tiles.size() + sizeof(VTask) * vtasks.size() +
                                sizeof(VFin) * vfins.size() + 16 * part_total + 16 * vpart_total + 16 * h->n_t +
                                8 * 3 * n);
     h->device_bytes = h->phase_bytes;
     // algorithmic bytes of each phase's main kernel: stored entries (unpadded),
     // index list (phase 2), gathered right-hand side, outputs
-    int64_t e1 = 0, e2 = 0, k1 = 0, k2 = 0, o1 = 0, o2 = 0;
-    for (size_t q = 0; q < rbs.size(); ++q) {
-        const HRowBlock& R = rbs[q];
-        if ((int64_t)q < h->n_rb1) {
-            e1 += (int64_t)R.m * R.K;
-            k1 += R.K;
-            o1 += R.m;
-        } else {
-            e2 += (int64_t)R.m * R.K;
-            k2 += R.K;
-            o2 += R.m;
-        }
+    // (phase 2: each output row once, also where it is formed from partial sums)
+    int64_t b2 = 0, k2 = 0;
+    for (int64_t q = h->n_rb1; q < (int64_t)rbs.size(); ++q) {
+        b2 += (int64_t)rbs[q].m * rbs[q].K * (rbs[q].lo ? (int64_t)sizeof(uint16_t) : es);
+        k2 += rbs[q].K;
     }
     // phase 1: V entries, each panel's x slice once, t
-    e1 = v_entries;
-    k1 = v_cols;
-    o1 = h->n_t;
-    h->alg_bytes1 = (int64_t)es * e1 + 8 * k1 + 8 * o1;
-    h->alg_bytes2 = (int64_t)es * e2 + 12 * k2 + 8 * o2;
+    h->alg_bytes1 = v_entries * (lo_far ? (int64_t)sizeof(uint16_t) : es) + 8 * v_cols + 8 * h->n_t;
+    h->alg_bytes2 = b2 + 12 * k2 + 8 * out_rows2;
     return QDFN_OK;
 }
 
@@ -834,7 +1041,7 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
     double* part = h->part.as<double>();
     const double* kd = h->kdiag.as<double>();
     double* t = h->t.as<double>();
-    const bool f32 = h->prec == QDFN_PREC_FP32;
+    const bool f32 = h->prec == QDFN_PREC_FP32, bf = h->prec == QDFN_PREC_BF16_FAR;
     if (ev0) QDFN_HIP(hipEventRecord(ev0, s));
     if (h->n_vtasks > 0) {
         const unsigned grid = (unsigned)((h->n_vtasks + kVThreads / 64 - 1) / (kVThreads / 64));
@@ -843,6 +1050,9 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
         if (f32)
             hipLaunchKernelGGL((k_vpanel<float>), dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
                                h->pool.as<float>(), d_x, t, vp, g);
+        else if (bf)
+            hipLaunchKernelGGL(k_vpanel_bf16, dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
+                               h->pool_lo.as<uint16_t>(), d_x, t, vp, g);
         else
             hipLaunchKernelGGL((k_vpanel<double>), dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
                                h->pool.as<double>(), d_x, t, vp, g);
@@ -857,7 +1067,11 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
     if (ev1) QDFN_HIP(hipEventRecord(ev1, s));
     const bool finish = h->n_multi2 > 0 && !finish_in_step;
     const int32_t mark_tiles = finish ? 0 : 1;
-    if (f32)
+    if (bf)
+        hipLaunchKernelGGL(k_hrow_mixed, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s, tiles + h->n_tiles1,
+                           rbs, part, h->pool.as<float>(), h->pool_lo.as<bf16_t>(), h->idx.as<int32_t>(), d_x, t,
+                           h->n, kd, form, d_yS, d_yN, mark_tiles, g);
+    else if (f32)
         hipLaunchKernelGGL((k_hrow<float, true>), dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
                            tiles + h->n_tiles1, rbs, part, h->pool.as<float>(), h->idx.as<int32_t>(), d_x, t, h->n,
                            kd, form, d_yS, d_yN, mark_tiles, g);
@@ -878,7 +1092,7 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
 // ---------------------------------------------------------------- build ----
 
 qdfn_hmat* new_hmat(int64_t n, int32_t prec) {
-    if (n <= 0 || (prec != QDFN_PREC_FP64 && prec != QDFN_PREC_FP32)) {
+    if (n <= 0 || (prec != QDFN_PREC_FP64 && prec != QDFN_PREC_FP32 && prec != QDFN_PREC_BF16_FAR)) {
         fail(QDFN_ERR_ARG, "H-matrix: bad size or precision");
         return nullptr;
     }

@@ -28,6 +28,7 @@ QDFN_ERR_EMPTY_DTMIN = -11
 
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_BF16_FAR = 2  # H operators: BF16 low-rank factors, FP32 dense blocks
 
 
 class QdfnError(RuntimeError):

@@ -1,11 +1,12 @@
 """Run the earthquake-cycle loop alone on the 20,000-triangle rough fault, for
 kernel traces of a whole attempt (product phases, local solves, decision).
 
-    python -m qdfn.profile_loop [--steps 1000] [--warmup 100] [--aca-tol 1e-6]
+    python -m qdfn.profile_loop [--steps 1000] [--warmup 100] [--aca-tol 1e-6] [--bf16-far]
 
-Builds the H operator (FP32 storage, leaves of 64, DistDiamRatioCrit 1.0) and
-the run's arrays the way the benchmark does, replays the captured attempt
-graph for `steps` accepted steps and prints the RHS evaluations per second.
+Builds the H operator (FP32 storage, or BF16 low-rank factors with --bf16-far;
+leaves of 64, DistDiamRatioCrit 1.0) and the run's arrays the way the
+benchmark does, replays the captured attempt graph for `steps` accepted steps
+and prints the RHS evaluations per second.
 """
 import argparse
 import sys
@@ -19,6 +20,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--aca-tol", type=float, default=1e-6)
+    ap.add_argument("--bf16-far", action="store_true")
     a = ap.parse_args()
     import qdfn
     from qdfn import configs
@@ -33,7 +35,7 @@ def main():
     p = st["perm"]
     geo = [x[p] for x in (P1, P2, P3, m["rake"], ctr, nrm, slip)]
     h = qdfn.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=a.aca_tol,
-                               prec=qdfn.PREC_FP32)
+                               prec=qdfn.PREC_BF16_FAR if a.bf16_far else qdfn.PREC_FP32)
     ks, _ = h.kself()
     hs, hn = h.matvec(np.ones(n), loading_form=False)
     prob = configs.problem_from_triangles(m, ks, hs, hn, order=p)

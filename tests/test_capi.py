@@ -65,3 +65,26 @@ def test_sampled_build_argument_checks_without_device():
                                           adm.ctypes.data_as(PI64), 1e-6, 0.0, _lib.PREC_FP64) is None
     assert "out of bounds" in _lib.last_error()
     assert L.qdfn_hmat_build_info(None, None) == _lib.QDFN_ERR_ARG
+
+
+def test_precision_checks_without_device():
+    import numpy as np
+    import qdfn
+    from qdfn import _lib
+    L = qdfn.lib()
+    PD, PI32, PI64 = _lib.PD, _lib.PI32, _lib.PI64
+    # BF16 low-rank factors are an H-operator storage: dense operators refuse it
+    K = np.eye(4, order="F")
+    assert L.qdfn_dense_create(K.ctypes.data_as(PD), K.ctypes.data_as(PD), 4, 4, _lib.PREC_BF16_FAR) is None
+    assert "prec" in _lib.last_error()
+    # an unknown storage precision is refused before the device is touched
+    rng = np.array([[1, 4, 1, 4]], dtype=np.int64)
+    adm = np.array([0], dtype=np.int64)
+    kind = np.zeros(1, dtype=np.int32)
+    rank = np.zeros(1, dtype=np.int64)
+    A = (PD * 1)(K.ctypes.data_as(PD))
+    B = (PD * 1)()
+    import ctypes as C
+    args = [kind.ctypes.data_as(PI32), rank.ctypes.data_as(PI64), C.cast(A, C.POINTER(PD)), C.cast(B, C.POINTER(PD))]
+    assert L.qdfn_hmat_create(4, 1, rng.ctypes.data_as(PI64), adm.ctypes.data_as(PI64), *args, *args, 3) is None
+    assert "precision" in _lib.last_error()
