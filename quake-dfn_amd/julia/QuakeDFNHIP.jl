@@ -134,7 +134,8 @@ end
 
 # H operator from the reference's ShearStiffness_H / NormalStiffness_H (RUN2_BUILD_INPUT.jl:362):
 # dense blocks pass as they are; a LowRankApprox PartialQR F (A[:, F[:p]] ~ F[:Q] F[:R]) passes as
-# U = F[:Q] and V with V[:, F[:p]] = F[:R].  prec: 0 FP64, 1 FP32 storage.
+# U = F[:Q] and V with V[:, F[:p]] = F[:R].  prec: 0 FP64, 1 FP32 storage, 2 BF16 low-rank
+# factors with FP32 dense blocks (QDFN_PREC_BF16_FAR; FP64 accumulation in every case).
 function HmatCreate(FaultCount, ElementRange_SR, Admissible, ShearStiffness_H, NormalStiffness_H; prec=0)
     nb = size(ElementRange_SR, 1)
     rng = Matrix{Int64}(permutedims(ElementRange_SR))          # 4 x nb = nb x 4 row-major
@@ -169,7 +170,7 @@ end
 # (ACA with full pivoting + SVD recompression, Frobenius tolerance relative to each block's
 # shear norm; Tolerance in the reference's absolute sense can be passed as atol with rtol = 0).
 # sampled = true: admissible blocks from sampled rows and columns, never assembled in full
-# (qdfn_hmat_build_rect_sampled; n is then not bounded by n^2 device memory).
+# (qdfn_hmat_build_rect_sampled; n is then not bounded by n^2 device memory).  prec as for HmatCreate.
 function HmatBuild_R_Device(ShearModulus, PoissonRatio, ElementRange_SR, Input_Segment, Admissible;
                             rtol=1e-6, atol=0.0, prec=1, sampled=false)
     seg = Matrix{Float64}(Input_Segment); n = size(seg, 1)
