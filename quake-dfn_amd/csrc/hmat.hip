@@ -168,7 +168,7 @@ __device__ inline void hmat_out(int kind, int64_t i, double s, const double* __r
 // power of two >= the chunk width / W (at most 64), so a wavefront load covers
 // 64 / LPR rows.  A row block with one tile writes its outputs; a split one
 // writes partial row sums, added by k_hfinish or the step kernel.
-template <class T, bool GATHER>
+template <class T>
 __device__ __forceinline__ void hrow_body(const HTile& tl, const HRowBlock& R, double* __restrict__ part,
                                           const T* __restrict__ pool, const int32_t* __restrict__ idx,
                                           const double* __restrict__ x, double* __restrict__ t, int64_t n,
@@ -194,25 +194,20 @@ __device__ __forceinline__ void hrow_body(const HTile& tl, const HRowBlock& R, d
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int32_t c = c0 + (int32_t)threadIdx.x + q * kPhase2Threads;
-            iv[q] = GATHER ? idx[R.zoff + min(c, K - 1)] : 0;
+            iv[q] = idx[R.zoff + min(c, K - 1)];
         }
     };
     auto load_vals = [&](int32_t c0) {
 #pragma unroll
         for (int q = 0; q < kPer; ++q) {
             const int32_t c = c0 + (int32_t)threadIdx.x + q * kPhase2Threads;
-            double val;
-            if (GATHER) {
-                const int32_t j = iv[q];
-                val = j < n ? x[j] : t[j - n];
-            } else {
-                val = x[R.zoff + min(c, K - 1)];
-            }
+            const int32_t j = iv[q];
+            const double val = j < n ? x[j] : t[j - n];
             gv[q] = c < K ? val : 0.0;
         }
     };
     if (K > 0) {
-        if (GATHER) load_idx(tl.c0);
+        load_idx(tl.c0);
         load_vals(tl.c0);
     } else {
 #pragma unroll
@@ -220,7 +215,7 @@ __device__ __forceinline__ void hrow_body(const HTile& tl, const HRowBlock& R, d
     }
 #pragma unroll
     for (int q = 0; q < kPer; ++q) zbuf[0][threadIdx.x + q * kPhase2Threads] = gv[q];
-    if (GATHER && tl.c0 + kChunk < KW) load_idx(tl.c0 + kChunk);
+    if (tl.c0 + kChunk < KW) load_idx(tl.c0 + kChunk);
     __syncthreads();
     double acc[kSlots];
 #pragma unroll
@@ -230,7 +225,7 @@ __device__ __forceinline__ void hrow_body(const HTile& tl, const HRowBlock& R, d
         const bool more = c0 + kChunk < KW;
         if (more) {
             load_vals(c0 + kChunk);
-            if (GATHER && c0 + 2 * kChunk < KW) load_idx(c0 + 2 * kChunk);
+            if (c0 + 2 * kChunk < KW) load_idx(c0 + 2 * kChunk);
         }
         const double* z = zbuf[buf];
         const int32_t lim = min((int32_t)kChunk, KW - c0);
@@ -273,7 +268,7 @@ __device__ __forceinline__ void hrow_body(const HTile& tl, const HRowBlock& R, d
     }
 }
 
-template <class T, bool GATHER>
+template <class T>
 __global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict__ tiles,
                                                          const HRowBlock* __restrict__ rbs,
                                                          double* __restrict__ part,
@@ -288,7 +283,7 @@ __global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict
     const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
     const HTile tl = tiles[blockIdx.x];
     const HRowBlock R = rbs[tl.rb];
-    hrow_body<T, GATHER>(tl, R, part, pool, idx, x, t, n, kdiag, form, yS, yN, zbuf);
+    hrow_body<T>(tl, R, part, pool, idx, x, t, n, kdiag, form, yS, yN, zbuf);
     if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
 }
 
@@ -312,9 +307,9 @@ __global__ __launch_bounds__(kPhase2Threads, 8) void k_hrow_mixed(const HTile* _
     const HTile tl = tiles[blockIdx.x];
     const HRowBlock R = rbs[tl.rb];
     if (R.lo)
-        hrow_body<bf16_t, true>(tl, R, part, pool_lo, idx, x, t, n, kdiag, form, yS, yN, zbuf);
+        hrow_body<bf16_t>(tl, R, part, pool_lo, idx, x, t, n, kdiag, form, yS, yN, zbuf);
     else
-        hrow_body<float, true>(tl, R, part, pool, idx, x, t, n, kdiag, form, yS, yN, zbuf);
+        hrow_body<float>(tl, R, part, pool, idx, x, t, n, kdiag, form, yS, yN, zbuf);
     if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
 }
 
@@ -1072,11 +1067,11 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
                            rbs, part, h->pool.as<float>(), h->pool_lo.as<bf16_t>(), h->idx.as<int32_t>(), d_x, t,
                            h->n, kd, form, d_yS, d_yN, mark_tiles, g);
     else if (f32)
-        hipLaunchKernelGGL((k_hrow<float, true>), dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
+        hipLaunchKernelGGL(k_hrow<float>, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
                            tiles + h->n_tiles1, rbs, part, h->pool.as<float>(), h->idx.as<int32_t>(), d_x, t, h->n,
                            kd, form, d_yS, d_yN, mark_tiles, g);
     else
-        hipLaunchKernelGGL((k_hrow<double, true>), dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
+        hipLaunchKernelGGL(k_hrow<double>, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
                            tiles + h->n_tiles1, rbs, part, h->pool.as<double>(), h->idx.as<int32_t>(), d_x, t, h->n,
                            kd, form, d_yS, d_yN, mark_tiles, g);
     QDFN_HIP(hipGetLastError());
