@@ -330,16 +330,23 @@ def adjoint_case(qd, args):
     return out
 
 
-def cpu_baseline(h32, st, prob_h, n_steps):
-    """The oracle's FP64 main_H (C restatement of HmatSolver_Pararllel with one
-    thread and of the loop) on the same stored blocks, for a bounded number of
-    steps."""
+def cpu_baseline(h32, st, prob_h, n_steps, threads):
+    """The oracle's FP64 main_H (C restatement of HmatSolver_Pararllel and of the
+    loop; products and local solves over OpenMP threads, outputs independent of
+    the count) on the same stored blocks, for a bounded number of steps, at each
+    thread count in `threads`."""
     from oracle import oracle as orc
     HS, HN = h32.blocks(st["range_SR"])
-    t0 = time.perf_counter()
-    status, r = orc.sim_hmat(prob_h, HS, HN, total_step=n_steps, record_step=0, trace=False)
-    el = time.perf_counter() - t0
-    return dict(value=r["rhs_evals"] / el, elapsed=el, steps=n_steps, status=status)
+    out = {}
+    prev = orc.set_threads(1)
+    for th in threads:
+        orc.set_threads(th)
+        t0 = time.perf_counter()
+        status, r = orc.sim_hmat(prob_h, HS, HN, total_step=n_steps, record_step=0, trace=False)
+        el = time.perf_counter() - t0
+        out[th] = dict(value=r["rhs_evals"] / el, elapsed=el, steps=n_steps, status=status)
+    orc.set_threads(prev)
+    return out
 
 
 def main():
@@ -351,7 +358,7 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=300, help="direct-launch steps with per-kernel events")
     ap.add_argument("--crit", type=float, default=1.0, help="DistDiamRatioCrit of the cluster tree")
     ap.add_argument("--aca-tol", type=float, default=1e-6)
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--network-steps", type=int, default=6000)
     ap.add_argument("--no-network", action="store_true")
@@ -384,11 +391,17 @@ def main():
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU baseline: oracle FP64 main_H, {args.cpu_steps} steps")
-        c = cpu_baseline(rf["h32"], rf["st"], rf["prob_h"], args.cpu_steps)
-        cpu = {"value": round(c["value"], 4), "unit": "RHS evals/s", "cores": 1, "kind": "port",
+        # all of the host's share of cores (16 per GPU on a shared box), and one
+        nall = min(int(os.environ.get("OMP_NUM_THREADS", "0") or os.cpu_count() or 1), os.cpu_count() or 1, 16)
+        cs = cpu_baseline(rf["h32"], rf["st"], rf["prob_h"], args.cpu_steps, sorted({1, nall}))
+        c, c1 = cs[nall], cs[1]
+        cpu = {"value": round(c["value"], 4), "unit": "RHS evals/s", "cores": nall, "kind": "port",
                "sample": f"{c['steps']} accepted steps of the 20,000-triangle H-matrix run, FP64 C restatement "
-                         f"of main_H / HmatSolver on the same blocks, one thread ({c['elapsed']:.1f}s)"}
-        log(f"[bench] CPU baseline {c['value']:.3f} RHS evals/s")
+                         f"of main_H / HmatSolver on the same blocks, {nall} OpenMP threads "
+                         f"({c['elapsed']:.1f}s; one thread: {c1['value']:.3f} RHS evals/s, "
+                         f"{c1['elapsed']:.1f}s)",
+               "single_thread": {"value": round(c1["value"], 4), "cores": 1}}
+        log(f"[bench] CPU baseline {c['value']:.3f} RHS evals/s on {nall} threads, {c1['value']:.3f} on one")
     rf["h32"].close()
     rk = rf["rk"]
     out = {

@@ -1,6 +1,7 @@
 /* ORACLE — test infrastructure only.  H-matrix parts of the reference:
  *   orc_hmat_matvec      HmatSolver_Pararllel (scripts/Functions_Hmatrix.jl:631-646):
- *                        y[rcv] += B * x[src] block by block (one thread's order);
+ *                        y[rcv] += B * x[src] block by block (each row's sum in
+ *                        block order, parallel over rows);
  *                        a low-rank block (PartialQR) applies as U * (V * x)
  *                        (Functions_Hmatrix.jl:637-638).
  *   orc_sim_hmat         main_H (scripts/Functions_RSFDFN3DMain_H.jl:2-383) through
@@ -12,42 +13,72 @@
  *                        line, including its interval test for blocks already
  *                        covered at a coarser level (:231-236). */
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "oracle.h"
 
+int orc_set_threads(int n) {
+    const int prev = omp_get_max_threads();
+    if (n > 0) omp_set_num_threads(n);
+    return prev;
+}
+
+/* Parallel over receiver rows with the serial summation order: t = V x of
+ * every low-rank block first (each as in the one-thread loop), then each row
+ * adds its blocks' contributions (each a sum over columns, or over the rank,
+ * in ascending order) in block order, as y[rcv] += B * x[src] block by block
+ * does.  The result is the same for any thread count. */
 void orc_hmat_matvec(const orc_hmat_t *h, const double *x, double *y) {
-    memset(y, 0, sizeof(double) * h->n);
-    for (int64_t q = 0; q < h->nblk; ++q) {
-        const int64_t s0 = h->range[4 * q] - 1, s1 = h->range[4 * q + 1] - 1;
-        const int64_t r0 = h->range[4 * q + 2] - 1, r1 = h->range[4 * q + 3] - 1;
-        const int64_t m = r1 - r0 + 1, k = s1 - s0 + 1;
-        const double *xs = x + s0;
-        if (h->kind[q] == 0) {
-            const double *A = h->a[q];
-            for (int64_t i = 0; i < m; ++i) {
-                double acc = 0.0;
-                for (int64_t j = 0; j < k; ++j) acc += A[i + j * m] * xs[j];
-                y[r0 + i] += acc;
-            }
-        } else {
-            const int64_t r = h->rank[q];
-            const double *U = h->a[q], *V = h->b[q];
-            double *t = (double *)calloc(r > 0 ? r : 1, sizeof(double));
-            for (int64_t p = 0; p < r; ++p) {
-                double acc = 0.0;
-                for (int64_t j = 0; j < k; ++j) acc += V[p + j * r] * xs[j];
-                t[p] = acc;
-            }
-            for (int64_t i = 0; i < m; ++i) {
-                double acc = 0.0;
-                for (int64_t p = 0; p < r; ++p) acc += U[i + p * m] * t[p];
-                y[r0 + i] += acc;
-            }
-            free(t);
+    const int64_t n = h->n, nb = h->nblk;
+    int64_t *toff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+    toff[0] = 0;
+    for (int64_t q = 0; q < nb; ++q) toff[q + 1] = toff[q] + (h->kind[q] == 0 ? 0 : h->rank[q]);
+    double *t = (double *)malloc(sizeof(double) * (size_t)(toff[nb] > 0 ? toff[nb] : 1));
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < nb; ++q) {
+        if (h->kind[q] == 0) continue;
+        const int64_t s0 = h->range[4 * q] - 1, k = h->range[4 * q + 1] - s0;
+        const int64_t r = h->rank[q];
+        const double *V = h->b[q], *xs = x + s0;
+        for (int64_t p = 0; p < r; ++p) {
+            double acc = 0.0;
+            for (int64_t j = 0; j < k; ++j) acc += V[p + j * r] * xs[j];
+            t[toff[q] + p] = acc;
         }
     }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i0 = 0; i0 < n; i0 += 64) {
+        const int64_t i1 = i0 + 64 < n ? i0 + 64 : n;
+        for (int64_t i = i0; i < i1; ++i) y[i] = 0.0;
+        for (int64_t q = 0; q < nb; ++q) {
+            const int64_t s0 = h->range[4 * q] - 1, s1 = h->range[4 * q + 1] - 1;
+            const int64_t r0 = h->range[4 * q + 2] - 1, r1 = h->range[4 * q + 3] - 1;
+            if (r1 < i0 || r0 >= i1) continue;
+            const int64_t m = r1 - r0 + 1, k = s1 - s0 + 1;
+            const int64_t lo = r0 > i0 ? r0 : i0, hi = r1 + 1 < i1 ? r1 + 1 : i1;
+            const double *xs = x + s0;
+            if (h->kind[q] == 0) {
+                const double *A = h->a[q];
+                for (int64_t i = lo; i < hi; ++i) {
+                    double acc = 0.0;
+                    for (int64_t j = 0; j < k; ++j) acc += A[(i - r0) + j * m] * xs[j];
+                    y[i] += acc;
+                }
+            } else {
+                const int64_t r = h->rank[q];
+                const double *U = h->a[q], *tq = t + toff[q];
+                for (int64_t i = lo; i < hi; ++i) {
+                    double acc = 0.0;
+                    for (int64_t p = 0; p < r; ++p) acc += U[(i - r0) + p * m] * tq[p];
+                    y[i] += acc;
+                }
+            }
+        }
+    }
+    free(t);
+    free(toff);
 }
 
 typedef struct {

@@ -100,6 +100,9 @@ typedef struct {
     const double *const *b;
 } orc_hmat_t;
 void orc_hmat_matvec(const orc_hmat_t *h, const double *x, double *y);
+/* OpenMP threads of the parallel loops (the products and the local solves);
+ * results do not depend on it.  Returns the previous count. */
+int orc_set_threads(int n);
 int orc_sim_hmat(const orc_hmat_t *shear_loading, const orc_hmat_t *normal, const orc_sim_in_t *in,
                  orc_sim_out_t *out);
 /* node arrays: begin/end 1-based inclusive, level, centre (n_nodes x 3 row-major),

@@ -50,6 +50,12 @@ def lib():
     return _LIB
 
 
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (dense and H products,
+    local solves); outputs do not depend on it.  Returns the previous count."""
+    return lib().orc_set_threads(int(n))
+
+
 def _p(a):
     if a is None:
         return None
@@ -104,6 +110,8 @@ class HmatOp(C.Structure):
 
 def _declare(L):
     L.orc_dc3d.argtypes = [D, I64, PD, PD, PD, D, D, D, D, D, D, D, D, D, PD, PD]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_threads.restype = C.c_int
     L.orc_stiffness_rect.argtypes = [PD, I64, I64, PD, I64, I64, D, D, PD, PD, I64]
     L.orc_evolution_law.argtypes = [D, D, D, D, D, D, D, D, C.c_int, C.c_int]
     L.orc_evolution_law.restype = D
@@ -360,7 +368,8 @@ def _hmat_struct(H):
 
 
 def hmat_matvec(H, x):
-    """HmatSolver_Pararllel (Functions_Hmatrix.jl:631-646) with one thread."""
+    """HmatSolver_Pararllel (Functions_Hmatrix.jl:631-646): each row's sum in
+    block order (the same for any thread count)."""
     s, keep = _hmat_struct(H)
     x = _f64(x)
     y = np.zeros(int(H["n"]))

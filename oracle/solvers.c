@@ -204,6 +204,8 @@ int64_t orc_solve_one_timestep(const orc_step_params_t *p, int64_t n, int64_t n_
                                double *Dt_All, int32_t *Instab, int32_t *Flag, int32_t *Iters) {
     int64_t stalls = 0;
     double DtOld = Dt;
+    /* elements are independent: any thread count gives the same outputs */
+#pragma omp parallel for schedule(static) reduction(+ : stalls)
     for (int64_t i = 0; i < n - n_loading; ++i) {
         orc_elem_in_t in;
         in.conv_crit = p->conv_crit; in.disp_old = DispOld[i]; in.friction_old = FrictionOld[i];
