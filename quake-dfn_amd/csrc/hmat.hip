@@ -754,6 +754,14 @@ int hmat_layout(qdfn_hmat* h) {
             const int64_t v = atoll(e);
             if (v >= 256 && v <= (1 << 20)) task_entries = v;
         }
+        // lanes per row at most: 16 (64 or 128 entries per row and step) measured
+        // faster than one step per row for wider panels (fewer cross-lane
+        // reductions per entry); QDFN_HMAT_VLPR sets it
+        int lpr_max = 16;
+        if (const char* e = getenv("QDFN_HMAT_VLPR")) {
+            const int v = atoi(e);
+            if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) lpr_max = v;
+        }
         int64_t tbase = 0;
         for (size_t gk = 0; gk < keys.size(); ++gk) {
             const int64_t s0 = keys[gk].first, k = keys[gk].second - s0 + 1, ld = round_al(k);
@@ -779,7 +787,7 @@ int hmat_layout(qdfn_hmat* h) {
             const int64_t nseg = (k + kVSeg - 1) / kVSeg;
             const int64_t segw = round_al((k + nseg - 1) / nseg);  // 16-byte aligned segment starts
             int lpr = 1;
-            while (lpr < 64 && al * lpr < segw) lpr <<= 1;
+            while (lpr < lpr_max && al * lpr < segw) lpr <<= 1;
             const int64_t gr = (int64_t)kVSlots * (64 / lpr);
             const int64_t chunk = std::max<int64_t>(gr, task_entries / std::max<int64_t>(segw, 1) / gr * gr);
             for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
