@@ -142,6 +142,30 @@ struct RowLoad<bf16_t> {
     }
 };
 
+// x moved between lanes by the DPP control CTRL (both 32-bit halves)
+template <int CTRL>
+__device__ inline double dpp_f64(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over aligned groups of lpr lanes (a power of two), valid in every lane of
+// the group: within 16-lane rows by DPP moves (quad permutations [1,0,3,2] and
+// [2,3,0,1], then the half-row and row mirrors, lane i with 7 - i and 15 - i:
+// after each round all lanes of the group so far hold its sum), across rows
+// by shuffles
+__device__ inline double group_sum(double v, int lpr) {
+    if (lpr >= 2) v += dpp_f64<0xB1>(v);
+    if (lpr >= 4) v += dpp_f64<0x4E>(v);
+    if (lpr >= 8) v += dpp_f64<0x141>(v);
+    if (lpr >= 16) v += dpp_f64<0x140>(v);
+    if (lpr >= 32) v += __shfl_xor(v, 16, 64);
+    if (lpr >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
 __device__ inline double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
@@ -383,8 +407,7 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ 
         if (ci == nc - 1) {
 #pragma unroll
             for (int r = 0; r < kVSlots; ++r) {
-                double v = acc[r];
-                for (int o = lpr >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                const double v = group_sum(acc[r], lpr);
                 const int32_t row = gi * gr + rl + r * rpw;
                 if (sub == 0 && row < tk.m) {
                     if (tk.part < 0)
@@ -464,8 +487,7 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restri
         if (ci == nc - 1) {
 #pragma unroll
             for (int r = 0; r < kVSlots; ++r) {
-                double v = acc[r];
-                for (int o = lpr >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+                const double v = group_sum(acc[r], lpr);
                 const int32_t row = gi * gr + rl + r * rpw;
                 if (sub == 0 && row < tk.m) {
                     if (tk.part < 0)
