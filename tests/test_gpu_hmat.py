@@ -188,3 +188,27 @@ def test_tri_build_and_product(orc, qd):
     assert np.abs(yS - orc.hmat_matvec(blocks[0], x)).max() <= 1e-12 * np.abs(yS).max()
     assert np.abs(yN - orc.hmat_matvec(blocks[1], x)).max() <= 1e-12 * np.abs(yN).max()
     H.close()
+
+
+@pytest.mark.parametrize("prec_name", ["PREC_FP64", "PREC_BF16_FAR"])
+def test_phase1_lanes_per_row(net, qd, monkeypatch, prec_name):
+    """The phase-1 layout with 1 to 64 lanes per V-panel row (QDFN_HMAT_VLPR;
+    row sums by DPP moves within 16 lanes, shuffles across rows) gives the
+    default layout's products to FP64 summation order."""
+    rs, ad = net["st"]["range_SR"], net["st"]["admissible"]
+    parts = []
+    for B in net["blocks"]:
+        parts.append([B["a"][q] if B["kind"][q] == 0 else (B["a"][q], B["b"][q]) for q in range(len(B["a"]))])
+    n = net["seg"].shape[0]
+    prec = getattr(qd, prec_name)
+    x = np.random.default_rng(10).standard_normal(n)
+    H0 = qd.HMatrix.from_blocks(n, rs, ad, parts[0], parts[1], prec=prec)
+    ref = H0.matvec(x, False)
+    H0.close()
+    for lanes in ("1", "2", "4", "8", "32", "64"):
+        monkeypatch.setenv("QDFN_HMAT_VLPR", lanes)
+        H = qd.HMatrix.from_blocks(n, rs, ad, parts[0], parts[1], prec=prec)
+        y = H.matvec(x, False)
+        H.close()
+        for u, v in zip(ref, y):
+            assert np.abs(u - v).max() <= 1e-12 * np.abs(u).max(), lanes
