@@ -218,9 +218,14 @@ __device__ __forceinline__ void setup_body(const SimArrays& A, SimState* __restr
 __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restrict__ st, unsigned blk, unsigned nblk) {
     __shared__ BlockPart shp[kStepBlock / 64];
     __shared__ int s_last;
-    if (st->done) return;
     const int64_t n = A.n;
     const int64_t i = (int64_t)blk * kStepBlock + threadIdx.x;
+    // element constants, issued before the run state is read (their latency
+    // overlaps the state's)
+    const int64_t ic = i < n ? i : n - 1;
+    const double c_f0 = A.f0[ic], c_a = A.a[ic], c_b = A.b[ic], c_dc = A.dc[ic], c_k = A.kself[ic];
+    const double c_omega = A.omega[ic], c_mass = A.mass[ic], c_sigma0 = A.sigma0[ic], c_tau0 = A.tau0[ic];
+    if (st->done) return;
     const int p = st->parity;
     const int64_t o = (int64_t)p * n, w = (int64_t)(1 - p) * n;
     const double dt = st->dt, told = st->told;
@@ -248,17 +253,17 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
                 if (!A.planar) sp_i = v;
             }
         }
-        double sig = sp_i + A.sigma0[i] + dsn;
+        double sig = sp_i + c_sigma0 + dsn;
         if (sig < A.min_sig) sig = A.min_sig;
         double vn;
         if (i < n - A.n_loading) {
             const int sw = fresh ? (v_old > A.switch_v ? 1 : 0) : A.sw[i];
             ElemIn e;
             e.conv = A.conv; e.d_old = A.D[o + i]; e.f_old = A.F[o + i]; e.th_old = A.TH[o + i];
-            e.v_old = v_old; e.f0 = A.f0[i]; e.a = A.a[i]; e.b = A.b[i]; e.dc = A.dc[i]; e.v0 = A.v0;
-            e.k = A.kself[i]; e.omega = A.omega[i]; e.mass = A.mass[i]; e.mu = A.mu; e.sig = sig;
+            e.v_old = v_old; e.f0 = c_f0; e.a = c_a; e.b = c_b; e.dc = c_dc; e.v0 = A.v0;
+            e.k = c_k; e.omega = c_omega; e.mass = c_mass; e.mu = A.mu; e.sig = sig;
             e.tl = 0.0 + tl_i; e.rho = A.rho; e.dtau = dtau; e.switch_v = A.switch_v; e.alpha_evo = A.alpha_evo;
-            e.sig_old = A.SIG[o + i]; e.tau0 = A.tau0[i]; e.dr = A.dr;
+            e.sig_old = A.SIG[o + i]; e.tau0 = c_tau0; e.dr = A.dr;
             const ElemOut r = (sw == 1) ? solve_high_v(e, dt) : solve_low_v(e, dt);
             A.V[w + i] = r.v; A.F[w + i] = r.f; A.D[w + i] = r.d; A.TH[w + i] = r.th; A.SIG[w + i] = r.sig;
             ok = r.ok;
