@@ -33,10 +33,10 @@ __global__ __launch_bounds__(256) void k_coupling(const T* __restrict__ E, const
                                                   int64_t n, int64_t cps, const double* __restrict__ xbase,
                                                   double* __restrict__ partS, double* __restrict__ partN,
                                                   int planar, CouplingGate g) {
-    const int run = g.gate ? *g.gate : 1;
+    const int run = g.gate ? ctrl_load(g.gate) : 1;
     if (g.ran_log && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) g.ran_log[g.slot] = run;
     if (!run) return;
-    const double* __restrict__ x = xbase + (g.parity ? (int64_t)(*g.parity) * g.parity_stride : 0);
+    const double* __restrict__ x = xbase + (g.parity ? (int64_t)ctrl_load(g.parity) * g.parity_stride : 0);
     using P = RowPack<T>;
     using V = typename P::V;
     constexpr int R = P::R;
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_coupling_finish(const double* __restric
                                                          const double* __restrict__ partN, int64_t ld, int64_t n,
                                                          int splits, int planar, double* __restrict__ tl,
                                                          double* __restrict__ sigprod, const int* __restrict__ gate) {
-    if (gate && *gate == 0) return;
+    if (gate && ctrl_load(gate) == 0) return;
     __shared__ double sh[2][4][64];
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int64_t i = (int64_t)blockIdx.x * 64 + lane;

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_ens_coupling(const EnsTile* __restrict_
     int any = 0;
 #pragma unroll
     for (int r = 0; r < RG; ++r) {
-        run[r] = force ? 1 : *insts[G.inst[r]].gate;
+        run[r] = force ? 1 : ctrl_load(insts[G.inst[r]].gate);
         any |= run[r];
     }
     if (!any) return;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_ens_coupling(const EnsTile* __restrict_
 #pragma unroll
     for (int r = 0; r < RG; ++r) {
         const EnsInst& I = insts[G.inst[r]];
-        x[r] = I.D + (int64_t)(*I.parity) * I.n;
+        x[r] = I.D + (int64_t)ctrl_load(I.parity) * I.n;
     }
     double aS[RG][4], aN[RG][4];
 #pragma unroll
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void k_ens_finish(const EnsBlk* __restrict__ f
     __shared__ double sh[2][4][64];
     const EnsBlk f = fins[blockIdx.x];
     const EnsInst& I = insts[f.inst];
-    if (!force && *I.gate == 0) return;
+    if (!force && ctrl_load(I.gate) == 0) return;
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int64_t i = (int64_t)f.blk * 64 + lane;
     double s = 0.0, r = 0.0;
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_ens_count(SimState* const* __restrict__
     int c = 0;
     // the flags were written by earlier kernels: volatile (vector) loads, never
     // the scalar cache, which a one-workgroup kernel can find holding an old line
-    for (int k = threadIdx.x; k < m; k += 256) c += *(const volatile int32_t*)&states[k]->done ? 0 : 1;
+    for (int k = threadIdx.x; k < m; k += 256) c += ctrl_load(&states[k]->done) ? 0 : 1;
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
     __syncthreads();

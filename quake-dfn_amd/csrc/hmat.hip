@@ -302,9 +302,9 @@ __global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict
                                                          int64_t n, const double* __restrict__ kdiag, int32_t form,
                                                          double* __restrict__ yS, double* __restrict__ yN,
                                                          int32_t mark, CouplingGate g) {
-    if (g.gate && *g.gate == 0) return;
+    if (g.gate && ctrl_load(g.gate) == 0) return;
     __shared__ double zbuf[2][kChunk];
-    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
     const HTile tl = tiles[blockIdx.x];
     const HRowBlock R = rbs[tl.rb];
     hrow_body<T>(tl, R, part, pool, idx, x, t, n, kdiag, form, yS, yN, zbuf);
@@ -325,9 +325,9 @@ __global__ __launch_bounds__(kPhase2Threads, 8) void k_hrow_mixed(const HTile* _
                                                                int64_t n, const double* __restrict__ kdiag,
                                                                int32_t form, double* __restrict__ yS,
                                                                double* __restrict__ yN, int32_t mark, CouplingGate g) {
-    if (g.gate && *g.gate == 0) return;
+    if (g.gate && ctrl_load(g.gate) == 0) return;
     __shared__ double zbuf[2][kChunk];
-    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
     const HTile tl = tiles[blockIdx.x];
     const HRowBlock R = rbs[tl.rb];
     if (R.lo)
@@ -359,11 +359,11 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ 
                                                       const T* __restrict__ pool, const double* __restrict__ x0,
                                                       double* __restrict__ t, double* __restrict__ vpart,
                                                       CouplingGate g) {
-    if (g.gate && *g.gate == 0) return;
+    if (g.gate && ctrl_load(g.gate) == 0) return;
     const int lane = threadIdx.x & 63;
     const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
     if (ti >= ntasks) return;
-    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
     const VTask tk = tasks[ti];
     const int lpr = tk.lpr, rpw = 64 / lpr, sub = lane & (lpr - 1), rl = lane / lpr;
     const int32_t K = tk.K, K4 = (K + 3) / 4 * 4;
@@ -441,11 +441,11 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restri
                                                            const uint16_t* __restrict__ pool,
                                                            const double* __restrict__ x0, double* __restrict__ t,
                                                            double* __restrict__ vpart, CouplingGate g) {
-    if (g.gate && *g.gate == 0) return;
+    if (g.gate && ctrl_load(g.gate) == 0) return;
     const int lane = threadIdx.x & 63;
     const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
     if (ti >= ntasks) return;
-    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
     const VTask tk = tasks[ti];
     const int lpr = tk.lpr, rpw = 64 / lpr, sub = lane & (lpr - 1), rl = lane / lpr;
     const int32_t K = tk.K, K8 = (K + 7) / 8 * 8;
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restri
 __global__ __launch_bounds__(256) void k_vfinish(const VFin* __restrict__ fins, int32_t nfin,
                                                  const double* __restrict__ vpart, double* __restrict__ t,
                                                  CouplingGate g) {
-    if (g.gate && *g.gate == 0) return;
+    if (g.gate && ctrl_load(g.gate) == 0) return;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     // fins are per row: i indexes a row of some split row chunk
     if (i >= nfin) return;
@@ -530,8 +530,8 @@ __global__ __launch_bounds__(64) void k_hfinish(const int32_t* __restrict__ mult
                                                 const double* __restrict__ kdiag, int32_t form,
                                                 double* __restrict__ t, double* __restrict__ yS,
                                                 double* __restrict__ yN, int32_t mark, CouplingGate g) {
-    if (g.gate && *g.gate == 0) return;
-    const double* x = g.parity ? x0 + (int64_t)(*g.parity) * g.parity_stride : x0;
+    if (g.gate && ctrl_load(g.gate) == 0) return;
+    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
     const HRowBlock R = rbs[multi[blockIdx.x]];
     const int row = threadIdx.x;
     if (row < R.m) {
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(64) void k_hfinish(const int32_t* __restrict__ mult
 }
 
 __global__ void k_mark_skipped(CouplingGate g) {
-    if (g.ran_log && g.gate && *g.gate == 0) g.ran_log[g.slot] = 0;
+    if (g.ran_log && g.gate && ctrl_load(g.gate) == 0) g.ran_log[g.slot] = 0;
 }
 
 // Adaptive cross approximation with full pivoting of one block (one workgroup).

@@ -72,6 +72,20 @@ __host__ __device__ inline double cos_deg(double x) {
 
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+// Control words — coupling gates, the run state's buffer parity, step count,
+// Dt and decision flags — are written by one kernel (plain vector stores) and
+// read by the next kernels on the stream.  Their addresses are uniform, so a
+// plain load may be compiled to a scalar load served by the scalar data cache,
+// which vector stores do not update; whether a kernel boundary invalidates that
+// cache depends on the acquire fence of the dispatch, which the runtime may
+// narrow between kernels of one queue or graph.  Every such read is therefore
+// a relaxed atomic load at agent scope: a vector load that the memory model
+// makes coherent with stores of earlier kernels on the device.
+template <typename T>
+__device__ __forceinline__ T ctrl_load(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Device-resident histories of a run (History_Time / History_Disp): time[n_rec],
 // disp column-major n_rec x n with leading dimension ld.  Valid until the run
 // is destroyed; work on them is ordered after the run's stream.

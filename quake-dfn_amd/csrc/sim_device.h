@@ -24,6 +24,30 @@ struct SimState {
     int32_t pad;
 };
 
+// The fields of the run state that kernels read, loaded coherently (the
+// previous kernel on the stream wrote them; see ctrl_load).
+__device__ __forceinline__ SimState load_state(const SimState* st) {
+    SimState S;
+    S.dt = ctrl_load(&st->dt);
+    S.told = ctrl_load(&st->told);
+    S.t_end = ctrl_load(&st->t_end);
+    S.dt_first = ctrl_load(&st->dt_first);
+    S.step = ctrl_load(&st->step);
+    S.target = ctrl_load(&st->target);
+    S.rhs_evals = ctrl_load(&st->rhs_evals);
+    S.retry_passes = ctrl_load(&st->retry_passes);
+    S.stalls = 0;  // only ever updated by atomics
+    S.rec = ctrl_load(&st->rec);
+    S.ntrace = ctrl_load(&st->ntrace);
+    S.need_coupling = 0;  // read by the coupling kernels only (through their gate)
+    S.iteration = ctrl_load(&st->iteration);
+    S.status = 0;  // read by the host only
+    S.done = ctrl_load(&st->done);
+    S.parity = ctrl_load(&st->parity);
+    S.pad = 0;
+    return S;
+}
+
 // Per-workgroup partial results of one attempt.
 struct BlockPart {
     int32_t ok, any4, found, pad;
@@ -178,9 +202,10 @@ __device__ inline double select_dt(const SimArrays& A, double dt_prev, double vm
 __device__ __forceinline__ void setup_body(const SimArrays& A, SimState* __restrict__ st) {
     __shared__ double shd[kSetupBlock / 64];
     __shared__ int shi[kSetupBlock / 64];
-    if (st->done) return;
+    const SimState S = load_state(st);
+    if (S.done) return;
     const int64_t n = A.n;
-    const int64_t o = (int64_t)st->parity * n;
+    const int64_t o = (int64_t)S.parity * n;
     double vmax = -INFINITY, dmin = INFINITY;
     int found = 0;
     for (int64_t i = threadIdx.x; i < n; i += kSetupBlock) {
@@ -203,11 +228,12 @@ __device__ __forceinline__ void setup_body(const SimArrays& A, SimState* __restr
             st->done = 1;
             st->need_coupling = 0;
         } else {
-            st->dt = select_dt(A, st->dt, vmax, dmin / A.dt_cut);
-            st->dt_first = st->dt;
+            const double dt = select_dt(A, S.dt, vmax, dmin / A.dt_cut);
+            st->dt = dt;
+            st->dt_first = dt;
             st->iteration = 1;
             st->need_coupling = 1;
-            st->rhs_evals += 1;
+            st->rhs_evals = S.rhs_evals + 1;
         }
     }
 }
@@ -225,14 +251,15 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
     const int64_t ic = i < n ? i : n - 1;
     const double c_f0 = A.f0[ic], c_a = A.a[ic], c_b = A.b[ic], c_dc = A.dc[ic], c_k = A.kself[ic];
     const double c_omega = A.omega[ic], c_mass = A.mass[ic], c_sigma0 = A.sigma0[ic], c_tau0 = A.tau0[ic];
-    if (st->done) return;
-    const int p = st->parity;
+    const SimState S = load_state(st);
+    if (S.done) return;
+    const int p = S.parity;
     const int64_t o = (int64_t)p * n, w = (int64_t)(1 - p) * n;
-    const double dt = st->dt, told = st->told;
-    const int fresh = st->iteration == 1;
-    const int64_t step = st->step;
-    const bool record = A.record_step > 0 && (step + 1) % A.record_step == 0 && st->rec < A.rec_cap;
-    const int64_t rec = st->rec;
+    const double dt = S.dt, told = S.told;
+    const int fresh = S.iteration == 1;
+    const int64_t step = S.step;
+    const bool record = A.record_step > 0 && (step + 1) % A.record_step == 0 && S.rec < A.rec_cap;
+    const int64_t rec = S.rec;
 
     int ok = 1, any4 = 0, found = 0;
     double mindt = INFINITY, vmax = -INFINITY, vmax_next = -INFINITY, dtmin_next = INFINITY;
@@ -341,32 +368,32 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
     if (!OK) {  // retry the step with the reduced Dt (:240-258)
         st->dt = ANY4 ? 1e-3 : MINDT;
         st->need_coupling = 0;
-        st->retry_passes += 1;
-        if (st->iteration > 100) {
+        st->retry_passes = S.retry_passes + 1;
+        if (S.iteration > 100) {
             st->status = QDFN_ERR_RETRIES;
             st->done = 1;
         }
-        st->iteration += 1;
+        st->iteration = S.iteration + 1;
         return;
     }
     // accept (:262-319); Dt = min(Dt_All) == dt when every element converged
     const double dta = MINDT;
-    if (st->ntrace < A.trace_cap) {
-        A.t_told[st->ntrace] = told;
-        A.t_dt[st->ntrace] = dta;
-        A.t_vmax[st->ntrace] = VMAX;
-        st->ntrace += 1;
+    if (S.ntrace < A.trace_cap) {
+        A.t_told[S.ntrace] = told;
+        A.t_dt[S.ntrace] = dta;
+        A.t_vmax[S.ntrace] = VMAX;
+        st->ntrace = S.ntrace + 1;
     }
     if (record) {
         A.h_time[rec] = told;
         A.h_dt[rec] = dta;
-        A.h_dtsel[rec] = st->dt_first;
+        A.h_dtsel[rec] = S.dt_first;
         st->rec = rec + 1;
     }
     st->told = told + dta;
     st->parity = 1 - p;
     st->step = step + 1;
-    if (step + 1 >= st->target || (st->t_end > 0.0 && told + dta >= st->t_end)) {
+    if (step + 1 >= S.target || (S.t_end > 0.0 && told + dta >= S.t_end)) {
         st->dt = dta;
         st->done = 1;
         st->need_coupling = 0;
@@ -379,11 +406,12 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
         st->need_coupling = 0;
         return;
     }
-    st->dt = select_dt(A, dta, VNEXT, DNEXT / A.dt_cut);
-    st->dt_first = st->dt;
+    const double dt_next = select_dt(A, dta, VNEXT, DNEXT / A.dt_cut);
+    st->dt = dt_next;
+    st->dt_first = dt_next;
     st->iteration = 1;
     st->need_coupling = 1;
-    st->rhs_evals += 1;
+    st->rhs_evals = S.rhs_evals + 1;
 }
 
 }  // namespace qdfn

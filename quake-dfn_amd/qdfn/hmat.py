@@ -212,7 +212,9 @@ class HMatrix:
     def matvec(self, x, loading_form=True):
         """(yS, yN): loading form yS = (H_load (0 - x)) ./ -K_Self; otherwise
         yS = H_S x.  yN = H_N x."""
-        x = _f64(x)
+        x = _f64(x).ravel()
+        if x.size != self.n:
+            raise ValueError(f"x has {x.size} entries; the operator has {self.n}")
         yS, yN = np.zeros(self.n), np.zeros(self.n)
         check(lib().qdfn_hmat_matvec(self.handle, ptr(x), ptr(yS), ptr(yN), 1 if loading_form else 0))
         return yS, yN

@@ -71,6 +71,8 @@ class DenseOperator:
     def matvec(self, x, loading_form=True):
         """(yS, yN): loading form yS = E*(0-x); otherwise yS = K_S*x.  yN = K_N*x."""
         x = _f64(x)
+        if x.size != self.n:
+            raise ValueError(f"x has {x.size} entries; the operator has {self.n}")
         yS, yN = np.zeros(self.n), np.zeros(self.n)
         check(lib().qdfn_matvec(self.handle, ptr(x), ptr(yS), ptr(yN), 1 if loading_form else 0))
         return yS, yN
@@ -176,6 +178,10 @@ class Simulation:
         if tgt.size != obs.size:
             raise ValueError("one target per observation element")
         vi, th = _f64(v_i), _f64(theta_i)
+        if vi.size != self.n or th.size != self.n:
+            raise ValueError("v_i and theta_i must have one entry per element")
+        if obs.size and (obs.min() < 0 or obs.max() >= self.n):
+            raise ValueError("observation indices must lie in [0, n)")
         ga, gb, gd = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
         loss = _lib.D()
         info = _lib.AdjointInfo()
