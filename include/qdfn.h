@@ -113,6 +113,19 @@ int qdfn_dense_normal_zero(qdfn_dense_t* op);
  * yN = K_N*x. */
 int qdfn_matvec(qdfn_dense_t* op, const double* x, double* yS, double* yN, int32_t loading_form);
 
+/* The coupling product of qdfn_matvec on DEVICE vectors (d_x in, d_yS and
+ * d_yN out, n doubles each), enqueued on `stream` (a hipStream_t; NULL is the
+ * null stream) without waiting for it, and allocating nothing: no hipMalloc,
+ * hipFree or host copy, so it can be called under HIP graph capture or in a
+ * loop that must not allocate.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory.  The split partial sums live
+ * in one static device array per GPU of 8 MiB (2 x splits x ld doubles must
+ * fit: QDFN_ERR_ARG otherwise), shared by every call on that GPU — calls that
+ * overlap on one GPU (several streams or threads) race on it.  The operator
+ * must live on the current device.  Results equal qdfn_matvec's bit for bit. */
+int qdfn_matvec_device_noalloc(qdfn_dense_t* op, const double* d_x, double* d_yS, double* d_yN,
+                               int32_t loading_form, void* stream);
+
 /* Average duration (ms, HIP events on the operator's stream) of `reps`
  * launches of the fused coupling kernel. */
 int qdfn_dense_time_coupling(qdfn_dense_t* op, int32_t reps, double* avg_ms);

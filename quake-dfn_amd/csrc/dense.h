@@ -23,6 +23,7 @@ struct qdfn_dense {
     qdfn::DevBuf partS, partN;  // splits x ld doubles
     qdfn::DevBuf x, yS, yN;     // scratch vectors for qdfn_matvec
     hipStream_t stream = nullptr;
+    int device = 0;             // HIP device the operator's memory lives on
 };
 
 namespace qdfn {
@@ -41,11 +42,14 @@ struct CouplingGate {
     int slot = 0;
 };
 
-// Launch the split coupling kernel: partial sums of E*x and K_N*x.
-int launch_coupling(const qdfn_dense* op, const double* d_x, const CouplingGate& g, hipStream_t s);
+// Launch the split coupling kernel: partial sums of E*x and K_N*x, into the
+// operator's partS / partN, or into part_s / part_n (splits x ld doubles
+// each) when given.
+int launch_coupling(const qdfn_dense* op, const double* d_x, const CouplingGate& g, hipStream_t s,
+                    double* part_s = nullptr, double* part_n = nullptr);
 // Finish: TL = -(sum of E*x partials), SigProd = sum of K_N*x partials
-// (TL = E*(0-x) as Functions_RSFDFN3DMain_D.jl:199).
+// (TL = E*(0-x) as Functions_RSFDFN3DMain_D.jl:199); partials as above.
 int launch_coupling_finish(const qdfn_dense* op, double* d_tl, double* d_sigprod, const int* d_gate,
-                           hipStream_t s);
+                           hipStream_t s, const double* part_s = nullptr, const double* part_n = nullptr);
 
 }  // namespace qdfn

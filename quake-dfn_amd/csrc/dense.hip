@@ -6,6 +6,8 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "assembly.h"
 #include "dense.h"
@@ -141,27 +143,28 @@ __global__ __launch_bounds__(256) void k_coupling_finish(const double* __restric
     }
 }
 
-int launch_coupling(const qdfn_dense* op, const double* d_x, const CouplingGate& g, hipStream_t s) {
+int launch_coupling(const qdfn_dense* op, const double* d_x, const CouplingGate& g, hipStream_t s,
+                    double* part_s, double* part_n) {
     const int R = op->prec == QDFN_PREC_FP64 ? 2 : 4;
     const int64_t rb = (op->ld + 256 * R - 1) / (256 * R);
     dim3 grid((unsigned)rb, (unsigned)op->splits);
+    double* ps = part_s ? part_s : op->partS.as<double>();
+    double* pn = part_n ? part_n : op->partN.as<double>();
     if (op->prec == QDFN_PREC_FP64)
         hipLaunchKernelGGL(k_coupling<double>, grid, dim3(256), 0, s, op->E.as<double>(), op->KN.as<double>(), op->ld,
-                           op->n, op->cols_per_split, d_x, op->partS.as<double>(), op->partN.as<double>(),
-                           op->normal_zero, g);
+                           op->n, op->cols_per_split, d_x, ps, pn, op->normal_zero, g);
     else
         hipLaunchKernelGGL(k_coupling<float>, grid, dim3(256), 0, s, op->E.as<float>(), op->KN.as<float>(), op->ld,
-                           op->n, op->cols_per_split, d_x, op->partS.as<double>(), op->partN.as<double>(),
-                           op->normal_zero, g);
+                           op->n, op->cols_per_split, d_x, ps, pn, op->normal_zero, g);
     QDFN_HIP(hipGetLastError());
     return QDFN_OK;
 }
 
 int launch_coupling_finish(const qdfn_dense* op, double* d_tl, double* d_sigprod, const int* d_gate,
-                           hipStream_t s) {
+                           hipStream_t s, const double* part_s, const double* part_n) {
     hipLaunchKernelGGL(k_coupling_finish, dim3((unsigned)((op->n + 63) / 64)), dim3(256), 0, s,
-                       op->partS.as<double>(), op->partN.as<double>(), op->ld, op->n, op->splits, op->normal_zero,
-                       d_tl, d_sigprod, d_gate);
+                       part_s ? part_s : op->partS.as<double>(), part_n ? part_n : op->partN.as<double>(), op->ld,
+                       op->n, op->splits, op->normal_zero, d_tl, d_sigprod, d_gate);
     QDFN_HIP(hipGetLastError());
     return QDFN_OK;
 }
@@ -259,6 +262,11 @@ static qdfn_dense* new_op(int64_t n, int32_t prec) {
         return nullptr;
     }
     qdfn_dense* op = new qdfn_dense();
+    if (hipGetDevice(&op->device) != hipSuccess) {
+        fail(QDFN_ERR_HIP, "dense operator: no current HIP device");
+        delete op;
+        return nullptr;
+    }
     op->n = n;
     op->ld = round_up(n, 64);
     op->prec = prec;
@@ -425,4 +433,71 @@ extern "C" int qdfn_dense_time_coupling(qdfn_dense_t* op, int32_t reps, double* 
     (void)hipEventDestroy(b);
     *avg_ms = (double)ms / reps;
     return rc;
+}
+
+// ---- coupling product that allocates nothing ------------------------------
+//
+// Split partial sums of qdfn_matvec_device_noalloc live in this file-scope
+// device array (one copy per GPU, held until the process ends).  On the host,
+// `scratch` names a host-side stand-in, not device memory: host code only
+// ever uses the device address that hipGetSymbolAddress returns for the
+// current device, kept per device below.
+
+namespace qdfn {
+
+constexpr int64_t kNoAllocScratchDoubles = int64_t(1) << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices];  // device address of `scratch` per device (null: not yet looked up)
+static std::mutex g_scratch_mu;
+
+static int scratch_for_current_device(int dev, double** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(QDFN_ERR_ARG, "qdfn_matvec_device_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(QDFN_ERR_HIP, std::string("qdfn_matvec_device_noalloc: hipGetSymbolAddress(scratch): ") +
+                                          hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return QDFN_OK;
+}
+
+}  // namespace qdfn
+
+extern "C" int qdfn_matvec_device_noalloc(qdfn_dense_t* op, const double* d_x, double* d_yS, double* d_yN,
+                                          int32_t loading_form, void* stream) {
+    if (!op || !d_x || !d_yS || !d_yN) return fail(QDFN_ERR_ARG, "qdfn_matvec_device_noalloc: null argument");
+    int dev = -1;
+    QDFN_HIP(hipGetDevice(&dev));
+    if (dev != op->device)
+        return fail(QDFN_ERR_ARG, "qdfn_matvec_device_noalloc: the operator lives on device " +
+                                      std::to_string(op->device) + ", the current device is " + std::to_string(dev));
+    const int64_t need = 2 * (int64_t)op->splits * op->ld;
+    if (need > kNoAllocScratchDoubles)
+        return fail(QDFN_ERR_ARG, "qdfn_matvec_device_noalloc: the operator's split partial sums need " +
+                                      std::to_string(need * 8) + " bytes of scratch, above the " +
+                                      std::to_string(kNoAllocScratchDoubles * 8) + "-byte maximum");
+    double* base = nullptr;
+    int rc = scratch_for_current_device(dev, &base);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double* ps = base;
+    double* pn = base + (int64_t)op->splits * op->ld;
+    // every split writes all ld rows of its partials before the finish reads
+    // them, so nothing has to start at zero
+    rc = launch_coupling(op, d_x, CouplingGate{}, s, ps, pn);
+    if (rc) return rc;
+    rc = launch_coupling_finish(op, d_yS, d_yN, nullptr, s, ps, pn);
+    if (rc) return rc;
+    if (!loading_form) {
+        hipLaunchKernelGGL(k_full_shear, dim3((unsigned)((op->n + 255) / 256)), dim3(256), 0, s, d_yS, d_x,
+                           op->diag.as<double>(), op->n, d_yS);
+        QDFN_HIP(hipGetLastError());
+    }
+    return QDFN_OK;
 }

@@ -89,6 +89,7 @@ SIGNATURES = {
     "qdfn_dense_kself": (C.c_int, [VP, PD, PD]),
     "qdfn_dense_normal_zero": (C.c_int, [VP]),
     "qdfn_matvec": (C.c_int, [VP, PD, PD, PD, I32]),
+    "qdfn_matvec_device_noalloc": (C.c_int, [VP, VP, VP, VP, I32, VP]),
     "qdfn_dense_time_coupling": (C.c_int, [VP, I32, PD]),
     "qdfn_solve_one_timestep": (I64, [C.POINTER(StepParams), I64, I64] + [PD] * 9 + [D] + [PD] * 4 + [PI32] +
                                 [PD] * 4 + [PD] * 6 + [PI32] * 3),

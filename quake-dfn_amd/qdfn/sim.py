@@ -77,6 +77,15 @@ class DenseOperator:
         check(lib().qdfn_matvec(self.handle, ptr(x), ptr(yS), ptr(yN), 1 if loading_form else 0))
         return yS, yN
 
+    def matvec_device_noalloc(self, d_x, d_yS, d_yN, loading_form=True, stream=0):
+        """The coupling product on device vectors (addresses as ints, n doubles
+        each), enqueued on `stream` (a hipStream_t as an int) without waiting
+        and without allocating; split partials in the library's static device
+        scratch (calls overlapping on one GPU race)."""
+        check(lib().qdfn_matvec_device_noalloc(self.handle, _lib.C.c_void_p(d_x), _lib.C.c_void_p(d_yS),
+                                               _lib.C.c_void_p(d_yN), 1 if loading_form else 0,
+                                               _lib.C.c_void_p(stream)))
+
     def time_coupling(self, reps=20):
         ms = _lib.D()
         check(lib().qdfn_dense_time_coupling(self.handle, reps, _lib.C.byref(ms)))
