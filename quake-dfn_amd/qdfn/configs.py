@@ -10,6 +10,7 @@ Workloads (all arrays seeded with numpy PCG64):
   fault_network  8 rectangular faults of 32 x 32 elements (8,192 elements)
   rough_fault    a non-planar strike-slip surface with sinusoidal roughness meshed
                  into triangles (100 x 100 cells of 50 m -> 20,000 triangles)
+  rough_fault_patch  the same mesh with a central velocity-weakening patch
   ensemble_members  256 planar faults of 500-4,000 elements of 50 m, seeded
                  per-element friction (the batched-ensemble workload)
 """
@@ -257,6 +258,24 @@ def rough_fault(nx=100, nz=100, h=50.0, top=100.0, seed=20240612, ratio=1e-2, n_
     rake = np.zeros(n)
     return dict(name=f"rough_fault_{n}", P1=P1, P2=P2, P3=P3, rake=rake, a=a, b=b, sigma=sigma,
                 dc=np.full(n, 1e-3), mu=mu, nu=nu, rho=mu / vs ** 2, min_normal_stress=2e6, h=h)
+
+
+def rough_fault_patch(nx=40, nz=30, h=50.0, top=100.0, seed=20240612, ratio=1e-2, patch=0.3, sigma=50e6):
+    """The rough-fault mesh of rough_fault() with the friction of the single
+    fault: a = 0.010 everywhere, b = 0.015 on a central velocity-weakening
+    patch (triangle centres within patch * length of the mid-point along
+    strike and along depth) and 0.006 elsewhere, uniform sigma_n, Dc = 1e-3 m.
+    Its first event is a well-conditioned function of the operator (the
+    loop parity tests on triangles use it)."""
+    m = rough_fault(nx=nx, nz=nz, h=h, top=top, seed=seed, ratio=ratio)
+    ctr = (m["P1"] + m["P2"] + m["P3"]) / 3
+    n = ctr.shape[0]
+    Lx, Lz = nx * h, nz * h
+    vw = (np.abs(ctr[:, 0]) <= patch * Lx) & (np.abs(-ctr[:, 2] - (top + Lz / 2)) <= patch * Lz)
+    b = np.full(n, 0.006)
+    b[vw] = 0.015
+    return dict(m, name=f"rough_fault_patch_{n}", a=np.full(n, 0.010), b=b, sigma=np.full(n, float(sigma)),
+                dc=np.full(n, 1e-3))
 
 
 def problem_from_triangles(m, kself, ks_row_sums, kn_row_sums, rate=1e-9, order=None, **overrides):
