@@ -320,7 +320,9 @@ qdfn_sim_t* qdfn_sim_create_hmat(qdfn_hmat_t* h, const qdfn_sim_params* p, const
                                  const double* b, const double* dc, const double* theta_i, const double* v_i,
                                  const double* friction_i, const double* sigma0, const double* loading_rate);
 /* External stress tables (Input_ExternalStressChange.jld2; InterpolateFromStressChange,
- * Functions_Solvers.jl:465): times[nt]; ext_n/ext_s are nt x n column-major. */
+ * Functions_Solvers.jl:465): times[nt], non-decreasing (QDFN_ERR_ARG otherwise: the
+ * device finds the reference's sample index by binary search); ext_n/ext_s are
+ * nt x n column-major. */
 int qdfn_sim_set_external(qdfn_sim_t* sim, int64_t nt, const double* times, const double* ext_n,
                           const double* ext_s);
 /* Advance n_steps accepted steps (the reference's TotalStep loop body).

@@ -102,28 +102,15 @@ __device__ inline void ext_and_slope(const AdjArgs& A, double t, int64_t i, doub
                                      double& ss) {
     dsn = dtau = sn = ss = 0.0;
     if (A.nt == 0) return;
-    int64_t idx = 0;
-    for (int64_t q = 1; q <= A.nt; ++q)
-        if (t >= A.ext_t[q - 1]) idx = q;
-    const int64_t nt = A.nt;
+    const int64_t nt = A.nt, n = A.n, idx = ext_index(A.ext_t, nt, t);
+    ext_interp_at(A.ext_t, A.ext_n, A.ext_s, nt, n, idx, t, i, dsn, dtau);
     if (idx == 0) {
-        const double td = t / A.ext_t[0];
-        dsn = td * (A.ext_n[0 + i * nt] - 0.0) + 0.0;
-        dtau = td * (A.ext_s[0 + i * nt] - 0.0) + 0.0;
-        sn = A.ext_n[0 + i * nt] / A.ext_t[0];
-        ss = A.ext_s[0 + i * nt] / A.ext_t[0];
-    } else if (idx == nt) {
-        dsn = A.ext_n[(idx - 1) + i * nt];
-        dtau = A.ext_s[(idx - 1) + i * nt];
-    } else {
-        const double td = t - A.ext_t[idx - 1];
+        sn = A.ext_n[i] / A.ext_t[0];
+        ss = A.ext_s[i] / A.ext_t[0];
+    } else if (idx < nt) {
         const double span = A.ext_t[idx] - A.ext_t[idx - 1];
-        const double ln = A.ext_n[(idx - 1) + i * nt], un = A.ext_n[idx + i * nt];
-        const double ls = A.ext_s[(idx - 1) + i * nt], us = A.ext_s[idx + i * nt];
-        dsn = td / span * (un - ln) + ln;
-        dtau = td / span * (us - ls) + ls;
-        sn = (un - ln) / span;
-        ss = (us - ls) / span;
+        sn = (A.ext_n[idx * n + i] - A.ext_n[(idx - 1) * n + i]) / span;
+        ss = (A.ext_s[idx * n + i] - A.ext_s[(idx - 1) * n + i]) / span;
     }
 }
 

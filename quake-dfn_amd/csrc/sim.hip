@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "dense.h"
@@ -247,11 +248,21 @@ extern "C" int qdfn_sim_set_external(qdfn_sim_t* s, int64_t nt, const double* ti
         s->A.nt = 0;
         return QDFN_OK;
     }
-    QDFN_HIP(s->ext.alloc(sizeof(double) * (nt + 2 * nt * n)));
+    for (int64_t k = 1; k < nt; ++k)
+        if (!(times[k] >= times[k - 1]))
+            return fail(QDFN_ERR_ARG, "qdfn_sim_set_external: sample times must be non-decreasing (time " +
+                                          std::to_string(k + 1) + " precedes time " + std::to_string(k) + ")");
+    // the caller's nt x n column-major tables, stored sample-major on the device
+    std::vector<double> h(nt + 2 * nt * n);
+    std::copy(times, times + nt, h.begin());
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t k = 0; k < nt; ++k) {
+            h[nt + k * n + i] = ext_n[k + i * nt];
+            h[nt + nt * n + k * n + i] = ext_s[k + i * nt];
+        }
+    QDFN_HIP(s->ext.alloc(sizeof(double) * h.size()));
     double* d = s->ext.as<double>();
-    QDFN_HIP(hipMemcpy(d, times, sizeof(double) * nt, hipMemcpyHostToDevice));
-    QDFN_HIP(hipMemcpy(d + nt, ext_n, sizeof(double) * nt * n, hipMemcpyHostToDevice));
-    QDFN_HIP(hipMemcpy(d + nt + nt * n, ext_s, sizeof(double) * nt * n, hipMemcpyHostToDevice));
+    QDFN_HIP(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
     s->A.nt = nt;
     s->A.ext_t = d;
     s->A.ext_n = d + nt;

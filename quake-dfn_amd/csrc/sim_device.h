@@ -77,7 +77,7 @@ struct SimArrays {
     int32_t* ran_log;  // per-attempt "coupling ran" marks of a timed batch
     BlockPart* parts;
     unsigned int* arrive;
-    // external stress (nt == 0: none)
+    // external stress (nt == 0: none); tables sample-major: ext_n[k * n + i]
     int64_t nt;
     const double *ext_t, *ext_n, *ext_s;
     // histories
@@ -104,24 +104,39 @@ __device__ inline double finish_row(const SimArrays& A, int32_t b, int64_t i) {
     return s;
 }
 
-// InterpolateFromStressChange (Functions_Solvers.jl:465-540) for one element.
-__device__ inline void ext_interp(const SimArrays& A, double time, int64_t i, double& dsn, double& dtau) {
-    int64_t idx = 0;
-    for (int64_t k = 1; k <= A.nt; ++k)
-        if (time >= A.ext_t[k - 1]) idx = k;
-    const int64_t nt = A.nt;
+// The reference's scan for InterpolateFromStressChange (Functions_Solvers.jl:
+// 465-540): the 1-based index of the last sample time not after `time` (0 if
+// none).  The times are non-decreasing (qdfn_sim_set_external checks), so a
+// binary search gives the same index: the number of samples <= time.  A NaN
+// time compares false everywhere and gives 0, as the scan does.
+__device__ inline int64_t ext_index(const double* __restrict__ ext_t, int64_t nt, double time) {
+    int64_t lo = 0, hi = nt;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (time >= ext_t[mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// InterpolateFromStressChange for element i of n at sample index idx
+// (ext_index).  Tables are sample-major on the device (ext[k * n + i]), so
+// the lanes of a wavefront read consecutive addresses.
+__device__ inline void ext_interp_at(const double* __restrict__ ext_t, const double* __restrict__ ext_n,
+                                     const double* __restrict__ ext_s, int64_t nt, int64_t n, int64_t idx,
+                                     double time, int64_t i, double& dsn, double& dtau) {
     if (idx == 0) {
-        const double td = time / A.ext_t[0];
-        dsn = td * (A.ext_n[0 + i * nt] - 0.0) + 0.0;
-        dtau = td * (A.ext_s[0 + i * nt] - 0.0) + 0.0;
+        const double td = time / ext_t[0];
+        dsn = td * (ext_n[i] - 0.0) + 0.0;
+        dtau = td * (ext_s[i] - 0.0) + 0.0;
     } else if (idx == nt) {
-        dsn = A.ext_n[(idx - 1) + i * nt];
-        dtau = A.ext_s[(idx - 1) + i * nt];
+        dsn = ext_n[(idx - 1) * n + i];
+        dtau = ext_s[(idx - 1) * n + i];
     } else {
-        const double td = time - A.ext_t[idx - 1];
-        const double span = A.ext_t[idx] - A.ext_t[idx - 1];
-        const double ln = A.ext_n[(idx - 1) + i * nt], un = A.ext_n[idx + i * nt];
-        const double ls = A.ext_s[(idx - 1) + i * nt], us = A.ext_s[idx + i * nt];
+        const double td = time - ext_t[idx - 1];
+        const double span = ext_t[idx] - ext_t[idx - 1];
+        const double ln = ext_n[(idx - 1) * n + i], un = ext_n[idx * n + i];
+        const double ls = ext_s[(idx - 1) * n + i], us = ext_s[idx * n + i];
         dsn = td / span * (un - ln) + ln;
         dtau = td / span * (us - ls) + ls;
     }
@@ -266,7 +281,10 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
     if (i < n) {
         const double v_old = A.V[o + i];
         double dsn = 0.0, dtau = 0.0;
-        if (A.nt > 0) ext_interp(A, A.ext_at_told ? told : told + dt, i, dsn, dtau);
+        if (A.nt > 0) {
+            const double te = A.ext_at_told ? told : told + dt;
+            ext_interp_at(A.ext_t, A.ext_n, A.ext_s, A.nt, n, ext_index(A.ext_t, A.nt, te), te, i, dsn, dtau);
+        }
         double tl_i = A.tl[i], sp_i = A.planar ? 0.0 : A.sigprod[i];
         if (A.fin_row) {  // rows of split H row blocks (the finish of the product, in the same order)
             const int32_t bs = A.fin_row[i], bn = A.fin_row[n + i];

@@ -199,6 +199,25 @@ def problem_from_segments(cfg, kself, ks_row_sums, kn_row_sums, rate=1e-9, **ove
     return prob
 
 
+def problem_from_bulk(seg, n_loading, mu, rho, min_normal_stress, kself, **overrides):
+    """Element arrays and run parameters of RunRSFDFN3D for an Input_Segment
+    table as ReadBulkInput returns it (QUAKEDFN_RunARG.jl:36-262): friction and
+    stress columns as written, loading faults last (n_loading rows moving at
+    their V_Const), FaultMass from the smallest element length, RUN3 defaults,
+    no external stress (the loading faults drive the run)."""
+    seg = np.asarray(seg, dtype=np.float64)
+    n = seg.shape[0]
+    d = dict(RUN3_DEFAULTS)
+    d.update(overrides)
+    mass, rec, rup = run_setup(seg, rho, kself, d["rupture_multiple"], d["vertical_length_scale"])
+    return dict(n=n, n_loading=int(n_loading), mass=mass, a=seg[:, 8].copy(), b=seg[:, 9].copy(),
+                dc=seg[:, 10].copy(), theta_i=seg[:, 11].copy(), v_i=seg[:, 12].copy(),
+                friction_i=seg[:, 13].copy(), sigma0=seg[:, 14].copy(), loading_rate=seg[:, 15].copy(), mu=mu,
+                rho=rho, switch_v=d["switch_v"], dt_cut=d["dt_cut"], rupture_dt=rup, maximum_dt=d["maximum_dt"],
+                min_normal_stress=min_normal_stress, alpha_evo=0.0, evolution_dr=1, planar=0, ext_at_told=0,
+                rec_time_step=rec, ext=None)
+
+
 def first_event_time(told, dt, vmax, threshold=1e-3):
     """Onset of the first event: the first accepted step whose max V reaches
     `threshold`, placed by linear interpolation in log V between that step and
