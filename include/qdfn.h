@@ -87,6 +87,14 @@ int qdfn_rotverts_unitvectors(int64_t n, double* P1, double* P2, double* P3, con
  * accumulation (about 2^-9 relative per factor entry; the error against the
  * FP64 product is reported per result, not bounded by the ACA tolerance) */
 #define QDFN_PREC_BF16_FAR 2
+/* H operators only: dense blocks and the leading components of each low-rank
+ * block in FP32; its trailing components (the smallest singular values) as
+ * FP16 unit vectors with an FP64 scale each, as many as fit in the part of
+ * the block's error budget (aca_rtol ||K_S,b||_F) that the ACA residual and
+ * the truncation leave unused, counting FP32 and FP16 rounding at their unit
+ * round-off.  FP64 accumulation.  Blocks that were not recompressed (rank
+ * below 2, aca_rtol = 0, or imported through qdfn_hmat_create) stay FP32. */
+#define QDFN_PREC_MIXED 3
 
 typedef struct qdfn_dense qdfn_dense_t;
 
@@ -235,6 +243,15 @@ int qdfn_hmat_get_ranks(qdfn_hmat_t* h, int32_t* kind_S, int64_t* rank_S, int32_
 /* One block's stored factors (which = 0 shear, 1 normal): dense m x k into a,
  * or U (m x r) into a and V (r x k) into b; FP64 values as stored. */
 int qdfn_hmat_get_block(qdfn_hmat_t* h, int64_t b, int32_t which, double* a, double* bv);
+/* Block b as the product applies it: the stored entries after rounding to the
+ * operator's storage (FP32 / BF16 / FP16 unit vectors times their FP64 scale
+ * folded into U; the shear diagonal of diagonal blocks stays FP64, as the
+ * product applies K_S,ii), in FP64, same shapes as qdfn_hmat_get_block.  No
+ * reference counterpart (the reference stores FP64 only). */
+int qdfn_hmat_get_block_stored(qdfn_hmat_t* h, int64_t b, int32_t which, double* a, double* bv);
+/* Per block, the first low-rank component held in the 16-bit pool (r_hi;
+ * rank: none, 0: all). */
+int qdfn_hmat_get_split(qdfn_hmat_t* h, int64_t* rhi_S, int64_t* rhi_N);
 /* K_Self = -K_S,ii from the diagonal blocks (StiffnessTransitionToLoading,
  * Functions_Hmatrix.jl:459); diag = K_S,ii (0 where no diagonal block). */
 int qdfn_hmat_kself(qdfn_hmat_t* h, double* kself, double* diag);

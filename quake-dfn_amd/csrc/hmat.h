@@ -18,6 +18,16 @@ struct HBlock {
     int kind[2] = {0, 0};
     int64_t rank[2] = {0, 0};
     std::vector<double> a[2], b[2];
+    // recompressed low-rank parts are in SVD form (U = Q Sigma with columns of
+    // decreasing norm sigma_j, V with orthonormal rows); slack = the part of
+    // its Frobenius error budget the truncation left unused (0: not in SVD form)
+    double slack[2] = {0.0, 0.0};
+    // components [0, r_hi) of a low-rank part are stored in the FP64 / FP32
+    // pool, [r_hi, rank) in the 16-bit pool (set by hmat_layout); with
+    // QDFN_PREC_MIXED, FP16 component r_hi + j is U's column times fu[j] and
+    // V's row times fv[j], its t row scaled by fc[j] (choose_fp16_tail)
+    int64_t r_hi[2] = {0, 0};
+    std::vector<double> fu[2], fv[2], fc[2];
 };
 
 // Row block of the phase-2 product: one receiver row group (up to 64 rows) of
@@ -28,7 +38,8 @@ struct HRowBlock {
     int64_t part_base;      // split row blocks: tile q's row sums at part[part_base + q * m]
     int32_t m, K, ld, kind;
     int32_t ntiles;
-    int32_t lo;             // entries in the BF16 pool (QDFN_PREC_BF16_FAR: rows of U)
+    int32_t lo;             // entries in the 16-bit pool (rows of U: BF16 with QDFN_PREC_BF16_FAR,
+                            // FP16 unit columns with QDFN_PREC_MIXED)
 };
 
 // work item: columns [c0, c1) of one row block (its q-th tile)
@@ -39,10 +50,10 @@ struct HTile {
 // phase-1 wavefront task: rows [0, m) of a row-major V panel slice at
 // data_off (leading dimension ld, K columns multiplying x[xoff + c]); row q
 // writes t[out + q], or vpart[part + q] when the panel is split into column
-// segments (part >= 0); lpr lanes per row
+// segments (part >= 0); lpr lanes per row; lo: the panel is in the 16-bit pool
 struct VTask {
     int64_t data_off;
-    int32_t xoff, out, m, K, ld, lpr, part, pad;
+    int32_t xoff, out, m, K, ld, lpr, part, lo;
 };
 
 // phase-1 finish of one row of a column-split panel:
@@ -62,7 +73,8 @@ struct qdfn_hmat {
     std::vector<double> kdiag_h;          // K_S,ii (0 where no diagonal block)
     qdfn::DevBuf kdiag;
     qdfn::DevBuf pool, idx, t;            // panel entries (V panels then row-group panels), gather indices, t
-    qdfn::DevBuf pool_lo;                 // QDFN_PREC_BF16_FAR: BF16 V panels, then row groups' U panels
+    qdfn::DevBuf pool_lo;                 // 16-bit V panels, then row groups' U panels (BF16_FAR: BF16; MIXED: FP16)
+    qdfn::DevBuf tscale;                  // QDFN_PREC_MIXED: scale of each t row (sigma of FP16 components, else 1)
     qdfn::DevBuf rbs, tiles, part, multi; // row blocks, tiles, partial sums, split row blocks (phase 2)
     qdfn::DevBuf vtasks, vfins, vpart;    // phase-1 wavefront tasks, split-panel finish, partial sums
     qdfn::DevBuf x, yS, yN;               // vectors for qdfn_hmat_matvec
@@ -110,8 +122,14 @@ void init_blocks(qdfn_hmat* h, int64_t nblk, const int64_t* range, const int64_t
 int hmat_layout(qdfn_hmat* h);
 // rank cap of the cross approximation of an m x k block
 int64_t aca_rank_cap(int64_t m, int64_t k);
-// U V (m x r, r x k) -> shortest factors within `budget` (Frobenius)
-void recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V, double budget);
+// U V (m x r, r x k) -> factors in SVD form within `budget` (Frobenius): the
+// shortest, or with fp16_tail those whose FP32 + FP16-tail storage is
+// smallest; returns the unused budget (budget - ||dropped tail||_F, >= 0), or
+// 0 when the factors were left as they were (r < 2 or no budget)
+double recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V, double budget,
+                  bool fp16_tail);
+// first FP16 component of a part in SVD form (see hmat.hip)
+int64_t fp16_tail_start(const double* sig, int64_t r, double slack, int64_t m, int64_t k);
 
 // part w of block q, held in device memory (column-major, leading dimension ld)
 struct DeviceBlock {

@@ -43,6 +43,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <atomic>
+#include <map>
+#include <tuple>
 #include <thread>
 #include <vector>
 
@@ -90,6 +92,24 @@ uint16_t to_bf16(double d) {
 struct bf16_t {
     uint16_t u;
 };
+// FP16 entry (IEEE binary16)
+struct fp16_t {
+    uint16_t u;
+};
+
+// the two 16-bit entries of a 32-bit word (entry 0 in the low half) as FP32
+template <class T>
+struct Dec16;
+template <>
+struct Dec16<bf16_t> {
+    __device__ static float lo(uint32_t w) { return __uint_as_float(w << 16); }
+    __device__ static float hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+};
+template <>
+struct Dec16<fp16_t> {
+    __device__ static float lo(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xffffu)); }
+    __device__ static float hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16)); }
+};
 
 template <class T>
 __device__ inline void load4raw(const T* p, T v[4]) {
@@ -130,17 +150,21 @@ struct RowLoad<double> {
     }
     __device__ double get(int j) const { return j == 0 ? a.x : j == 1 ? a.y : j == 2 ? b.x : b.y; }
 };
-template <>
-struct RowLoad<bf16_t> {
+template <class T16>
+struct RowLoad16 {
     static constexpr int W = 8;
     uint4 v;
-    __device__ void load(const bf16_t* p) { v = *reinterpret_cast<const uint4*>(p); }
+    __device__ void load(const T16* p) { v = *reinterpret_cast<const uint4*>(p); }
     // entry 2k is the low half of word k, entry 2k + 1 the high half
     __device__ double get(int j) const {
         const uint32_t w = (j >> 1) == 0 ? v.x : (j >> 1) == 1 ? v.y : (j >> 1) == 2 ? v.z : v.w;
-        return (double)__uint_as_float((j & 1) ? (w & 0xffff0000u) : (w << 16));
+        return (double)((j & 1) ? Dec16<T16>::hi(w) : Dec16<T16>::lo(w));
     }
 };
+template <>
+struct RowLoad<bf16_t> : RowLoad16<bf16_t> {};
+template <>
+struct RowLoad<fp16_t> : RowLoad16<fp16_t> {};
 
 // x moved between lanes by the DPP control CTRL (both 32-bit halves)
 template <int CTRL>
@@ -311,15 +335,17 @@ __global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict
     if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
 }
 
-// Phase 2 with BF16 low-rank factors: FP32 panels (dense blocks) and BF16
-// panels (rows of U) in one launch, tiles in one size-ordered list; the row
-// block's flag picks the body (uniform per workgroup).  At most 64 VGPRs, so
-// that two 1,024-thread workgroups fit a CU as for k_hrow.
+// Phase 2 with 16-bit panels: FP32 panels (dense blocks, and the leading
+// low-rank components with QDFN_PREC_MIXED) and 16-bit panels (rows of U:
+// BF16, or FP16 unit columns) in one launch, tiles in one size-ordered list;
+// the row block's flag picks the body (uniform per workgroup).  At most 64
+// VGPRs, so that two 1,024-thread workgroups fit a CU as for k_hrow.
+template <class T16>
 __global__ __launch_bounds__(kPhase2Threads, 8) void k_hrow_mixed(const HTile* __restrict__ tiles,
                                                                const HRowBlock* __restrict__ rbs,
                                                                double* __restrict__ part,
                                                                const float* __restrict__ pool,
-                                                               const bf16_t* __restrict__ pool_lo,
+                                                               const T16* __restrict__ pool_lo,
                                                                const int32_t* __restrict__ idx,
                                                                const double* __restrict__ x0, double* __restrict__ t,
                                                                int64_t n, const double* __restrict__ kdiag,
@@ -331,7 +357,7 @@ __global__ __launch_bounds__(kPhase2Threads, 8) void k_hrow_mixed(const HTile* _
     const HTile tl = tiles[blockIdx.x];
     const HRowBlock R = rbs[tl.rb];
     if (R.lo)
-        hrow_body<bf16_t>(tl, R, part, pool_lo, idx, x, t, n, kdiag, form, yS, yN, zbuf);
+        hrow_body<T16>(tl, R, part, pool_lo, idx, x, t, n, kdiag, form, yS, yN, zbuf);
     else
         hrow_body<float>(tl, R, part, pool, idx, x, t, n, kdiag, form, yS, yN, zbuf);
     if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
@@ -354,17 +380,20 @@ struct VStep {
     double z[4];
 };
 
+// t row `out` of a finished phase-1 row: the sum, times the row's scale when
+// the panel holds FP16 unit vectors (QDFN_PREC_MIXED)
+__device__ inline void vpanel_out(const VTask& tk, int32_t row, double v, double* __restrict__ t,
+                                  double* __restrict__ vpart, const double* __restrict__ tscale) {
+    if (tk.part < 0)
+        t[tk.out + row] = tscale ? v * tscale[tk.out + row] : v;
+    else
+        vpart[tk.part + row] = v;
+}
+
 template <class T>
-__global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ tasks, int32_t ntasks,
-                                                      const T* __restrict__ pool, const double* __restrict__ x0,
-                                                      double* __restrict__ t, double* __restrict__ vpart,
-                                                      CouplingGate g) {
-    if (g.gate && ctrl_load(g.gate) == 0) return;
+__device__ __forceinline__ void vpanel_body(const VTask& tk, const T* __restrict__ pool, const double* __restrict__ x,
+                                            double* __restrict__ t, double* __restrict__ vpart) {
     const int lane = threadIdx.x & 63;
-    const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
-    if (ti >= ntasks) return;
-    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
-    const VTask tk = tasks[ti];
     const int lpr = tk.lpr, rpw = 64 / lpr, sub = lane & (lpr - 1), rl = lane / lpr;
     const int32_t K = tk.K, K4 = (K + 3) / 4 * 4;
     const int32_t cstep = 4 * lpr, nc = (K4 + cstep - 1) / cstep;  // column steps per row group
@@ -409,12 +438,7 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ 
             for (int r = 0; r < kVSlots; ++r) {
                 const double v = group_sum(acc[r], lpr);
                 const int32_t row = gi * gr + rl + r * rpw;
-                if (sub == 0 && row < tk.m) {
-                    if (tk.part < 0)
-                        t[tk.out + row] = v;
-                    else
-                        vpart[tk.part + row] = v;
-                }
+                if (sub == 0 && row < tk.m) vpanel_out(tk, row, v, t, vpart, nullptr);
                 acc[r] = 0.0;
             }
         }
@@ -429,24 +453,32 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ 
     }
 }
 
-// Phase 1 over BF16 V panels: k_vpanel's schedule with eight entries of a row
-// per lane and step (one 16-byte load), so a step covers twice the columns.
-// Panel leading dimensions and segment starts are multiples of eight entries.
+template <class T>
+__global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ tasks, int32_t ntasks,
+                                                      const T* __restrict__ pool, const double* __restrict__ x0,
+                                                      double* __restrict__ t, double* __restrict__ vpart,
+                                                      CouplingGate g) {
+    if (g.gate && ctrl_load(g.gate) == 0) return;
+    const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
+    if (ti >= ntasks) return;
+    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
+    vpanel_body<T>(tasks[ti], pool, x, t, vpart);
+}
+
+// Phase 1 over 16-bit V panels: k_vpanel's schedule with eight entries of a
+// row per lane and step (one 16-byte load), so a step covers twice the
+// columns.  Panel leading dimensions and segment starts are multiples of
+// eight entries.
 struct VStep8 {
     uint4 w[kVSlots];
     double z[8];
 };
 
-__global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restrict__ tasks, int32_t ntasks,
-                                                           const uint16_t* __restrict__ pool,
-                                                           const double* __restrict__ x0, double* __restrict__ t,
-                                                           double* __restrict__ vpart, CouplingGate g) {
-    if (g.gate && ctrl_load(g.gate) == 0) return;
+template <class T16>
+__device__ __forceinline__ void vpanel16_body(const VTask& tk, const uint16_t* __restrict__ pool,
+                                              const double* __restrict__ x, double* __restrict__ t,
+                                              double* __restrict__ vpart, const double* __restrict__ tscale) {
     const int lane = threadIdx.x & 63;
-    const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
-    if (ti >= ntasks) return;
-    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
-    const VTask tk = tasks[ti];
     const int lpr = tk.lpr, rpw = 64 / lpr, sub = lane & (lpr - 1), rl = lane / lpr;
     const int32_t K = tk.K, K8 = (K + 7) / 8 * 8;
     const int32_t cstep = 8 * lpr, nc = (K8 + cstep - 1) / cstep;
@@ -480,8 +512,8 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restri
             const uint32_t wd[4] = {S.w[r].x, S.w[r].y, S.w[r].z, S.w[r].w};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                acc[r] = fma((double)__uint_as_float(wd[j] << 16), S.z[2 * j], acc[r]);
-                acc[r] = fma((double)__uint_as_float(wd[j] & 0xffff0000u), S.z[2 * j + 1], acc[r]);
+                acc[r] = fma((double)Dec16<T16>::lo(wd[j]), S.z[2 * j], acc[r]);
+                acc[r] = fma((double)Dec16<T16>::hi(wd[j]), S.z[2 * j + 1], acc[r]);
             }
         }
         if (ci == nc - 1) {
@@ -489,12 +521,7 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restri
             for (int r = 0; r < kVSlots; ++r) {
                 const double v = group_sum(acc[r], lpr);
                 const int32_t row = gi * gr + rl + r * rpw;
-                if (sub == 0 && row < tk.m) {
-                    if (tk.part < 0)
-                        t[tk.out + row] = v;
-                    else
-                        vpart[tk.part + row] = v;
-                }
+                if (sub == 0 && row < tk.m) vpanel_out(tk, row, v, t, vpart, tscale);
                 acc[r] = 0.0;
             }
         }
@@ -509,10 +536,41 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restri
     }
 }
 
+__global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restrict__ tasks, int32_t ntasks,
+                                                           const uint16_t* __restrict__ pool,
+                                                           const double* __restrict__ x0, double* __restrict__ t,
+                                                           double* __restrict__ vpart, CouplingGate g) {
+    if (g.gate && ctrl_load(g.gate) == 0) return;
+    const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
+    if (ti >= ntasks) return;
+    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
+    vpanel16_body<bf16_t>(tasks[ti], pool, x, t, vpart, nullptr);
+}
+
+// Phase 1 of QDFN_PREC_MIXED: FP32 panels (leading components) and FP16
+// panels (unit vectors of the trailing components, their t rows scaled) in
+// one launch; the task's flag picks the body (uniform per wavefront).
+__global__ __launch_bounds__(kVThreads) void k_vpanel_mixed(const VTask* __restrict__ tasks, int32_t ntasks,
+                                                            const float* __restrict__ pool,
+                                                            const uint16_t* __restrict__ pool_lo,
+                                                            const double* __restrict__ tscale,
+                                                            const double* __restrict__ x0, double* __restrict__ t,
+                                                            double* __restrict__ vpart, CouplingGate g) {
+    if (g.gate && ctrl_load(g.gate) == 0) return;
+    const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
+    if (ti >= ntasks) return;
+    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
+    const VTask tk = tasks[ti];
+    if (tk.lo)
+        vpanel16_body<fp16_t>(tk, pool_lo, x, t, vpart, tscale);
+    else
+        vpanel_body<float>(tk, pool, x, t, vpart);
+}
+
 // Phase-1 finish: t rows of column-split panels, segments added in order.
 __global__ __launch_bounds__(256) void k_vfinish(const VFin* __restrict__ fins, int32_t nfin,
                                                  const double* __restrict__ vpart, double* __restrict__ t,
-                                                 CouplingGate g) {
+                                                 const double* __restrict__ tscale, CouplingGate g) {
     if (g.gate && ctrl_load(g.gate) == 0) return;
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     // fins are per row: i indexes a row of some split row chunk
@@ -520,7 +578,7 @@ __global__ __launch_bounds__(256) void k_vfinish(const VFin* __restrict__ fins, 
     const VFin f = fins[i];
     double s = 0.0;
     for (int32_t q = 0; q < f.nseg; ++q) s += vpart[f.part0 + (int64_t)q * f.stride];
-    t[f.out] = s;
+    t[f.out] = tscale ? s * tscale[f.out] : s;
 }
 
 // finish of the split row blocks: one 64-lane workgroup each, one lane per row,
@@ -677,6 +735,57 @@ __global__ __launch_bounds__(kAcaThreads) void k_aca(const AcaJob* __restrict__ 
 
 // ------------------------------------------------------ host-side layout ----
 
+// d rounded once to FP16 (11 significant bits, ties to even; quantum 2^-24
+// below 2^-14), for |d| <= 1
+uint16_t to_fp16(double d) {
+    double r;
+    if (std::fabs(d) < 0x1p-14) {
+        r = std::ldexp(std::nearbyint(std::ldexp(d, 24)), -24);
+    } else {
+        int e;
+        std::frexp(d, &e);
+        r = std::ldexp(std::nearbyint(std::ldexp(d, 11 - e)), e - 11);
+    }
+    const _Float16 f = (_Float16)r;  // exact: r is an FP16 value
+    uint16_t b;
+    std::memcpy(&b, &f, 2);
+    return b;
+}
+
+// QDFN_PREC_MIXED: the FP16 tail of low-rank part w of b, which is in SVD form
+// (column a of U has norm sigma_a, decreasing; V has orthonormal rows), within
+// b.slack[w] (fp16_tail_start); b.r_hi[w] is its first component.  Per FP16
+// component: fu, fv scale its U column and V row to unit vectors whose largest
+// entry lies in [0.5, 1), fc = 1 / (fu fv) scales its t row.
+void choose_fp16_tail(HBlock& b, int w, std::vector<double>& fu, std::vector<double>& fv, std::vector<double>& fc) {
+    const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1, r = b.rank[w];
+    const std::vector<double>& U = b.a[w];
+    const std::vector<double>& V = b.b[w];
+    std::vector<double> sig(r);
+    for (int64_t a = 0; a < r; ++a) {
+        double q = 0.0;
+        for (int64_t i = 0; i < m; ++i) q += U[i + a * m] * U[i + a * m];
+        sig[a] = std::sqrt(q);
+    }
+    const int64_t r_hi = fp16_tail_start(sig.data(), r, b.slack[w], m, k);
+    b.r_hi[w] = r_hi;
+    fu.assign(r - r_hi, 0.0);
+    fv.assign(r - r_hi, 0.0);
+    fc.assign(r - r_hi, 0.0);
+    for (int64_t a = r_hi; a < r; ++a) {
+        if (!(sig[a] > 0.0)) continue;  // a zero component stays zero
+        double mu = 0.0, mv = 0.0;
+        for (int64_t i = 0; i < m; ++i) mu = std::max(mu, std::fabs(U[i + a * m]));
+        for (int64_t c = 0; c < k; ++c) mv = std::max(mv, std::fabs(V[a + c * r]));
+        int eu, ev;
+        std::frexp(mu / sig[a], &eu);  // largest entry of the unit column = f 2^eu, f in [0.5, 1)
+        std::frexp(mv, &ev);
+        fu[a - r_hi] = std::ldexp(1.0 / sig[a], -eu);
+        fv[a - r_hi] = std::ldexp(1.0, -ev);
+        fc[a - r_hi] = std::ldexp(sig[a], eu + ev);
+    }
+}
+
 int hmat_layout(qdfn_hmat* h) {
     ScopeMs timer(&h->build_phase_ms[4]);
     const int64_t n = h->n;
@@ -695,13 +804,24 @@ int hmat_layout(qdfn_hmat* h) {
             for (double e : *v)
                 if (e != 0.0) nz = 0;
     h->normal_zero = nz;
-    // entries of the FP64 / FP32 panels (pool) and, with BF16 low-rank factors,
-    // of the V panels and the row groups' U panels (lo)
-    const bool lo_far = h->prec == QDFN_PREC_BF16_FAR;
+    // entries of the FP64 / FP32 panels (pool) and of the 16-bit panels (lo):
+    // BF16 low-rank factors (QDFN_PREC_BF16_FAR: every component of every
+    // low-rank part) or FP16 unit vectors (QDFN_PREC_MIXED: the trailing
+    // components of each recompressed part, choose_fp16_tail)
+    const bool lo_far = h->prec == QDFN_PREC_BF16_FAR, mixed = h->prec == QDFN_PREC_MIXED;
+    const bool has_lo = lo_far || mixed;
     std::vector<double> pool, lo;
-    std::vector<double>& vdst = lo_far ? lo : pool;
     std::vector<int32_t> idx;
     std::vector<HRowBlock> rbs;
+    for (HBlock& b : B)
+        for (int w = 0; w < 2; ++w) {
+            b.r_hi[w] = lo_far ? 0 : b.rank[w];
+            b.fu[w].clear();
+            b.fv[w].clear();
+            b.fc[w].clear();
+            if (mixed && b.kind[w] == QDFN_HBLK_LOWRANK && b.rank[w] >= 2 && b.slack[w] > 0.0)
+                choose_fp16_tail(b, w, b.fu[w], b.fv[w], b.fc[w]);
+        }
     if (const char* e = getenv("QDFN_HMAT_TILE")) {
         const int v = atoi(e);
         if (v >= kChunk && v % kChunk == 0) h->tile_w = v;
@@ -710,7 +830,8 @@ int hmat_layout(qdfn_hmat* h) {
     std::vector<int32_t> multi;
     std::vector<VTask> vtasks;
     std::vector<VFin> vfins;
-    int64_t v_entries = 0, v_cols = 0, vpart_total = 0;
+    std::vector<double> tsc;  // scale of each t row (QDFN_PREC_MIXED)
+    int64_t v_entries = 0, v_entries_lo = 0, v_cols = 0, vpart_total = 0;
     int64_t part_total = 0;
     // tiles of row block rb (columns [0, K)) numbered from q into tl; returns the next q
     auto add_tiles = [&](int32_t rb, int32_t K, int32_t q, std::vector<HTile>& tl) {
@@ -737,7 +858,7 @@ int hmat_layout(qdfn_hmat* h) {
         }
         rbs.push_back(R);
     };
-    // the FP32 and the BF16 panel of one row group: one split row (tiles of Ra
+    // the FP32 and the 16-bit panel of one row group: one split row (tiles of Ra
     // then of Rb, partial sums in one area, finished through Ra)
     auto add_pair = [&](HRowBlock Ra, HRowBlock Rb) {
         const int32_t ia = (int32_t)rbs.size();
@@ -751,30 +872,39 @@ int hmat_layout(qdfn_hmat* h) {
         rbs.push_back(Ra);
         rbs.push_back(Rb);
     };
-    // phase 1: V panels, one per source range, in order of first appearance
-    std::vector<int64_t> toff(2 * B.size(), -1);
+    // phase 1: V panels, one per source range and pool (components [0, r_hi)
+    // in FP64 / FP32, [r_hi, rank) in 16 bits), in order of first appearance;
+    // toff / toff_lo: first t row of each part's components in either pool
+    std::vector<int64_t> toff(2 * B.size(), -1), toff_lo(2 * B.size(), -1);
     {
-        std::vector<std::pair<int64_t, int64_t>> keys;
+        std::vector<std::tuple<int64_t, int64_t, int>> keys;
         std::vector<std::vector<std::pair<size_t, int>>> members;
+        std::map<std::tuple<int64_t, int64_t, int>, size_t> key_index;
         for (size_t bi = 0; bi < B.size(); ++bi)
             for (int w = 0; w < 2; ++w) {
                 const HBlock& b = B[bi];
                 if (b.kind[w] != QDFN_HBLK_LOWRANK || b.rank[w] == 0) continue;
-                size_t gk = 0;
-                while (gk < keys.size() && keys[gk] != std::make_pair(b.s0, b.s1)) ++gk;
-                if (gk == keys.size()) {
-                    keys.push_back({b.s0, b.s1});
-                    members.emplace_back();
+                for (int glo = 0; glo < 2; ++glo) {
+                    const int64_t rows = glo ? b.rank[w] - b.r_hi[w] : b.r_hi[w];
+                    if (rows == 0) continue;
+                    const auto key = std::make_tuple(b.s0, b.s1, glo);
+                    auto it = key_index.find(key);
+                    size_t gk;
+                    if (it == key_index.end()) {
+                        gk = keys.size();
+                        key_index[key] = gk;
+                        keys.push_back(key);
+                        members.emplace_back();
+                    } else {
+                        gk = it->second;
+                    }
+                    members[gk].push_back({bi, w});
                 }
-                members[gk].push_back({bi, w});
             }
-        // BF16 panels are read eight entries per lane (k_vpanel_bf16), the others four
-        const int64_t al = lo_far ? 8 : 4;
-        auto round_al = [&](int64_t v) { return (v + al - 1) / al * al; };
-        int64_t task_entries = lo_far ? 2 * kVTaskEntries : kVTaskEntries;
+        int64_t task_env = 0;
         if (const char* e = getenv("QDFN_HMAT_VTASK")) {
             const int64_t v = atoll(e);
-            if (v >= 256 && v <= (1 << 20)) task_entries = v;
+            if (v >= 256 && v <= (1 << 20)) task_env = v;
         }
         // lanes per row at most: 16 (64 or 128 entries per row and step) measured
         // faster than one step per row for wider panels (fewer cross-lane
@@ -786,25 +916,39 @@ int hmat_layout(qdfn_hmat* h) {
         }
         int64_t tbase = 0;
         for (size_t gk = 0; gk < keys.size(); ++gk) {
-            const int64_t s0 = keys[gk].first, k = keys[gk].second - s0 + 1, ld = round_al(k);
+            const int64_t s0 = std::get<0>(keys[gk]), k = std::get<1>(keys[gk]) - s0 + 1;
+            const int glo = std::get<2>(keys[gk]);
+            // 16-bit panels are read eight entries per lane, the others four
+            const int64_t al = glo ? 8 : 4;
+            auto round_al = [&](int64_t v) { return (v + al - 1) / al * al; };
+            const int64_t task_entries = task_env ? task_env : (glo ? 2 * kVTaskEntries : kVTaskEntries);
+            const int64_t ld = round_al(k);
+            std::vector<double>& vdst = glo ? lo : pool;
             int64_t rows = 0;
             for (auto& f : members[gk]) {
-                toff[2 * f.first + f.second] = tbase + rows;
-                rows += B[f.first].rank[f.second];
+                const HBlock& b = B[f.first];
+                (glo ? toff_lo : toff)[2 * f.first + f.second] = tbase + rows;
+                rows += glo ? b.rank[f.second] - b.r_hi[f.second] : b.r_hi[f.second];
             }
             const int64_t off = (int64_t)vdst.size();
             vdst.resize(off + rows * ld, 0.0);
+            tsc.resize(tbase + rows, 1.0);
             int64_t row = 0;
             for (auto& f : members[gk]) {
                 const HBlock& b = B[f.first];
-                const int64_t r = b.rank[f.second];
+                const int64_t r = b.rank[f.second], a0 = glo ? b.r_hi[f.second] : 0, a1 = glo ? r : b.r_hi[f.second];
                 const std::vector<double>& V = b.b[f.second];
-                for (int64_t q = 0; q < r; ++q, ++row)
-                    for (int64_t c = 0; c < k; ++c) vdst[off + row * ld + c] = V[q + c * r];
+                const std::vector<double>& sv = b.fv[f.second];
+                const std::vector<double>& sc = b.fc[f.second];
+                for (int64_t a = a0; a < a1; ++a, ++row) {
+                    const double mult = (glo && mixed) ? sv[a - a0] : 1.0;
+                    for (int64_t c = 0; c < k; ++c) vdst[off + row * ld + c] = V[a + c * r] * mult;
+                    if (glo && mixed) tsc[tbase + row] = sc[a - a0];
+                }
             }
             // wavefront tasks: column segments of at most kVSeg columns, row chunks of
             // about task_entries entries (whole row groups; twice kVTaskEntries for
-            // BF16 panels, whose steps cover twice the columns; QDFN_HMAT_VTASK sets
+            // 16-bit panels, whose steps cover twice the columns; QDFN_HMAT_VTASK sets
             // it); narrow panels read several rows per wavefront load
             const int64_t nseg = (k + kVSeg - 1) / kVSeg;
             const int64_t segw = round_al((k + nseg - 1) / nseg);  // 16-byte aligned segment starts
@@ -826,6 +970,7 @@ int hmat_layout(qdfn_hmat* h) {
                     T.ld = (int32_t)ld;
                     T.lpr = lpr;
                     T.part = nseg > 1 ? (int32_t)(vpart_total + sg * mr) : -1;
+                    T.lo = glo;
                     vtasks.push_back(T);
                 }
                 if (nseg > 1) {
@@ -835,12 +980,13 @@ int hmat_layout(qdfn_hmat* h) {
                         F.stride = (int32_t)mr;
                         F.nseg = (int32_t)nseg;
                         F.out = (int32_t)(tbase + r0 + q);
+                        F.pad = 0;
                         vfins.push_back(F);
                     }
                     vpart_total += nseg * mr;
                 }
             }
-            v_entries += rows * k;
+            (glo ? v_entries_lo : v_entries) += rows * k;
             v_cols += k;
             tbase += rows;
         }
@@ -874,23 +1020,33 @@ int hmat_layout(qdfn_hmat* h) {
         for (int w = 0; w < 2; ++w) {
             const int64_t m = g0[gi + 1] - g0[gi], row0 = g0[gi];
             out_rows2 += m;
-            // the row group's panel in dst over the columns of its dense blocks
-            // (sel 1), of its low-rank factors (sel 2) or of both (sel 0); BF16
-            // panels (in lo) have leading dimensions of multiples of eight
-            auto panel = [&](int sel, std::vector<double>& dst) {
-                const int64_t al = &dst == &lo ? 8 : 4;
-                auto take = [&](const HBlock& b) {
-                    const bool dense = b.kind[w] == QDFN_HBLK_DENSE;
-                    return sel == 0 || (sel == 1) == dense;
+            // the row group's panel: in the FP64 / FP32 pool the columns of its
+            // dense blocks and the components [0, r_hi) of its low-rank parts, in
+            // the 16-bit pool (lopanel; leading dimensions of multiples of eight)
+            // the components [r_hi, rank)
+            auto panel = [&](bool lopanel, std::vector<double>& dst) {
+                const int64_t al = lopanel ? 8 : 4;
+                auto span = [&](const HBlock& b, int64_t& a0, int64_t& a1) {
+                    if (b.kind[w] == QDFN_HBLK_DENSE) {
+                        a0 = 0;
+                        a1 = lopanel ? 0 : b.s1 - b.s0 + 1;
+                    } else {
+                        a0 = lopanel ? b.r_hi[w] : 0;
+                        a1 = lopanel ? b.rank[w] : b.r_hi[w];
+                    }
                 };
                 const int64_t i0 = (int64_t)idx.size();
                 for (int32_t bi : cover[gi]) {
                     const HBlock& b = B[bi];
-                    if (!take(b)) continue;
-                    if (b.kind[w] == QDFN_HBLK_DENSE)
-                        for (int64_t c = 0; c <= b.s1 - b.s0; ++c) idx.push_back((int32_t)(b.s0 + c));
-                    else
-                        for (int64_t c = 0; c < b.rank[w]; ++c) idx.push_back((int32_t)(n + toff[2 * bi + w] + c));
+                    int64_t a0, a1;
+                    span(b, a0, a1);
+                    for (int64_t c = a0; c < a1; ++c) {
+                        if (b.kind[w] == QDFN_HBLK_DENSE)
+                            idx.push_back((int32_t)(b.s0 + c));
+                        else
+                            idx.push_back((int32_t)(n + (lopanel ? toff_lo[2 * bi + w] + (c - a0)
+                                                                 : toff[2 * bi + w] + c)));
+                    }
                 }
                 const int64_t K = (int64_t)idx.size() - i0, ld = (K + al - 1) / al * al;
                 const int64_t off = (int64_t)dst.size();
@@ -898,18 +1054,22 @@ int hmat_layout(qdfn_hmat* h) {
                 int64_t col = 0;
                 for (int32_t bi : cover[gi]) {
                     const HBlock& b = B[bi];
-                    if (!take(b)) continue;
+                    int64_t a0, a1;
+                    span(b, a0, a1);
+                    if (a1 == a0) continue;
                     const int64_t mb = b.r1 - b.r0 + 1, roff = row0 - b.r0;
                     const bool diag = (w == 0 && b.s0 == b.r0 && b.s1 == b.r1 && b.kind[0] == QDFN_HBLK_DENSE);
-                    const int64_t nc = b.kind[w] == QDFN_HBLK_DENSE ? (b.s1 - b.s0 + 1) : b.rank[w];
+                    const bool scaled = lopanel && mixed && b.kind[w] == QDFN_HBLK_LOWRANK;
+                    const std::vector<double>& su = b.fu[w];
                     const std::vector<double>& A = b.a[w];
                     for (int64_t i = 0; i < m; ++i)
-                        for (int64_t c = 0; c < nc; ++c) {
+                        for (int64_t c = a0; c < a1; ++c) {
                             double v = A[(roff + i) + c * mb];
                             if (diag && row0 + i == b.s0 + c) v = 0.0;  // loading form: zero diagonal
-                            dst[off + i * ld + col + c] = v;
+                            if (scaled) v *= su[c - a0];
+                            dst[off + i * ld + col + (c - a0)] = v;
                         }
-                    col += nc;
+                    col += a1 - a0;
                 }
                 HRowBlock R;
                 R.data_off = off;
@@ -921,11 +1081,11 @@ int hmat_layout(qdfn_hmat* h) {
                 R.kind = 1 + w;
                 return R;
             };
-            if (!lo_far) {
-                add_rowblock(panel(0, pool), false);
+            if (!has_lo) {
+                add_rowblock(panel(false, pool), false);
                 continue;
             }
-            const HRowBlock Ra = panel(1, pool), Rb = panel(2, lo);
+            const HRowBlock Ra = panel(false, pool), Rb = panel(true, lo);
             if (Rb.K == 0)
                 add_rowblock(Ra, false);
             else if (Ra.K == 0)
@@ -982,11 +1142,16 @@ int hmat_layout(qdfn_hmat* h) {
     // upload
     if (pool.size() < 4) pool.resize(4, 0.0);
     QDFN_HIP(h->pool.alloc(es * pool.size()));
-    if (lo_far) {
+    if (has_lo) {
         std::vector<uint16_t> b(std::max<size_t>(lo.size(), 4), 0);
-        for (size_t q = 0; q < lo.size(); ++q) b[q] = to_bf16(lo[q]);
+        for (size_t q = 0; q < lo.size(); ++q) b[q] = lo_far ? to_bf16(lo[q]) : to_fp16(lo[q]);
         QDFN_HIP(h->pool_lo.alloc(sizeof(uint16_t) * b.size()));
         QDFN_HIP(hipMemcpy(h->pool_lo.p, b.data(), sizeof(uint16_t) * b.size(), hipMemcpyHostToDevice));
+    }
+    if (mixed) {
+        tsc.resize(std::max<int64_t>(h->n_t, 1), 1.0);
+        QDFN_HIP(h->tscale.alloc(sizeof(double) * tsc.size()));
+        QDFN_HIP(hipMemcpy(h->tscale.p, tsc.data(), sizeof(double) * tsc.size(), hipMemcpyHostToDevice));
     }
     if (h->prec != QDFN_PREC_FP64) {
         std::vector<float> f(pool.begin(), pool.end());
@@ -1042,7 +1207,7 @@ int hmat_layout(qdfn_hmat* h) {
     h->phase_bytes = (int64_t)(es * pool.size() + sizeof(uint16_t) * lo.size() + sizeof(int32_t) * idx.size() +
                                sizeof(HRowBlock) * rbs.size() + sizeof(HTile) * tiles.size() + sizeof(VTask) * vtasks.size() +
                                sizeof(VFin) * vfins.size() + 16 * part_total + 16 * vpart_total + 16 * h->n_t +
-                               8 * 3 * n);
+                               (mixed ? 8 * h->n_t : 0) + 8 * 3 * n);
     h->device_bytes = h->phase_bytes;
     // algorithmic bytes of each phase's main kernel: stored entries (unpadded),
     // index list (phase 2), gathered right-hand side, outputs
@@ -1053,7 +1218,7 @@ int hmat_layout(qdfn_hmat* h) {
         k2 += rbs[q].K;
     }
     // phase 1: V entries, each panel's x slice once, t
-    h->alg_bytes1 = v_entries * (lo_far ? (int64_t)sizeof(uint16_t) : es) + 8 * v_cols + 8 * h->n_t;
+    h->alg_bytes1 = v_entries * es + v_entries_lo * (int64_t)sizeof(uint16_t) + 8 * v_cols + 8 * h->n_t;
     h->alg_bytes2 = b2 + 12 * k2 + 8 * out_rows2;
     return QDFN_OK;
 }
@@ -1066,13 +1231,17 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
     double* part = h->part.as<double>();
     const double* kd = h->kdiag.as<double>();
     double* t = h->t.as<double>();
-    const bool f32 = h->prec == QDFN_PREC_FP32, bf = h->prec == QDFN_PREC_BF16_FAR;
+    const bool f32 = h->prec == QDFN_PREC_FP32, bf = h->prec == QDFN_PREC_BF16_FAR, mx = h->prec == QDFN_PREC_MIXED;
+    const double* tsc = mx ? h->tscale.as<double>() : nullptr;
     if (ev0) QDFN_HIP(hipEventRecord(ev0, s));
     if (h->n_vtasks > 0) {
         const unsigned grid = (unsigned)((h->n_vtasks + kVThreads / 64 - 1) / (kVThreads / 64));
         const VTask* vt = h->vtasks.as<VTask>();
         double* vp = h->vpart.as<double>();
-        if (f32)
+        if (mx)
+            hipLaunchKernelGGL(k_vpanel_mixed, dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
+                               h->pool.as<float>(), h->pool_lo.as<uint16_t>(), tsc, d_x, t, vp, g);
+        else if (f32)
             hipLaunchKernelGGL((k_vpanel<float>), dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
                                h->pool.as<float>(), d_x, t, vp, g);
         else if (bf)
@@ -1084,7 +1253,7 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
         QDFN_HIP(hipGetLastError());
         if (h->n_vfins > 0) {
             hipLaunchKernelGGL(k_vfinish, dim3((unsigned)((h->n_vfins + 255) / 256)), dim3(256), 0, s,
-                               h->vfins.as<VFin>(), (int32_t)h->n_vfins, vp, t, g);
+                               h->vfins.as<VFin>(), (int32_t)h->n_vfins, vp, t, tsc, g);
             QDFN_HIP(hipGetLastError());
         }
     }
@@ -1093,9 +1262,13 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
     const bool finish = h->n_multi2 > 0 && !finish_in_step;
     const int32_t mark_tiles = finish ? 0 : 1;
     if (bf)
-        hipLaunchKernelGGL(k_hrow_mixed, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s, tiles + h->n_tiles1,
-                           rbs, part, h->pool.as<float>(), h->pool_lo.as<bf16_t>(), h->idx.as<int32_t>(), d_x, t,
-                           h->n, kd, form, d_yS, d_yN, mark_tiles, g);
+        hipLaunchKernelGGL(k_hrow_mixed<bf16_t>, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
+                           tiles + h->n_tiles1, rbs, part, h->pool.as<float>(), h->pool_lo.as<bf16_t>(),
+                           h->idx.as<int32_t>(), d_x, t, h->n, kd, form, d_yS, d_yN, mark_tiles, g);
+    else if (mx)
+        hipLaunchKernelGGL(k_hrow_mixed<fp16_t>, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
+                           tiles + h->n_tiles1, rbs, part, h->pool.as<float>(), h->pool_lo.as<fp16_t>(),
+                           h->idx.as<int32_t>(), d_x, t, h->n, kd, form, d_yS, d_yN, mark_tiles, g);
     else if (f32)
         hipLaunchKernelGGL(k_hrow<float>, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
                            tiles + h->n_tiles1, rbs, part, h->pool.as<float>(), h->idx.as<int32_t>(), d_x, t, h->n,
@@ -1117,7 +1290,8 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
 // ---------------------------------------------------------------- build ----
 
 qdfn_hmat* new_hmat(int64_t n, int32_t prec) {
-    if (n <= 0 || (prec != QDFN_PREC_FP64 && prec != QDFN_PREC_FP32 && prec != QDFN_PREC_BF16_FAR)) {
+    if (n <= 0 || (prec != QDFN_PREC_FP64 && prec != QDFN_PREC_FP32 && prec != QDFN_PREC_BF16_FAR &&
+                   prec != QDFN_PREC_MIXED)) {
         fail(QDFN_ERR_ARG, "H-matrix: bad size or precision");
         return nullptr;
     }
@@ -1236,9 +1410,34 @@ static void svd_jacobi(int64_t r, std::vector<double> C, std::vector<double>& W,
 
 // U (m x r), V (r x k) -> the shortest factors whose product differs from U V
 // by at most `budget` in the Frobenius norm.
-void recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V,
-                       double budget) {
-    if (r < 2 || budget <= 0) return;
+// First FP16 component of an m x k low-rank part in SVD form with singular
+// values sig[0..r) (decreasing) and rounding budget `slack`.  Stored component
+// a contributes sigma_a (u_a + du)(v_a + dv)^T: rounding a unit vector of
+// length l to FP16 after scaling its largest entry into [0.5, 1) errs by at
+// most 2^-11 + sqrt(l) 2^-24 in the 2-norm (relative rounding above 2^-14,
+// the subnormal quantum below); rounding U's column and V's row to FP32 by
+// 2^-24 each.  The trailing components move to FP16, from the smallest up,
+// while the sum over all components of sigma_a (|du| + |dv| + |du| |dv|)
+// stays within the slack (none when even FP32 rounding alone exceeds it).
+int64_t fp16_tail_start(const double* sig, int64_t r, double slack, int64_t m, int64_t k) {
+    const double e32 = 2 * 0x1p-24 + 0x1p-48;
+    const double du = 0x1p-11 + std::sqrt((double)m) * 0x1p-24, dv = 0x1p-11 + std::sqrt((double)k) * 0x1p-24;
+    const double e16 = du + dv + du * dv;
+    double E = 0.0;
+    for (int64_t a = 0; a < r; ++a) E += e32 * sig[a];
+    int64_t r_hi = r;
+    while (r_hi > 0) {
+        const double E2 = E + (e16 - e32) * sig[r_hi - 1];
+        if (!(E2 <= slack)) break;
+        E = E2;
+        --r_hi;
+    }
+    return r_hi;
+}
+
+double recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V,
+                  double budget, bool fp16_tail) {
+    if (r < 2 || budget <= 0) return 0.0;
     std::vector<double> Vt(k * r);
     for (int64_t i = 0; i < r; ++i)
         for (int64_t c = 0; c < k; ++c) Vt[c + i * k] = V[i + c * r];
@@ -1265,7 +1464,30 @@ void recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::v
         tail += s2;
         --keep;
     }
-    if (keep >= r) return;
+    if (fp16_tail) {
+        // with FP16 trailing components, keeping a few more components leaves
+        // more of the budget for rounding: keep the count whose storage
+        // (4 bytes per FP32 entry, 2 per FP16 entry) is smallest
+        std::vector<double> ss(r), tail2(r + 1, 0.0);
+        for (int64_t a = 0; a < r; ++a) ss[a] = sig[ord[a]];
+        for (int64_t a = r - 1; a >= 0; --a) tail2[a] = tail2[a + 1] + ss[a] * ss[a];
+        auto bytes = [&](int64_t kk) {
+            const int64_t rh = fp16_tail_start(ss.data(), kk, budget - std::sqrt(tail2[kk]), m, k);
+            return (m + k) * (4 * rh + 2 * (kk - rh));
+        };
+        int64_t best = keep, best_b = bytes(keep);
+        for (int64_t kk = keep + 1; kk <= std::min<int64_t>(r, keep + 32); ++kk) {
+            const int64_t b = bytes(kk);
+            if (b < best_b) {
+                best = kk;
+                best_b = b;
+            }
+        }
+        keep = best;
+        tail = tail2[keep];
+    }
+    // SVD form even when nothing is dropped: the storage of QDFN_PREC_MIXED
+    // rounds components by their singular values
     std::vector<double> U2(m * keep, 0.0), V2(keep * k, 0.0);
     for (int64_t a = 0; a < keep; ++a) {
         const int64_t j = ord[a];
@@ -1281,6 +1503,7 @@ void recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::v
     U.swap(U2);
     V.swap(V2);
     r = keep;
+    return std::max(0.0, budget - std::sqrt(tail));
 }
 
 __global__ __launch_bounds__(256) void k_pack(const PackItem* __restrict__ items, double* __restrict__ dst) {
@@ -1494,7 +1717,7 @@ void recompress_parallel(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, const
             HBlock& b = h->blocks[lr[items[i]].q];
             const int w = lr[items[i]].w;
             const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
-            recompress(m, k, b.rank[w], b.a[w], b.b[w], budgets[i]);
+            b.slack[w] = recompress(m, k, b.rank[w], b.a[w], b.b[w], budgets[i], h->prec == QDFN_PREC_MIXED);
             ++done;
             if (main_thread && verbose &&
                 std::chrono::steady_clock::now() - t_last > std::chrono::seconds(10)) {
@@ -1760,6 +1983,66 @@ extern "C" int qdfn_hmat_get_ranks(qdfn_hmat_t* h, int32_t* kind_S, int64_t* ran
         if (rank_S) rank_S[q] = b.rank[0];
         if (kind_N) kind_N[q] = b.kind[1];
         if (rank_N) rank_N[q] = b.rank[1];
+    }
+    return QDFN_OK;
+}
+
+// value of a 16-bit pool entry
+static double from_lo(uint16_t b, bool bf16) {
+    if (bf16) {
+        const uint32_t w = (uint32_t)b << 16;
+        float f;
+        std::memcpy(&f, &w, 4);
+        return f;
+    }
+    _Float16 f;
+    std::memcpy(&f, &b, 2);
+    return (double)f;
+}
+
+extern "C" int qdfn_hmat_get_block_stored(qdfn_hmat_t* h, int64_t bi, int32_t which, double* a, double* bv) {
+    if (!h || bi < 0 || bi >= (int64_t)h->blocks.size() || which < 0 || which > 1)
+        return fail(QDFN_ERR_ARG, "qdfn_hmat_get_block_stored: bad arguments");
+    const HBlock& b = h->blocks[bi];
+    const int w = which;
+    const bool f64 = h->prec == QDFN_PREC_FP64, bf = h->prec == QDFN_PREC_BF16_FAR;
+    auto hi = [&](double v) { return f64 ? v : (double)(float)v; };
+    const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+    if (b.kind[w] == QDFN_HBLK_DENSE) {
+        // the product zeroes the shear diagonal of diagonal blocks and applies
+        // K_S,ii in FP64 (kdiag), so that diagonal is not rounded
+        const bool diag = w == 0 && b.s0 == b.r0 && b.s1 == b.r1;
+        if (a)
+            for (int64_t c = 0; c < k; ++c)
+                for (int64_t i = 0; i < m; ++i)
+                    a[i + c * m] = (diag && i == c) ? b.a[w][i + c * m] : hi(b.a[w][i + c * m]);
+        return QDFN_OK;
+    }
+    const int64_t r = b.rank[w], r_hi = b.r_hi[w];
+    for (int64_t c = 0; c < r; ++c) {
+        const bool lo = c >= r_hi;
+        const bool scaled = lo && !bf;
+        const double mu = scaled ? b.fu[w][c - r_hi] : 1.0, mv = scaled ? b.fv[w][c - r_hi] : 1.0;
+        const double sc = scaled ? b.fc[w][c - r_hi] : 1.0;
+        if (a)
+            for (int64_t i = 0; i < m; ++i) {
+                const double v = b.a[w][i + c * m];
+                a[i + c * m] = lo ? from_lo(bf ? to_bf16(v) : to_fp16(v * mu), bf) * sc : hi(v);
+            }
+        if (bv)
+            for (int64_t j = 0; j < k; ++j) {
+                const double v = b.b[w][c + j * r];
+                bv[c + j * r] = lo ? from_lo(bf ? to_bf16(v) : to_fp16(v * mv), bf) : hi(v);
+            }
+    }
+    return QDFN_OK;
+}
+
+extern "C" int qdfn_hmat_get_split(qdfn_hmat_t* h, int64_t* rhi_S, int64_t* rhi_N) {
+    if (!h) return fail(QDFN_ERR_ARG, "qdfn_hmat_get_split: null handle");
+    for (size_t q = 0; q < h->blocks.size(); ++q) {
+        if (rhi_S) rhi_S[q] = h->blocks[q].r_hi[0];
+        if (rhi_N) rhi_N[q] = h->blocks[q].r_hi[1];
     }
     return QDFN_OK;
 }

@@ -29,6 +29,7 @@ QDFN_ERR_EMPTY_DTMIN = -11
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16_FAR = 2  # H operators: BF16 low-rank factors, FP32 dense blocks
+PREC_MIXED = 3  # H operators: FP32, trailing low-rank components FP16 within each block's error budget
 
 
 class QdfnError(RuntimeError):
@@ -106,6 +107,8 @@ SIGNATURES = {
     "qdfn_hmat_time_phases": (C.c_int, [VP, I32, PD, PD, PD]),
     "qdfn_hmat_get_ranks": (C.c_int, [VP, PI32, PI64, PI32, PI64]),
     "qdfn_hmat_get_block": (C.c_int, [VP, I64, I32, PD, PD]),
+    "qdfn_hmat_get_block_stored": (C.c_int, [VP, I64, I32, PD, PD]),
+    "qdfn_hmat_get_split": (C.c_int, [VP, PI64, PI64]),
     "qdfn_hmat_kself": (C.c_int, [VP, PD, PD]),
     "qdfn_hmat_normal_zero": (C.c_int, [VP]),
     "qdfn_hmat_matvec": (C.c_int, [VP, PD, PD, PD, I32]),

@@ -175,9 +175,18 @@ class HMatrix:
                                         kN.ctypes.data_as(PI32), rN.ctypes.data_as(PI64)))
         return {"kind_S": kS, "rank_S": rS, "kind_N": kN, "rank_N": rN}
 
-    def blocks(self, range_SR):
-        """The stored blocks in the oracle's dict format, one dict per matrix
-        (n, range, kind, rank, a, b)."""
+    def split(self):
+        """Per block, the first low-rank component in the 16-bit pool (shear, normal)."""
+        nb = self.info()["blocks"]
+        s, nn = np.zeros(nb, dtype=np.int64), np.zeros(nb, dtype=np.int64)
+        check(lib().qdfn_hmat_get_split(self.handle, s.ctypes.data_as(PI64), nn.ctypes.data_as(PI64)))
+        return s, nn
+
+    def blocks(self, range_SR, stored=False):
+        """The blocks in the oracle's dict format, one dict per matrix
+        (n, range, kind, rank, a, b): FP64 values as built, or (stored=True)
+        as the product applies them, rounded to the operator's storage."""
+        get = lib().qdfn_hmat_get_block_stored if stored else lib().qdfn_hmat_get_block
         rs = _i64(range_SR)
         rk = self.ranks()
         out = []
@@ -188,14 +197,14 @@ class HMatrix:
                 k = rs[q, 1] - rs[q, 0] + 1
                 if kk[q] == 0:
                     a = np.zeros((m, k), order="F")
-                    check(lib().qdfn_hmat_get_block(self.handle, q, w, a.ctypes.data_as(PD), None))
+                    check(get(self.handle, q, w, a.ctypes.data_as(PD), None))
                     A.append(a)
                     B.append(None)
                 else:
                     r = int(rr[q])
                     a = np.zeros((m, r), order="F")
                     b = np.zeros((r, k), order="F")
-                    check(lib().qdfn_hmat_get_block(self.handle, q, w, a.ctypes.data_as(PD), b.ctypes.data_as(PD)))
+                    check(get(self.handle, q, w, a.ctypes.data_as(PD), b.ctypes.data_as(PD)))
                     A.append(a)
                     B.append(b)
             out.append({"n": self.n, "range": rs, "kind": kk.astype(np.int64), "rank": rr, "a": A, "b": B})

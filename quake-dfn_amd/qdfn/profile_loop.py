@@ -1,12 +1,14 @@
 """Run the earthquake-cycle loop alone on the 20,000-triangle rough fault, for
 kernel traces of a whole attempt (product phases, local solves, decision).
 
-    python -m qdfn.profile_loop [--steps 1000] [--warmup 100] [--aca-tol 1e-6] [--bf16-far]
+    python -m qdfn.profile_loop [--steps 1000] [--warmup 100] [--aca-tol 1e-6]
+                                [--prec fp32|mixed|bf16|fp64] [--crit 1.5]
 
-Builds the H operator (FP32 storage, or BF16 low-rank factors with --bf16-far;
-leaves of 64, DistDiamRatioCrit 1.0) and the run's arrays the way the
-benchmark does, replays the captured attempt graph for `steps` accepted steps
-and prints the RHS evaluations per second.
+Builds the H operator (storage --prec: FP32; FP32 with FP16 trailing low-rank
+components; BF16 low-rank factors; FP64 — leaves of 64, DistDiamRatioCrit
+--crit) and the run's arrays the way the benchmark does, replays the captured
+attempt graph for `steps` accepted steps and prints the RHS evaluations per
+second.
 """
 import argparse
 import sys
@@ -20,7 +22,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--aca-tol", type=float, default=1e-6)
-    ap.add_argument("--bf16-far", action="store_true")
+    ap.add_argument("--prec", choices=("fp32", "mixed", "bf16", "fp64"), default="fp32")
+    ap.add_argument("--crit", type=float, default=1.5)
     a = ap.parse_args()
     import qdfn
     from qdfn import configs
@@ -30,12 +33,13 @@ def main():
     m = dict(m, P1=P1, P2=P2, P3=P3)
     ctr = (P1 + P2 + P3) / 3
     lam = 2 * m["mu"] * m["nu"] / (1 - 2 * m["nu"])
-    st = qdfn.hmat_structure(ctr, P1, min_elements_to_cut=64, max_levels=40, dist_diam_ratio=1.0,
+    st = qdfn.hmat_structure(ctr, P1, min_elements_to_cut=64, max_levels=40, dist_diam_ratio=a.crit,
                              arrange_point=(0.0, 0.0, 0.0))
     p = st["perm"]
     geo = [x[p] for x in (P1, P2, P3, m["rake"], ctr, nrm, slip)]
     h = qdfn.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=a.aca_tol,
-                               prec=qdfn.PREC_BF16_FAR if a.bf16_far else qdfn.PREC_FP32)
+                               prec={"fp32": qdfn.PREC_FP32, "mixed": qdfn.PREC_MIXED, "bf16": qdfn.PREC_BF16_FAR,
+                                     "fp64": qdfn.PREC_FP64}[a.prec])
     ks, _ = h.kself()
     hs, hn = h.matvec(np.ones(n), loading_form=False)
     prob = configs.problem_from_triangles(m, ks, hs, hn, order=p)
@@ -48,9 +52,13 @@ def main():
     el = time.perf_counter() - t0
     s1 = sim.stats()
     sim.close()
+    info = h.info()
+    ph = h.time_phases(20)
     h.close()
-    print(f"loop n={n}: {s1['rhs_evals'] - s0['rhs_evals']} RHS evals in {el:.3f} s = "
-          f"{(s1['rhs_evals'] - s0['rhs_evals']) / el:.1f}/s, status {s1['status']}", flush=True)
+    print(f"loop n={n} prec {a.prec} crit {a.crit}: {info['bytes_per_product'] / 1e6:.1f} MB per product, "
+          f"{s1['rhs_evals'] - s0['rhs_evals']} RHS evals in {el:.3f} s = "
+          f"{(s1['rhs_evals'] - s0['rhs_evals']) / el:.1f}/s, status {s1['status']}; product phases (ms): "
+          f"phase 1 {ph[0]:.4f}, phase 2 {ph[1]:.4f}, total {ph[2]:.4f}", flush=True)
     return 0 if s1["status"] == 0 else 1
 
 
