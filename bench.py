@@ -1,22 +1,30 @@
 #!/usr/bin/env python3
-"""Benchmark: RHS evaluations per second of the device-resident earthquake-cycle
-loop, mixed precision against the FP64 plain form, at a stated event-time error.
+"""Benchmark: RHS evaluations per second and simulated years per hour of the
+device-resident earthquake-cycle loop, mixed precision against the FP64 plain
+form, at a stated event-time error.
 
 Headline workload: the rough strike-slip fault of 20,000 triangles (100 x 100
-cells of 50 m, sinusoidal roughness), H-matrix operator (cluster leaves of 64,
-ACA tolerance 1e-6) stored in FP32 with FP64 accumulation and FP64 state,
-against the dense FP64 operator ("plain" form, main() of
-scripts/Functions_RSFDFN3DMain_D.jl).  Both loops are main_H/main semantics on
-the device; each run is loaded by back-slip tables built from its own operator
-(UniformBackslip.jl).  `bf16_far_h` reports the same H build with its low-rank
-factors stored in BF16 (dense blocks FP32), with its own operator error.
-Secondary workload: the 8-fault, 8,192-element dense network, FP32 storage
-against FP64.
+cells of 50 m, three seeded sinusoidal roughness modes), H-matrix operator
+with cluster leaves of 64, admissibility eta = 2 read through the reference's
+criterion as DistDiamRatioCrit 1.5, ACA tolerance 1e-6; storage FP32 for dense
+blocks and leading low-rank components, FP16 unit vectors for the trailing
+components of each block within its error budget; FP64 accumulation, state,
+local solves and time.  Plain form: the dense FP64 operator (main() of
+scripts/Functions_RSFDFN3DMain_D.jl).  Both loops have main_H / main semantics
+on the device; each run is loaded by back-slip tables built from its own
+operator (UniformBackslip.jl).  Reported beside: the same H structure in FP32
+storage, the reference's default DistDiamRatioCrit 1.0, and (--bf16-far)
+BF16 low-rank factors.
 
 A step is one accepted time step (one coupling product plus one or more
-local-solve passes).  `value` = RHS evaluations / s of the mixed-precision H run
-over the timed steps (replayed from a captured HIP graph), inputs resident in
-HBM.  Prints one JSON line on stdout; progress goes to stderr.
+local-solve passes).  `value` = RHS evaluations / s of the headline run over
+exactly --steps accepted steps replayed from a captured HIP graph, inputs
+resident in HBM.  The accuracy half of the metric does not depend on --steps:
+fixed-duration runs to the first event (or the configuration's duration)
+give the first-event time of the mixed run, of the FP64 run and of FP64 runs
+whose initial state is perturbed at rounding level (the spread that measures
+the workload's conditioning).  Prints one JSON line on stdout; progress on
+stderr.
 """
 import argparse
 import json
@@ -32,12 +40,13 @@ import numpy as np  # noqa: E402
 
 METRIC = "RHS evals/sec + sim-years/hour, mixed vs FP64 on GPU at stated event-time error"
 HBM_PEAK_GBS = 8000.0
-PROFILE = os.path.join(ROOT, "profiles", "hmat_phase2_rough_fault_20000.md")
+PROFILE = os.path.join(ROOT, "profiles", "hmat_phase2_mixed_rough_fault_20000.md")
+YEAR = 3.15576e7
 
 
 def measured_traffic():
-    """HBM bytes per launch of the phase-2 H kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated); None if absent."""
+    """HBM bytes per launch of the headline phase-2 kernel from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, calibrated); None if absent."""
     try:
         for line in open(PROFILE):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -68,32 +77,120 @@ def run_timed(qd, op, prob, steps, warmup, graph_attempts, kernel_steps):
     if kernel_steps:
         sim.run(kernel_steps, timed=True)
     s2 = sim.stats()
-    tr = sim.trace()
     sim.close()
     rhs = s1["rhs_evals"] - s0["rhs_evals"]
     return dict(elapsed=el, rhs=rhs, retries=s1["retry_passes"] - s0["retry_passes"],
                 sim_seconds=s1["t"] - s0["t"], coupling_ms=s2["coupling_ms_avg"],
-                coupling_timed=s2["coupling_timed"], trace=tr, status=s2["status"],
-                evals_per_s=rhs / el)
+                coupling_timed=s2["coupling_timed"], status=s2["status"], evals_per_s=rhs / el)
 
 
-def summarize(name, r, sec_per_year):
-    r["sim_years_per_hour"] = r["sim_seconds"] / sec_per_year / (r["elapsed"] / 3600.0)
+def summarize(name, r):
+    r["sim_years_per_hour"] = r["sim_seconds"] / YEAR / (r["elapsed"] / 3600.0)
     log(f"[bench] {name}: {r['rhs']} RHS evals in {r['elapsed']:.3f}s = {r['evals_per_s']:.1f}/s, "
         f"coupling {r['coupling_ms']*1e3:.1f} us over {r['coupling_timed']}, "
         f"{r['sim_years_per_hour']:.3f} sim-yr/h, retries {r['retries']}")
 
 
-def divergence(tr, ref, rel=1e-3):
-    """First step at which the simulated time of a run departs from the FP64
-    plain run by more than `rel` (relative), with the plain run's time there."""
-    a, b = np.asarray(tr["told"]), np.asarray(ref["told"])
-    m = min(a.size, b.size)
-    d = np.abs(a[:m] - b[:m]) > rel * np.maximum(np.abs(b[:m]), 1.0)
-    if not d.any():
-        return None
-    k = int(np.argmax(d))
-    return {"step": k, "sim_time_s": float(b[k])}
+def onsets(told, dt, vmax, threshold=1e-3, rearm=1e-5):
+    """Event onsets: accepted steps whose max V reaches `threshold` after the
+    run has been below `rearm` since the last onset (so the oscillations of
+    one rupture around the threshold count once), placed by linear
+    interpolation in log V (time axis = end of step)."""
+    from qdfn import configs
+    v = np.asarray(vmax)
+    out = []
+    armed = True
+    for k in range(v.size):
+        if armed and v[k] >= threshold:
+            if k == 0:
+                out.append(float(told[0] + dt[0]))
+            else:
+                out.append(configs.first_event_time(told[k - 1:k + 1], dt[k - 1:k + 1], v[k - 1:k + 1], threshold))
+            armed = False
+        elif not armed and v[k] < rearm:
+            armed = True
+    return out
+
+
+def run_window(qd, op, prob, years, stop_at_event, max_steps, chunk=2000, graph_attempts=32):
+    """One run from t = 0 to `years` simulated years (or, with stop_at_event,
+    to its first event), max_steps at most, launched in chunks of accepted
+    steps replayed from the captured attempt graph."""
+    from qdfn import configs
+    sim = qd.Simulation(op, prob, record_step=0, record_capacity=0, trace_capacity=max_steps)
+    t_end = years * YEAR
+    ev = None
+    wall = 0.0
+    while True:
+        s = sim.stats()
+        left = max_steps - s["steps"]
+        if left <= 0 or s["t"] >= t_end:
+            break
+        t0 = time.perf_counter()
+        sim.run_until(t_end, min(chunk, left), graph_attempts=graph_attempts)
+        wall += time.perf_counter() - t0
+        if stop_at_event:
+            tr = sim.trace()
+            ev = configs.first_event_time(tr["told"], tr["dt"], tr["vmax"])
+            if ev is not None:
+                break
+    s = sim.stats()
+    tr = sim.trace()
+    sim.close()
+    if ev is None:
+        ev = configs.first_event_time(tr["told"], tr["dt"], tr["vmax"])
+    return dict(first_event_s=ev, steps=s["steps"], sim_seconds=s["t"], wall_s=wall, rhs=s["rhs_evals"],
+                retries=s["retry_passes"], onsets=onsets(tr["told"], tr["dt"], tr["vmax"]), status=s["status"],
+                sim_years_per_hour=(s["t"] / YEAR) / (wall / 3600.0) if wall > 0 else None,
+                rhs_evals_per_s=s["rhs_evals"] / wall if wall > 0 else None)
+
+
+def event_accuracy(qd, ops, probs, years, n_pert, max_steps):
+    """First-event times (onset of V >= 1e-3 m/s) within `years` of the FP64
+    plain run, of n_pert FP64 plain runs whose initial theta is multiplied by
+    1 + 1e-15 xi (xi standard normal, seeds 1..n_pert: rounding-level
+    perturbations of the reference's own path) and of each listed mixed run."""
+    base = run_window(qd, ops["fp64_dense"], probs["fp64_dense"], years, True, max_steps)
+    log(f"[bench] event: fp64_dense {fmt_ev(base['first_event_s'])} in {base['steps']} steps, {base['wall_s']:.1f}s")
+    pert = []
+    for seed in range(1, n_pert + 1):
+        p = dict(probs["fp64_dense"])
+        xi = np.random.default_rng(seed).standard_normal(p["theta_i"].size)
+        p["theta_i"] = p["theta_i"] * (1.0 + 1e-15 * xi)
+        r = run_window(qd, ops["fp64_dense"], p, years, True, max_steps)
+        pert.append(r["first_event_s"])
+        log(f"[bench] event: fp64_dense perturbed (seed {seed}) {fmt_ev(r['first_event_s'])}")
+    out = {"window_years": years, "fp64_plain_s": base["first_event_s"], "fp64_perturbed_s": pert}
+    evs = [t for t in pert + [base["first_event_s"]] if t is not None]
+    out["fp64_spread"] = {"min_s": min(evs) if evs else None, "max_s": max(evs) if evs else None,
+                          "runs_without_event": sum(1 for t in pert + [base["first_event_s"]] if t is None),
+                          "runs": n_pert + 1,
+                          "rel_spread": (max(evs) - min(evs)) / base["first_event_s"]
+                          if evs and base["first_event_s"] else None}
+    for name in ops:
+        if name == "fp64_dense":
+            continue
+        r = run_window(qd, ops[name], probs[name], years, True, max_steps)
+        t, t64 = r["first_event_s"], base["first_event_s"]
+        entry = {"first_event_s": t}
+        if t is not None and t64 is not None:
+            entry["rel_err"] = abs(t - t64) / t64
+        else:
+            entry["rel_err"] = None
+            entry["status"] = (f"no event within {years} years" if t is None else "FP64 run has no event") + \
+                (f" (FP64: {t64 / YEAR:.3f} years)" if t64 else "")
+        lo, hi = out["fp64_spread"]["min_s"], out["fp64_spread"]["max_s"]
+        none_seen = out["fp64_spread"]["runs_without_event"] > 0
+        entry["within_fp64_perturbation_range"] = bool(
+            (t is None and none_seen) or (t is not None and lo is not None and lo <= t <= (np.inf if none_seen else hi)))
+        out[name] = entry
+        log(f"[bench] event: {name} {fmt_ev(t)}; rel. error {entry['rel_err']}; within the FP64 spread "
+            f"{entry['within_fp64_perturbation_range']}")
+    return out, base
+
+
+def fmt_ev(t):
+    return "no event" if t is None else f"{t / YEAR:.4f} years"
 
 
 def _build_summary(h):
@@ -113,52 +210,67 @@ def rough_fault(qd, args):
     m = dict(m, P1=P1, P2=P2, P3=P3)
     ctr = (P1 + P2 + P3) / 3
     lam = 2 * m["mu"] * m["nu"] / (1 - 2 * m["nu"])
-    st = qd.hmat_structure(ctr, P1, min_elements_to_cut=64, max_levels=40, dist_diam_ratio=args.crit,
-                           arrange_point=(0.0, 0.0, 0.0))
+    t0 = time.perf_counter()
+    structs = {c: qd.hmat_structure(ctr, P1, min_elements_to_cut=64, max_levels=40, dist_diam_ratio=c,
+                                    arrange_point=(0.0, 0.0, 0.0)) for c in sorted({args.crit, 1.0})}
+    st = structs[args.crit]
     p = st["perm"]
     geo = [a[p] for a in (P1, P2, P3, m["rake"], ctr, nrm, slip)]
-    t0 = time.perf_counter()
     d64 = qd.DenseOperator.from_triangles(*geo, m["mu"], lam, qd.PREC_FP64)
-    log(f"[bench] rough fault n={n}: dense FP64 operator in {time.perf_counter() - t0:.2f}s")
-    t0 = time.perf_counter()
-    h32 = qd.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=args.aca_tol,
-                               prec=qd.PREC_FP32)
-    info = h32.info()
-    rk = h32.ranks()
-    log(f"[bench] H FP32 operator in {time.perf_counter() - t0:.2f}s: {info['blocks']} blocks, "
-        f"{int(st['admissible'].sum())} admissible, {info['stored_entries'] / 1e6:.1f} M entries, "
-        f"{info['bytes_per_product'] / 1e6:.1f} MB per product")
-    builds = {"full": _build_summary(h32)}
-    if not args.no_sampled_build:
-        # the same operator from sampled rows and columns (no full assembly)
-        hs = qd.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=args.aca_tol,
-                                  prec=qd.PREC_FP32, sampled=True)
-        builds["sampled"] = _build_summary(hs)
-        xr = np.random.default_rng(6).standard_normal(n)
-        a, b, c = h32.matvec(xr, loading_form=False), hs.matvec(xr, loading_form=False), d64.matvec(xr, False)
-        builds["product_rel_err_vs_dense_fp64"] = {
-            k: [float(np.linalg.norm(y[w] - c[w]) / np.linalg.norm(c[w])) for w in (0, 1)]
-            for k, y in (("full", a), ("sampled", b))}
-        hs.close()
-        log(f"[bench] H build full {builds['full']['ms_build']:.0f} ms ({builds['full']['entries_evaluated']} "
-            f"pairs), sampled {builds['sampled']['ms_build']:.0f} ms ({builds['sampled']['entries_evaluated']} "
-            f"pairs); product errors {builds['product_rel_err_vs_dense_fp64']}")
-    hb = None
-    if not args.no_bf16_far:
-        # the same build with the low-rank factors stored in BF16
+    log(f"[bench] rough fault n={n}: structures {time.perf_counter() - t0:.2f}s (dense FP64 operator included)")
+
+    def build(prec, crit=args.crit, sampled=False):
+        s = structs[crit]
+        g = [a[s["perm"]] for a in (P1, P2, P3, m["rake"], ctr, nrm, slip)]
         t0 = time.perf_counter()
-        hb = qd.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=args.aca_tol,
-                                  prec=qd.PREC_BF16_FAR)
-        log(f"[bench] H BF16-factor operator in {time.perf_counter() - t0:.2f}s: "
-            f"{hb.info()['bytes_per_product'] / 1e6:.1f} MB per product")
-    d32 = qd.DenseOperator.from_triangles(*geo, m["mu"], lam, qd.PREC_FP32)
+        h = qd.HMatrix.build_tri(*g, m["mu"], lam, s["range_SR"], s["admissible"], aca_rtol=args.aca_tol,
+                                 prec=prec, sampled=sampled)
+        info = h.info()
+        log(f"[bench] H (crit {crit}, prec {prec}{', sampled' if sampled else ''}) in {time.perf_counter() - t0:.2f}s: "
+            f"{info['blocks']} blocks, {int(s['admissible'].sum())} admissible, {info['stored_entries'] / 1e6:.1f} M "
+            f"entries, {info['bytes_per_product'] / 1e6:.1f} MB per product")
+        return h, s
+
+    hm, _ = build(qd.PREC_MIXED)
+    h32, _ = build(qd.PREC_FP32)
+    hm1, st1 = build(qd.PREC_MIXED, crit=1.0) if args.crit != 1.0 else (None, None)
+    builds = {"full": _build_summary(hm)}
+    xr = np.random.default_rng(6).standard_normal(n)
+    ref_prod = d64.matvec(xr, False)
+
+    def prod_err(h, x=xr, c=ref_prod):
+        y = h.matvec(x, loading_form=False)
+        return [float(np.linalg.norm(y[w] - c[w]) / np.linalg.norm(c[w])) for w in (0, 1)]
+
+    builds["product_rel_err_vs_dense_fp64"] = {"full": prod_err(hm)}
+    if not args.no_sampled_build:
+        hs, _ = build(qd.PREC_MIXED, sampled=True)
+        builds["sampled"] = _build_summary(hs)
+        builds["product_rel_err_vs_dense_fp64"]["sampled"] = prod_err(hs)
+        hs.close()
+    hb = build(qd.PREC_BF16_FAR)[0] if args.bf16_far else None
     ks, _ = d64.kself()
     ksum, knsum = d64.matvec(np.ones(n), loading_form=False)
     prob = configs.problem_from_triangles(m, ks, ksum, knsum, order=p)
-    prob["ext_at_told"] = 1  # main_H reads the external tables at TOld; the plain runs use the same timing
-    hsum, hnsum = h32.matvec(np.ones(n), loading_form=False)
-    prob_h = dict(prob, ext=configs.backslip_tables(hsum, hnsum))
-    # operator accuracy against the dense FP64 product
+    prob["ext_at_told"] = 1  # main_H reads the external tables at TOld; the plain run uses the same timing
+
+    def own_loading(h, order=p):
+        hsum, hnsum = h.matvec(np.ones(n), loading_form=False)
+        kk, _ = h.kself()
+        pr = configs.problem_from_triangles(m, kk, hsum, hnsum, order=order)
+        pr["ext_at_told"] = 1
+        return pr
+
+    probs = {"fp64_dense": prob, "mixed_h": own_loading(hm), "fp32_h": own_loading(h32)}
+    ops = {"fp64_dense": d64, "mixed_h": hm, "fp32_h": h32}
+    if hm1 is not None:
+        probs["mixed_h_crit1"] = own_loading(hm1, st1["perm"])
+        ops["mixed_h_crit1"] = hm1
+    if hb is not None:
+        probs["bf16_far_h"] = own_loading(hb)
+        ops["bf16_far_h"] = hb
+
+    # operator accuracy against the dense FP64 product (same element order)
     def accuracy(h):
         rng = np.random.default_rng(5)
         acc = {}
@@ -169,70 +281,84 @@ def rough_fault(qd, args):
                           "sigma_rel": float(np.abs(a[1] - b[1]).max() / np.abs(a[1]).max())}
         return acc
 
-    acc = accuracy(h32)
-    ph1, ph2, tot = h32.time_phases(20)
-    runs = [("fp64_dense", d64, prob), ("mixed_h", h32, prob_h), ("fp32_dense", d32, prob)]
-    bf = None
+    acc = {"mixed_h": accuracy(hm), "fp32_h": accuracy(h32)}
     if hb is not None:
-        bsum, bnsum = hb.matvec(np.ones(n), loading_form=False)
-        runs.append(("bf16_far_h", hb, dict(prob, ext=configs.backslip_tables(bsum, bnsum))))
-        bi = hb.info()
-        bf = {"operator_rel_err": accuracy(hb), "bytes_per_product": bi["bytes_per_product"],
-              "phase_bytes": [bi["phase1_bytes"], bi["phase2_bytes"]],
-              "h_product_us": [round(v * 1e3, 2) for v in hb.time_phases(20)]}
+        acc["bf16_far_h"] = accuracy(hb)
     res = {}
-    for name, op, pr in runs:
+    for name in ops:
         log(f"[bench] {name}: warmup {args.warmup}, timed {args.steps} steps")
-        r = run_timed(qd, op, pr, args.steps, args.warmup, args.graph_attempts, args.kernel_steps)
+        r = run_timed(qd, ops[name], probs[name], args.steps, args.warmup, args.graph_attempts, args.kernel_steps)
         if r["status"] != 0:
             raise SystemExit(f"{name} run failed with status {r['status']}")
-        summarize(name, r, configs.SEC_PER_YEAR)
-        r["first_event"] = configs.first_event_time(r["trace"]["told"], r["trace"]["dt"], r["trace"]["vmax"])
+        summarize(name, r)
         res[name] = r
-    if hb is not None:
-        r = res["bf16_far_h"]
-        bf.update(rhs_evals_per_s=round(r["evals_per_s"], 2), sim_years_per_hour=round(r["sim_years_per_hour"], 4),
-                  first_event_s=r["first_event"], divergence_from_fp64=divergence(r["trace"], res["fp64_dense"]["trace"]))
-        bf["h_product_us"] = dict(zip(("phase1", "phase2", "total"), bf["h_product_us"]))
-        hb.close()
-    out = dict(n=n, info=info, st=st, rk=rk, acc=acc, phases=(ph1, ph2, tot), res=res, h32=h32, prob_h=prob_h,
-               builds=builds, bf16_far=bf)
-    d64.close()
-    d32.close()
+    phases = {k: [round(v * 1e3, 2) for v in ops[k].time_phases(20)] for k in ops if k != "fp64_dense"}
+    hinfo = {k: dict(ops[k].info(), bytes_by_part=ops[k].bytes_by_part()) for k in ops if k != "fp64_dense"}
+    out = dict(n=n, st=st, probs=probs, ops=ops, res=res, acc=acc, phases=phases, hinfo=hinfo, builds=builds,
+               d64=d64, hm=hm, d64_bytes=d64.bytes_per_product())
     return out
 
 
 def network(qd, args):
+    """Config 1: 8 rectangular faults, 8,192 elements, dense FP64 against
+    FP32 storage; first events within 100 years and their FP64 perturbation
+    spread (as for the rough fault)."""
     from qdfn import configs
     cfg = configs.fault_network()
     seg = cfg["seg"]
     ops = {name: qd.DenseOperator.from_rect_segments(seg, cfg["mu"], cfg["nu"], prec)
-           for name, prec in (("fp64", qd.PREC_FP64), ("fp32", qd.PREC_FP32))}
-    ks, _ = ops["fp64"].kself()
-    ksum, knsum = ops["fp64"].matvec(np.ones(seg.shape[0]), loading_form=False)
+           for name, prec in (("fp64_dense", qd.PREC_FP64), ("fp32_dense", qd.PREC_FP32))}
+    ks, _ = ops["fp64_dense"].kself()
+    ksum, knsum = ops["fp64_dense"].matvec(np.ones(seg.shape[0]), loading_form=False)
     prob = configs.problem_from_segments(cfg, ks, ksum, knsum)
     res = {}
-    for name in ("fp64", "fp32"):
+    for name in ops:
         r = run_timed(qd, ops[name], prob, args.network_steps, args.warmup, args.graph_attempts, 0)
         if r["status"] != 0:
             raise SystemExit(f"network {name} run failed with status {r['status']}")
-        summarize("network " + name, r, configs.SEC_PER_YEAR)
+        summarize("network " + name, r)
         res[name] = r
+    acc, base = event_accuracy(qd, ops, {k: prob for k in ops}, 100.0, args.perturbations, 60000)
+    out = {"workload": "fault_network_8192", "fp64_rhs_evals_per_s": round(res["fp64_dense"]["evals_per_s"], 2),
+           "mixed_rhs_evals_per_s": round(res["fp32_dense"]["evals_per_s"], 2),
+           "speedup_mixed_vs_fp64": round(res["fp32_dense"]["evals_per_s"] / res["fp64_dense"]["evals_per_s"], 3),
+           "steps_timed": args.network_steps, "coupling_us": {k: round(res[k]["coupling_ms"] * 1e3, 2) for k in res},
+           "event_time": acc,
+           "tolerance": "first-event time of FP32 storage within the spread of rounding-level perturbations of "
+                        "the FP64 run"}
     for o in ops.values():
         o.close()
-    t = [configs.first_event_time(res[k]["trace"]["told"], res[k]["trace"]["dt"], res[k]["trace"]["vmax"])
-         for k in ("fp64", "fp32")]
-    err = abs(t[1] - t[0]) / t[0] if (t[0] and t[1]) else None
-    return {"workload": "fault_network_8192", "fp64_rhs_evals_per_s": round(res["fp64"]["evals_per_s"], 2),
-            "mixed_rhs_evals_per_s": round(res["fp32"]["evals_per_s"], 2),
-            "speedup_mixed_vs_fp64": round(res["fp32"]["evals_per_s"] / res["fp64"]["evals_per_s"], 3),
-            "event_time_rel_err": err, "steps_timed": args.network_steps}
+    return out
+
+
+def single_fault(qd, args):
+    """Config 0: the 512-element single fault, dense FP64, run toward 200
+    simulated years (at most args.single_steps accepted steps): RHS evals/s
+    and sim-years/hour over whole cycles (first to last onset in the window),
+    event count and recurrence."""
+    from qdfn import configs
+    cfg = configs.single_fault()
+    op = qd.DenseOperator.from_rect_segments(cfg["seg"], cfg["mu"], cfg["nu"], qd.PREC_FP64)
+    ks, _ = op.kself()
+    ksum, knsum = op.matvec(np.ones(op.n), loading_form=False)
+    prob = configs.problem_from_segments(cfg, ks, ksum, knsum)
+    r = run_window(qd, op, prob, 200.0, False, args.single_steps, chunk=100_000)
+    op.close()
+    ev = r["onsets"]
+    log(f"[bench] single fault: {r['sim_seconds'] / YEAR:.1f} years in {r['steps']} steps, {r['wall_s']:.2f}s = "
+        f"{r['rhs_evals_per_s']:.0f} RHS evals/s, {r['sim_years_per_hour']:.0f} sim-yr/h, {len(ev)} events")
+    return {"workload": "single_fault_512", "years": r["sim_seconds"] / YEAR, "target_years": 200.0,
+            "steps": r["steps"], "step_cap": args.single_steps, "rhs_evals": r["rhs"], "retry_passes": r["retries"],
+            "wall_s": round(r["wall_s"], 3), "rhs_evals_per_s": round(r["rhs_evals_per_s"], 1),
+            "sim_years_per_hour": round(r["sim_years_per_hour"], 1), "events": len(ev),
+            "mean_recurrence_years": float(np.mean(np.diff(ev)) / YEAR) if len(ev) > 1 else None,
+            "first_event_years": ev[0] / YEAR if ev else None, "dtype": "f64"}
 
 
 def ensemble(qd, args):
     """256 independent planar faults of 500-4,000 elements (FP32 operators, one
-    per distinct geometry) advanced together by qdfn_ensemble_run for up to
-    20 simulated years each (step cap args.ensemble_steps per member)."""
+    per distinct geometry) advanced together by qdfn_ensemble_run to 20
+    simulated years each (step cap args.ensemble_steps per member)."""
     from qdfn import configs
     from qdfn.ensemble import problem_for_member
     t0 = time.perf_counter()
@@ -242,7 +368,7 @@ def ensemble(qd, args):
     log(f"[bench] ensemble: {len(members)} members, {sum(m['seg'].shape[0] for m in members)} elements, "
         f"{info['groups']} operator groups (max_rhs {info['max_rhs']}, autotuned), built in "
         f"{time.perf_counter() - t0:.1f}s")
-    t_end = 20 * configs.SEC_PER_YEAR
+    t_end = 20 * YEAR
     s0 = e.stats()
     t0 = time.perf_counter()
     e.run(args.ensemble_steps, t_end=t_end, graph_attempts=16)
@@ -250,7 +376,7 @@ def ensemble(qd, args):
     s1 = e.stats()
     rhs = sum(b["rhs_evals"] - a["rhs_evals"] for a, b in zip(s0, s1))
     steps = sum(b["steps"] - a["steps"] for a, b in zip(s0, s1))
-    years = sum(b["t"] - a["t"] for a, b in zip(s0, s1)) / configs.SEC_PER_YEAR
+    years = sum(b["t"] - a["t"] for a, b in zip(s0, s1)) / YEAR
     done = sum(1 for b in s1 if b["t"] >= t_end)
     log(f"[bench] ensemble: {rhs} RHS evals ({steps} steps) in {el:.2f}s = {rhs / el:.0f}/s; "
         f"{done}/{len(members)} members reached 20 years")
@@ -258,8 +384,9 @@ def ensemble(qd, args):
            "elements": int(sum(m["seg"].shape[0] for m in members)), "operator_groups": info["groups"],
            "max_rhs": info["max_rhs"], "matrix_bytes_per_sweep": info["bytes_per_sweep"],
            "rhs_evals_per_s": round(rhs / el, 1), "steps": steps, "elapsed_s": round(el, 3),
-           "sim_years_per_hour": round(years / (el / 3600.0), 1),
-           "members_at_20_years": done, "step_cap_per_member": args.ensemble_steps, "dtype": "f64 state, fp32 operator"}
+           "sim_years_per_hour": round(years / (el / 3600.0), 1), "members_at_20_years": done,
+           "steps_per_member": {"min": int(min(b["steps"] for b in s1)), "max": int(max(b["steps"] for b in s1))},
+           "step_cap_per_member": args.ensemble_steps, "dtype": "f64 state, fp32 operator"}
     e.close()
     return out
 
@@ -268,11 +395,10 @@ def adjoint_case(qd, args):
     """64 x 64 elements of 50 m (4,096), per-element a, b, Dc; loss = squared
     slip misfit at 32 seeded elements (+ squared first-event-time misfit when
     the window holds the onset) over a window of accepted steps; gradient by
-    qdfn_sim_adjoint, checked against central differences on 16 random
-    parameters.  Windows: args.adjoint_steps steps (through the first event;
-    the rupture's sensitivity grows exponentially, so differences are taken on
-    4 parameters only, to show it), the onset plus 38 steps (the first HighV
-    solves) and the interseismic steps before the onset."""
+    qdfn_sim_adjoint, checked against central differences on random
+    parameters.  Windows: the interseismic steps before the onset, the onset
+    plus 38 steps (the first HighV solves), and args.adjoint_steps steps
+    through the first event."""
     from qdfn import adjoint, configs
     m = configs.adjoint_fault()
     seg = m["seg"]
@@ -323,26 +449,25 @@ def adjoint_case(qd, args):
                      "fd": [{k: (float(f"{v:.9g}") if isinstance(v, float) else v) for k, v in c.items()}
                             for c in chk]}
         if name == "full_window" and info["onset_step"] > 2:
-            # through the onset and the first 30-40 steps of the event (HighV solves)
             windows.append(("onset_window", min(K, int(info["onset_step"]) + 38), 16))
             windows.append(("interseismic", int(info["onset_step"]) - 2, 16))
     op.close()
     return out
 
 
-def cpu_baseline(h32, st, prob_h, n_steps, threads):
+def cpu_baseline(h, st, prob, n_steps, threads):
     """The oracle's FP64 main_H (C restatement of HmatSolver_Pararllel and of the
     loop; products and local solves over OpenMP threads, outputs independent of
-    the count) on the same stored blocks, for a bounded number of steps, at each
-    thread count in `threads`."""
+    the count) on the headline operator's stored blocks, for a bounded number
+    of steps, at each thread count in `threads`."""
     from oracle import oracle as orc
-    HS, HN = h32.blocks(st["range_SR"])
+    HS, HN = h.blocks(st["range_SR"], stored=True)
     out = {}
     prev = orc.set_threads(1)
     for th in threads:
         orc.set_threads(th)
         t0 = time.perf_counter()
-        status, r = orc.sim_hmat(prob_h, HS, HN, total_step=n_steps, record_step=0, trace=False)
+        status, r = orc.sim_hmat(prob, HS, HN, total_step=n_steps, record_step=0, trace=False)
         el = time.perf_counter() - t0
         out[th] = dict(value=r["rhs_evals"] / el, elapsed=el, steps=n_steps, status=status)
     orc.set_threads(prev)
@@ -352,22 +477,26 @@ def cpu_baseline(h32, st, prob_h, n_steps, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8000, help="timed accepted steps per run (rough fault)")
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=4000, help="timed accepted steps per run (rough fault)")
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--graph-attempts", type=int, default=32)
     ap.add_argument("--kernel-steps", type=int, default=300, help="direct-launch steps with per-kernel events")
-    ap.add_argument("--crit", type=float, default=1.0, help="DistDiamRatioCrit of the cluster tree")
+    ap.add_argument("--crit", type=float, default=1.5, help="DistDiamRatioCrit of the cluster tree (eta = 2)")
     ap.add_argument("--aca-tol", type=float, default=1e-6)
+    ap.add_argument("--event-years", type=float, default=50.0, help="accuracy runs: first event within this window")
+    ap.add_argument("--perturbations", type=int, default=4, help="rounding-level perturbed FP64 runs")
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--network-steps", type=int, default=6000)
     ap.add_argument("--no-network", action="store_true")
-    ap.add_argument("--ensemble-steps", type=int, default=20000, help="accepted-step cap per ensemble member")
+    ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--single-steps", type=int, default=300000, help="step cap of the 200-year single-fault run")
+    ap.add_argument("--ensemble-steps", type=int, default=200000, help="accepted-step cap per ensemble member")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-sampled-build", action="store_true")
-    ap.add_argument("--no-bf16-far", action="store_true")
+    ap.add_argument("--bf16-far", action="store_true", help="also build and run BF16 low-rank factors")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU layout is "
@@ -379,31 +508,35 @@ def main():
     rf = rough_fault(qdfn, args)
     res = rf["res"]
     mixed, plain = res["mixed_h"], res["fp64_dense"]
-    t64, th, t32 = plain["first_event"], mixed["first_event"], res["fp32_dense"]["first_event"]
-    ev_err = abs(th - t64) / t64 if (th and t64) else None
-    ev_ref = abs(t32 - t64) / t64 if (t32 and t64) else None
-    ph1, ph2, tot = rf["phases"]
-    bytes2 = rf["info"]["phase2_bytes"]
-    achieved = bytes2 / (mixed["coupling_ms"] * 1e-3) / 1e9
+    hb = rf["hinfo"]["mixed_h"]["bytes_by_part"]
+    t2 = mixed["coupling_ms"] * 1e-3  # phase-2 kernel, HIP events on the run's stream
+    achieved = hb["phase2_entries"] / t2 / 1e9
+    # accuracy: fixed-duration runs, independent of --steps
+    ev_ops = {k: rf["ops"][k] for k in rf["ops"] if k in ("fp64_dense", "mixed_h", "fp32_h", "mixed_h_crit1")}
+    acc, base = event_accuracy(qdfn, ev_ops, rf["probs"], args.event_years, args.perturbations, 200000)
+    # the headline run over the configuration's duration (50 years), through its events
+    full = run_window(qdfn, rf["ops"]["mixed_h"], rf["probs"]["mixed_h"], args.event_years, False, 2_000_000,
+                      chunk=20000)
+    log(f"[bench] mixed_h over {args.event_years} years: {full['steps']} steps in {full['wall_s']:.2f}s, "
+        f"{full['sim_years_per_hour']:.1f} sim-yr/h, {len(full['onsets'])} events")
     net = None if args.no_network else network(qdfn, args)
+    single = None if args.no_single else single_fault(qdfn, args)
     ens = None if args.no_ensemble else ensemble(qdfn, args)
     adj = None if args.no_adjoint else adjoint_case(qdfn, args)
     cpu = None
     if not args.no_cpu:
-        log(f"[bench] CPU baseline: oracle FP64 main_H, {args.cpu_steps} steps")
+        log(f"[bench] CPU baseline: oracle FP64 main_H on the headline operator's blocks, {args.cpu_steps} steps")
         # all of the host's share of cores (16 per GPU on a shared box), and one
         nall = min(int(os.environ.get("OMP_NUM_THREADS", "0") or os.cpu_count() or 1), os.cpu_count() or 1, 16)
-        cs = cpu_baseline(rf["h32"], rf["st"], rf["prob_h"], args.cpu_steps, sorted({1, nall}))
+        cs = cpu_baseline(rf["hm"], rf["st"], rf["probs"]["mixed_h"], args.cpu_steps, sorted({1, nall}))
         c, c1 = cs[nall], cs[1]
         cpu = {"value": round(c["value"], 4), "unit": "RHS evals/s", "cores": nall, "kind": "port",
                "sample": f"{c['steps']} accepted steps of the 20,000-triangle H-matrix run, FP64 C restatement "
-                         f"of main_H / HmatSolver on the same blocks, {nall} OpenMP threads "
-                         f"({c['elapsed']:.1f}s; one thread: {c1['value']:.3f} RHS evals/s, "
-                         f"{c1['elapsed']:.1f}s)",
+                         f"of main_H / HmatSolver on the headline operator's stored blocks, {nall} OpenMP threads "
+                         f"({c['elapsed']:.1f}s; one thread: {c1['value']:.3f} RHS evals/s, {c1['elapsed']:.1f}s)",
                "single_thread": {"value": round(c1["value"], 4), "cores": 1}}
         log(f"[bench] CPU baseline {c['value']:.3f} RHS evals/s on {nall} threads, {c1['value']:.3f} on one")
-    rf["h32"].close()
-    rk = rf["rk"]
+    hi = rf["hinfo"]["mixed_h"]
     out = {
         "metric": METRIC,
         "value": round(mixed["evals_per_s"], 2),
@@ -415,40 +548,53 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64 state/solve/accumulate; operator storage fp32 + fp16 tail (H), plain f64",
         "data": "synthetic (seeded rough fault, back-slip loading)",
-        "config": {"workload": "rough_fault_20000", "n_elements": rf["n"], "operator": "H-matrix, FP32 storage",
-                   "leaf": 64, "dist_diam_ratio": args.crit, "aca_tol": args.aca_tol, "rank_cap": 1024,
-                   "plain": "dense FP64", "steps_timed": args.steps},
+        "config": {"workload": "rough_fault_20000", "n_elements": rf["n"],
+                   "operator": "H-matrix: FP32 dense blocks and leading low-rank components, FP16 trailing "
+                               "components within each block's ACA budget, FP64 accumulation",
+                   "leaf": 64, "admissibility_eta": 2, "dist_diam_ratio": args.crit, "aca_tol": args.aca_tol,
+                   "rank_cap": 1024, "plain": "dense FP64", "steps_timed": args.steps},
         "fp64_plain": {"rhs_evals_per_s": round(plain["evals_per_s"], 2),
                        "sim_years_per_hour": round(plain["sim_years_per_hour"], 4),
-                       "coupling_us": round(plain["coupling_ms"] * 1e3, 2)},
+                       "coupling_us": round(plain["coupling_ms"] * 1e3, 2),
+                       "coupling_hbm_frac": round(rf["d64_bytes"] / (plain["coupling_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                                  4)},
         "mixed": {"rhs_evals_per_s": round(mixed["evals_per_s"], 2),
                   "sim_years_per_hour": round(mixed["sim_years_per_hour"], 4),
                   "phase2_kernel_us": round(mixed["coupling_ms"] * 1e3, 2),
-                  "h_product_us": {"phase1": round(ph1 * 1e3, 2), "phase2": round(ph2 * 1e3, 2),
-                                   "total": round(tot * 1e3, 2)},
-                  "stored_entries": rf["info"]["stored_entries"],
-                  "bytes_per_product": rf["info"]["bytes_per_product"],
-                  "mean_rank_shear": round(float(rk["rank_S"][rk["kind_S"] == 1].mean()), 2),
-                  "mean_rank_normal": round(float(rk["rank_N"][rk["kind_N"] == 1].mean()), 2)},
-        "fp32_dense": {"rhs_evals_per_s": round(res["fp32_dense"]["evals_per_s"], 2)},
+                  "h_product_us": dict(zip(("phase1", "phase2", "total"), rf["phases"]["mixed_h"])),
+                  "stored_entries": hi["stored_entries"], "bytes_per_product": hi["bytes_per_product"],
+                  "bytes_by_part": hb},
         "speedup_mixed_vs_fp64": round(mixed["evals_per_s"] / plain["evals_per_s"], 3),
-        "event_time_rel_err": ev_err,
-        "event_time_rel_err_fp32_dense": ev_ref,
-        "first_event_s": {"fp64_plain": t64, "mixed_h": th, "fp32_dense": t32},
-        "divergence_from_fp64": {"mixed_h": divergence(mixed["trace"], plain["trace"]),
-                                 "fp32_dense": divergence(res["fp32_dense"]["trace"], plain["trace"])},
+        "compared": {k: {"rhs_evals_per_s": round(res[k]["evals_per_s"], 2),
+                         "bytes_per_product": rf["hinfo"][k]["bytes_per_product"] if k in rf["hinfo"] else None,
+                         "h_product_us": rf["phases"].get(k)} for k in res},
+        "event_time_rel_err": acc["mixed_h"]["rel_err"],
+        "event_time": acc,
+        "mixed_over_duration": {"years": full["sim_seconds"] / YEAR, "steps": full["steps"],
+                                "wall_s": round(full["wall_s"], 3),
+                                "rhs_evals_per_s": round(full["rhs_evals_per_s"], 1),
+                                "sim_years_per_hour": round(full["sim_years_per_hour"], 2),
+                                "events": len(full["onsets"]), "retry_passes": full["retries"]},
+        "fp64_to_first_event": {"years": base["sim_seconds"] / YEAR, "steps": base["steps"],
+                                "wall_s": round(base["wall_s"], 3), "rhs_evals_per_s": round(base["rhs_evals_per_s"], 1),
+                                "sim_years_per_hour": round(base["sim_years_per_hour"], 2)},
         "operator_rel_err": rf["acc"],
-        "bf16_far_h": rf["bf16_far"],
         "h_build": rf["builds"],
         "network_8192": net,
+        "single_fault_512": single,
         "ensemble_256": ens,
         "adjoint_4096": adj,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic()},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
+                     "kernel": "k_hrow_mixed<fp16_t> (H product phase 2)",
+                     "bytes": "stored panel entries only", "bytes_per_launch": hb["phase2_entries"],
+                     "index_and_gather_bytes": hb["phase2_index"] + hb["phase2_gather"]},
         "cpu_baseline": cpu,
     }
+    for o in rf["ops"].values():
+        o.close()
     print(json.dumps(out), flush=True)
 
 

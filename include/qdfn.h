@@ -238,6 +238,10 @@ void qdfn_hmat_destroy(qdfn_hmat_t* h);
  * (y = panels [x; t]) kernels. */
 int qdfn_hmat_info(qdfn_hmat_t* h, int64_t* n, int64_t* nblk, int64_t* device_bytes, int64_t* stored_entries,
                    int64_t* phase_bytes);
+/* Algorithmic bytes of one product by part: out[0] phase-1 stored entries,
+ * out[1] phase-1 x slices and t rows, out[2] phase-2 stored entries, out[3]
+ * phase-2 index list, out[4] phase-2 gathered right-hand side, out[5] outputs. */
+int qdfn_hmat_bytes(qdfn_hmat_t* h, int64_t* out);
 /* Per block: kind and rank of the shear and normal parts (Ranks_Shear/Ranks_Normal). */
 int qdfn_hmat_get_ranks(qdfn_hmat_t* h, int32_t* kind_S, int64_t* rank_S, int32_t* kind_N, int64_t* rank_N);
 /* One block's stored factors (which = 0 shear, 1 normal): dense m x k into a,
@@ -348,6 +352,26 @@ int qdfn_sim_set_external(qdfn_sim_t* sim, int64_t nt, const double* times, cons
  * kernel (fills coupling_ms_avg; for an H operator the phase-2 kernel).
  * Returns QDFN_OK or the run's status. */
 int qdfn_sim_run(qdfn_sim_t* sim, int64_t n_steps, int32_t mode, int32_t graph_attempts);
+/* Record callback of qdfn_sim_run_until: records first .. first + count - 1
+ * of the run (counted from the first record handed over), as host arrays
+ * laid out like the reference's History_* (time[count], dt[count]; V, Disp,
+ * Theta, Sigma count x n column-major), valid during the call.  Return 0 to
+ * continue, nonzero to stop the run after this batch. */
+typedef int (*qdfn_record_cb)(void* user, int64_t first, int64_t count, const double* time, const double* dt,
+                              const double* V, const double* Disp, const double* Theta, const double* Sigma);
+/* main()'s TotalStep loop with its periodic save (Functions_RSFDFN3DMain_D.jl:
+ * 146-297, main_H the same): advance until TOld reaches t_end (> 0; the step
+ * that crosses it is completed) or max_steps accepted steps, replaying
+ * captured attempts (graph_attempts per replay).  A record is taken every
+ * record_step steps as in the reference; with cb, the run's history buffer
+ * (record_capacity records, at least graph_attempts / record_step + 1) is a
+ * staging area handed to cb whenever the next replay could overflow it and at
+ * the end, so runs of any length keep every record (the reference's
+ * save(SaveResultFileName, ...) every SaveStep steps).  cb = NULL keeps the
+ * records on the device up to record_capacity.  Returns QDFN_OK or the run's
+ * status. */
+int qdfn_sim_run_until(qdfn_sim_t* sim, double t_end, int64_t max_steps, int32_t graph_attempts, qdfn_record_cb cb,
+                       void* user);
 int qdfn_sim_get_stats(qdfn_sim_t* sim, qdfn_sim_stats* st);
 /* Current state (after the last accepted step): V, Disp, Theta, Friction,
  * EffNormalStress, Accel (each length n, any may be NULL). */

@@ -86,6 +86,10 @@ struct qdfn_hmat {
     int64_t device_bytes = 0;
     int64_t phase_bytes = 0;              // HBM bytes one product reads and writes
     int64_t alg_bytes1 = 0, alg_bytes2 = 0; // algorithmic bytes of the phase-1 / phase-2 main kernels
+    // their parts: stored entries (unpadded), x slices and t (phase 1); stored
+    // entries, index list (4 B per column), gathered right-hand side (8 B per
+    // column), outputs (phase 2)
+    int64_t p1_entries = 0, p1_other = 0, p2_entries = 0, p2_index = 0, p2_gather = 0, p2_out = 0;
     hipStream_t stream = nullptr;
     // construction statistics
     int64_t build_entries = 0, build_steps = 0, build_sampled = 0, build_fallback = 0;

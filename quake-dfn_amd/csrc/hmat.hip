@@ -831,7 +831,7 @@ int hmat_layout(qdfn_hmat* h) {
     std::vector<VTask> vtasks;
     std::vector<VFin> vfins;
     std::vector<double> tsc;  // scale of each t row (QDFN_PREC_MIXED)
-    int64_t v_entries = 0, v_entries_lo = 0, v_cols = 0, vpart_total = 0;
+    int64_t v_entries = 0, v_entries_lo = 0, v_entries_lo_rows = 0, v_cols = 0, vpart_total = 0;
     int64_t part_total = 0;
     // tiles of row block rb (columns [0, K)) numbered from q into tl; returns the next q
     auto add_tiles = [&](int32_t rb, int32_t K, int32_t q, std::vector<HTile>& tl) {
@@ -987,6 +987,7 @@ int hmat_layout(qdfn_hmat* h) {
                 }
             }
             (glo ? v_entries_lo : v_entries) += rows * k;
+            if (glo) v_entries_lo_rows += rows;
             v_cols += k;
             tbase += rows;
         }
@@ -1218,7 +1219,13 @@ int hmat_layout(qdfn_hmat* h) {
         k2 += rbs[q].K;
     }
     // phase 1: V entries, each panel's x slice once, t
-    h->alg_bytes1 = v_entries * es + v_entries_lo * (int64_t)sizeof(uint16_t) + 8 * v_cols + 8 * h->n_t;
+    h->p1_entries = v_entries * es + v_entries_lo * (int64_t)sizeof(uint16_t);
+    h->p1_other = 8 * v_cols + 8 * h->n_t + (mixed ? 8 * v_entries_lo_rows : 0);
+    h->p2_entries = b2;
+    h->p2_index = 4 * k2;
+    h->p2_gather = 8 * k2;
+    h->p2_out = 8 * out_rows2;
+    h->alg_bytes1 = h->p1_entries + h->p1_other;
     h->alg_bytes2 = b2 + 12 * k2 + 8 * out_rows2;
     return QDFN_OK;
 }
@@ -1971,6 +1978,13 @@ extern "C" int qdfn_hmat_info(qdfn_hmat_t* h, int64_t* n, int64_t* nblk, int64_t
         phase_bytes[0] = h->alg_bytes1;
         phase_bytes[1] = h->alg_bytes2;
     }
+    return QDFN_OK;
+}
+
+extern "C" int qdfn_hmat_bytes(qdfn_hmat_t* h, int64_t* out) {
+    if (!h || !out) return fail(QDFN_ERR_ARG, "qdfn_hmat_bytes: null argument");
+    const int64_t v[6] = {h->p1_entries, h->p1_other, h->p2_entries, h->p2_index, h->p2_gather, h->p2_out};
+    std::memcpy(out, v, sizeof(v));
     return QDFN_OK;
 }
 

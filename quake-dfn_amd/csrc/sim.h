@@ -36,5 +36,6 @@ struct qdfn_sim {
     // timing of coupling kernels (mode 1)
     double coup_ms_sum = 0.0;
     int64_t coup_count = 0;
+    int64_t rec_flushed = 0;  // records already handed to a record callback
 };
 

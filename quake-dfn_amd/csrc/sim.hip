@@ -270,14 +270,46 @@ extern "C" int qdfn_sim_set_external(qdfn_sim_t* s, int64_t nt, const double* ti
     return QDFN_OK;
 }
 
-extern "C" int qdfn_sim_run(qdfn_sim_t* s, int64_t n_steps, int32_t mode, int32_t graph_attempts) {
-    if (!s || n_steps < 0) return fail(QDFN_ERR_ARG, "qdfn_sim_run: bad args");
+// Hands the records held on the device (s->h_st.rec of them) to cb as host
+// arrays and empties the device history; cb's return value is returned.
+static int flush_records(qdfn_sim* s, qdfn_record_cb cb, void* user, int* cb_rc) {
+    const int64_t cnt = s->h_st.rec, n = s->P.n;
+    *cb_rc = 0;
+    if (cnt == 0) return QDFN_OK;
+    std::vector<double> t(cnt), dt(cnt), V(cnt * n), D(cnt * n), TH(cnt * n), SIG(cnt * n);
+    int rc = qdfn_sim_get_history(s, cnt, t.data(), dt.data(), V.data(), D.data(), TH.data(), SIG.data());
+    if (rc) return rc;
+    *cb_rc = cb(user, s->rec_flushed, cnt, t.data(), dt.data(), V.data(), D.data(), TH.data(), SIG.data());
+    s->rec_flushed += cnt;
+    s->h_st.rec = 0;
+    QDFN_HIP(hipMemcpy(s->d_st, &s->h_st, sizeof(SimState), hipMemcpyHostToDevice));
+    return QDFN_OK;
+}
+
+// Advance up to n_steps accepted steps, or until TOld reaches t_end (> 0).
+// With cb, the device history is a staging area handed to cb whenever the
+// next batch of attempts could overflow it, and at the end.
+static int run_impl(qdfn_sim* s, const char* who, int64_t n_steps, double t_end, int32_t mode,
+                    int32_t graph_attempts, qdfn_record_cb cb, void* user) {
     if (n_steps == 0) return QDFN_OK;
     QDFN_HIP(hipMemcpy(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost));
-    if (s->h_st.status != QDFN_OK) return fail(s->h_st.status, "qdfn_sim_run: run is in an error state");
+    if (s->h_st.status != QDFN_OK) return fail(s->h_st.status, std::string(who) + ": run is in an error state");
+    if (t_end > 0.0 && s->h_st.told >= t_end) return QDFN_OK;
     s->h_st.target = s->h_st.step + n_steps;
+    s->h_st.t_end = t_end;
     s->h_st.done = 0;
     QDFN_HIP(hipMemcpy(s->d_st, &s->h_st, sizeof(SimState), hipMemcpyHostToDevice));
+    // records one batch of K attempts can add (each accepts at most one step)
+    const int64_t rstep = s->P.record_step, cap = s->P.record_capacity;
+    auto room_for = [&](int64_t attempts) { return rstep > 0 ? attempts / rstep + 1 : 0; };
+    int cb_rc = 0;
+    auto maybe_flush = [&](int64_t attempts, bool force) -> int {
+        if (!cb || rstep <= 0 || cb_rc) return QDFN_OK;
+        if (force || cap - s->h_st.rec < room_for(attempts)) return flush_records(s, cb, user, &cb_rc);
+        return QDFN_OK;
+    };
+    if (cb && rstep > 0 && cap < room_for(std::max(1, graph_attempts > 0 ? graph_attempts : 32)))
+        return fail(QDFN_ERR_ARG, std::string(who) + ": record_capacity must hold the records of one batch of attempts");
     s->started = true;
     // step setup for the first step of this call
     hipLaunchKernelGGL(k_setup, dim3(1), dim3(kSetupBlock), 0, s->stream, s->A, s->d_st);
@@ -311,6 +343,9 @@ extern "C" int qdfn_sim_run(qdfn_sim_t* s, int64_t n_steps, int32_t mode, int32_
             gs.attempts = K;
         }
         for (;;) {
+            int rc = maybe_flush(K, false);
+            if (rc) return rc;
+            if (cb_rc) break;
             QDFN_HIP(hipGraphLaunch(gs.exec, s->stream));
             QDFN_HIP(hipMemcpyAsync(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost, s->stream));
             QDFN_HIP(hipStreamSynchronize(s->stream));
@@ -323,6 +358,8 @@ extern "C" int qdfn_sim_run(qdfn_sim_t* s, int64_t n_steps, int32_t mode, int32_
         std::vector<int32_t> ran(K);
         int rc = QDFN_OK;
         for (;;) {
+            rc = maybe_flush(K, false);
+            if (rc || cb_rc) break;
             // K attempts, each with HIP events around its coupling kernel
             for (int k = 0; k < K && rc == QDFN_OK; ++k) rc = enqueue_attempt(s, k, ev[2 * k], ev[2 * k + 1]);
             if (rc) break;
@@ -335,13 +372,28 @@ extern "C" int qdfn_sim_run(qdfn_sim_t* s, int64_t n_steps, int32_t mode, int32_
         for (auto& e : ev) (void)hipEventDestroy(e);
         if (rc) return rc;
     }
+    {
+        int rc = maybe_flush(0, true);
+        if (rc) return rc;
+    }
     if (s->h_st.status != QDFN_OK) {
         const char* what = s->h_st.status == QDFN_ERR_RETRIES
                                ? "more than 100 retries in one step (reference blocks in readline)"
                                : "empty candidate set for min |V/Accel|";
-        return fail(s->h_st.status, std::string("qdfn_sim_run: ") + what);
+        return fail(s->h_st.status, std::string(who) + ": " + what);
     }
     return QDFN_OK;
+}
+
+extern "C" int qdfn_sim_run(qdfn_sim_t* s, int64_t n_steps, int32_t mode, int32_t graph_attempts) {
+    if (!s || n_steps < 0) return fail(QDFN_ERR_ARG, "qdfn_sim_run: bad args");
+    return run_impl(s, "qdfn_sim_run", n_steps, 0.0, mode, graph_attempts, nullptr, nullptr);
+}
+
+extern "C" int qdfn_sim_run_until(qdfn_sim_t* s, double t_end, int64_t max_steps, int32_t graph_attempts,
+                                  qdfn_record_cb cb, void* user) {
+    if (!s || max_steps < 0 || !(t_end > 0.0)) return fail(QDFN_ERR_ARG, "qdfn_sim_run_until: bad args");
+    return run_impl(s, "qdfn_sim_run_until", max_steps, t_end, 0, graph_attempts, cb, user);
 }
 
 extern "C" int qdfn_sim_get_stats(qdfn_sim_t* s, qdfn_sim_stats* o) {

@@ -73,6 +73,9 @@ class HmatParams(C.Structure):
 
 PI64 = C.POINTER(I64)
 PPD = C.POINTER(PD)
+# int cb(void* user, int64_t first, int64_t count, const double* time, const double* dt, const double* V,
+#        const double* Disp, const double* Theta, const double* Sigma)
+RECORD_CB = C.CFUNCTYPE(C.c_int, VP, I64, I64, PD, PD, PD, PD, PD, PD)
 
 # name -> (restype, argtypes); mirrors include/qdfn.h
 SIGNATURES = {
@@ -109,6 +112,7 @@ SIGNATURES = {
     "qdfn_hmat_get_block": (C.c_int, [VP, I64, I32, PD, PD]),
     "qdfn_hmat_get_block_stored": (C.c_int, [VP, I64, I32, PD, PD]),
     "qdfn_hmat_get_split": (C.c_int, [VP, PI64, PI64]),
+    "qdfn_hmat_bytes": (C.c_int, [VP, PI64]),
     "qdfn_hmat_kself": (C.c_int, [VP, PD, PD]),
     "qdfn_hmat_normal_zero": (C.c_int, [VP]),
     "qdfn_hmat_matvec": (C.c_int, [VP, PD, PD, PD, I32]),
@@ -117,6 +121,7 @@ SIGNATURES = {
     "qdfn_sim_create_hmat": (VP, [VP, C.POINTER(SimParams)] + [PD] * 9),
     "qdfn_sim_set_external": (C.c_int, [VP, I64, PD, PD, PD]),
     "qdfn_sim_run": (C.c_int, [VP, I64, I32, I32]),
+    "qdfn_sim_run_until": (C.c_int, [VP, D, I64, I32, VP, VP]),
     "qdfn_sim_get_stats": (C.c_int, [VP, C.POINTER(SimStats)]),
     "qdfn_sim_get_state": (C.c_int, [VP, PD, PD, PD, PD, PD, PD]),
     "qdfn_sim_get_history": (C.c_int, [VP, I64, PD, PD, PD, PD, PD, PD]),

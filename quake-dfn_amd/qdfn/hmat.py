@@ -151,6 +151,13 @@ class HMatrix:
         return {"n": n.value, "blocks": nb.value, "bytes_per_product": db.value, "stored_entries": se.value,
                 "phase1_bytes": pb[0], "phase2_bytes": pb[1]}
 
+    def bytes_by_part(self):
+        """Algorithmic bytes of one product by part (qdfn_hmat_bytes)."""
+        out = np.zeros(6, dtype=np.int64)
+        check(lib().qdfn_hmat_bytes(self.handle, out.ctypes.data_as(PI64)))
+        keys = ("phase1_entries", "phase1_x_t", "phase2_entries", "phase2_index", "phase2_gather", "outputs")
+        return dict(zip(keys, (int(v) for v in out)))
+
     def build_info(self):
         """Entries evaluated, sampled-approximation steps, parts finished by
         sampling / evaluated in full, build wall time (ms)."""

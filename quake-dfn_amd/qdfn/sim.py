@@ -158,6 +158,35 @@ class Simulation:
             raise _lib.QdfnError(rc, _lib.last_error())
         return self.stats()
 
+    def run_until(self, t_end, max_steps, on_records=None, graph_attempts=32):
+        """Advance until TOld reaches t_end seconds or max_steps accepted steps
+        (qdfn_sim_run_until).  on_records(first, hist) receives every batch of
+        records as a dict of History_Time / History_Dt / History_V /
+        History_Disp / History_Theta / History_NormalStress arrays (records x n)
+        and may return True to stop the run."""
+        cb = None
+        if on_records is not None:
+            n = self.n
+
+            def _cb(user, first, count, t, dt, V, D, TH, SIG):
+                try:
+                    get = lambda p, k: np.ctypeslib.as_array(p, shape=(k,)).copy()
+                    mat = lambda p: get(p, count * n).reshape((count, n), order="F")
+                    hist = {"History_Time": get(t, count), "History_Dt": get(dt, count), "History_V": mat(V),
+                            "History_Disp": mat(D), "History_Theta": mat(TH), "History_NormalStress": mat(SIG)}
+                    return 1 if on_records(int(first), hist) else 0
+                except Exception:  # a Python error stops the run instead of crossing the C boundary
+                    self._cb_error = True
+                    return 1
+            cb = _lib.RECORD_CB(_cb)
+        self._cb_error = False
+        rc = lib().qdfn_sim_run_until(self._h, float(t_end), int(max_steps), int(graph_attempts), cb, None)
+        if rc < 0:
+            raise _lib.QdfnError(rc, _lib.last_error())
+        if self._cb_error:
+            raise RuntimeError("the record callback raised; the run was stopped")
+        return self.stats()
+
     def stats(self):
         s = SimStats()
         check(lib().qdfn_sim_get_stats(self._h, s))
