@@ -100,7 +100,11 @@ typedef struct qdfn_dense qdfn_dense_t;
 
 /* Upload K_S, K_N (n x n, leading dimension ld) and build the operator of
  * scripts/Functions_RSFDFN3DMain_D.jl:54-63: E_ij = K_S,ij/K_S,ii (i != j),
- * E_ii = 0, K_Self = |K_S,ii|, stored at precision prec. */
+ * E_ii = 0, K_Self = |K_S,ii|, stored at precision prec.  FP64 operators of
+ * 6,144 elements or more are stored split-tiled for the coupling kernel (each
+ * workgroup's rows x columns contiguous), the others column-major; the
+ * environment variable QDFN_DENSE_TILED=0/1 overrides.  Products are the
+ * same bit for bit either way. */
 qdfn_dense_t* qdfn_dense_create(const double* KS, const double* KN, int64_t n, int64_t ld, int32_t prec);
 /* Assemble the stiffness of n rectangular segments on the device (no host
  * round trip) and build the operator as above. */

@@ -324,16 +324,16 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_elem(AdjArgs A, double t_old,
 // ld_prev[j] += sum_i (-E[i, j] ltl[i] + K_N[i, j] lsig[i]): one workgroup per column
 template <class T>
 __global__ __launch_bounds__(256) void k_adj_transpose(const T* __restrict__ E, const T* __restrict__ KN, int64_t ld,
-                                                       int64_t n, int32_t planar, const double* __restrict__ ltl,
+                                                       int64_t n, int32_t planar, int32_t tiled, DenseTiles lay,
+                                                       const double* __restrict__ ltl,
                                                        const double* __restrict__ lsig, double* __restrict__ ld_prev) {
     __shared__ double sh[4];
     const int64_t j = blockIdx.x;
-    const T* ce = E + j * ld;
-    const T* ck = KN + j * ld;
     double s = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += 256) {
-        s += -(double)ce[i] * ltl[i];
-        if (!planar) s += (double)ck[i] * lsig[i];
+        const int64_t o = tiled ? lay.index(i, j) : i + j * ld;
+        s += -(double)E[o] * ltl[i];
+        if (!planar) s += (double)KN[o] * lsig[i];
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
@@ -538,10 +538,12 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
         hipLaunchKernelGGL(k_adj_elem, dim3(gb), dim3(kAdjBlock), 0, strm, A, h_time[k], h_dt[k]);
         if (op->prec == QDFN_PREC_FP64)
             hipLaunchKernelGGL(k_adj_transpose<double>, dim3((unsigned)n), dim3(256), 0, strm, op->E.as<double>(),
-                               op->KN.as<double>(), op->ld, n, op->normal_zero, A.ltl, A.lsig, A.ld_prev);
+                               op->KN.as<double>(), op->ld, n, op->normal_zero, op->tiled, dense_tiles(op), A.ltl,
+                               A.lsig, A.ld_prev);
         else
             hipLaunchKernelGGL(k_adj_transpose<float>, dim3((unsigned)n), dim3(256), 0, strm, op->E.as<float>(),
-                               op->KN.as<float>(), op->ld, n, op->normal_zero, A.ltl, A.lsig, A.ld_prev);
+                               op->KN.as<float>(), op->ld, n, op->normal_zero, op->tiled, dense_tiles(op), A.ltl,
+                               A.lsig, A.ld_prev);
         hipLaunchKernelGGL(k_adj_scalar, dim3(1), dim3(64), 0, strm, A, nblk, retry_factor[k]);
         QDFN_HIP(hipGetLastError());
         cur ^= 1;

@@ -1,8 +1,8 @@
 // Dense coupling operator: device layout and launch helpers.
 //
-// HBM layout: E (zero-diagonal K_S,ij/K_S,ii) and K_N are stored
-// column-major with a padded leading dimension ld = round_up(n, 64), in FP64
-// or FP32.  The coupling kernel streams both matrices once per RHS
+// HBM layout: E (zero-diagonal K_S,ij/K_S,ii) and K_N, ld = round_up(n, 64)
+// rows, in FP64 or FP32, column-major or (large operators) split-tiled
+// (DenseTiles).  The coupling kernel streams both matrices once per RHS
 // evaluation, each lane owning 2 (FP64) or 4 (FP32) consecutive rows and
 // reading them with one 16-byte load per column, so a wavefront moves a
 // contiguous 1 KiB per load instruction.  The column range is split across
@@ -18,6 +18,7 @@ struct qdfn_dense {
     int32_t normal_zero = 0;
     int32_t splits = 1;
     int64_t cols_per_split = 0;
+    int32_t tiled = 0;          // 1: split-tiled layout (qdfn::DenseTiles), 0: column-major ld x n
     qdfn::DevBuf E, KN;         // ld x n, element type by prec
     qdfn::DevBuf kself, diag;   // n doubles: |K_ii| and signed K_ii
     qdfn::DevBuf partS, partN;  // splits x ld doubles
@@ -27,6 +28,27 @@ struct qdfn_dense {
 };
 
 namespace qdfn {
+
+// Split-tiled layout of an ld x n operator (FP64 operators of at least
+// kTiledMinN elements; 8,192 elements: coupling 168 -> 158 us): the columns are cut into the coupling kernel's splits
+// of cps columns and the rows into its row blocks of rb_rows rows; the tile of
+// (row block b, split s) holds rows [b rb_rows, +rows) x columns [s cps, +w)
+// column-major with leading dimension rows, and the tiles follow each other
+// split by split, row block by row block.  A coupling workgroup then streams
+// one contiguous region instead of w columns ld entries apart.
+constexpr int64_t kTiledMinN = 6144;
+struct DenseTiles {
+    int64_t ld, n, cps, rb_rows;
+    __host__ __device__ int64_t index(int64_t i, int64_t j) const {
+        const int64_t j0 = (j / cps) * cps, w = (n - j0 < cps) ? n - j0 : cps;
+        const int64_t r0 = (i / rb_rows) * rb_rows, rows = (ld - r0 < rb_rows) ? ld - r0 : rb_rows;
+        return j0 * ld + r0 * w + (j - j0) * rows + (i - r0);
+    }
+};
+inline DenseTiles dense_tiles(const qdfn_dense* op) {
+    const int64_t R = op->prec == QDFN_PREC_FP64 ? 2 : 4;
+    return DenseTiles{op->ld, op->n, op->cols_per_split, 256 * R};
+}
 
 // Optional device-side control of a coupling launch:
 //   gate   - when non-null and *gate == 0 the kernel returns at once (the

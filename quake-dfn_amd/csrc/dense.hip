@@ -6,6 +6,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <string>
 
@@ -34,7 +35,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_coupling(const T* __restrict__ E, const T* __restrict__ KN, int64_t ld,
                                                   int64_t n, int64_t cps, const double* __restrict__ xbase,
                                                   double* __restrict__ partS, double* __restrict__ partN,
-                                                  int planar, CouplingGate g) {
+                                                  int planar, int tiled, CouplingGate g) {
     const int run = g.gate ? ctrl_load(g.gate) : 1;
     if (g.ran_log && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) g.ran_log[g.slot] = run;
     if (!run) return;
@@ -42,23 +43,28 @@ __global__ __launch_bounds__(256) void k_coupling(const T* __restrict__ E, const
     using P = RowPack<T>;
     using V = typename P::V;
     constexpr int R = P::R;
-    const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) * R;
+    const int64_t r0 = (int64_t)blockIdx.x * 256 * R;
+    const int64_t row = r0 + (int64_t)threadIdx.x * R;
     const int64_t j0 = (int64_t)blockIdx.y * cps;
     const int64_t j1 = (j0 + cps < n) ? j0 + cps : n;
     if (row >= ld) return;
     double aS[R], aN[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) aS[r] = aN[r] = 0.0;
-    const T* pe = E + row + j0 * ld;
-    const T* pk = KN + row + j0 * ld;
+    // column stride: ld (column-major) or the tile's row count (DenseTiles)
+    const int64_t rows = (ld - r0 < 256 * R) ? ld - r0 : 256 * R;
+    const int64_t off = tiled ? j0 * ld + r0 * (j1 - j0) + (row - r0) : row + j0 * ld;
+    const int64_t cs = tiled ? rows : ld;
+    const T* pe = E + off;
+    const T* pk = KN + off;
     int64_t j = j0;
     if (!planar) {
         for (; j + kUnroll <= j1; j += kUnroll) {
             V ve[kUnroll], vk[kUnroll];
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
-                ve[u] = __builtin_nontemporal_load((const V*)(pe + u * ld));
-                vk[u] = __builtin_nontemporal_load((const V*)(pk + u * ld));
+                ve[u] = __builtin_nontemporal_load((const V*)(pe + u * cs));
+                vk[u] = __builtin_nontemporal_load((const V*)(pk + u * cs));
             }
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
@@ -69,8 +75,8 @@ __global__ __launch_bounds__(256) void k_coupling(const T* __restrict__ E, const
                     aN[r] = fma((double)vk[u][r], xj, aN[r]);
                 }
             }
-            pe += kUnroll * ld;
-            pk += kUnroll * ld;
+            pe += kUnroll * cs;
+            pk += kUnroll * cs;
         }
         for (; j < j1; ++j) {
             const V ve = __builtin_nontemporal_load((const V*)pe);
@@ -81,28 +87,28 @@ __global__ __launch_bounds__(256) void k_coupling(const T* __restrict__ E, const
                 aS[r] = fma((double)ve[r], xj, aS[r]);
                 aN[r] = fma((double)vk[r], xj, aN[r]);
             }
-            pe += ld;
-            pk += ld;
+            pe += cs;
+            pk += cs;
         }
     } else {
         for (; j + kUnroll <= j1; j += kUnroll) {
             V ve[kUnroll];
 #pragma unroll
-            for (int u = 0; u < kUnroll; ++u) ve[u] = __builtin_nontemporal_load((const V*)(pe + u * ld));
+            for (int u = 0; u < kUnroll; ++u) ve[u] = __builtin_nontemporal_load((const V*)(pe + u * cs));
 #pragma unroll
             for (int u = 0; u < kUnroll; ++u) {
                 const double xj = x[j + u];
 #pragma unroll
                 for (int r = 0; r < R; ++r) aS[r] = fma((double)ve[u][r], xj, aS[r]);
             }
-            pe += kUnroll * ld;
+            pe += kUnroll * cs;
         }
         for (; j < j1; ++j) {
             const V ve = __builtin_nontemporal_load((const V*)pe);
             const double xj = x[j];
 #pragma unroll
             for (int r = 0; r < R; ++r) aS[r] = fma((double)ve[r], xj, aS[r]);
-            pe += ld;
+            pe += cs;
         }
     }
     double* ps = partS + (int64_t)blockIdx.y * ld + row;
@@ -152,10 +158,10 @@ int launch_coupling(const qdfn_dense* op, const double* d_x, const CouplingGate&
     double* pn = part_n ? part_n : op->partN.as<double>();
     if (op->prec == QDFN_PREC_FP64)
         hipLaunchKernelGGL(k_coupling<double>, grid, dim3(256), 0, s, op->E.as<double>(), op->KN.as<double>(), op->ld,
-                           op->n, op->cols_per_split, d_x, ps, pn, op->normal_zero, g);
+                           op->n, op->cols_per_split, d_x, ps, pn, op->normal_zero, op->tiled, g);
     else
         hipLaunchKernelGGL(k_coupling<float>, grid, dim3(256), 0, s, op->E.as<float>(), op->KN.as<float>(), op->ld,
-                           op->n, op->cols_per_split, d_x, ps, pn, op->normal_zero, g);
+                           op->n, op->cols_per_split, d_x, ps, pn, op->normal_zero, op->tiled, g);
     QDFN_HIP(hipGetLastError());
     return QDFN_OK;
 }
@@ -186,7 +192,8 @@ __global__ void k_diag(const double* __restrict__ KS, int64_t ld, int64_t n, dou
 template <typename T>
 __global__ __launch_bounds__(256) void k_build(const double* __restrict__ KS, const double* __restrict__ KNin,
                                                int64_t ld, int64_t n, const double* __restrict__ diag,
-                                               T* __restrict__ E, T* __restrict__ KN, int* __restrict__ nz) {
+                                               T* __restrict__ E, T* __restrict__ KN, int* __restrict__ nz,
+                                               int tiled, DenseTiles lay) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
         int nonzero = 0;
@@ -198,25 +205,50 @@ __global__ __launch_bounds__(256) void k_build(const double* __restrict__ KS, co
                 k = KNin[idx];
                 nonzero = k != 0.0;
             }
-            E[idx] = (T)e;
-            KN[idx] = (T)k;
+            const int64_t o = tiled ? lay.index(i, j) : idx;
+            E[o] = (T)e;
+            KN[o] = (T)k;
         }
         if (__syncthreads_or(nonzero) && threadIdx.x == 0) atomicOr(nz, 1);
     }
 }
 
+// Column splits of the coupling kernel.  Each CU ends up with floor or ceil
+// of (workgroups / 256) of them, so the kernel runs for the ceiling:
+// workgroup counts just above a multiple of 256 leave most CUs idle at the end
+// (20,000 elements in FP64: 280 workgroups of 11.7 MB took 1,270 us, 2,080
+// workgroups 1,018 us; 8,192 elements: 256 workgroups 168 us, 320 took 190 us).
+// Among the split counts giving 1.5-8 MB per workgroup (smaller ones where
+// that leaves fewer than 256 workgroups), the one whose count wastes the
+// least of the last round is taken (ties: nearest 3.5 MB per workgroup).
 static int choose_splits(qdfn_dense* op) {
     const int R = op->prec == QDFN_PREC_FP64 ? 2 : 4;
     const int64_t rb = (op->ld + 256 * R - 1) / (256 * R);
-    int64_t target = 256;
-    if (const char* env = getenv("QDFN_COUPLING_WGS")) target = std::max<int64_t>(1, atoll(env));
-    int64_t splits = std::max<int64_t>(1, (target + rb - 1) / rb);
-    splits = std::min<int64_t>(splits, std::max<int64_t>(1, op->n / 32));
-    int64_t cps = (op->n + splits - 1) / splits;
-    cps = round_up(cps, kUnroll);
-    splits = (op->n + cps - 1) / cps;
-    op->splits = (int32_t)std::max<int64_t>(1, splits);
-    op->cols_per_split = cps;
+    const int64_t esz = op->prec == QDFN_PREC_FP64 ? 8 : 4;
+    const double total = 2.0 * esz * op->ld * op->n;
+    const int64_t smax = std::max<int64_t>(1, op->n / 32);
+    auto cps_of = [&](int64_t splits) { return round_up((op->n + splits - 1) / splits, kUnroll); };
+    int64_t best_cps = cps_of(std::min<int64_t>(smax, std::max<int64_t>(1, (256 + rb - 1) / rb)));
+    double best_waste = 1e30, best_dist = 1e30;
+    for (int64_t sp = 1; sp <= smax; ++sp) {
+        const int64_t cps = cps_of(sp), se = (op->n + cps - 1) / cps, W = rb * se;
+        const double per = total / (double)W;
+        if (per < std::min(1.5e6, 0.999 * total / 256.0) || per > 8e6) continue;
+        const int64_t rounds = (W + 255) / 256;
+        const double waste = (double)(rounds * 256 - W) / (double)W;
+        const double dist = std::fabs(std::log(per / 3.5e6));
+        if (waste < best_waste - 1e-12 || (waste <= best_waste + 1e-12 && dist < best_dist)) {
+            best_waste = waste;
+            best_dist = dist;
+            best_cps = cps;
+        }
+    }
+    if (const char* env = getenv("QDFN_COUPLING_WGS")) {
+        const int64_t target = std::max<int64_t>(1, atoll(env));
+        best_cps = cps_of(std::min<int64_t>(smax, std::max<int64_t>(1, (target + rb - 1) / rb)));
+    }
+    op->cols_per_split = best_cps;
+    op->splits = (int32_t)std::max<int64_t>(1, (op->n + best_cps - 1) / best_cps);
     return QDFN_OK;
 }
 
@@ -235,19 +267,22 @@ static int finish_build(qdfn_dense* op, double* d_KS, double* d_KN) {
     const size_t esz = op->prec == QDFN_PREC_FP64 ? sizeof(double) : sizeof(float);
     QDFN_HIP(op->E.alloc(esz * ld * n));
     QDFN_HIP(op->KN.alloc(esz * ld * n));
+    choose_splits(op);
+    op->tiled = (n >= kTiledMinN && op->prec == QDFN_PREC_FP64) ? 1 : 0;
+    if (const char* e = getenv("QDFN_DENSE_TILED")) op->tiled = atoi(e) ? 1 : 0;
+    const DenseTiles lay = dense_tiles(op);
     dim3 grid((unsigned)((ld + 255) / 256), (unsigned)std::min<int64_t>(n, 32768));
     if (op->prec == QDFN_PREC_FP64)
         hipLaunchKernelGGL(k_build<double>, grid, dim3(256), 0, s, d_KS, d_KN, ld, n, op->diag.as<double>(),
-                           op->E.as<double>(), op->KN.as<double>(), nz.as<int>());
+                           op->E.as<double>(), op->KN.as<double>(), nz.as<int>(), op->tiled, lay);
     else
         hipLaunchKernelGGL(k_build<float>, grid, dim3(256), 0, s, d_KS, d_KN, ld, n, op->diag.as<double>(),
-                           op->E.as<float>(), op->KN.as<float>(), nz.as<int>());
+                           op->E.as<float>(), op->KN.as<float>(), nz.as<int>(), op->tiled, lay);
     QDFN_HIP(hipGetLastError());
     int hnz = 0;
     QDFN_HIP(hipMemcpyAsync(&hnz, nz.p, sizeof(int), hipMemcpyDeviceToHost, s));
     QDFN_HIP(hipStreamSynchronize(s));
     op->normal_zero = hnz ? 0 : 1;
-    choose_splits(op);
     QDFN_HIP(op->partS.alloc(sizeof(double) * op->splits * ld));
     QDFN_HIP(op->partN.alloc(sizeof(double) * op->splits * ld));
     QDFN_HIP(op->x.alloc(sizeof(double) * ld));

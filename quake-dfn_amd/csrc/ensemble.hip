@@ -368,8 +368,9 @@ extern "C" qdfn_ensemble_t* qdfn_ensemble_create(int64_t m, qdfn_sim_t* const* s
     }
     for (int64_t k = 0; k < m; ++k) {
         const qdfn_sim* s = sims[k];
-        if (!s || !s->op || s->op->prec != QDFN_PREC_FP32) {
-            fail(QDFN_ERR_ARG, "qdfn_ensemble_create: every run must be a dense run with an FP32 operator");
+        if (!s || !s->op || s->op->prec != QDFN_PREC_FP32 || s->op->tiled) {
+            fail(QDFN_ERR_ARG, "qdfn_ensemble_create: every run must be a dense run with a column-major FP32 "
+                               "operator");
             return nullptr;
         }
         for (int64_t q = 0; q < k; ++q)
