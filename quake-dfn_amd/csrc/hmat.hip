@@ -390,9 +390,12 @@ __device__ inline void vpanel_out(const VTask& tk, int32_t row, double v, double
         vpart[tk.part + row] = v;
 }
 
-template <class T>
-__device__ __forceinline__ void vpanel_body(const VTask& tk, const T* __restrict__ pool, const double* __restrict__ x,
-                                            double* __restrict__ t, double* __restrict__ vpart) {
+// ONE: the task's columns fit one column step (nc == 1, most leaf-cluster
+// panels), so every row group multiplies the same x values: they are loaded
+// once and each step loads panel rows only.
+template <class T, bool ONE>
+__device__ __forceinline__ void vpanel_steps(const VTask& tk, const T* __restrict__ pool, const double* __restrict__ x,
+                                             double* __restrict__ t, double* __restrict__ vpart) {
     const int lane = threadIdx.x & 63;
     const int lpr = tk.lpr, rpw = 64 / lpr, sub = lane & (lpr - 1), rl = lane / lpr;
     const int32_t K = tk.K, K4 = (K + 3) / 4 * 4;
@@ -401,18 +404,24 @@ __device__ __forceinline__ void vpanel_body(const VTask& tk, const T* __restrict
     const int32_t total = nc * ng;
     const T* base = pool + tk.data_off;
     const double* xs = x + tk.xoff;
-    auto load = [&](int32_t it, VStep<T>& S) {
-        const int32_t gi = it / nc, ci = it - gi * nc;
-        const int32_t c = ci * cstep + 4 * sub;
+    // x values of column c + q (masked in the integer domain so that the load
+    // stays unconditional: a select lets the compiler move the load into a
+    // branch that waits for every load in flight)
+    auto xload = [&](int32_t c, double z[4]) {
         const int32_t cc = c < K4 ? c : 0;
-        // masked in the integer domain so that the load stays unconditional (a
-        // select lets the compiler move the load into a branch that waits for
-        // every load in flight)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const long long bits = __double_as_longlong(xs[min(cc + q, K - 1)]);
-            S.z[q] = __longlong_as_double((c < K4 && cc + q < K) ? bits : 0LL);
+            z[q] = __longlong_as_double((c < K4 && cc + q < K) ? bits : 0LL);
         }
+    };
+    double z1[4];
+    if (ONE) xload(4 * sub, z1);
+    auto load = [&](int32_t it, VStep<T>& S) {
+        const int32_t gi = ONE ? it : it / nc, ci = ONE ? 0 : it - gi * nc;
+        const int32_t c = ci * cstep + 4 * sub;
+        const int32_t cc = c < K4 ? c : 0;
+        if (!ONE) xload(c, S.z);
 #pragma unroll
         for (int r = 0; r < kVSlots; ++r) {
             const int32_t row = min(gi * gr + rl + r * rpw, tk.m - 1);
@@ -425,13 +434,14 @@ __device__ __forceinline__ void vpanel_body(const VTask& tk, const T* __restrict
     // accumulate step it; at a group's last column step, reduce over the row's
     // lanes, write and restart
     auto consume = [&](int32_t it, const VStep<T>& S) {
-        const int32_t gi = it / nc, ci = it - gi * nc;
+        const int32_t gi = ONE ? it : it / nc, ci = ONE ? 0 : it - gi * nc;
+        const double* z = ONE ? z1 : S.z;
 #pragma unroll
         for (int r = 0; r < kVSlots; ++r) {
-            acc[r] = fma((double)S.w[r][0], S.z[0], acc[r]);
-            acc[r] = fma((double)S.w[r][1], S.z[1], acc[r]);
-            acc[r] = fma((double)S.w[r][2], S.z[2], acc[r]);
-            acc[r] = fma((double)S.w[r][3], S.z[3], acc[r]);
+            acc[r] = fma((double)S.w[r][0], z[0], acc[r]);
+            acc[r] = fma((double)S.w[r][1], z[1], acc[r]);
+            acc[r] = fma((double)S.w[r][2], z[2], acc[r]);
+            acc[r] = fma((double)S.w[r][3], z[3], acc[r]);
         }
         if (ci == nc - 1) {
 #pragma unroll
@@ -451,6 +461,16 @@ __device__ __forceinline__ void vpanel_body(const VTask& tk, const T* __restrict
         load(min(it + 2, total - 1), A);
         if (it + 1 < total) consume(it + 1, B);
     }
+}
+
+template <class T>
+__device__ __forceinline__ void vpanel_body(const VTask& tk, const T* __restrict__ pool, const double* __restrict__ x,
+                                            double* __restrict__ t, double* __restrict__ vpart) {
+    const int32_t K4 = (tk.K + 3) / 4 * 4;
+    if (K4 <= 4 * tk.lpr)
+        vpanel_steps<T, true>(tk, pool, x, t, vpart);
+    else
+        vpanel_steps<T, false>(tk, pool, x, t, vpart);
 }
 
 template <class T>
@@ -474,10 +494,10 @@ struct VStep8 {
     double z[8];
 };
 
-template <class T16>
-__device__ __forceinline__ void vpanel16_body(const VTask& tk, const uint16_t* __restrict__ pool,
-                                              const double* __restrict__ x, double* __restrict__ t,
-                                              double* __restrict__ vpart, const double* __restrict__ tscale) {
+template <class T16, bool ONE>
+__device__ __forceinline__ void vpanel16_steps(const VTask& tk, const uint16_t* __restrict__ pool,
+                                               const double* __restrict__ x, double* __restrict__ t,
+                                               double* __restrict__ vpart, const double* __restrict__ tscale) {
     const int lane = threadIdx.x & 63;
     const int lpr = tk.lpr, rpw = 64 / lpr, sub = lane & (lpr - 1), rl = lane / lpr;
     const int32_t K = tk.K, K8 = (K + 7) / 8 * 8;
@@ -486,15 +506,21 @@ __device__ __forceinline__ void vpanel16_body(const VTask& tk, const uint16_t* _
     const int32_t total = nc * ng;
     const uint16_t* base = pool + tk.data_off;
     const double* xs = x + tk.xoff;
-    auto load = [&](int32_t it, VStep8& S) {
-        const int32_t gi = it / nc, ci = it - gi * nc;
-        const int32_t c = ci * cstep + 8 * sub;
+    auto xload = [&](int32_t c, double z[8]) {
         const int32_t cc = c < K8 ? c : 0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const long long bits = __double_as_longlong(xs[min(cc + q, K - 1)]);
-            S.z[q] = __longlong_as_double((c < K8 && cc + q < K) ? bits : 0LL);
+            z[q] = __longlong_as_double((c < K8 && cc + q < K) ? bits : 0LL);
         }
+    };
+    double z1[8];
+    if (ONE) xload(8 * sub, z1);
+    auto load = [&](int32_t it, VStep8& S) {
+        const int32_t gi = ONE ? it : it / nc, ci = ONE ? 0 : it - gi * nc;
+        const int32_t c = ci * cstep + 8 * sub;
+        const int32_t cc = c < K8 ? c : 0;
+        if (!ONE) xload(c, S.z);
 #pragma unroll
         for (int r = 0; r < kVSlots; ++r) {
             const int32_t row = min(gi * gr + rl + r * rpw, tk.m - 1);
@@ -506,14 +532,15 @@ __device__ __forceinline__ void vpanel16_body(const VTask& tk, const uint16_t* _
     for (int r = 0; r < kVSlots; ++r) acc[r] = 0.0;
     // entry 2j of a step is the low half of word j, entry 2j + 1 the high half
     auto consume = [&](int32_t it, const VStep8& S) {
-        const int32_t gi = it / nc, ci = it - gi * nc;
+        const int32_t gi = ONE ? it : it / nc, ci = ONE ? 0 : it - gi * nc;
+        const double* z = ONE ? z1 : S.z;
 #pragma unroll
         for (int r = 0; r < kVSlots; ++r) {
             const uint32_t wd[4] = {S.w[r].x, S.w[r].y, S.w[r].z, S.w[r].w};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                acc[r] = fma((double)Dec16<T16>::lo(wd[j]), S.z[2 * j], acc[r]);
-                acc[r] = fma((double)Dec16<T16>::hi(wd[j]), S.z[2 * j + 1], acc[r]);
+                acc[r] = fma((double)Dec16<T16>::lo(wd[j]), z[2 * j], acc[r]);
+                acc[r] = fma((double)Dec16<T16>::hi(wd[j]), z[2 * j + 1], acc[r]);
             }
         }
         if (ci == nc - 1) {
@@ -534,6 +561,17 @@ __device__ __forceinline__ void vpanel16_body(const VTask& tk, const uint16_t* _
         load(min(it + 2, total - 1), A);
         if (it + 1 < total) consume(it + 1, B);
     }
+}
+
+template <class T16>
+__device__ __forceinline__ void vpanel16_body(const VTask& tk, const uint16_t* __restrict__ pool,
+                                              const double* __restrict__ x, double* __restrict__ t,
+                                              double* __restrict__ vpart, const double* __restrict__ tscale) {
+    const int32_t K8 = (tk.K + 7) / 8 * 8;
+    if (K8 <= 8 * tk.lpr)
+        vpanel16_steps<T16, true>(tk, pool, x, t, vpart, tscale);
+    else
+        vpanel16_steps<T16, false>(tk, pool, x, t, vpart, tscale);
 }
 
 __global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restrict__ tasks, int32_t ntasks,
