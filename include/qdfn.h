@@ -407,10 +407,19 @@ void qdfn_sim_destroy(qdfn_sim_t* sim);
  * Functions_Solvers.jl:26-327): the implicit-function derivative of each
  * converged local solve, transposed coupling products, the Dt controller's
  * argmin and growth rules, with the forward's discrete choices held fixed.
- * It has no counterpart in the reference (which has no gradients). */
+ * It has no counterpart in the reference (which has no gradients).
+ * Validity: through a rupture the discrete step map amplifies perturbations
+ * (RuptureDt oscillator steps), and so does its adjoint: lambda_growth is the
+ * largest max|adjoint of Disp| over the backward sweep divided by its seed
+ * value, growth_step the forward step at which it first exceeded 1e8 (-1:
+ * never).  Interseismic windows and windows that end shortly after an onset
+ * stay below that; beyond it the gradient is that of a chaotic discrete path,
+ * which no difference quotient can check (verified: see the tests). */
 typedef struct {
     int64_t steps, output_mismatches, select_mismatches, onset_step, retried_steps;
     double t_first, ms_backward;
+    double lambda_growth;
+    int64_t growth_step;
 } qdfn_adjoint_info;
 int qdfn_sim_adjoint(qdfn_sim_t* sim, const double* v_i, const double* theta_i, int64_t n_obs, const int64_t* obs,
                      const double* d_target, double w_time, double t_target, double v_onset, double* loss,

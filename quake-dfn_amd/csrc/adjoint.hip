@@ -369,6 +369,21 @@ __global__ void k_adj_add(double* p, int64_t i, double v, AdjScal* sc, double la
     if (sc) *(volatile double*)&sc->lam_t_new += lam_t;
 }
 
+// out[0] = max_i |lam[i]| (one 1,024-thread workgroup; the adjoint sweep's growth record)
+__global__ __launch_bounds__(1024) void k_adj_maxabs(const double* __restrict__ lam, int64_t n, double* __restrict__ out) {
+    __shared__ double sh[16];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(lam[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sh[0];
+        for (int k = 1; k < 16; ++k) r = fmax(r, sh[k]);
+        out[0] = r;
+    }
+}
+
 // gradient of f0 = Friction_i - a ln(V_i / V0) - b ln(Theta_i V0 / Dc) into a, b, Dc
 __global__ __launch_bounds__(kAdjBlock) void k_adj_f0(AdjArgs A) {
     const int64_t i = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
@@ -477,7 +492,8 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
         seeds.push_back({0, -1, 0.0, 2.0 * w_time * r});
     }
     // device work space
-    DevBuf vec, grads, parts, scal, mm, init;
+    DevBuf vec, grads, parts, scal, mm, init, lnorm;
+    QDFN_HIP(lnorm.alloc(sizeof(double) * (K + 1)));
     const int64_t nblk = (n + kAdjBlock - 1) / kAdjBlock;
     if (vec.alloc(sizeof(double) * 15 * n) || grads.alloc(sizeof(double) * 4 * n) ||
         parts.alloc(sizeof(double) * 2 * nblk) || scal.alloc(sizeof(AdjScal)) || mm.alloc(sizeof(int32_t)) ||
@@ -513,6 +529,8 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
     A.ext_at_told = SA.ext_at_told; A.planar = SA.planar;
     // seed: adjoint of Disp after the last step
     QDFN_HIP(hipMemcpy(lbuf[0][1], lam_dK.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    double seed_norm = 0.0;
+    for (double v : lam_dK) seed_norm = std::max(seed_norm, std::fabs(v));
     hipEvent_t e0, e1;
     QDFN_HIP(hipEventCreate(&e0));
     QDFN_HIP(hipEventCreate(&e1));
@@ -545,6 +563,7 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
                                op->KN.as<float>(), op->ld, n, op->normal_zero, op->tiled, dense_tiles(op), A.ltl,
                                A.lsig, A.ld_prev);
         hipLaunchKernelGGL(k_adj_scalar, dim3(1), dim3(64), 0, strm, A, nblk, retry_factor[k]);
+        hipLaunchKernelGGL(k_adj_maxabs, dim3(1), dim3(1024), 0, strm, A.ld_prev, n, lnorm.as<double>() + k);
         QDFN_HIP(hipGetLastError());
         cur ^= 1;
     }
@@ -563,6 +582,17 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
     int32_t mism = 0;
     QDFN_HIP(hipMemcpy(&hs, A.sc, sizeof(AdjScal), hipMemcpyDeviceToHost));
     QDFN_HIP(hipMemcpy(&mism, A.mismatch, sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<double> hn(K);
+    QDFN_HIP(hipMemcpy(hn.data(), lnorm.p, sizeof(double) * K, hipMemcpyDeviceToHost));
+    // growth of the Disp adjoint over the backward sweep (k = K - 1 down to 0)
+    double growth = 0.0;
+    int64_t growth_step = -1;
+    const double ref = seed_norm > 0.0 ? seed_norm : 1.0;
+    for (int64_t k = K - 1; k >= 0; --k) {
+        const double gk = hn[k] / ref;
+        if (gk > growth || std::isnan(gk)) growth = std::isnan(gk) ? INFINITY : gk;
+        if (growth_step < 0 && !(gk <= 1e8)) growth_step = k;
+    }
     *loss = L;
     if (info) {
         memset(info, 0, sizeof(*info));
@@ -573,6 +603,8 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
         info->retried_steps = retried;
         info->t_first = t_first;
         info->ms_backward = ms;
+        info->lambda_growth = growth;
+        info->growth_step = growth_step;
     }
     return QDFN_OK;
 }

@@ -135,7 +135,9 @@ end
 # H operator from the reference's ShearStiffness_H / NormalStiffness_H (RUN2_BUILD_INPUT.jl:362):
 # dense blocks pass as they are; a LowRankApprox PartialQR F (A[:, F[:p]] ~ F[:Q] F[:R]) passes as
 # U = F[:Q] and V with V[:, F[:p]] = F[:R].  prec: 0 FP64, 1 FP32 storage, 2 BF16 low-rank
-# factors with FP32 dense blocks (QDFN_PREC_BF16_FAR; FP64 accumulation in every case).
+# factors with FP32 dense blocks (QDFN_PREC_BF16_FAR), 3 FP32 with the trailing components of each
+# recompressed low-rank block as FP16 unit vectors within the block's ACA budget (QDFN_PREC_MIXED;
+# imported blocks are not recompressed and stay FP32); FP64 accumulation in every case.
 function HmatCreate(FaultCount, ElementRange_SR, Admissible, ShearStiffness_H, NormalStiffness_H; prec=0)
     nb = size(ElementRange_SR, 1)
     rng = Matrix{Int64}(permutedims(ElementRange_SR))          # 4 x nb = nb x 4 row-major
@@ -283,6 +285,32 @@ SimRun(sim, n_steps; mode=0, graph_attempts=32) =
     check(ccall((:qdfn_sim_run, LIB), Cint, (Ptr{Cvoid}, Int64, Int32, Int32), sim, n_steps, Int32(mode),
                 Int32(graph_attempts)))
 
+# main()'s loop to a simulated time with its periodic save (Functions_RSFDFN3DMain_D.jl:265-297):
+# on_records(first, History_Time, History_Dt, History_V, History_Disp, History_Theta,
+# History_NormalStress) receives every batch of records (copies); return true to stop.
+function _record_trampoline(user::Ptr{Cvoid}, first::Int64, count::Int64, t::Ptr{Float64}, dt::Ptr{Float64},
+                            V::Ptr{Float64}, D::Ptr{Float64}, Th::Ptr{Float64}, S::Ptr{Float64})::Cint
+    f = unsafe_pointer_to_objref(user)::Base.RefValue{Any}
+    n = f[][2]
+    w(p, k) = copy(unsafe_wrap(Array, p, k))
+    m(p) = reshape(w(p, count * n), count, n)
+    stop = f[][1](first, w(t, count), w(dt, count), m(V), m(D), m(Th), m(S))
+    return stop === true ? Cint(1) : Cint(0)
+end
+
+function SimRunUntil(sim, t_end, max_steps, n; on_records=nothing, graph_attempts=32)
+    if on_records === nothing
+        return check(ccall((:qdfn_sim_run_until, LIB), Cint, (Ptr{Cvoid}, Float64, Int64, Int32, Ptr{Cvoid}, Ptr{Cvoid}),
+                           sim, t_end, max_steps, Int32(graph_attempts), C_NULL, C_NULL))
+    end
+    box = Ref{Any}((on_records, n))
+    cb = @cfunction(_record_trampoline, Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                                               Ptr{Float64}, Ptr{Float64}, Ptr{Float64}))
+    GC.@preserve box check(ccall((:qdfn_sim_run_until, LIB), Cint,
+                                 (Ptr{Cvoid}, Float64, Int64, Int32, Ptr{Cvoid}, Ptr{Cvoid}),
+                                 sim, t_end, max_steps, Int32(graph_attempts), cb, pointer_from_objref(box)))
+end
+
 # History_Time, History_Dt, History_V, History_Disp, History_Theta, History_NormalStress (count x n).
 function SimHistory(sim, count, n)
     t = zeros(count); dt = zeros(count); V = zeros(count, n); D = zeros(count, n); Th = zeros(count, n)
@@ -314,6 +342,7 @@ EnsembleDestroy(e) = ccall((:qdfn_ensemble_destroy, LIB), Cvoid, (Ptr{Cvoid},), 
 struct AdjointInfo
     steps::Int64; output_mismatches::Int64; select_mismatches::Int64; onset_step::Int64; retried_steps::Int64
     t_first::Float64; ms_backward::Float64
+    lambda_growth::Float64; growth_step::Int64
 end
 
 function SimAdjoint(sim, V_i, Theta_i, obs, target; w_time=0.0, t_target=0.0, v_onset=1e-3)

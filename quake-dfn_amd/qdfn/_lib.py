@@ -58,7 +58,8 @@ class SimStats(C.Structure):
 
 class AdjointInfo(C.Structure):
     _fields_ = [("steps", I64), ("output_mismatches", I64), ("select_mismatches", I64), ("onset_step", I64),
-                ("retried_steps", I64), ("t_first", D), ("ms_backward", D)]
+                ("retried_steps", I64), ("t_first", D), ("ms_backward", D), ("lambda_growth", D),
+                ("growth_step", I64)]
 
 
 class HmatBuildStats(C.Structure):

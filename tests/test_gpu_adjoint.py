@@ -136,3 +136,94 @@ def test_adjoint_same_on_tiled_layout(qd, case, setup, monkeypatch):
         grads.append(g)
     for k in ("a", "b", "dc"):
         np.testing.assert_array_equal(grads[0][k], grads[1][k])
+
+
+def _dual_directional(od, prob, KS, KN, K, obs, target, delta, w_time=0.0, t_target=0.0, v_onset=1e-3):
+    """dL/ds along delta by the forward-mode (dual-number) oracle: the stand-in
+    for a ForwardDiff pass through the reference's CPU code."""
+    st, r = od.sim_dense(prob, KS, KN, total_step=K, record_step=1, seed=delta)
+    assert st == 0
+    D, dD = r["hist_disp"][-1], r["d_hist_disp"][-1]
+    res = D[obs] - target
+    L = float(np.sum(res ** 2))
+    dL = float(np.sum(2 * res * dD[obs]))
+    if w_time > 0:
+        v, dv = r["trace_vmax"], r["d_trace_vmax"]
+        te, dte = r["trace_told"] + r["trace_dt"], r["d_trace_told"] + r["d_trace_dt"]
+        k = int(np.nonzero(v >= v_onset)[0][0])
+        assert k >= 1
+        l0, l1, lt = np.log10(v[k - 1]), np.log10(v[k]), np.log10(v_onset)
+        dl0, dl1 = dv[k - 1] / (v[k - 1] * np.log(10)), dv[k] / (v[k] * np.log(10))
+        f = (lt - l0) / (l1 - l0)
+        df = (-dl0 * (l1 - l0) - (lt - l0) * (dl1 - dl0)) / (l1 - l0) ** 2
+        t = te[k - 1] + f * (te[k] - te[k - 1])
+        dt = dte[k - 1] + df * (te[k] - te[k - 1]) + f * (dte[k] - dte[k - 1])
+        L += w_time * (t - t_target) ** 2
+        dL += 2 * w_time * (t - t_target) * dt
+    return L, dL
+
+
+def test_adjoint_dot_product_with_dual_forward(qd, case, setup):
+    """<grad L, delta> from the device adjoint equals the forward-mode
+    derivative of the oracle's main() along delta (random direction in a, b and
+    Dc, each entry scaled by its parameter) to 1e-6 relative, on the
+    interseismic window (two directions)."""
+    from oracle import oracle_dual as od
+    prob, K = setup["prob"], setup["K"]
+    sim = qd.Simulation(setup["op"], prob, record_step=1, record_capacity=K)
+    sim.run(K)
+    L, g, info = sim.adjoint(prob["v_i"], prob["theta_i"], setup["obs"], setup["target"])
+    sim.close()
+    assert info["lambda_growth"] < 1e8 and info["growth_step"] == -1
+    for seed in (1, 2):
+        rng = np.random.default_rng(seed)
+        delta = {k: prob[k] * rng.standard_normal(prob["n"]) for k in ("a", "b", "dc")}
+        ad = sum(float(np.dot(g[k], delta[k])) for k in delta)
+        Ld, dd = _dual_directional(od, prob, case["KS"], case["KN"], K, setup["obs"], setup["target"], delta)
+        print(f"[adjoint] interseismic window ({K} steps): adjoint {ad:.12e} dual {dd:.12e} "
+              f"rel {abs(ad - dd) / abs(dd):.2e}")
+        assert abs(L - Ld) <= 1e-9 * abs(Ld)
+        assert abs(ad - dd) <= 1e-6 * abs(dd)
+
+
+@pytest.mark.parametrize("after", [3])
+def test_adjoint_dot_product_through_event(qd, case, after):
+    """The same identity on a window through the fault's first event (onset plus
+    `after` HighV steps at RuptureDt), with the event-time term.  Deeper into
+    the rupture the device and the CPU oracle stop following one discrete path:
+    the secant solves stop at |V_trial / V - 1| < 1e-9, so ulp-level differences
+    between the two libm implementations change iteration counts, which the
+    rupture amplifies (losses apart by 3.7e-8 three steps after the onset,
+    2.7e-5 after eight, 1.3e-3 after thirty) -- each derivative is then exact
+    for its own path."""
+    from oracle import oracle_dual as od
+    from qdfn import adjoint
+    prob = case["prob"]
+    op = qd.DenseOperator.from_matrices(case["KS"], case["KN"], qd.PREC_FP64)
+    probe = qd.Simulation(op, prob, trace_capacity=4000)
+    probe.run(4000)
+    tr = probe.trace()
+    probe.close()
+    on = int(np.nonzero(tr["vmax"] >= 1e-3)[0][0])
+    K = on + after
+    obs = np.sort(np.random.default_rng(5).choice(prob["n"], 16, replace=False))
+    sim = qd.Simulation(op, prob, record_step=1, record_capacity=K)
+    sim.run(K)
+    h = sim.history()
+    dK = h["History_Disp"][-1]
+    target = dK[obs] * 1.1 + 1e-3
+    _, t_first = adjoint.misfit(h, 0, obs, target)
+    t_target = t_first * 1.05
+    w_time = float(np.sum((dK[obs] - target) ** 2)) / (t_first - t_target) ** 2
+    L, g, info = sim.adjoint(prob["v_i"], prob["theta_i"], obs, target, w_time, t_target)
+    sim.close()
+    op.close()
+    rng = np.random.default_rng(3)
+    delta = {k: prob[k] * rng.standard_normal(prob["n"]) for k in ("a", "b", "dc")}
+    ad = sum(float(np.dot(g[k], delta[k])) for k in delta)
+    Ld, dd = _dual_directional(od, prob, case["KS"], case["KN"], K, obs, target, delta, w_time, t_target)
+    print(f"[adjoint] window through the event ({K} steps, onset {on}): adjoint {ad:.12e} dual {dd:.12e} "
+          f"rel {abs(ad - dd) / abs(dd):.2e}, lambda growth {info['lambda_growth']:.3e}")
+    # the two forwards agree to ~4e-8 (onset + 3 steps); the derivatives of their paths to ~1e-7
+    assert abs(L - Ld) <= 1e-7 * abs(Ld)
+    assert abs(ad - dd) <= 1e-5 * abs(dd)
