@@ -396,13 +396,16 @@ def adjoint_case(qd, args):
     slip misfit at 32 seeded elements (+ squared first-event-time misfit when
     the window holds the onset) over a window of accepted steps; gradient by
     qdfn_sim_adjoint, checked against central differences on random
-    parameters.  Windows: the interseismic steps before the onset, the onset
-    plus 38 steps (the first HighV solves), and args.adjoint_steps steps
-    through the first event."""
+    parameters, and by the dot-product identity against the forward-mode
+    (dual-number) oracle along random directions.  Windows: args.adjoint_steps
+    steps through the first event (its adjoint growth reported), the onset plus
+    38 steps (the first HighV solves), the onset plus 3 steps and the
+    interseismic steps before the onset."""
     from qdfn import adjoint, configs
     m = configs.adjoint_fault()
     seg = m["seg"]
     op = qd.DenseOperator.from_rect_segments(seg, m["mu"], m["nu"], qd.PREC_FP64)
+    KS = KN = None
     ks, _ = op.kself()
     ksum, knsum = op.matvec(np.ones(op.n), loading_form=False)
     prob = configs.problem_from_segments(m, ks, ksum, knsum)
@@ -440,7 +443,39 @@ def adjoint_case(qd, args):
             f"{info['retried_steps']}, max |grad| {max(np.abs(v).max() for v in g.values()):.3e}; central "
             f"differences on {n_fd} parameters: rel. error median {np.median(errs):.2e} max {errs.max():.2e} "
             f"(one-sided differences disagree by {np.median(smooth):.1e} at the best step)")
+        dual = None
+        if name in ("interseismic", "onset_plus_3") and not args.no_dual:
+            # dot-product identity against the forward-mode (dual-number) oracle
+            from oracle import oracle_dual as od
+            if KS is None:
+                KS, KN = qd.StiffnessMatrix_ByParts_Calculation_Rec(seg, seg, m["mu"], m["nu"])
+            dual = []
+            for sd in (1, 2):
+                r3 = np.random.default_rng(sd)
+                delta = {k: prob[k] * r3.standard_normal(op.n) for k in ("a", "b", "dc")}
+                adj_dd = sum(float(np.dot(g[k], delta[k])) for k in delta)
+                t0 = time.perf_counter()
+                st_d, rd = od.sim_dense(prob, KS, KN, total_step=K, record_step=1, seed=delta)
+                Dd, dDd = rd["hist_disp"][-1], rd["d_hist_disp"][-1]
+                dd = float(np.sum(2 * (Dd[obs] - target) * dDd[obs]))
+                if w_time > 0:
+                    v, dv = rd["trace_vmax"], rd["d_trace_vmax"]
+                    te, dte = rd["trace_told"] + rd["trace_dt"], rd["d_trace_told"] + rd["d_trace_dt"]
+                    k = int(np.nonzero(v >= 1e-3)[0][0])
+                    l0, l1, lt = np.log10(v[k - 1]), np.log10(v[k]), np.log10(1e-3)
+                    dl0, dl1 = dv[k - 1] / (v[k - 1] * np.log(10)), dv[k] / (v[k] * np.log(10))
+                    f = (lt - l0) / (l1 - l0)
+                    df = (-dl0 * (l1 - l0) - (lt - l0) * (dl1 - dl0)) / (l1 - l0) ** 2
+                    tf = te[k - 1] + f * (te[k] - te[k - 1])
+                    dd += 2 * w_time * (tf - t_target) * (dte[k - 1] + df * (te[k] - te[k - 1]) +
+                                                          f * (dte[k] - dte[k - 1]))
+                dual.append({"adjoint": adj_dd, "dual_forward": dd, "rel_err": abs(adj_dd - dd) / abs(dd),
+                             "cpu_s": round(time.perf_counter() - t0, 2)})
+            log(f"[bench] adjoint ({name}): dot-product check against the dual-number oracle: rel. errors "
+                + ", ".join(f"{d_['rel_err']:.2e}" for d_ in dual))
         out[name] = {"steps": K, "loss": L, "time_term": w_time > 0.0, "forward_ms": round(fwd_s * 1e3, 2),
+                     "lambda_growth": info["lambda_growth"], "growth_step": info["growth_step"],
+                     "verifiable": info["growth_step"] < 0, "dual_dot_product": dual,
                      "backward_ms": round(info["ms_backward"], 2), "backward_wall_ms": round(bwd_s * 1e3, 2),
                      "onset_step": info["onset_step"], "retried_steps": info["retried_steps"],
                      "grad_max_abs": float(max(np.abs(v).max() for v in g.values())),
@@ -450,6 +485,11 @@ def adjoint_case(qd, args):
                             for c in chk]}
         if name == "full_window" and info["onset_step"] > 2:
             windows.append(("onset_window", min(K, int(info["onset_step"]) + 38), 16))
+            # the device and the CPU oracle follow one discrete path only a few
+            # rupture steps past the onset (their secant solves stop at 1e-9 and
+            # the rupture amplifies ulp differences), so the dual-number check
+            # runs there and on the interseismic window
+            windows.append(("onset_plus_3", min(K, int(info["onset_step"]) + 3), 4))
             windows.append(("interseismic", int(info["onset_step"]) - 2, 16))
     op.close()
     return out
@@ -495,6 +535,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-dual", action="store_true", help="skip the adjoint's dual-number oracle check")
     ap.add_argument("--no-sampled-build", action="store_true")
     ap.add_argument("--bf16-far", action="store_true", help="also build and run BF16 low-rank factors")
     args = ap.parse_args()
