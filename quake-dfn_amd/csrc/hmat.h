@@ -124,8 +124,9 @@ int check_blocks(int64_t n, int64_t nblk, const int64_t* range, const int64_t* a
 void init_blocks(qdfn_hmat* h, int64_t nblk, const int64_t* range, const int64_t* adm);
 // device panels and product tasks from h->blocks
 int hmat_layout(qdfn_hmat* h);
-// rank cap of the cross approximation of an m x k block
-int64_t aca_rank_cap(int64_t m, int64_t k);
+// rank cap of the cross approximation of an m x k block (fp16_tail: the
+// break-even rank of FP32 + FP16-tail factors against the dense FP32 block)
+int64_t aca_rank_cap(int64_t m, int64_t k, bool fp16_tail = false);
 // U V (m x r, r x k) -> factors in SVD form within `budget` (Frobenius): the
 // shortest, or with fp16_tail those whose FP32 + FP16-tail storage is
 // smallest; returns the unused budget (budget - ||dropped tail||_F, >= 0), or

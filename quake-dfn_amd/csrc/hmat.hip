@@ -1596,13 +1596,16 @@ static void set_dense(HBlock& b, int w, const double* a) {
 // stores large admissible blocks dense: 256 made the 20,000-triangle operator
 // 811 MB instead of 632 MB (ranks up to ~1000 at ACA 1e-6 with
 // DistDiamRatioCrit 1.0); 2048 and 4096 give the 1024 operator there.
-int64_t aca_rank_cap(int64_t m, int64_t k) {
+int64_t aca_rank_cap(int64_t m, int64_t k, bool fp16_tail) {
     static const int64_t ceiling = [] {
         const char* e = getenv("QDFN_HMAT_RANK_CAP");
         const long long v = e ? atoll(e) : 0;
         return v >= 16 && v <= 4096 ? (int64_t)v : int64_t(1024);
     }();
-    return std::min<int64_t>(ceiling, m * k / (m + k));
+    // FP32 factors break even with the dense FP32 block at mk/(m+k); with an
+    // FP16 tail they can be cheaper up to twice that (all-FP16 components)
+    const int64_t even = m * k / (m + k);
+    return std::min<int64_t>(ceiling, fp16_tail ? std::min<int64_t>(std::min(m, k), 2 * even) : even);
 }
 
 int block_norms(const std::vector<const double*>& src, const std::vector<int64_t>& ld,
@@ -1633,6 +1636,7 @@ int block_norms(const std::vector<const double*>& src, const std::vector<int64_t
 int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, const std::vector<double>& bnorm,
                            double rtol, double atol) {
     const double rc0 = h->build_phase_ms[3];
+    const bool tail = h->prec == QDFN_PREC_MIXED;
     double total = 0.0;  // full-pivot time: this scope minus the recompression inside it
     {
         ScopeMs whole(&total);
@@ -1654,7 +1658,7 @@ int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, con
         while (end < lr.size()) {
             const HBlock& b = h->blocks[lr[end].q];
             const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
-            const int64_t rmax = aca_rank_cap(m, k);
+            const int64_t rmax = aca_rank_cap(m, k, tail);
             const int64_t need = m * k + m * rmax + rmax * k;
             if (end > pos && (used + need) * 8 > budget) break;
             off.push_back(used);
@@ -1666,7 +1670,7 @@ int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, con
         for (size_t q = pos; q < end; ++q) {
             const HBlock& b = h->blocks[lr[q].q];
             const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
-            const int64_t rmax = aca_rank_cap(m, k);
+            const int64_t rmax = aca_rank_cap(m, k, tail);
             AcaJob J;
             J.src = lr[q].src;
             J.scale = dscale.as<double>() + lr[q].q;
@@ -1744,6 +1748,42 @@ int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, con
             budgets[i] = rtol * bnorm[lr[q].q] - res[q - pos];
         }
         recompress_parallel(h, lr, to_recompress, budgets);
+        if (tail) {
+            // parts whose FP32 + FP16-tail factors would take at least the dense
+            // block's FP32 bytes are stored dense (from the assembled source)
+            std::vector<PackItem> dense_items;
+            std::vector<size_t> dense_q;
+            for (size_t q = pos; q < end; ++q) {
+                HBlock& b = h->blocks[lr[q].q];
+                const int w = lr[q].w;
+                if (b.kind[w] != QDFN_HBLK_LOWRANK) continue;
+                const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1, r = b.rank[w];
+                int64_t r_hi = r;
+                if (r >= 2 && b.slack[w] > 0.0) {
+                    std::vector<double> sg(r);
+                    for (int64_t a = 0; a < r; ++a) {
+                        double t2 = 0.0;
+                        for (int64_t i = 0; i < m; ++i) t2 += b.a[w][i + a * m] * b.a[w][i + a * m];
+                        sg[a] = std::sqrt(t2);
+                    }
+                    r_hi = fp16_tail_start(sg.data(), r, b.slack[w], m, k);
+                }
+                if ((m + k) * (4 * r_hi + 2 * (r - r_hi)) >= 4 * m * k) {
+                    dense_items.push_back({lr[q].src, lr[q].ld, (int32_t)m, (int32_t)k, 0});
+                    dense_q.push_back(q);
+                }
+            }
+            if (!dense_items.empty()) {
+                std::vector<double> hd;
+                int rcd = pack_to_host(dense_items, hd, h->stream);
+                if (rcd) return rcd;
+                for (size_t i = 0; i < dense_q.size(); ++i) {
+                    HBlock& b = h->blocks[lr[dense_q[i]].q];
+                    set_dense(b, lr[dense_q[i]].w, hd.data() + dense_items[i].dst);
+                    b.slack[lr[dense_q[i]].w] = 0.0;
+                }
+            }
+        }
         pos = end;
     }
     }
