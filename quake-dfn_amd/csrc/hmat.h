@@ -47,10 +47,11 @@ struct HTile {
     int32_t rb, c0, c1, q;
 };
 
-// phase-1 wavefront task: rows [0, m) of a row-major V panel slice at
-// data_off (leading dimension ld, K columns multiplying x[xoff + c]); row q
-// writes t[out + q], or vpart[part + q] when the panel is split into column
-// segments (part >= 0); lpr lanes per row; lo: the panel is in the 16-bit pool
+// phase-1 wavefront task: rows [0, m) of a column-major V sub-panel slice at
+// data_off (column stride ld, a multiple of the entries per lane load; K
+// columns multiplying x[xoff + c]); row q writes t[out + q], or vpart[part + q]
+// when the sub-panel is split into column segments (part >= 0); lpr = lanes per
+// column; lo: the sub-panel is in the 16-bit pool
 struct VTask {
     int64_t data_off;
     int32_t xoff, out, m, K, ld, lpr, part, lo;

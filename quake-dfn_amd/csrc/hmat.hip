@@ -61,9 +61,8 @@ constexpr int kPhase2Threads = 1024;
 constexpr int kSlots = 64 / (kPhase2Threads / 64);  // row slots per lane (64 rows over 16 waves)
 constexpr int kChunk = 1024;   // right-hand-side columns staged in LDS per pass (double-buffered)
 constexpr int kVThreads = 256; // phase 1: four independent wavefronts per workgroup
-constexpr int kVSlots = 4;     // phase 1: row slots per lane and step
-constexpr int kVSeg = 1024;    // phase 1: widest column segment of one task
-constexpr int kVTaskEntries = 4096;   // phase 1: target entries per task
+constexpr int kVDepth = 4;     // phase 1: steps in flight per wavefront
+constexpr int kVTaskEntries = 4096;   // phase 1: target entries per task (FP32 and FP64; twice in 16 bits)
 
 int64_t round4(int64_t v) { return (v + 3) / 4 * 4; }
 
@@ -363,21 +362,26 @@ __global__ __launch_bounds__(kPhase2Threads, 8) void k_hrow_mixed(const HTile* _
     if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
 }
 
-// Phase 1 (t = V x over the V panels): one wavefront per task, a task being
-// rows [0, m) of one panel over a column segment of at most kVSeg columns.
-// The task is walked as a flat sequence of steps (row group, column step),
-// software-pipelined two deep: the loads of step i + 1 (panel entries and the
-// x values they multiply) are issued before step i is accumulated, so a
-// wavefront always has a step's loads in flight.  A row group is kVSlots rows
-// per lane; each row is read by lpr lanes with 16-byte loads; loads are
-// branch-free (clamped, masked).  Wavefronts never synchronise; x is read
-// straight from global memory (L2-resident, shared by the panel's tasks).
-// Tasks of panels wider than kVSeg write partial sums (vpart) that
-// k_vfinish adds in segment order.
+// Phase 1 (t = V x over the V panels).  A V panel (the components, in one
+// pool, of the low-rank parts that share a source cluster) is stored as
+// column-major sub-panels of at most 64 W rows, W = RowLoad<T>::W entries per
+// lane load (4 in FP32 and FP64, 8 in 16 bits), rows padded to a multiple of
+// W with zeros: column c of a sub-panel is its ld rows at c ld, so any column
+// range is one contiguous run.  One wavefront per task (a sub-panel's columns,
+// or a segment of them): L = ld / W lanes cover a column (lane j its rows
+// [jW, jW + W)), and the wavefront's G = 64 / L lane groups take G consecutive
+// columns per step, so a step reads G ld contiguous entries and G values of x
+// (each lane the one of its column).  Steps are software-pipelined kVDepth
+// deep: each register slot is reloaded as soon as it is consumed.  Loads are
+// branch-free (column clamped, x masked to zero past the task's columns and in
+// the lanes past G L).  At the end the G group sums of each row are added over
+// a fixed tree of lane shuffles (the same order every run), and the lanes of
+// group 0 write W consecutive t rows (times the row's scale for the FP16
+// panels of QDFN_PREC_MIXED), or partial sums for column-split panels.
 template <class T>
-struct VStep {
-    T w[kVSlots][4];
-    double z[4];
+struct VColStep {
+    RowLoad<T> w;
+    double z;
 };
 
 // t row `out` of a finished phase-1 row: the sum, times the row's scale when
@@ -390,87 +394,62 @@ __device__ inline void vpanel_out(const VTask& tk, int32_t row, double v, double
         vpart[tk.part + row] = v;
 }
 
-// ONE: the task's columns fit one column step (nc == 1, most leaf-cluster
-// panels), so every row group multiplies the same x values: they are loaded
-// once and each step loads panel rows only.
-template <class T, bool ONE>
-__device__ __forceinline__ void vpanel_steps(const VTask& tk, const T* __restrict__ pool, const double* __restrict__ x,
-                                             double* __restrict__ t, double* __restrict__ vpart) {
-    const int lane = threadIdx.x & 63;
-    const int lpr = tk.lpr, rpw = 64 / lpr, sub = lane & (lpr - 1), rl = lane / lpr;
-    const int32_t K = tk.K, K4 = (K + 3) / 4 * 4;
-    const int32_t cstep = 4 * lpr, nc = (K4 + cstep - 1) / cstep;  // column steps per row group
-    const int32_t gr = kVSlots * rpw, ng = (tk.m + gr - 1) / gr;     // rows per group, groups
-    const int32_t total = nc * ng;
-    const T* base = pool + tk.data_off;
-    const double* xs = x + tk.xoff;
-    // x values of column c + q (masked in the integer domain so that the load
-    // stays unconditional: a select lets the compiler move the load into a
-    // branch that waits for every load in flight)
-    auto xload = [&](int32_t c, double z[4]) {
-        const int32_t cc = c < K4 ? c : 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const long long bits = __double_as_longlong(xs[min(cc + q, K - 1)]);
-            z[q] = __longlong_as_double((c < K4 && cc + q < K) ? bits : 0LL);
-        }
-    };
-    double z1[4];
-    if (ONE) xload(4 * sub, z1);
-    auto load = [&](int32_t it, VStep<T>& S) {
-        const int32_t gi = ONE ? it : it / nc, ci = ONE ? 0 : it - gi * nc;
-        const int32_t c = ci * cstep + 4 * sub;
-        const int32_t cc = c < K4 ? c : 0;
-        if (!ONE) xload(c, S.z);
-#pragma unroll
-        for (int r = 0; r < kVSlots; ++r) {
-            const int32_t row = min(gi * gr + rl + r * rpw, tk.m - 1);
-            load4raw<T>(base + (uint32_t)(row * tk.ld + cc), S.w[r]);
-        }
-    };
-    double acc[kVSlots];
-#pragma unroll
-    for (int r = 0; r < kVSlots; ++r) acc[r] = 0.0;
-    // accumulate step it; at a group's last column step, reduce over the row's
-    // lanes, write and restart
-    auto consume = [&](int32_t it, const VStep<T>& S) {
-        const int32_t gi = ONE ? it : it / nc, ci = ONE ? 0 : it - gi * nc;
-        const double* z = ONE ? z1 : S.z;
-#pragma unroll
-        for (int r = 0; r < kVSlots; ++r) {
-            acc[r] = fma((double)S.w[r][0], z[0], acc[r]);
-            acc[r] = fma((double)S.w[r][1], z[1], acc[r]);
-            acc[r] = fma((double)S.w[r][2], z[2], acc[r]);
-            acc[r] = fma((double)S.w[r][3], z[3], acc[r]);
-        }
-        if (ci == nc - 1) {
-#pragma unroll
-            for (int r = 0; r < kVSlots; ++r) {
-                const double v = group_sum(acc[r], lpr);
-                const int32_t row = gi * gr + rl + r * rpw;
-                if (sub == 0 && row < tk.m) vpanel_out(tk, row, v, t, vpart, nullptr);
-                acc[r] = 0.0;
-            }
-        }
-    };
-    VStep<T> A, B;
-    load(0, A);
-    for (int32_t it = 0; it < total; it += 2) {
-        load(min(it + 1, total - 1), B);
-        consume(it, A);
-        load(min(it + 2, total - 1), A);
-        if (it + 1 < total) consume(it + 1, B);
-    }
-}
-
 template <class T>
-__device__ __forceinline__ void vpanel_body(const VTask& tk, const T* __restrict__ pool, const double* __restrict__ x,
-                                            double* __restrict__ t, double* __restrict__ vpart) {
-    const int32_t K4 = (tk.K + 3) / 4 * 4;
-    if (K4 <= 4 * tk.lpr)
-        vpanel_steps<T, true>(tk, pool, x, t, vpart);
-    else
-        vpanel_steps<T, false>(tk, pool, x, t, vpart);
+__device__ __forceinline__ void vcol_body(const VTask& tk, const T* __restrict__ pool, const double* __restrict__ x,
+                                          double* __restrict__ t, double* __restrict__ vpart,
+                                          const double* __restrict__ tscale) {
+    constexpr int W = RowLoad<T>::W;
+    const int lane = threadIdx.x & 63;
+    const int L = tk.lpr, G = 64 / L;
+    const int g = lane / L, j = lane - g * L;  // g == G: idle lane (j < L still)
+    const int32_t K = tk.K, ns = (K + G - 1) / G;
+    const int32_t nsd = (ns + kVDepth - 1) / kVDepth * kVDepth;  // steps, masked past ns
+    const T* base = pool + tk.data_off + j * W;
+    const double* xs = x + tk.xoff;
+    auto load = [&](int32_t s, VColStep<T>& S) {
+        const int32_t c = s * G + g;
+        const bool ok = c < K && g < G;
+        const int32_t cc = ok ? c : K - 1;
+        S.w.load(base + (uint32_t)cc * (uint32_t)tk.ld);
+        const long long bits = __double_as_longlong(xs[cc]);
+        S.z = __longlong_as_double(ok ? bits : 0LL);
+    };
+    double acc[W];
+#pragma unroll
+    for (int v = 0; v < W; ++v) acc[v] = 0.0;
+    auto consume = [&](const VColStep<T>& S) {
+#pragma unroll
+        for (int v = 0; v < W; ++v) acc[v] = fma(S.w.get(v), S.z, acc[v]);
+    };
+    VColStep<T> S[kVDepth];
+#pragma unroll
+    for (int d = 0; d < kVDepth; ++d) load(d, S[d]);
+    for (int32_t s0 = 0; s0 + kVDepth < nsd; s0 += kVDepth) {
+#pragma unroll
+        for (int d = 0; d < kVDepth; ++d) {
+            consume(S[d]);
+            load(s0 + kVDepth + d, S[d]);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < kVDepth; ++d) consume(S[d]);
+    // lane (g, j) adds lane (g + o, j) where g is a multiple of 2o
+    for (int o = 1; o < G; o <<= 1) {
+        const int src = min(lane + o * L, 63);
+        const bool take = (g & (2 * o - 1)) == 0 && g + o < G;
+#pragma unroll
+        for (int v = 0; v < W; ++v) {
+            const double u = __shfl(acc[v], src, 64);
+            if (take) acc[v] += u;
+        }
+    }
+    if (g == 0) {
+#pragma unroll
+        for (int v = 0; v < W; ++v) {
+            const int32_t row = j * W + v;
+            if (row < tk.m) vpanel_out(tk, row, acc[v], t, vpart, tscale);
+        }
+    }
 }
 
 template <class T>
@@ -482,115 +461,15 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ 
     const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
     if (ti >= ntasks) return;
     const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
-    vpanel_body<T>(tasks[ti], pool, x, t, vpart);
+    vcol_body<T>(tasks[ti], pool, x, t, vpart, nullptr);
 }
 
-// Phase 1 over 16-bit V panels: k_vpanel's schedule with eight entries of a
-// row per lane and step (one 16-byte load), so a step covers twice the
-// columns.  Panel leading dimensions and segment starts are multiples of
-// eight entries.
-struct VStep8 {
-    uint4 w[kVSlots];
-    double z[8];
-};
-
-template <class T16, bool ONE>
-__device__ __forceinline__ void vpanel16_steps(const VTask& tk, const uint16_t* __restrict__ pool,
-                                               const double* __restrict__ x, double* __restrict__ t,
-                                               double* __restrict__ vpart, const double* __restrict__ tscale) {
-    const int lane = threadIdx.x & 63;
-    const int lpr = tk.lpr, rpw = 64 / lpr, sub = lane & (lpr - 1), rl = lane / lpr;
-    const int32_t K = tk.K, K8 = (K + 7) / 8 * 8;
-    const int32_t cstep = 8 * lpr, nc = (K8 + cstep - 1) / cstep;
-    const int32_t gr = kVSlots * rpw, ng = (tk.m + gr - 1) / gr;
-    const int32_t total = nc * ng;
-    const uint16_t* base = pool + tk.data_off;
-    const double* xs = x + tk.xoff;
-    auto xload = [&](int32_t c, double z[8]) {
-        const int32_t cc = c < K8 ? c : 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const long long bits = __double_as_longlong(xs[min(cc + q, K - 1)]);
-            z[q] = __longlong_as_double((c < K8 && cc + q < K) ? bits : 0LL);
-        }
-    };
-    double z1[8];
-    if (ONE) xload(8 * sub, z1);
-    auto load = [&](int32_t it, VStep8& S) {
-        const int32_t gi = ONE ? it : it / nc, ci = ONE ? 0 : it - gi * nc;
-        const int32_t c = ci * cstep + 8 * sub;
-        const int32_t cc = c < K8 ? c : 0;
-        if (!ONE) xload(c, S.z);
-#pragma unroll
-        for (int r = 0; r < kVSlots; ++r) {
-            const int32_t row = min(gi * gr + rl + r * rpw, tk.m - 1);
-            S.w[r] = *reinterpret_cast<const uint4*>(base + (uint32_t)(row * tk.ld + cc));
-        }
-    };
-    double acc[kVSlots];
-#pragma unroll
-    for (int r = 0; r < kVSlots; ++r) acc[r] = 0.0;
-    // entry 2j of a step is the low half of word j, entry 2j + 1 the high half
-    auto consume = [&](int32_t it, const VStep8& S) {
-        const int32_t gi = ONE ? it : it / nc, ci = ONE ? 0 : it - gi * nc;
-        const double* z = ONE ? z1 : S.z;
-#pragma unroll
-        for (int r = 0; r < kVSlots; ++r) {
-            const uint32_t wd[4] = {S.w[r].x, S.w[r].y, S.w[r].z, S.w[r].w};
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                acc[r] = fma((double)Dec16<T16>::lo(wd[j]), z[2 * j], acc[r]);
-                acc[r] = fma((double)Dec16<T16>::hi(wd[j]), z[2 * j + 1], acc[r]);
-            }
-        }
-        if (ci == nc - 1) {
-#pragma unroll
-            for (int r = 0; r < kVSlots; ++r) {
-                const double v = group_sum(acc[r], lpr);
-                const int32_t row = gi * gr + rl + r * rpw;
-                if (sub == 0 && row < tk.m) vpanel_out(tk, row, v, t, vpart, tscale);
-                acc[r] = 0.0;
-            }
-        }
-    };
-    VStep8 A, B;
-    load(0, A);
-    for (int32_t it = 0; it < total; it += 2) {
-        load(min(it + 1, total - 1), B);
-        consume(it, A);
-        load(min(it + 2, total - 1), A);
-        if (it + 1 < total) consume(it + 1, B);
-    }
-}
-
-template <class T16>
-__device__ __forceinline__ void vpanel16_body(const VTask& tk, const uint16_t* __restrict__ pool,
-                                              const double* __restrict__ x, double* __restrict__ t,
-                                              double* __restrict__ vpart, const double* __restrict__ tscale) {
-    const int32_t K8 = (tk.K + 7) / 8 * 8;
-    if (K8 <= 8 * tk.lpr)
-        vpanel16_steps<T16, true>(tk, pool, x, t, vpart, tscale);
-    else
-        vpanel16_steps<T16, false>(tk, pool, x, t, vpart, tscale);
-}
-
-__global__ __launch_bounds__(kVThreads) void k_vpanel_bf16(const VTask* __restrict__ tasks, int32_t ntasks,
-                                                           const uint16_t* __restrict__ pool,
-                                                           const double* __restrict__ x0, double* __restrict__ t,
-                                                           double* __restrict__ vpart, CouplingGate g) {
-    if (g.gate && ctrl_load(g.gate) == 0) return;
-    const int32_t ti = (int32_t)blockIdx.x * (kVThreads / 64) + (int32_t)(threadIdx.x >> 6);
-    if (ti >= ntasks) return;
-    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
-    vpanel16_body<bf16_t>(tasks[ti], pool, x, t, vpart, nullptr);
-}
-
-// Phase 1 of QDFN_PREC_MIXED: FP32 panels (leading components) and FP16
-// panels (unit vectors of the trailing components, their t rows scaled) in
-// one launch; the task's flag picks the body (uniform per wavefront).
+// Phase 1 of QDFN_PREC_MIXED: FP32 sub-panels (leading components) and FP16
+// sub-panels (unit vectors of the trailing components, their t rows scaled)
+// in one launch; the task's flag picks the body (uniform per wavefront).
 __global__ __launch_bounds__(kVThreads) void k_vpanel_mixed(const VTask* __restrict__ tasks, int32_t ntasks,
                                                             const float* __restrict__ pool,
-                                                            const uint16_t* __restrict__ pool_lo,
+                                                            const fp16_t* __restrict__ pool_lo,
                                                             const double* __restrict__ tscale,
                                                             const double* __restrict__ x0, double* __restrict__ t,
                                                             double* __restrict__ vpart, CouplingGate g) {
@@ -600,9 +479,9 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel_mixed(const VTask* __restr
     const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
     const VTask tk = tasks[ti];
     if (tk.lo)
-        vpanel16_body<fp16_t>(tk, pool_lo, x, t, vpart, tscale);
+        vcol_body<fp16_t>(tk, pool_lo, x, t, vpart, tscale);
     else
-        vpanel_body<float>(tk, pool, x, t, vpart);
+        vcol_body<float>(tk, pool, x, t, vpart, nullptr);
 }
 
 // Phase-1 finish: t rows of column-split panels, segments added in order.
@@ -944,84 +823,80 @@ int hmat_layout(qdfn_hmat* h) {
             const int64_t v = atoll(e);
             if (v >= 256 && v <= (1 << 20)) task_env = v;
         }
-        // lanes per row at most: 16 (64 or 128 entries per row and step) measured
-        // faster than one step per row for wider panels (fewer cross-lane
-        // reductions per entry); QDFN_HMAT_VLPR sets it
-        int lpr_max = 16;
-        if (const char* e = getenv("QDFN_HMAT_VLPR")) {
-            const int v = atoi(e);
-            if (v >= 1 && v <= 64 && (v & (v - 1)) == 0) lpr_max = v;
-        }
+        // lanes per column: the largest count <= the target whose lane groups
+        // fill at least 60 of a wavefront's 64 lanes
+        auto good_lanes = [](int64_t L) {
+            L = std::max<int64_t>(1, std::min<int64_t>(64, L));
+            while (L > 1 && (64 / L) * L < 60) --L;
+            return L;
+        };
         int64_t tbase = 0;
+        std::vector<std::pair<size_t, int64_t>> rowsrc;  // (member, component) of each panel row
         for (size_t gk = 0; gk < keys.size(); ++gk) {
             const int64_t s0 = std::get<0>(keys[gk]), k = std::get<1>(keys[gk]) - s0 + 1;
             const int glo = std::get<2>(keys[gk]);
-            // 16-bit panels are read eight entries per lane, the others four
+            // 16-bit panels are read eight entries per lane load, the others four
             const int64_t al = glo ? 8 : 4;
             auto round_al = [&](int64_t v) { return (v + al - 1) / al * al; };
             const int64_t task_entries = task_env ? task_env : (glo ? 2 * kVTaskEntries : kVTaskEntries);
-            const int64_t ld = round_al(k);
             std::vector<double>& vdst = glo ? lo : pool;
-            int64_t rows = 0;
-            for (auto& f : members[gk]) {
-                const HBlock& b = B[f.first];
-                (glo ? toff_lo : toff)[2 * f.first + f.second] = tbase + rows;
-                rows += glo ? b.rank[f.second] - b.r_hi[f.second] : b.r_hi[f.second];
+            rowsrc.clear();
+            for (size_t f = 0; f < members[gk].size(); ++f) {
+                const HBlock& b = B[members[gk][f].first];
+                const int w = members[gk][f].second;
+                (glo ? toff_lo : toff)[2 * members[gk][f].first + w] = tbase + (int64_t)rowsrc.size();
+                const int64_t a0 = glo ? b.r_hi[w] : 0, a1 = glo ? b.rank[w] : b.r_hi[w];
+                for (int64_t a = a0; a < a1; ++a) rowsrc.push_back({f, a});
             }
-            const int64_t off = (int64_t)vdst.size();
-            vdst.resize(off + rows * ld, 0.0);
+            const int64_t rows = (int64_t)rowsrc.size();
             tsc.resize(tbase + rows, 1.0);
-            int64_t row = 0;
-            for (auto& f : members[gk]) {
-                const HBlock& b = B[f.first];
-                const int64_t r = b.rank[f.second], a0 = glo ? b.r_hi[f.second] : 0, a1 = glo ? r : b.r_hi[f.second];
-                const std::vector<double>& V = b.b[f.second];
-                const std::vector<double>& sv = b.fv[f.second];
-                const std::vector<double>& sc = b.fc[f.second];
-                for (int64_t a = a0; a < a1; ++a, ++row) {
-                    const double mult = (glo && mixed) ? sv[a - a0] : 1.0;
-                    for (int64_t c = 0; c < k; ++c) vdst[off + row * ld + c] = V[a + c * r] * mult;
-                    if (glo && mixed) tsc[tbase + row] = sc[a - a0];
+            // sub-panels of L al rows, L chosen for about task_entries entries per task
+            const int64_t L = good_lanes(task_entries / (k * al));
+            const int64_t sub_rows = L * al;
+            for (int64_t ra = 0; ra < rows; ra += sub_rows) {
+                const int64_t rs = std::min<int64_t>(sub_rows, rows - ra), ld = round_al(rs);
+                const int64_t off = (int64_t)vdst.size();
+                vdst.resize(off + ld * k, 0.0);
+                for (int64_t q = 0; q < rs; ++q) {
+                    const auto& m = members[gk][rowsrc[ra + q].first];
+                    const HBlock& b = B[m.first];
+                    const int w = m.second;
+                    const int64_t r = b.rank[w], a = rowsrc[ra + q].second, a0 = glo ? b.r_hi[w] : 0;
+                    const double mult = (glo && mixed) ? b.fv[w][a - a0] : 1.0;
+                    const std::vector<double>& V = b.b[w];
+                    for (int64_t c = 0; c < k; ++c) vdst[off + c * ld + q] = V[a + c * r] * mult;
+                    if (glo && mixed) tsc[tbase + ra + q] = b.fc[w][a - a0];
                 }
-            }
-            // wavefront tasks: column segments of at most kVSeg columns, row chunks of
-            // about task_entries entries (whole row groups; twice kVTaskEntries for
-            // 16-bit panels, whose steps cover twice the columns; QDFN_HMAT_VTASK sets
-            // it); narrow panels read several rows per wavefront load
-            const int64_t nseg = (k + kVSeg - 1) / kVSeg;
-            const int64_t segw = round_al((k + nseg - 1) / nseg);  // 16-byte aligned segment starts
-            int lpr = 1;
-            while (lpr < lpr_max && al * lpr < segw) lpr <<= 1;
-            const int64_t gr = (int64_t)kVSlots * (64 / lpr);
-            const int64_t chunk = std::max<int64_t>(gr, task_entries / std::max<int64_t>(segw, 1) / gr * gr);
-            for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
-                const int64_t mr = std::min<int64_t>(chunk, rows - r0);
+                // wavefront tasks: all columns, or (sub-panels of W rows past four
+                // times the target) column segments whose partial sums k_vfinish adds
+                const int64_t nseg = std::max<int64_t>(1, (ld * k + 4 * task_entries - 1) / (4 * task_entries));
+                const int64_t segw = (k + nseg - 1) / nseg;
                 for (int64_t sg = 0; sg < nseg; ++sg) {
                     const int64_t cs = sg * segw, ce = std::min<int64_t>(k, cs + segw);
                     if (cs >= ce) return fail(QDFN_ERR_STATE, "H-matrix: empty phase-1 segment");
                     VTask T;
-                    T.data_off = off + r0 * ld + cs;
+                    T.data_off = off + cs * ld;
                     T.xoff = (int32_t)(s0 + cs);
-                    T.out = (int32_t)(tbase + r0);
-                    T.m = (int32_t)mr;
+                    T.out = (int32_t)(tbase + ra);
+                    T.m = (int32_t)rs;
                     T.K = (int32_t)(ce - cs);
                     T.ld = (int32_t)ld;
-                    T.lpr = lpr;
-                    T.part = nseg > 1 ? (int32_t)(vpart_total + sg * mr) : -1;
+                    T.lpr = (int32_t)(ld / al);
+                    T.part = nseg > 1 ? (int32_t)(vpart_total + sg * rs) : -1;
                     T.lo = glo;
                     vtasks.push_back(T);
                 }
                 if (nseg > 1) {
-                    for (int64_t q = 0; q < mr; ++q) {
+                    for (int64_t q = 0; q < rs; ++q) {
                         VFin F;
                         F.part0 = vpart_total + q;
-                        F.stride = (int32_t)mr;
+                        F.stride = (int32_t)rs;
                         F.nseg = (int32_t)nseg;
-                        F.out = (int32_t)(tbase + r0 + q);
+                        F.out = (int32_t)(tbase + ra + q);
                         F.pad = 0;
                         vfins.push_back(F);
                     }
-                    vpart_total += nseg * mr;
+                    vpart_total += nseg * rs;
                 }
             }
             (glo ? v_entries_lo : v_entries) += rows * k;
@@ -1145,7 +1020,7 @@ int hmat_layout(qdfn_hmat* h) {
     std::stable_sort(tiles.begin(), tiles.begin() + h->n_tiles1, by_size);
     std::stable_sort(tiles.begin() + h->n_tiles1, tiles.end(), by_size);
     std::stable_sort(vtasks.begin(), vtasks.end(), [](const VTask& a, const VTask& b) {
-        return (int64_t)a.m * a.ld > (int64_t)b.m * b.ld;
+        return (int64_t)a.K * a.ld > (int64_t)b.K * b.ld;
     });
     h->n_vtasks = (int64_t)vtasks.size();
     h->n_vfins = (int64_t)vfins.size();
@@ -1285,13 +1160,13 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
         double* vp = h->vpart.as<double>();
         if (mx)
             hipLaunchKernelGGL(k_vpanel_mixed, dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
-                               h->pool.as<float>(), h->pool_lo.as<uint16_t>(), tsc, d_x, t, vp, g);
+                               h->pool.as<float>(), h->pool_lo.as<fp16_t>(), tsc, d_x, t, vp, g);
         else if (f32)
             hipLaunchKernelGGL((k_vpanel<float>), dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
                                h->pool.as<float>(), d_x, t, vp, g);
         else if (bf)
-            hipLaunchKernelGGL(k_vpanel_bf16, dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
-                               h->pool_lo.as<uint16_t>(), d_x, t, vp, g);
+            hipLaunchKernelGGL((k_vpanel<bf16_t>), dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
+                               h->pool_lo.as<bf16_t>(), d_x, t, vp, g);
         else
             hipLaunchKernelGGL((k_vpanel<double>), dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
                                h->pool.as<double>(), d_x, t, vp, g);
