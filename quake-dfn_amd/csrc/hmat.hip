@@ -45,6 +45,7 @@
 #include <atomic>
 #include <map>
 #include <tuple>
+#include <type_traits>
 #include <thread>
 #include <vector>
 
@@ -267,6 +268,9 @@ __device__ __forceinline__ void hrow_body(const HTile& tl, const HRowBlock& R, d
     double acc[kSlots];
 #pragma unroll
     for (int r = 0; r < kSlots; ++r) acc[r] = 0.0;
+    // row slots this wavefront reads (slot r starts at row r step_rows + wave
+    // rpw; slots wholly past the row block are skipped, wave-uniformly)
+    const int nslots = min(kSlots, max(0, (R.m - wave * rpw + step_rows - 1) / step_rows));
     int buf = 0;
     for (int32_t c0 = tl.c0; c0 < KW; c0 += kChunk) {
         const bool more = c0 + kChunk < KW;
@@ -276,23 +280,34 @@ __device__ __forceinline__ void hrow_body(const HTile& tl, const HRowBlock& R, d
         }
         const double* z = zbuf[buf];
         const int32_t lim = min((int32_t)kChunk, KW - c0);
-        for (int32_t cb = 0; cb < lim; cb += W * lpr) {
-            // lanes past the chunk read its first columns again and add zero
-            const int32_t c = cb + sub * W < lim ? cb + sub * W : 0;
-            const double zm = cb + sub * W < lim ? 1.0 : 0.0;
-            double zz[W];
+        auto stream = [&](auto slots) {
+            constexpr int NS = decltype(slots)::value;
+            for (int32_t cb = 0; cb < lim; cb += W * lpr) {
+                // lanes past the chunk read its first columns again and add zero
+                const int32_t c = cb + sub * W < lim ? cb + sub * W : 0;
+                const double zm = cb + sub * W < lim ? 1.0 : 0.0;
+                double zz[W];
 #pragma unroll
-            for (int j = 0; j < W; ++j) zz[j] = zm * z[c + j];
-            RowLoad<T> w[kSlots];
+                for (int j = 0; j < W; ++j) zz[j] = zm * z[c + j];
+                RowLoad<T> w[NS];
 #pragma unroll
-            for (int r = 0; r < kSlots; ++r) {
-                const int row = min(r * step_rows + wave * rpw + rl, R.m - 1);
-                w[r].load(base + (uint32_t)(row * R.ld + c0 + c));
+                for (int r = 0; r < NS; ++r) {
+                    const int row = min(r * step_rows + wave * rpw + rl, R.m - 1);
+                    w[r].load(base + (uint32_t)(row * R.ld + c0 + c));
+                }
+#pragma unroll
+                for (int r = 0; r < NS; ++r)
+#pragma unroll
+                    for (int j = 0; j < W; ++j) acc[r] = fma(w[r].get(j), zz[j], acc[r]);
             }
-#pragma unroll
-            for (int r = 0; r < kSlots; ++r)
-#pragma unroll
-                for (int j = 0; j < W; ++j) acc[r] = fma(w[r].get(j), zz[j], acc[r]);
+        };
+        static_assert(kSlots == 4, "row-slot dispatch below");
+        switch (nslots) {
+            case 4: stream(std::integral_constant<int, 4>{}); break;
+            case 3: stream(std::integral_constant<int, 3>{}); break;
+            case 2: stream(std::integral_constant<int, 2>{}); break;
+            case 1: stream(std::integral_constant<int, 1>{}); break;
+            default: break;
         }
         if (more) {
 #pragma unroll
