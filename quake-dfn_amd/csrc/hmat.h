@@ -30,21 +30,15 @@ struct HBlock {
     std::vector<double> fu[2], fv[2], fc[2];
 };
 
-// Row block of the phase-2 product: one receiver row group (up to 64 rows) of
-// one matrix (kind 1 shear, 2 normal) as a row-major panel; column c multiplies
-// w[idx[zoff + c]] with w = [x; t]; row q writes row out + q of yS / yN.
-struct HRowBlock {
-    int64_t data_off, zoff, out;
-    int64_t part_base;      // split row blocks: tile q's row sums at part[part_base + q * m]
-    int32_t m, K, ld, kind;
-    int32_t ntiles;
-    int32_t lo;             // entries in the 16-bit pool (rows of U: BF16 with QDFN_PREC_BF16_FAR,
-                            // FP16 unit columns with QDFN_PREC_MIXED)
-};
-
-// work item: columns [c0, c1) of one row block (its q-th tile)
-struct HTile {
-    int32_t rb, c0, c1, q;
+// Phase-2 unit (one workgroup): columns [0, k) of one panel of a row group,
+// the group being rows [out, out + m) of one matrix (kind 1 shear, 2
+// normal).  The columns are column-major at off in the FP64 / FP32 pool
+// (lo 0; column stride ld = m rounded up to four) or in the 16-bit pool (lo
+// 1; ld = m rounded up to eight); column c multiplies w[idx[z + c]], w = [x;
+// t].  Unit u of its group writes its row sums to slot u.
+struct HUnit {
+    int64_t off, z;
+    int32_t k, ld, lo, out, m, kind, u, pad;
 };
 
 // phase-1 wavefront task: rows [0, m) of a column-major V sub-panel slice at
@@ -73,16 +67,13 @@ struct qdfn_hmat {
     std::vector<qdfn::HBlock> blocks;
     std::vector<double> kdiag_h;          // K_S,ii (0 where no diagonal block)
     qdfn::DevBuf kdiag;
-    qdfn::DevBuf pool, idx, t;            // panel entries (V panels then row-group panels), gather indices, t
-    qdfn::DevBuf pool_lo;                 // 16-bit V panels, then row groups' U panels (BF16_FAR: BF16; MIXED: FP16)
+    qdfn::DevBuf pool, idx, t;            // panel entries (V sub-panels then row-group panels), gather indices, t
+    qdfn::DevBuf pool_lo;                 // 16-bit V sub-panels, then row groups' U panels (BF16_FAR: BF16; MIXED: FP16)
     qdfn::DevBuf tscale;                  // QDFN_PREC_MIXED: scale of each t row (sigma of FP16 components, else 1)
-    qdfn::DevBuf rbs, tiles, part, multi; // row blocks, tiles, partial sums, split row blocks (phase 2)
+    qdfn::DevBuf units, part;             // phase-2 units, their slots (n_slots x [y_S; y_N] row sums)
     qdfn::DevBuf vtasks, vfins, vpart;    // phase-1 wavefront tasks, split-panel finish, partial sums
     qdfn::DevBuf x, yS, yN;               // vectors for qdfn_hmat_matvec
-    qdfn::DevBuf fin_row, fin_blk;        // rows of split phase-2 row blocks (see RowFinish)
-    int64_t n_rb1 = 0, n_rb2 = 0, n_t = 0;
-    int64_t n_tiles1 = 0, n_tiles2 = 0, n_multi1 = 0, n_multi2 = 0, n_vtasks = 0, n_vfins = 0;
-    int32_t tile_w = 4096;                // columns per tile (phase 2)
+    int64_t n_t = 0, n_units = 0, n_slots = 0, n_vtasks = 0, n_vfins = 0;
     int64_t stored_entries = 0;
     int64_t device_bytes = 0;
     int64_t phase_bytes = 0;              // HBM bytes one product reads and writes
@@ -110,13 +101,11 @@ struct ScopeMs {
 // Both phases of one product on stream s.  form 1: yS = (H_load (0 - x)) ./ K_ii,
 // form 0: yS = H_S x; yN = H_N x.  The gate skips the product as for the dense
 // coupling.
-// Events (optional): ev0 before phase 1, ev1 between the phases, ev2 after the
-// phase-2 main kernel (before its finish).
-// finish_in_step: the rows of split row blocks are left as partial sums for the
-// step kernel (fin_row / fin_blk), and the finish kernel is not launched.
+// Events (optional): ev0 before phase 1, ev1 between the phases, ev2 after
+// phase 2.  finish: k_hsum forms yS / yN from the unit slots; otherwise the
+// caller's next kernel does (the step kernel: the slots' sums in slot order).
 int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, int32_t form, double* d_yS,
-                double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2,
-                bool finish_in_step = false);
+                double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2, bool finish = true);
 
 // ---- construction helpers shared by the full and the sampled builds ----
 qdfn_hmat* new_hmat(int64_t n, int32_t prec);

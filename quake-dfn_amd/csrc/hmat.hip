@@ -19,23 +19,23 @@
 //
 // Product layout (HBM): the rows are cut into row groups of at most 64 rows
 // whose boundaries include every block's receiver boundaries.  Each row group
-// owns one row-major panel per matrix holding, side by side, the rows of every
-// block that covers it: dense blocks contribute their columns, low-rank blocks
-// the matching rows of U.  A product is then
+// owns column-major panels per matrix holding, side by side, the columns of
+// every block that covers it: dense blocks contribute their columns, low-rank
+// blocks the matching rows of U.  A product is then
 //   phase 1: t = V_b x[src_b] for every low-rank factor; the V rows of all
-//            factors that share a source range form one row-major panel over
-//            that (contiguous) range of x;
+//            factors that share a source range form one panel over that
+//            (contiguous) range of x, stored as column-major sub-panels;
 //   phase 2: y = panel * [x; t] per row group, the right-hand side gathered
 //            through an index list.
-// Phase 1 runs one wavefront per task (k_vpanel); phase 2 cuts each row-group
-// panel into tiles of at most tile_w columns, one workgroup each (k_hrow).  Both
-// stream their rows with wide loads, FP64 accumulation and wavefront
-// reductions; split rows are added tile by tile in column order by a small
-// finish kernel (no atomics: results are deterministic run to run).
-// Storage: FP64, FP32, or BF16 low-rank factors with FP32 dense blocks
-// (QDFN_PREC_BF16_FAR); then every row group with both kinds of columns has an
-// FP32 panel (dense blocks) and a BF16 panel (rows of U) whose tiles write
-// partial sums that the finish adds in tile order, FP32 tiles first.
+// Phase 1 runs one wavefront per task (k_vpanel), phase 2 one workgroup per
+// row group and matrix (k_hgroup).  Both stream contiguous column runs with
+// wide loads and FP64 accumulation; sums are combined over fixed shuffle trees
+// and (phase 2) through LDS in wavefront order, with no atomics, so results
+// are the same run to run.
+// Storage: FP64, FP32, FP32 with BF16 low-rank factors (QDFN_PREC_BF16_FAR),
+// or FP32 with FP16 trailing components (QDFN_PREC_MIXED); a row group with
+// 16-bit columns has a second panel in the 16-bit pool, streamed by some of
+// its workgroup's wavefronts.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -58,9 +58,11 @@ namespace qdfn {
 namespace {
 
 constexpr int kRowsPerGroup = 64;
-constexpr int kPhase2Threads = 1024;
-constexpr int kSlots = 64 / (kPhase2Threads / 64);  // row slots per lane (64 rows over 16 waves)
-constexpr int kChunk = 1024;   // right-hand-side columns staged in LDS per pass (double-buffered)
+constexpr int kP2Waves = 4;    // phase 2: wavefronts per workgroup (one unit of a row group)
+constexpr int64_t kP2Unit = 192 << 10; // phase 2: unit size (panel bytes) a row group is split into
+constexpr int kP2MaxSlots = 8;          // phase 2: most units of one row group (larger units beyond)
+constexpr int kP2Threads = 64 * kP2Waves;
+constexpr int kP2Depth = 4;    // phase 2: steps in flight per wavefront
 constexpr int kVThreads = 256; // phase 1: four independent wavefronts per workgroup
 constexpr int kVDepth = 4;     // phase 1: steps in flight per wavefront
 constexpr int kVTaskEntries = 4096;   // phase 1: target entries per task (FP32 and FP64; twice in 16 bits)
@@ -133,11 +135,20 @@ __device__ inline void load4raw(const T* p, T v[4]) {
 // BF16, two for FP64) and their FP64 values
 template <class T>
 struct RowLoad;
+// Panel entries are read once per product: FP32 and 16-bit loads are
+// nontemporal (streamed past the caches that keep x, t and the indices;
+// measured 5-13% faster than plain loads here), FP64 loads plain (its
+// nontemporal pair of 16-byte loads measured slower).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int nt_u4 __attribute__((ext_vector_type(4)));
 template <>
 struct RowLoad<float> {
     static constexpr int W = 4;
     float4 v;
-    __device__ void load(const float* p) { v = *reinterpret_cast<const float4*>(p); }
+    __device__ void load(const float* p) {
+        const nt_f4 q = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+        v = make_float4(q.x, q.y, q.z, q.w);
+    }
     __device__ double get(int j) const { return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w; }
 };
 template <>
@@ -154,7 +165,10 @@ template <class T16>
 struct RowLoad16 {
     static constexpr int W = 8;
     uint4 v;
-    __device__ void load(const T16* p) { v = *reinterpret_cast<const uint4*>(p); }
+    __device__ void load(const T16* p) {
+        const nt_u4 q = __builtin_nontemporal_load(reinterpret_cast<const nt_u4*>(p));
+        v = make_uint4(q.x, q.y, q.z, q.w);
+    }
     // entry 2k is the low half of word k, entry 2k + 1 the high half
     __device__ double get(int j) const {
         const uint32_t w = (j >> 1) == 0 ? v.x : (j >> 1) == 1 ? v.y : (j >> 1) == 2 ? v.z : v.w;
@@ -198,183 +212,201 @@ __device__ inline double wave_sum(double v) {
 // Output of one row (main_H :241-243 for the shear loading form:
 // (H_load * (0 - x)) ./ (-K_Self) with -K_Self = K_ii).
 __device__ inline void hmat_out(int kind, int64_t i, double s, const double* __restrict__ x,
-                                const double* __restrict__ kdiag, int32_t form, double* __restrict__ t,
-                                double* __restrict__ yS, double* __restrict__ yN) {
-    if (kind == 0)
-        t[i] = s;
-    else if (kind == 1)
+                                const double* __restrict__ kdiag, int32_t form, double* __restrict__ yS,
+                                double* __restrict__ yN) {
+    if (kind == 1)
         yS[i] = form ? (-s) / kdiag[i] : s + kdiag[i] * x[i];
     else
         yN[i] = s;
 }
 
-// One workgroup per tile (columns [c0, c1) of a row block).  The right-hand
-// side is staged in LDS in chunks of kChunk columns, double-buffered: the
-// gather of chunk i + 1 (through the index list from [x; t]) is issued before
-// chunk i is streamed.  Each row is read by LPR lanes, W entries each (one
-// 16-byte load: W = 4 in FP32, 8 in BF16; two loads in FP64), LPR the smallest
-// power of two >= the chunk width / W (at most 64), so a wavefront load covers
-// 64 / LPR rows.  A row block with one tile writes its outputs; a split one
-// writes partial row sums, added by k_hfinish or the step kernel.
+// Phase 2 (y = panels [x; t], one workgroup per unit: a column range of one
+// panel of one row group and matrix, at most kP2Unit panel bytes, so that no
+// workgroup holds up the end of the launch).  A row group's panels are
+// column-major: panel 0 (FP64 / FP32 pool) holds the
+// columns of its dense blocks and the leading components of its low-rank
+// parts with column stride ld0 = m rounded up to four, panel 1 (16-bit pool)
+// the trailing components with ld1 = m rounded up to eight (padding rows
+// zero).  Column c of a panel multiplies w[idx[z + c]], w = [x; t].  Each of
+// the kP2Waves wavefronts streams one contiguous column slice of one panel,
+// as phase 1 does: L = ld / W lanes cover a column (W = 4 entries per lane
+// load in FP32 and FP64, 8 in 16 bits; lane j rows [jW, jW + W)), G = 64 / L
+// lane groups take G consecutive columns per step.  The slice's right-hand
+// side is gathered into the wavefront's LDS chunk by chunk (double-buffered,
+// the next chunk's gather in flight while the current one is streamed), so a
+// step issues one panel load per lane; steps are software-pipelined
+// kP2Depth deep (half that in FP64), loads branch-free.  The lane groups'
+// sums are added over a fixed shuffle tree, the wavefronts' sums through LDS
+// in wavefront order, the units' sums in unit order, so every row is formed
+// in a fixed order (no finish kernel, results the same run to run).
 template <class T>
-__device__ __forceinline__ void hrow_body(const HTile& tl, const HRowBlock& R, double* __restrict__ part,
-                                          const T* __restrict__ pool, const int32_t* __restrict__ idx,
-                                          const double* __restrict__ x, double* __restrict__ t, int64_t n,
-                                          const double* __restrict__ kdiag, int32_t form, double* __restrict__ yS,
-                                          double* __restrict__ yN, double (*zbuf)[kChunk]) {
+struct P2Step {
+    RowLoad<T> w;
+};
+
+constexpr int kP2Chunk = 256;  // right-hand-side values staged per wavefront and buffer
+
+// Columns [0, k) of a panel slice at `panel` (gather indices zi); zs: this
+// wavefront's two LDS buffers of kP2Chunk values.  acc[v]: this lane's sum for
+// row Wj + v over its lane group's columns, then (lanes of group 0) over the
+// whole slice.
+template <class T>
+__device__ __forceinline__ void p2_slice(const T* __restrict__ panel, const int32_t* __restrict__ zi, int32_t k,
+                                         int32_t ld, const double* __restrict__ x, const double* __restrict__ t,
+                                         int64_t n, double (*zs)[kP2Chunk], double acc[RowLoad<T>::W]) {
     constexpr int W = RowLoad<T>::W;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int32_t K = tl.c1, KW = (K + W - 1) / W * W;  // columns [tl.c0, K) of the row block
-    const int32_t cw = min(KW - tl.c0, (int32_t)kChunk);
-    int lpr = 1;
-    while (lpr < 64 && W * lpr < cw) lpr <<= 1;
-    constexpr int kWaves = kPhase2Threads / 64;
-    const int rpw = 64 / lpr, step_rows = kWaves * rpw, sub = lane & (lpr - 1), rl = lane / lpr;
-    const T* base = pool + R.data_off;
-    constexpr int kPer = kChunk / kPhase2Threads;
-    // right-hand side: index loads run one chunk ahead of the value loads, so a
-    // chunk's gather never waits on its own index loads
+    constexpr int D = sizeof(T) == 8 ? kP2Depth / 2 : kP2Depth;  // FP64: 32 bytes per lane and step
+    constexpr int kPer = kP2Chunk / 64;                          // gathered values per lane and chunk
+    const int lane = threadIdx.x & 63;
+    const int L = ld / W, G = 64 / L;
+    const int g = lane / L, j = lane - g * L;  // g == G: idle lane (j < L still)
+    // a chunk is SC steps (a multiple of D) of G columns
+    const int32_t SC = D * max(1, kP2Chunk / (G * D)), C = SC * G;
+    const int32_t ns = (k + G - 1) / G, nsd = (ns + SC - 1) / SC * SC, nchunk = nsd / SC;
+    const T* base = panel + j * W;
+    // gather of chunk q: indices, then values (zero past the slice)
+    auto load_idx = [&](int32_t q, int32_t iv[kPer]) {
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int32_t c = q * C + lane + 64 * i;
+            iv[i] = zi[c < k ? c : k - 1];
+        }
+    };
+    auto load_vals = [&](int32_t q, const int32_t iv[kPer], double zv[kPer]) {
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int32_t c = q * C + lane + 64 * i;
+            const double* zp = iv[i] < n ? x + iv[i] : t + (iv[i] - n);
+            const long long bits = __double_as_longlong(*zp);
+            zv[i] = __longlong_as_double((c < k && lane + 64 * i < C) ? bits : 0LL);
+        }
+    };
+    auto store_vals = [&](int32_t q, const double zv[kPer]) {
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) zs[q & 1][lane + 64 * i] = zv[i];
+    };
+    auto load = [&](int32_t s, P2Step<T>& S) {
+        const int32_t c = s * G + g;
+        S.w.load(base + (uint32_t)((c < k && g < G) ? c : k - 1) * (uint32_t)ld);
+    };
+    // step s of chunk q reads its column's value at zq[s G] (zq set per chunk);
+    // columns past the slice read a staged zero, idle lanes a zero weight
+    const double gm = g < G ? 1.0 : 0.0;
+    auto consume = [&](const double* zq, int32_t s, const P2Step<T>& S) {
+        const double z = gm * zq[s * G];
+#pragma unroll
+        for (int v = 0; v < W; ++v) acc[v] = fma(S.w.get(v), z, acc[v]);
+    };
     int32_t iv[kPer];
-    double gv[kPer];
-    // loads are branch-free (indices clamped, values masked) so that they issue
-    // back to back instead of one at a time
-    auto load_idx = [&](int32_t c0) {
+    double zv[kPer];
+    load_idx(0, iv);
+    P2Step<T> S[D];
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            const int32_t c = c0 + (int32_t)threadIdx.x + q * kPhase2Threads;
-            iv[q] = idx[R.zoff + min(c, K - 1)];
-        }
-    };
-    auto load_vals = [&](int32_t c0) {
+    for (int d = 0; d < D; ++d) load(d, S[d]);
+    load_vals(0, iv, zv);
+    load_idx(1, iv);
+    store_vals(0, zv);
+    for (int32_t q = 0; q < nchunk; ++q) {
+        // gather of chunk q + 1 in flight while chunk q is streamed
+        load_vals(q + 1, iv, zv);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int32_t s_beg = q * SC, s_end = s_beg + SC;
+        const double* zq = zs[q & 1] + min(g, G - 1) - s_beg * G;
+        for (int32_t s0 = s_beg; s0 < s_end; s0 += D) {
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            const int32_t c = c0 + (int32_t)threadIdx.x + q * kPhase2Threads;
-            const int32_t j = iv[q];
-            const double val = j < n ? x[j] : t[j - n];
-            gv[q] = c < K ? val : 0.0;
-        }
-    };
-    if (K > 0) {
-        load_idx(tl.c0);
-        load_vals(tl.c0);
-    } else {
-#pragma unroll
-        for (int q = 0; q < kPer; ++q) gv[q] = 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) zbuf[0][threadIdx.x + q * kPhase2Threads] = gv[q];
-    if (tl.c0 + kChunk < KW) load_idx(tl.c0 + kChunk);
-    __syncthreads();
-    double acc[kSlots];
-#pragma unroll
-    for (int r = 0; r < kSlots; ++r) acc[r] = 0.0;
-    // row slots this wavefront reads (slot r starts at row r step_rows + wave
-    // rpw; slots wholly past the row block are skipped, wave-uniformly)
-    const int nslots = min(kSlots, max(0, (R.m - wave * rpw + step_rows - 1) / step_rows));
-    int buf = 0;
-    for (int32_t c0 = tl.c0; c0 < KW; c0 += kChunk) {
-        const bool more = c0 + kChunk < KW;
-        if (more) {
-            load_vals(c0 + kChunk);
-            if (c0 + 2 * kChunk < KW) load_idx(c0 + 2 * kChunk);
-        }
-        const double* z = zbuf[buf];
-        const int32_t lim = min((int32_t)kChunk, KW - c0);
-        auto stream = [&](auto slots) {
-            constexpr int NS = decltype(slots)::value;
-            for (int32_t cb = 0; cb < lim; cb += W * lpr) {
-                // lanes past the chunk read its first columns again and add zero
-                const int32_t c = cb + sub * W < lim ? cb + sub * W : 0;
-                const double zm = cb + sub * W < lim ? 1.0 : 0.0;
-                double zz[W];
-#pragma unroll
-                for (int j = 0; j < W; ++j) zz[j] = zm * z[c + j];
-                RowLoad<T> w[NS];
-#pragma unroll
-                for (int r = 0; r < NS; ++r) {
-                    const int row = min(r * step_rows + wave * rpw + rl, R.m - 1);
-                    w[r].load(base + (uint32_t)(row * R.ld + c0 + c));
-                }
-#pragma unroll
-                for (int r = 0; r < NS; ++r)
-#pragma unroll
-                    for (int j = 0; j < W; ++j) acc[r] = fma(w[r].get(j), zz[j], acc[r]);
+            for (int d = 0; d < D; ++d) {
+                consume(zq, s0 + d, S[d]);
+                load(min(s0 + D + d, nsd - 1), S[d]);
             }
-        };
-        static_assert(kSlots == 4, "row-slot dispatch below");
-        switch (nslots) {
-            case 4: stream(std::integral_constant<int, 4>{}); break;
-            case 3: stream(std::integral_constant<int, 3>{}); break;
-            case 2: stream(std::integral_constant<int, 2>{}); break;
-            case 1: stream(std::integral_constant<int, 1>{}); break;
-            default: break;
         }
-        if (more) {
-#pragma unroll
-            for (int q = 0; q < kPer; ++q) zbuf[buf ^ 1][threadIdx.x + q * kPhase2Threads] = gv[q];
-        }
-        __syncthreads();
-        buf ^= 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        store_vals(q + 1, zv);
+        load_idx(q + 2, iv);
     }
+    // lane (g, j) adds lane (g + o, j) where g is a multiple of 2o
+    for (int o = 1; o < G; o <<= 1) {
+        const int src = min(lane + o * L, 63);
+        const bool take = (g & (2 * o - 1)) == 0 && g + o < G;
 #pragma unroll
-    for (int r = 0; r < kSlots; ++r) {
-        double v = acc[r];
-        for (int o = lpr >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        const int row = r * step_rows + wave * rpw + rl;
-        if (sub == 0 && row < R.m) {
-            if (R.ntiles == 1)
-                hmat_out(R.kind, R.out + row, v, x, kdiag, form, t, yS, yN);
-            else
-                part[R.part_base + (int64_t)tl.q * R.m + row] = v;
+        for (int v = 0; v < W; ++v) {
+            const double u = __shfl(acc[v], src, 64);
+            if (take) acc[v] += u;
         }
     }
 }
 
-template <class T>
-__global__ __launch_bounds__(kPhase2Threads) void k_hrow(const HTile* __restrict__ tiles,
-                                                         const HRowBlock* __restrict__ rbs,
-                                                         double* __restrict__ part,
-                                                         const T* __restrict__ pool,
-                                                         const int32_t* __restrict__ idx,
-                                                         const double* __restrict__ x0, double* __restrict__ t,
-                                                         int64_t n, const double* __restrict__ kdiag, int32_t form,
-                                                         double* __restrict__ yS, double* __restrict__ yN,
-                                                         int32_t mark, CouplingGate g) {
+// One workgroup per unit: the unit's columns in kP2Waves equal slices, the
+// slices' row sums added through LDS in wavefront order and written to the
+// unit's slot: part[u 2n + r] for row r of [y_S; y_N] (r = out + row, plus n
+// for the normal matrix).  Slots of units a group does not have stay zero, so
+// a row is the sum over every slot in slot order (k_hsum, or the step kernel;
+// no atomics, results the same run to run).  T0: panel-0 entries (float,
+// double), T1: panel-1 entries (16-bit; unused when no unit is in panel 1).
+template <class T0, class T1>
+__global__ __launch_bounds__(kP2Threads, 6) void k_hunit(const HUnit* __restrict__ units,
+                                                        const T0* __restrict__ pool, const T1* __restrict__ pool_lo,
+                                                        const int32_t* __restrict__ idx,
+                                                        const double* __restrict__ x0, const double* __restrict__ t,
+                                                        int64_t n, double* __restrict__ part, CouplingGate g) {
     if (g.gate && ctrl_load(g.gate) == 0) return;
-    __shared__ double zbuf[2][kChunk];
+    __shared__ double red[kP2Waves][kRowsPerGroup];
+    __shared__ double zs[kP2Waves][2][kP2Chunk];
     const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
-    const HTile tl = tiles[blockIdx.x];
-    const HRowBlock R = rbs[tl.rb];
-    hrow_body<T>(tl, R, part, pool, idx, x, t, n, kdiag, form, yS, yN, zbuf);
-    if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
+    const HUnit R = units[blockIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int32_t c0 = (int32_t)((int64_t)R.k * wave / kP2Waves), c1 = (int32_t)((int64_t)R.k * (wave + 1) / kP2Waves);
+    // lanes of lane group 0 hold the slice's row sums (a slice with no columns: zeros)
+    if (R.lo) {
+        constexpr int W = RowLoad<T1>::W;
+        double acc[W];
+#pragma unroll
+        for (int v = 0; v < W; ++v) acc[v] = 0.0;
+        if (c1 > c0)
+            p2_slice<T1>(pool_lo + R.off + (int64_t)c0 * R.ld, idx + R.z + c0, c1 - c0, R.ld, x, t, n, zs[wave], acc);
+        if (lane < R.ld / W) {
+#pragma unroll
+            for (int v = 0; v < W; ++v) red[wave][W * lane + v] = acc[v];
+        }
+    } else {
+        constexpr int W = RowLoad<T0>::W;
+        double acc[W];
+#pragma unroll
+        for (int v = 0; v < W; ++v) acc[v] = 0.0;
+        if (c1 > c0)
+            p2_slice<T0>(pool + R.off + (int64_t)c0 * R.ld, idx + R.z + c0, c1 - c0, R.ld, x, t, n, zs[wave], acc);
+        if (lane < R.ld / W) {
+#pragma unroll
+            for (int v = 0; v < W; ++v) red[wave][W * lane + v] = acc[v];
+        }
+    }
+    __syncthreads();
+    const int row = (int)threadIdx.x;
+    if (row < R.m) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kP2Waves; ++w) s += red[w][row];
+        part[(int64_t)R.u * 2 * n + (R.kind == 2 ? n : 0) + R.out + row] = s;
+    }
+    if (g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
 }
 
-// Phase 2 with 16-bit panels: FP32 panels (dense blocks, and the leading
-// low-rank components with QDFN_PREC_MIXED) and 16-bit panels (rows of U:
-// BF16, or FP16 unit columns) in one launch, tiles in one size-ordered list;
-// the row block's flag picks the body (uniform per workgroup).  At most 64
-// VGPRs, so that two 1,024-thread workgroups fit a CU as for k_hrow.
-template <class T16>
-__global__ __launch_bounds__(kPhase2Threads, 8) void k_hrow_mixed(const HTile* __restrict__ tiles,
-                                                               const HRowBlock* __restrict__ rbs,
-                                                               double* __restrict__ part,
-                                                               const float* __restrict__ pool,
-                                                               const T16* __restrict__ pool_lo,
-                                                               const int32_t* __restrict__ idx,
-                                                               const double* __restrict__ x0, double* __restrict__ t,
-                                                               int64_t n, const double* __restrict__ kdiag,
-                                                               int32_t form, double* __restrict__ yS,
-                                                               double* __restrict__ yN, int32_t mark, CouplingGate g) {
+// Rows of the product from the unit slots (slot order), one lane per row of
+// [y_S; y_N].
+__global__ __launch_bounds__(256) void k_hsum(const double* __restrict__ part, int32_t slots, int64_t n,
+                                              const double* __restrict__ x0, const double* __restrict__ kdiag,
+                                              int32_t form, double* __restrict__ yS, double* __restrict__ yN,
+                                              CouplingGate g) {
     if (g.gate && ctrl_load(g.gate) == 0) return;
-    __shared__ double zbuf[2][kChunk];
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= 2 * n) return;
     const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
-    const HTile tl = tiles[blockIdx.x];
-    const HRowBlock R = rbs[tl.rb];
-    if (R.lo)
-        hrow_body<T16>(tl, R, part, pool_lo, idx, x, t, n, kdiag, form, yS, yN, zbuf);
-    else
-        hrow_body<float>(tl, R, part, pool, idx, x, t, n, kdiag, form, yS, yN, zbuf);
-    if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
+    double s = 0.0;
+    for (int32_t u = 0; u < slots; ++u) s += part[(int64_t)u * 2 * n + r];
+    hmat_out(r < n ? 1 : 2, r < n ? r : r - n, s, x, kdiag, form, yS, yN);
 }
 
 // Phase 1 (t = V x over the V panels).  A V panel (the components, in one
@@ -511,25 +543,6 @@ __global__ __launch_bounds__(256) void k_vfinish(const VFin* __restrict__ fins, 
     double s = 0.0;
     for (int32_t q = 0; q < f.nseg; ++q) s += vpart[f.part0 + (int64_t)q * f.stride];
     t[f.out] = tscale ? s * tscale[f.out] : s;
-}
-
-// finish of the split row blocks: one 64-lane workgroup each, one lane per row,
-// tiles added in column order
-__global__ __launch_bounds__(64) void k_hfinish(const int32_t* __restrict__ multi, const HRowBlock* __restrict__ rbs,
-                                                const double* __restrict__ part, const double* __restrict__ x0,
-                                                const double* __restrict__ kdiag, int32_t form,
-                                                double* __restrict__ t, double* __restrict__ yS,
-                                                double* __restrict__ yN, int32_t mark, CouplingGate g) {
-    if (g.gate && ctrl_load(g.gate) == 0) return;
-    const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
-    const HRowBlock R = rbs[multi[blockIdx.x]];
-    const int row = threadIdx.x;
-    if (row < R.m) {
-        double s = 0.0;
-        for (int32_t q = 0; q < R.ntiles; ++q) s += part[R.part_base + (int64_t)q * R.m + row];
-        hmat_out(R.kind, R.out + row, s, x, kdiag, form, t, yS, yN);
-    }
-    if (mark && g.ran_log && blockIdx.x == 0 && threadIdx.x == 0) g.ran_log[g.slot] = 1;
 }
 
 __global__ void k_mark_skipped(CouplingGate g) {
@@ -744,7 +757,6 @@ int hmat_layout(qdfn_hmat* h) {
     const bool has_lo = lo_far || mixed;
     std::vector<double> pool, lo;
     std::vector<int32_t> idx;
-    std::vector<HRowBlock> rbs;
     for (HBlock& b : B)
         for (int w = 0; w < 2; ++w) {
             b.r_hi[w] = lo_far ? 0 : b.rank[w];
@@ -754,56 +766,12 @@ int hmat_layout(qdfn_hmat* h) {
             if (mixed && b.kind[w] == QDFN_HBLK_LOWRANK && b.rank[w] >= 2 && b.slack[w] > 0.0)
                 choose_fp16_tail(b, w, b.fu[w], b.fv[w], b.fc[w]);
         }
-    if (const char* e = getenv("QDFN_HMAT_TILE")) {
-        const int v = atoi(e);
-        if (v >= kChunk && v % kChunk == 0) h->tile_w = v;
-    }
-    std::vector<HTile> tiles;
-    std::vector<int32_t> multi;
+    std::vector<HUnit> units;
+    int64_t slots = 1, n_gid = 0;  // unit slots (most units of a row group), row groups
     std::vector<VTask> vtasks;
     std::vector<VFin> vfins;
     std::vector<double> tsc;  // scale of each t row (QDFN_PREC_MIXED)
     int64_t v_entries = 0, v_entries_lo = 0, v_entries_lo_rows = 0, v_cols = 0, vpart_total = 0;
-    int64_t part_total = 0;
-    // tiles of row block rb (columns [0, K)) numbered from q into tl; returns the next q
-    auto add_tiles = [&](int32_t rb, int32_t K, int32_t q, std::vector<HTile>& tl) {
-        int32_t c = 0;
-        do {
-            HTile T;
-            T.rb = rb;
-            T.c0 = c;
-            T.c1 = std::min<int32_t>(K, c + h->tile_w);
-            T.q = q++;
-            tl.push_back(T);
-            c = T.c1;
-        } while (c < K);
-        return q;
-    };
-    auto add_rowblock = [&](HRowBlock R, bool in_lo) {
-        R.ntiles = add_tiles((int32_t)rbs.size(), R.K, 0, tiles);
-        R.lo = in_lo;
-        R.part_base = -1;
-        if (R.ntiles > 1) {
-            R.part_base = part_total;
-            part_total += (int64_t)R.ntiles * R.m;
-            multi.push_back((int32_t)rbs.size());
-        }
-        rbs.push_back(R);
-    };
-    // the FP32 and the 16-bit panel of one row group: one split row (tiles of Ra
-    // then of Rb, partial sums in one area, finished through Ra)
-    auto add_pair = [&](HRowBlock Ra, HRowBlock Rb) {
-        const int32_t ia = (int32_t)rbs.size();
-        const int32_t q = add_tiles(ia + 1, Rb.K, add_tiles(ia, Ra.K, 0, tiles), tiles);
-        Ra.ntiles = Rb.ntiles = q;
-        Ra.lo = 0;
-        Rb.lo = 1;
-        Ra.part_base = Rb.part_base = part_total;
-        part_total += (int64_t)q * Ra.m;
-        multi.push_back(ia);
-        rbs.push_back(Ra);
-        rbs.push_back(Rb);
-    };
     // phase 1: V panels, one per source range and pool (components [0, r_hi)
     // in FP64 / FP32, [r_hi, rank) in 16 bits), in order of first appearance;
     // toff / toff_lo: first t row of each part's components in either pool
@@ -921,9 +889,6 @@ int hmat_layout(qdfn_hmat* h) {
         }
         h->n_t = tbase;
     }
-    h->n_rb1 = (int64_t)rbs.size();
-    h->n_tiles1 = (int64_t)tiles.size();
-    h->n_multi1 = (int64_t)multi.size();
     if (n + h->n_t > 0x7fffffffLL) return fail(QDFN_ERR_ARG, "H-matrix: index space exceeds int32");
     // phase 2: row groups of at most 64 rows whose boundaries include every receiver boundary
     std::vector<int64_t> cut = {0, n};
@@ -944,17 +909,23 @@ int hmat_layout(qdfn_hmat* h) {
         int64_t gi = std::lower_bound(g0.begin(), g0.begin() + G, b.r0) - g0.begin();
         for (; gi < G && g0[gi + 1] - 1 <= b.r1; ++gi) cover[gi].push_back(bi);
     }
-    int64_t out_rows2 = 0;
+    int64_t out_rows2 = 0, b2 = 0, k2 = 0;  // output rows, stored entry bytes, columns
+    const int64_t es = h->prec == QDFN_PREC_FP64 ? sizeof(double) : sizeof(float);  // pool entries
+    int64_t unit_bytes = kP2Unit;
+    if (const char* e = getenv("QDFN_HMAT_UNIT")) {
+        const long long v = atoll(e);
+        if (v >= 4096 && v <= (1LL << 30)) unit_bytes = v;
+    }
     for (int64_t gi = 0; gi < G; ++gi)
         for (int w = 0; w < 2; ++w) {
             const int64_t m = g0[gi + 1] - g0[gi], row0 = g0[gi];
             out_rows2 += m;
-            // the row group's panel: in the FP64 / FP32 pool the columns of its
-            // dense blocks and the components [0, r_hi) of its low-rank parts, in
-            // the 16-bit pool (lopanel; leading dimensions of multiples of eight)
-            // the components [r_hi, rank)
-            auto panel = [&](bool lopanel, std::vector<double>& dst) {
-                const int64_t al = lopanel ? 8 : 4;
+            // the row group's panels (column-major, column stride ld): in the
+            // FP64 / FP32 pool the columns of its dense blocks and the components
+            // [0, r_hi) of its low-rank parts, in the 16-bit pool (lopanel) the
+            // components [r_hi, rank); returns (offset, first index, columns)
+            auto panel = [&](bool lopanel, std::vector<double>& dst, int64_t& off, int64_t& i0) {
+                const int64_t ld = lopanel ? (m + 7) / 8 * 8 : round4(m);
                 auto span = [&](const HBlock& b, int64_t& a0, int64_t& a1) {
                     if (b.kind[w] == QDFN_HBLK_DENSE) {
                         a0 = 0;
@@ -964,7 +935,7 @@ int hmat_layout(qdfn_hmat* h) {
                         a1 = lopanel ? b.rank[w] : b.r_hi[w];
                     }
                 };
-                const int64_t i0 = (int64_t)idx.size();
+                i0 = (int64_t)idx.size();
                 for (int32_t bi : cover[gi]) {
                     const HBlock& b = B[bi];
                     int64_t a0, a1;
@@ -977,9 +948,9 @@ int hmat_layout(qdfn_hmat* h) {
                                                                  : toff[2 * bi + w] + c)));
                     }
                 }
-                const int64_t K = (int64_t)idx.size() - i0, ld = (K + al - 1) / al * al;
-                const int64_t off = (int64_t)dst.size();
-                dst.resize(off + m * ld, 0.0);
+                const int64_t K = (int64_t)idx.size() - i0;
+                off = (int64_t)dst.size();
+                dst.resize(off + K * ld, 0.0);
                 int64_t col = 0;
                 for (int32_t bi : cover[gi]) {
                     const HBlock& b = B[bi];
@@ -991,49 +962,66 @@ int hmat_layout(qdfn_hmat* h) {
                     const bool scaled = lopanel && mixed && b.kind[w] == QDFN_HBLK_LOWRANK;
                     const std::vector<double>& su = b.fu[w];
                     const std::vector<double>& A = b.a[w];
-                    for (int64_t i = 0; i < m; ++i)
-                        for (int64_t c = a0; c < a1; ++c) {
+                    for (int64_t c = a0; c < a1; ++c)
+                        for (int64_t i = 0; i < m; ++i) {
                             double v = A[(roff + i) + c * mb];
                             if (diag && row0 + i == b.s0 + c) v = 0.0;  // loading form: zero diagonal
                             if (scaled) v *= su[c - a0];
-                            dst[off + i * ld + col + (c - a0)] = v;
+                            dst[off + (col + (c - a0)) * ld + i] = v;
                         }
                     col += a1 - a0;
                 }
-                HRowBlock R;
-                R.data_off = off;
-                R.zoff = i0;
-                R.out = row0;
-                R.m = (int32_t)m;
-                R.K = (int32_t)K;
-                R.ld = (int32_t)ld;
-                R.kind = 1 + w;
-                return R;
+                return K;
             };
-            if (!has_lo) {
-                add_rowblock(panel(false, pool), false);
-                continue;
+            const int64_t ld0 = round4(m), ld1 = (m + 7) / 8 * 8;
+            int64_t off0, z0, off1 = 0, z1 = 0;
+            const int64_t k0 = panel(false, pool, off0, z0);
+            const int64_t k1 = has_lo ? panel(true, lo, off1, z1) : 0;
+            // units of at most unit_bytes panel bytes (at least one per group);
+            // unit u of the group writes slot u
+            const int64_t B0 = ld0 * k0 * es, B1 = ld1 * k1 * (int64_t)sizeof(uint16_t);
+            int64_t ub = unit_bytes, S0, S1;
+            for (;;) {
+                S0 = (B0 + ub - 1) / ub;
+                S1 = (B1 + ub - 1) / ub;
+                if (S0 + S1 <= kP2MaxSlots) break;
+                ub += ub / 4;
             }
-            const HRowBlock Ra = panel(false, pool), Rb = panel(true, lo);
-            if (Rb.K == 0)
-                add_rowblock(Ra, false);
-            else if (Ra.K == 0)
-                add_rowblock(Rb, true);
-            else
-                add_pair(Ra, Rb);
+            if (S0 + S1 == 0) S0 = 1;
+            const int64_t S = S0 + S1;
+            slots = std::max(slots, S);
+            for (int64_t u = 0; u < S; ++u) {
+                const bool in1 = u >= S0;
+                const int64_t Sp = in1 ? S1 : S0, up = in1 ? u - S0 : u, kp = in1 ? k1 : k0;
+                const int64_t ca = kp * up / Sp, cb = kp * (up + 1) / Sp, ldp = in1 ? ld1 : ld0;
+                HUnit U;
+                U.off = (in1 ? off1 : off0) + ca * ldp;
+                U.z = (in1 ? z1 : z0) + ca;
+                U.k = (int32_t)(cb - ca);
+                U.ld = (int32_t)ldp;
+                U.lo = in1 ? 1 : 0;
+                U.out = (int32_t)row0;
+                U.m = (int32_t)m;
+                U.kind = 1 + w;
+                U.u = (int32_t)u;
+                U.pad = 0;
+                units.push_back(U);
+            }
+            ++n_gid;
+            b2 += m * (k0 * es + k1 * (int64_t)sizeof(uint16_t));
+            k2 += k0 + k1;
         }
-    h->n_rb2 = (int64_t)rbs.size() - h->n_rb1;
-    h->n_tiles2 = (int64_t)tiles.size() - h->n_tiles1;
-    h->n_multi2 = (int64_t)multi.size() - h->n_multi1;
-    // largest tiles first within each phase (the dispatcher hands out workgroups
-    // in order, so the tail of a launch is made of small tiles)
-    const int64_t es = h->prec == QDFN_PREC_FP64 ? sizeof(double) : sizeof(float);  // pool entries
-    auto tile_bytes = [&](const HTile& T) {
-        return (int64_t)rbs[T.rb].m * (T.c1 - T.c0) * (rbs[T.rb].lo ? (int64_t)sizeof(uint16_t) : es);
+    // largest units first (the dispatcher hands out workgroups in order, so
+    // the tail of a launch is made of small units)
+    auto unit_bytes_of = [&](const HUnit& U) {
+        return (int64_t)U.ld * U.k * (U.lo ? (int64_t)sizeof(uint16_t) : es);
     };
-    auto by_size = [&](const HTile& a, const HTile& b) { return tile_bytes(a) > tile_bytes(b); };
-    std::stable_sort(tiles.begin(), tiles.begin() + h->n_tiles1, by_size);
-    std::stable_sort(tiles.begin() + h->n_tiles1, tiles.end(), by_size);
+    std::stable_sort(units.begin(), units.end(),
+                     [&](const HUnit& a, const HUnit& b) { return unit_bytes_of(a) > unit_bytes_of(b); });
+    h->n_units = (int64_t)units.size();
+    h->n_slots = slots;
+    int64_t slot_rows = 0;  // rows the units write to their slots
+    for (const HUnit& U : units) slot_rows += U.m;
     std::stable_sort(vtasks.begin(), vtasks.end(), [](const VTask& a, const VTask& b) {
         return (int64_t)a.K * a.ld > (int64_t)b.K * b.ld;
     });
@@ -1048,22 +1036,13 @@ int hmat_layout(qdfn_hmat* h) {
                 "max %lld\n", sz.size(), (long long)v_entries, (long long)sz.front(), (long long)sz[sz.size() / 2],
                 (long long)sz[sz.size() * 9 / 10], (long long)sz.back());
     }
-    if (getenv("QDFN_HMAT_VERBOSE")) {
-        for (int ph = 0; ph < 2; ++ph) {
-            const size_t b0 = ph ? h->n_tiles1 : 0, b1 = ph ? tiles.size() : h->n_tiles1;
-            if (b1 == b0) continue;
-            std::vector<int64_t> sz;
-            int64_t tot = 0;
-            for (size_t q = b0; q < b1; ++q) {
-                sz.push_back(tile_bytes(tiles[q]));
-                tot += sz.back();
-            }
-            std::sort(sz.begin(), sz.end());
-            fprintf(stderr, "[hmat] phase %d: %zu tiles, %lld bytes, per tile min %lld median %lld p90 %lld max %lld, "
-                    "row blocks %lld (split %lld)\n", ph + 1, sz.size(), (long long)tot, (long long)sz.front(),
-                    (long long)sz[sz.size() / 2], (long long)sz[sz.size() * 9 / 10], (long long)sz.back(),
-                    (long long)(ph ? h->n_rb2 : h->n_rb1), (long long)(ph ? h->n_multi2 : h->n_multi1));
-        }
+    if (getenv("QDFN_HMAT_VERBOSE") && !units.empty()) {
+        std::vector<int64_t> sz;
+        for (const HUnit& U : units) sz.push_back(unit_bytes_of(U));
+        std::sort(sz.begin(), sz.end());
+        fprintf(stderr, "[hmat] phase 2: %lld row groups, %zu units, %lld bytes, per unit min %lld median %lld p90 %lld "
+                "max %lld\n", (long long)n_gid, sz.size(), (long long)b2, (long long)sz.front(),
+                (long long)sz[sz.size() / 2], (long long)sz[sz.size() * 9 / 10], (long long)sz.back());
     }
     h->stored_entries = 0;
     for (const HBlock& b : B)
@@ -1088,10 +1067,12 @@ int hmat_layout(qdfn_hmat* h) {
     } else {
         QDFN_HIP(hipMemcpy(h->pool.p, pool.data(), es * pool.size(), hipMemcpyHostToDevice));
     }
-    QDFN_HIP(h->rbs.alloc(sizeof(HRowBlock) * rbs.size()));
-    QDFN_HIP(hipMemcpy(h->rbs.p, rbs.data(), sizeof(HRowBlock) * rbs.size(), hipMemcpyHostToDevice));
-    QDFN_HIP(h->tiles.alloc(sizeof(HTile) * tiles.size()));
-    QDFN_HIP(hipMemcpy(h->tiles.p, tiles.data(), sizeof(HTile) * tiles.size(), hipMemcpyHostToDevice));
+    QDFN_HIP(h->units.alloc(sizeof(HUnit) * std::max<size_t>(units.size(), 1)));
+    if (!units.empty())
+        QDFN_HIP(hipMemcpy(h->units.p, units.data(), sizeof(HUnit) * units.size(), hipMemcpyHostToDevice));
+    // unit slots: zero once here; a slot no unit writes stays zero
+    QDFN_HIP(h->part.alloc(sizeof(double) * slots * 2 * n));
+    QDFN_HIP(hipMemset(h->part.p, 0, sizeof(double) * slots * 2 * n));
     QDFN_HIP(h->vtasks.alloc(sizeof(VTask) * std::max<size_t>(vtasks.size(), 1)));
     if (!vtasks.empty())
         QDFN_HIP(hipMemcpy(h->vtasks.p, vtasks.data(), sizeof(VTask) * vtasks.size(), hipMemcpyHostToDevice));
@@ -1099,33 +1080,8 @@ int hmat_layout(qdfn_hmat* h) {
     if (!vfins.empty())
         QDFN_HIP(hipMemcpy(h->vfins.p, vfins.data(), sizeof(VFin) * vfins.size(), hipMemcpyHostToDevice));
     QDFN_HIP(h->vpart.alloc(sizeof(double) * std::max<int64_t>(vpart_total, 1)));
-    QDFN_HIP(h->part.alloc(sizeof(double) * std::max<int64_t>(part_total, 1)));
-    QDFN_HIP(h->multi.alloc(sizeof(int32_t) * std::max<size_t>(multi.size(), 1)));
-    if (!multi.empty())
-        QDFN_HIP(hipMemcpy(h->multi.p, multi.data(), sizeof(int32_t) * multi.size(), hipMemcpyHostToDevice));
     QDFN_HIP(h->idx.alloc(sizeof(int32_t) * std::max<size_t>(idx.size(), 1)));
     if (!idx.empty()) QDFN_HIP(hipMemcpy(h->idx.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
-    // rows of split phase-2 row blocks, for a step kernel that forms them itself
-    {
-        std::vector<int32_t> fr(2 * n, -1);
-        std::vector<RowFinish> fb;
-        for (int64_t q = h->n_multi1; q < (int64_t)multi.size(); ++q) {
-            const HRowBlock& R = rbs[multi[q]];
-            RowFinish F;
-            F.part_base = R.part_base;
-            F.m = R.m;
-            F.ntiles = R.ntiles;
-            F.out = (int32_t)R.out;
-            F.pad = 0;
-            for (int32_t r = 0; r < R.m; ++r) fr[(R.kind == 1 ? 0 : n) + R.out + r] = (int32_t)fb.size();
-            fb.push_back(F);
-        }
-        QDFN_HIP(h->fin_row.alloc(sizeof(int32_t) * 2 * n));
-        QDFN_HIP(hipMemcpy(h->fin_row.p, fr.data(), sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice));
-        QDFN_HIP(h->fin_blk.alloc(sizeof(RowFinish) * std::max<size_t>(fb.size(), 1)));
-        if (!fb.empty())
-            QDFN_HIP(hipMemcpy(h->fin_blk.p, fb.data(), sizeof(RowFinish) * fb.size(), hipMemcpyHostToDevice));
-    }
     QDFN_HIP(h->t.alloc(sizeof(double) * std::max<int64_t>(h->n_t, 1)));
     QDFN_HIP(h->kdiag.alloc(sizeof(double) * n));
     QDFN_HIP(hipMemcpy(h->kdiag.p, h->kdiag_h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
@@ -1134,18 +1090,13 @@ int hmat_layout(qdfn_hmat* h) {
     QDFN_HIP(h->yN.alloc(sizeof(double) * n));
     // HBM bytes of one product: panel entries, indices, descriptors, t (write + read), x, yS, yN
     h->phase_bytes = (int64_t)(es * pool.size() + sizeof(uint16_t) * lo.size() + sizeof(int32_t) * idx.size() +
-                               sizeof(HRowBlock) * rbs.size() + sizeof(HTile) * tiles.size() + sizeof(VTask) * vtasks.size() +
-                               sizeof(VFin) * vfins.size() + 16 * part_total + 16 * vpart_total + 16 * h->n_t +
+                               sizeof(HUnit) * units.size() + sizeof(VTask) * vtasks.size() +
+                               sizeof(VFin) * vfins.size() + 8 * slot_rows + 8 * slots * 2 * n +
+                               16 * vpart_total + 16 * h->n_t +
                                (mixed ? 8 * h->n_t : 0) + 8 * 3 * n);
     h->device_bytes = h->phase_bytes;
     // algorithmic bytes of each phase's main kernel: stored entries (unpadded),
     // index list (phase 2), gathered right-hand side, outputs
-    // (phase 2: each output row once, also where it is formed from partial sums)
-    int64_t b2 = 0, k2 = 0;
-    for (int64_t q = h->n_rb1; q < (int64_t)rbs.size(); ++q) {
-        b2 += (int64_t)rbs[q].m * rbs[q].K * (rbs[q].lo ? (int64_t)sizeof(uint16_t) : es);
-        k2 += rbs[q].K;
-    }
     // phase 1: V entries, each panel's x slice once, t
     h->p1_entries = v_entries * es + v_entries_lo * (int64_t)sizeof(uint16_t);
     h->p1_other = 8 * v_cols + 8 * h->n_t + (mixed ? 8 * v_entries_lo_rows : 0);
@@ -1159,11 +1110,7 @@ int hmat_layout(qdfn_hmat* h) {
 }
 
 int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, int32_t form, double* d_yS,
-                double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2, bool finish_in_step) {
-    const HRowBlock* rbs = h->rbs.as<HRowBlock>();
-    const HTile* tiles = h->tiles.as<HTile>();
-    const int32_t* multi = h->multi.as<int32_t>();
-    double* part = h->part.as<double>();
+                double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2, bool finish) {
     const double* kd = h->kdiag.as<double>();
     double* t = h->t.as<double>();
     const bool f32 = h->prec == QDFN_PREC_FP32, bf = h->prec == QDFN_PREC_BF16_FAR, mx = h->prec == QDFN_PREC_MIXED;
@@ -1194,29 +1141,28 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
     }
     if (g.ran_log) hipLaunchKernelGGL(k_mark_skipped, dim3(1), dim3(1), 0, s, g);
     if (ev1) QDFN_HIP(hipEventRecord(ev1, s));
-    const bool finish = h->n_multi2 > 0 && !finish_in_step;
-    const int32_t mark_tiles = finish ? 0 : 1;
+    const HUnit* un = h->units.as<HUnit>();
+    const unsigned nu = (unsigned)h->n_units;
+    const int32_t* ix = h->idx.as<int32_t>();
+    double* pt = h->part.as<double>();
+    if (nu == 0) return fail(QDFN_ERR_STATE, "H-matrix: no phase-2 units");
     if (bf)
-        hipLaunchKernelGGL(k_hrow_mixed<bf16_t>, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
-                           tiles + h->n_tiles1, rbs, part, h->pool.as<float>(), h->pool_lo.as<bf16_t>(),
-                           h->idx.as<int32_t>(), d_x, t, h->n, kd, form, d_yS, d_yN, mark_tiles, g);
+        hipLaunchKernelGGL((k_hunit<float, bf16_t>), dim3(nu), dim3(kP2Threads), 0, s, un, h->pool.as<float>(),
+                           h->pool_lo.as<bf16_t>(), ix, d_x, t, h->n, pt, g);
     else if (mx)
-        hipLaunchKernelGGL(k_hrow_mixed<fp16_t>, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
-                           tiles + h->n_tiles1, rbs, part, h->pool.as<float>(), h->pool_lo.as<fp16_t>(),
-                           h->idx.as<int32_t>(), d_x, t, h->n, kd, form, d_yS, d_yN, mark_tiles, g);
+        hipLaunchKernelGGL((k_hunit<float, fp16_t>), dim3(nu), dim3(kP2Threads), 0, s, un, h->pool.as<float>(),
+                           h->pool_lo.as<fp16_t>(), ix, d_x, t, h->n, pt, g);
     else if (f32)
-        hipLaunchKernelGGL(k_hrow<float>, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
-                           tiles + h->n_tiles1, rbs, part, h->pool.as<float>(), h->idx.as<int32_t>(), d_x, t, h->n,
-                           kd, form, d_yS, d_yN, mark_tiles, g);
+        hipLaunchKernelGGL((k_hunit<float, fp16_t>), dim3(nu), dim3(kP2Threads), 0, s, un, h->pool.as<float>(),
+                           (const fp16_t*)nullptr, ix, d_x, t, h->n, pt, g);
     else
-        hipLaunchKernelGGL(k_hrow<double>, dim3((unsigned)h->n_tiles2), dim3(kPhase2Threads), 0, s,
-                           tiles + h->n_tiles1, rbs, part, h->pool.as<double>(), h->idx.as<int32_t>(), d_x, t, h->n,
-                           kd, form, d_yS, d_yN, mark_tiles, g);
+        hipLaunchKernelGGL((k_hunit<double, fp16_t>), dim3(nu), dim3(kP2Threads), 0, s, un, h->pool.as<double>(),
+                           (const fp16_t*)nullptr, ix, d_x, t, h->n, pt, g);
     QDFN_HIP(hipGetLastError());
     if (ev2) QDFN_HIP(hipEventRecord(ev2, s));
     if (finish) {
-        hipLaunchKernelGGL(k_hfinish, dim3((unsigned)h->n_multi2), dim3(64), 0, s, multi + h->n_multi1, rbs, part,
-                           d_x, kd, form, t, d_yS, d_yN, 1, g);
+        hipLaunchKernelGGL(k_hsum, dim3((unsigned)((2 * h->n + 255) / 256)), dim3(256), 0, s, pt,
+                           (int32_t)h->n_slots, h->n, d_x, kd, form, d_yS, d_yN, g);
         QDFN_HIP(hipGetLastError());
     }
     return QDFN_OK;

@@ -57,7 +57,7 @@ static int enqueue_attempt(qdfn_sim* s, int slot, hipEvent_t ev0, hipEvent_t ev1
     int rc;
     if (s->hop) {
         // timed mode: events around the phase-2 kernel (the dominant one)
-        rc = launch_hmat(s->hop, s->A.D, g, 1, s->A.tl, s->A.sigprod, s->stream, nullptr, ev0, ev1, true);
+        rc = launch_hmat(s->hop, s->A.D, g, 1, s->A.tl, s->A.sigprod, s->stream, nullptr, ev0, ev1, false);
         if (rc) return rc;
     } else {
         if (ev0) QDFN_HIP(hipEventRecord(ev0, s->stream));
@@ -189,10 +189,9 @@ static qdfn_sim* sim_create(const char* who, int64_t n_op, const double* kself_h
     A.maximum_dt = p->maximum_dt; A.min_sig = p->min_normal_stress; A.alpha_evo = p->alpha_evo;
     A.conv = 1e-9; A.v0 = V0; A.dr = p->evolution_dr; A.planar = p->planar || normal_zero;
     A.ext_at_told = p->ext_at_told;
-    A.fin_row = nullptr;
-    A.fin_blk = nullptr;
-    A.fin_part = nullptr;
-    A.fin_kdiag = nullptr;
+    A.hpart = nullptr;
+    A.hkdiag = nullptr;
+    A.hslots = 0;
     return s;
 }
 
@@ -230,10 +229,9 @@ extern "C" qdfn_sim_t* qdfn_sim_create_hmat(qdfn_hmat_t* h, const qdfn_sim_param
                              theta_i, v_i, friction_i, sigma0, loading_rate);
     if (s) {
         s->hop = h;
-        s->A.fin_row = h->fin_row.as<int32_t>();
-        s->A.fin_blk = h->fin_blk.as<RowFinish>();
-        s->A.fin_part = h->part.as<double>();
-        s->A.fin_kdiag = h->kdiag.as<double>();
+        s->A.hpart = h->part.as<double>();
+        s->A.hkdiag = h->kdiag.as<double>();
+        s->A.hslots = h->n_slots;
     }
     return s;
 }

@@ -54,15 +54,6 @@ struct BlockPart {
     double mindt, vmax, vmax_next, dtmin_next;
 };
 
-// A row block of an H operator whose rows are summed from tile partials:
-// row i (out <= i < out + m) is the sum over q < ntiles, in order, of
-// part[part_base + q * m + (i - out)].  With an H operator the step kernel
-// forms these rows itself (the product's separate finish is not launched).
-struct RowFinish {
-    int64_t part_base;
-    int32_t m, ntiles, out, pad;
-};
-
 // Device arrays of a run.  Double-buffered state arrays are [2][n].
 struct SimArrays {
     int64_t n, n_loading;
@@ -86,23 +77,16 @@ struct SimArrays {
     double* h_dtsel;  // selected Dt of each recorded step (before any retry)
     int8_t* h_sw;     // solver of each element in the recorded step's accepted attempt (1: HighV)
     double *t_told, *t_dt, *t_vmax;
-    // H operator rows formed here (null otherwise): fin_row[i] / fin_row[n + i]
-    // index fin_blk for the shear / normal row i (-1: the product wrote it)
-    const int32_t* fin_row;
-    const RowFinish* fin_blk;
-    const double* fin_part;
-    const double* fin_kdiag;  // K_S,ii of the operator (loading form: TL = (-sum) / K_S,ii)
+    // H operator (null otherwise): its product's unit slots, hslots x [2][n]
+    // row sums that this kernel adds in slot order (TL = (-sum_S) / K_S,ii,
+    // SigmaProd = sum_N, the loading form of the product)
+    const double* hpart;
+    const double* hkdiag;
+    int64_t hslots;
     // parameters
     double mu, rho, switch_v, dt_cut, rupture_dt, maximum_dt, min_sig, alpha_evo, conv, v0;
     int32_t dr, planar, ext_at_told, pad;
 };
-
-__device__ inline double finish_row(const SimArrays& A, int32_t b, int64_t i) {
-    const RowFinish F = A.fin_blk[b];
-    double s = 0.0;
-    for (int32_t q = 0; q < F.ntiles; ++q) s += A.fin_part[F.part_base + (int64_t)q * F.m + (i - F.out)];
-    return s;
-}
 
 // The reference's scan for InterpolateFromStressChange (Functions_Solvers.jl:
 // 465-540): the 1-based index of the last sample time not after `time` (0 if
@@ -285,18 +269,20 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
             const double te = A.ext_at_told ? told : told + dt;
             ext_interp_at(A.ext_t, A.ext_n, A.ext_s, A.nt, n, ext_index(A.ext_t, A.nt, te), te, i, dsn, dtau);
         }
-        double tl_i = A.tl[i], sp_i = A.planar ? 0.0 : A.sigprod[i];
-        if (A.fin_row) {  // rows of split H row blocks (the finish of the product, in the same order)
-            const int32_t bs = A.fin_row[i], bn = A.fin_row[n + i];
-            if (bs >= 0) {
-                tl_i = (-finish_row(A, bs, i)) / A.fin_kdiag[i];
-                A.tl[i] = tl_i;
+        double tl_i, sp_i;
+        if (A.hpart) {
+            double sS = 0.0, sN = 0.0;
+            for (int64_t u = 0; u < A.hslots; ++u) {
+                sS += A.hpart[u * 2 * n + i];
+                sN += A.hpart[u * 2 * n + n + i];
             }
-            if (bn >= 0) {
-                const double v = finish_row(A, bn, i);
-                A.sigprod[i] = v;
-                if (!A.planar) sp_i = v;
-            }
+            tl_i = (-sS) / A.hkdiag[i];
+            A.tl[i] = tl_i;
+            A.sigprod[i] = sN;
+            sp_i = A.planar ? 0.0 : sN;
+        } else {
+            tl_i = A.tl[i];
+            sp_i = A.planar ? 0.0 : A.sigprod[i];
         }
         double sig = sp_i + c_sigma0 + dsn;
         if (sig < A.min_sig) sig = A.min_sig;

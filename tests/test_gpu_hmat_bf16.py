@@ -108,8 +108,9 @@ def _parts(blocks):
 
 
 def test_synthetic_layout_cases(qd, monkeypatch):
-    # tiles of 1,024 columns so that moderate panels split on both sides
-    monkeypatch.setenv("QDFN_HMAT_TILE", "1024")
+    # phase-2 units of 64 KiB so that moderate panels split into several units
+    # on both sides (several slots per row group)
+    monkeypatch.setenv("QDFN_HMAT_UNIT", "65536")
     rng = np.random.default_rng(31)
     nr, w = 5, 600
     n = nr * w
@@ -119,8 +120,8 @@ def test_synthetic_layout_cases(qd, monkeypatch):
             rs.append([bj * w + 1, (bj + 1) * w, bi * w + 1, (bi + 1) * w])
             ad.append(0 if bi == bj else 1)
     rs, ad = np.array(rs, dtype=np.int64), np.array(ad, dtype=np.int64)
-    # block row 0: dense only (3,000 columns: three FP32 tiles, no BF16 panel)
-    # block row 1: dense diagonal + four factors of rank 300 (1,200 BF16 columns: two tiles)
+    # block row 0: dense only (3,000 FP32 columns: several units, no BF16 panel)
+    # block row 1: dense diagonal + four factors of rank 300 (1,200 BF16 columns: two units)
     # block row 2: dense diagonal and (2, 3), factors of ranks 0, 7 and 64
     # block row 3: dense diagonal + factors of ranks 1..4 with scales 1e-6..1e6
     # block row 4: factors only (normal matrix; the shear diagonal stays dense)
