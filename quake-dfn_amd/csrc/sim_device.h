@@ -240,6 +240,66 @@ __device__ __forceinline__ void setup_body(const SimArrays& A, SimState* __restr
 // One attempt of SolveOneTimeStep (Functions_Solvers.jl:334-380) with the
 // effective normal stress of main() (:212-226), plus the step decision, for
 // workgroup blk of the nblk (kStepBlock threads each) that cover the run.
+// The retry-or-accept decision of main() (:240-319) from the reduction `all`
+// of every workgroup's part, taken by one thread: the run state S as the
+// attempt read it.
+__device__ inline void decide(const SimArrays& A, SimState* __restrict__ st, const SimState& S, const BlockPart& all) {
+    const int p = S.parity;
+    const double told = S.told;
+    const int64_t step = S.step;
+    const bool record = A.record_step > 0 && (step + 1) % A.record_step == 0 && S.rec < A.rec_cap;
+    const int64_t rec = S.rec;
+    const int OK = all.ok, ANY4 = all.any4, FOUND = all.found;
+    const double MINDT = all.mindt, VMAX = all.vmax, VNEXT = all.vmax_next, DNEXT = all.dtmin_next;
+    if (!OK) {  // retry the step with the reduced Dt (:240-258)
+        st->dt = ANY4 ? 1e-3 : MINDT;
+        st->need_coupling = 0;
+        st->retry_passes = S.retry_passes + 1;
+        if (S.iteration > 100) {
+            st->status = QDFN_ERR_RETRIES;
+            st->done = 1;
+        }
+        st->iteration = S.iteration + 1;
+        return;
+    }
+    // accept (:262-319); Dt = min(Dt_All) == dt when every element converged
+    const double dta = MINDT;
+    if (S.ntrace < A.trace_cap) {
+        A.t_told[S.ntrace] = told;
+        A.t_dt[S.ntrace] = dta;
+        A.t_vmax[S.ntrace] = VMAX;
+        st->ntrace = S.ntrace + 1;
+    }
+    if (record) {
+        A.h_time[rec] = told;
+        A.h_dt[rec] = dta;
+        A.h_dtsel[rec] = S.dt_first;
+        st->rec = rec + 1;
+    }
+    st->told = told + dta;
+    st->parity = 1 - p;
+    st->step = step + 1;
+    if (step + 1 >= S.target || (S.t_end > 0.0 && told + dta >= S.t_end)) {
+        st->dt = dta;
+        st->done = 1;
+        st->need_coupling = 0;
+        return;
+    }
+    if (!FOUND) {
+        st->dt = dta;
+        st->status = QDFN_ERR_EMPTY_DTMIN;
+        st->done = 1;
+        st->need_coupling = 0;
+        return;
+    }
+    const double dt_next = select_dt(A, dta, VNEXT, DNEXT / A.dt_cut);
+    st->dt = dt_next;
+    st->dt_first = dt_next;
+    st->iteration = 1;
+    st->need_coupling = 1;
+    st->rhs_evals = S.rhs_evals + 1;
+}
+
 __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restrict__ st, unsigned blk, unsigned nblk) {
     __shared__ BlockPart shp[kStepBlock / 64];
     __shared__ int s_last;
@@ -367,55 +427,7 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
     all = block_reduce_part<kStepBlock>(all, shp);
     if (threadIdx.x != 0) return;
     *A.arrive = 0u;
-    const int OK = all.ok, ANY4 = all.any4, FOUND = all.found;
-    const double MINDT = all.mindt, VMAX = all.vmax, VNEXT = all.vmax_next, DNEXT = all.dtmin_next;
-    if (!OK) {  // retry the step with the reduced Dt (:240-258)
-        st->dt = ANY4 ? 1e-3 : MINDT;
-        st->need_coupling = 0;
-        st->retry_passes = S.retry_passes + 1;
-        if (S.iteration > 100) {
-            st->status = QDFN_ERR_RETRIES;
-            st->done = 1;
-        }
-        st->iteration = S.iteration + 1;
-        return;
-    }
-    // accept (:262-319); Dt = min(Dt_All) == dt when every element converged
-    const double dta = MINDT;
-    if (S.ntrace < A.trace_cap) {
-        A.t_told[S.ntrace] = told;
-        A.t_dt[S.ntrace] = dta;
-        A.t_vmax[S.ntrace] = VMAX;
-        st->ntrace = S.ntrace + 1;
-    }
-    if (record) {
-        A.h_time[rec] = told;
-        A.h_dt[rec] = dta;
-        A.h_dtsel[rec] = S.dt_first;
-        st->rec = rec + 1;
-    }
-    st->told = told + dta;
-    st->parity = 1 - p;
-    st->step = step + 1;
-    if (step + 1 >= S.target || (S.t_end > 0.0 && told + dta >= S.t_end)) {
-        st->dt = dta;
-        st->done = 1;
-        st->need_coupling = 0;
-        return;
-    }
-    if (!FOUND) {
-        st->dt = dta;
-        st->status = QDFN_ERR_EMPTY_DTMIN;
-        st->done = 1;
-        st->need_coupling = 0;
-        return;
-    }
-    const double dt_next = select_dt(A, dta, VNEXT, DNEXT / A.dt_cut);
-    st->dt = dt_next;
-    st->dt_first = dt_next;
-    st->iteration = 1;
-    st->need_coupling = 1;
-    st->rhs_evals = S.rhs_evals + 1;
+    decide(A, st, S, all);
 }
 
 }  // namespace qdfn
