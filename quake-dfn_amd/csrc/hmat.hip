@@ -262,7 +262,8 @@ __device__ __forceinline__ void p2_slice(const T* __restrict__ panel, const int3
     const int g = lane / L, j = lane - g * L;  // g == G: idle lane (j < L still)
     // a chunk is SC steps (a multiple of D) of G columns
     const int32_t SC = D * max(1, kP2Chunk / (G * D)), C = SC * G;
-    const int32_t ns = (k + G - 1) / G, nsd = (ns + SC - 1) / SC * SC, nchunk = nsd / SC;
+    // steps rounded up to the pipeline depth only: the last chunk stops there
+    const int32_t ns = (k + G - 1) / G, nsd = (ns + D - 1) / D * D, nchunk = (ns + SC - 1) / SC;
     const T* base = panel + j * W;
     // gather of chunk q: indices, then values (zero past the slice)
     auto load_idx = [&](int32_t q, int32_t iv[kPer]) {
@@ -312,7 +313,7 @@ __device__ __forceinline__ void p2_slice(const T* __restrict__ panel, const int3
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int32_t s_beg = q * SC, s_end = s_beg + SC;
+        const int32_t s_beg = q * SC, s_end = min(s_beg + SC, nsd);
         const double* zq = zs[q & 1] + min(g, G - 1) - s_beg * G;
         for (int32_t s0 = s_beg; s0 < s_end; s0 += D) {
 #pragma unroll
