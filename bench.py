@@ -42,6 +42,7 @@ METRIC = "RHS evals/sec + sim-years/hour, mixed vs FP64 on GPU at stated event-t
 HBM_PEAK_GBS = 8000.0
 PROFILE = os.path.join(ROOT, "profiles", "hmat_phase2_mixed_rough_fault_20000.md")
 YEAR = 3.15576e7
+CHECKPOINT_EVERY = 50  # adjoint workload: forward checkpoints, segments recomputed in the backward sweep
 
 
 def measured_traffic():
@@ -416,14 +417,16 @@ def adjoint_case(qd, args):
     windows = [("full_window", K, 4)]
     while windows:
         name, K, n_fd = windows.pop(0)
-        sim = qd.Simulation(op, prob, record_step=1, record_capacity=K)
+        # forward with a checkpoint every 50 steps; the backward sweep
+        # recomputes each 50-step segment from its checkpoint
+        sim = qd.Simulation(op, prob, record_step=0, record_capacity=0, trace_capacity=K)
         t0 = time.perf_counter()
-        sim.run(K)
+        sim.run_checkpointed(K, CHECKPOINT_EVERY)
         fwd_s = time.perf_counter() - t0
-        h = sim.history()
-        dK = h["History_Disp"][-1]
+        dK = sim.state()["Disp"]
         target = dK[obs] * 1.1 + 1e-3
-        _, t_first = adjoint.misfit(h, 0, obs, target)
+        tr = sim.trace()
+        t_first = adjoint._onset(tr["told"], tr["dt"], tr["vmax"], 1e-3)
         if t_first is not None:
             t_target = t_first * 1.05
             w_time = float(np.sum((dK[obs] - target) ** 2)) / (t_first - t_target) ** 2
@@ -473,7 +476,8 @@ def adjoint_case(qd, args):
                              "cpu_s": round(time.perf_counter() - t0, 2)})
             log(f"[bench] adjoint ({name}): dot-product check against the dual-number oracle: rel. errors "
                 + ", ".join(f"{d_['rel_err']:.2e}" for d_ in dual))
-        out[name] = {"steps": K, "loss": L, "time_term": w_time > 0.0, "forward_ms": round(fwd_s * 1e3, 2),
+        out[name] = {"steps": K, "checkpoint_every": CHECKPOINT_EVERY, "loss": L, "time_term": w_time > 0.0,
+                     "forward_ms": round(fwd_s * 1e3, 2),
                      "lambda_growth": info["lambda_growth"], "growth_step": info["growth_step"],
                      "verifiable": info["growth_step"] < 0, "dual_dot_product": dual,
                      "backward_ms": round(info["ms_backward"], 2), "backward_wall_ms": round(bwd_s * 1e3, 2),

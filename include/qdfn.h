@@ -376,6 +376,20 @@ typedef int (*qdfn_record_cb)(void* user, int64_t first, int64_t count, const do
  * status. */
 int qdfn_sim_run_until(qdfn_sim_t* sim, double t_end, int64_t max_steps, int32_t graph_attempts, qdfn_record_cb cb,
                        void* user);
+/* ConvergenceCrit of the local solves (Solver_LowV_D / Solver_HighV_D stop
+ * when |V_trial / V - 1| falls below it; 1e-9 in the reference,
+ * Functions_RSFDFN3DMain_D.jl, the default here).  Before the run starts.
+ * A tighter value makes the run a smoother function of its parameters (the
+ * finite-difference checks of the adjoint use it). */
+int qdfn_sim_set_convergence(qdfn_sim_t* sim, double conv_crit);
+/* Advance a dense run from its start by n_steps accepted steps, keeping the
+ * run state every `every` steps (checkpoints; 11 n doubles each) and the
+ * per-step trace (trace_capacity >= n_steps), for qdfn_sim_adjoint's
+ * checkpointed recomputation: each segment of `every` steps is replayed from
+ * its checkpoint (bitwise the forward's steps) just before its backward
+ * sweep, so the tape takes `every` records instead of n_steps.  No reference
+ * counterpart (the reference has no gradients). */
+int qdfn_sim_run_checkpointed(qdfn_sim_t* sim, int64_t n_steps, int64_t every, int32_t graph_attempts);
 int qdfn_sim_get_stats(qdfn_sim_t* sim, qdfn_sim_stats* st);
 /* Current state (after the last accepted step): V, Disp, Theta, Friction,
  * EffNormalStress, Accel (each length n, any may be NULL). */
@@ -397,8 +411,9 @@ void qdfn_sim_destroy(qdfn_sim_t* sim);
  * Gradient with respect to the per-element a, b and Dc of
  *   L = sum_o (Disp_K[obs_o] - d_target_o)^2 + w_time (t_first - t_target)^2
  * for the K accepted steps a dense run has taken since its creation (it must
- * record every step: record_step = 1, record_capacity >= K; aging law with
- * alpha = 0).  t_first: the first step whose max V over the
+ * record every step: record_step = 1, record_capacity >= K, or have been
+ * advanced by qdfn_sim_run_checkpointed, whose segments are recomputed from
+ * their checkpoints; aging law with alpha = 0).  t_first: the first step whose max V over the
  * non-loading elements reaches v_onset, interpolated in log10 V between the
  * end times of that step and the one before (the term is dropped if
  * w_time = 0 or no onset occurs).  v_i / theta_i: the run's initial V and

@@ -72,22 +72,25 @@ struct AdjArgs {
     const double *a, *b, *dc, *f0, *kself, *omega, *mass, *tau0, *sigma0, *lrate;
     const int8_t* cand;
     const double *v_init, *th_init;
+    // state before the first step of the tape held in h_* (seg_d null: zero)
+    const double *seg_v, *seg_d, *seg_th;
     int64_t nt;
     const double *ext_t, *ext_n, *ext_s;
     double mu, rho, switch_v, dt_cut, rupture_dt, maximum_dt, min_sig, v0;
     int32_t ext_at_told, planar;
 };
 
-// state at record k (k = -1: the initial state)
+// the inputs and the accepted V of the step at tape record k (record -1: the
+// state before the tape's first step)
 __global__ __launch_bounds__(kAdjBlock) void k_adj_gather(AdjArgs A) {
     const int64_t i = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
     if (i >= A.n) return;
     const int64_t k = A.k;
     A.v_new[i] = A.h_v[k + i * A.rec_cap];
     if (k == 0) {
-        A.v_old[i] = A.v_init[i];
-        A.d_old[i] = 0.0;
-        A.th_old[i] = A.th_init[i];
+        A.v_old[i] = A.seg_v[i];
+        A.d_old[i] = A.seg_d ? A.seg_d[i] : 0.0;
+        A.th_old[i] = A.seg_th[i];
     } else {
         A.v_old[i] = A.h_v[(k - 1) + i * A.rec_cap];
         A.d_old[i] = A.h_d[(k - 1) + i * A.rec_cap];
@@ -399,6 +402,14 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_f0(AdjArgs A) {
 
 using namespace qdfn;
 
+namespace {
+// the replay run of checkpointed recomputation, destroyed on every return
+struct ReplayRun {
+    qdfn_sim* r = nullptr;
+    ~ReplayRun() { qdfn_sim_destroy(r); }
+};
+}  // namespace
+
 extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* theta_i, int64_t n_obs,
                                 const int64_t* obs, const double* d_target, double w_time, double t_target,
                                 double v_onset, double* loss, double* grad_a, double* grad_b, double* grad_dc,
@@ -416,31 +427,39 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
     QDFN_HIP(hipMemcpy(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost));
     const SimState st = s->h_st;
     if (st.status != QDFN_OK) return fail(QDFN_ERR_STATE, "qdfn_sim_adjoint: the run is in an error state");
-    if (P.record_step != 1 || st.rec != st.step || st.step < 1)
-        return fail(QDFN_ERR_STATE, "qdfn_sim_adjoint: the run must record every step from its start");
-    const int64_t K = st.step, cap = P.record_capacity;
+    // tape: the run's own history of every step (record_step 1), or the
+    // checkpoints of qdfn_sim_run_checkpointed, each segment recomputed from
+    // its checkpoint into a replay run just before its backward sweep
+    const bool ckm = s->ck_every > 0;
+    const int64_t K = st.step, C = s->ck_every;
+    if (ckm) {
+        if (K < 1 || (int64_t)s->ck_state.size() != (K + C - 1) / C || st.ntrace != K)
+            return fail(QDFN_ERR_STATE, "qdfn_sim_adjoint: the run was advanced after qdfn_sim_run_checkpointed");
+    } else if (P.record_step != 1 || st.rec != st.step || st.step < 1) {
+        return fail(QDFN_ERR_STATE, "qdfn_sim_adjoint: the run must record every step from its start "
+                                    "(or be made by qdfn_sim_run_checkpointed)");
+    }
+    const int64_t cap = P.record_capacity;
     hipStream_t strm = s->stream;
     const SimArrays& SA = s->A;
-    // host copies of the scalar history and V / Disp rows needed for the loss
+    // per-step scalars and the V / Disp rows needed for the loss
     std::vector<double> h_time(K), h_dt(K), h_dtsel(K), dK(n), retry_factor(K);
-    QDFN_HIP(hipMemcpy(h_time.data(), SA.h_time, sizeof(double) * K, hipMemcpyDeviceToHost));
-    QDFN_HIP(hipMemcpy(h_dt.data(), SA.h_dt, sizeof(double) * K, hipMemcpyDeviceToHost));
-    QDFN_HIP(hipMemcpy(h_dtsel.data(), SA.h_dtsel, sizeof(double) * K, hipMemcpyDeviceToHost));
-    int64_t retried = 0;
-    for (int64_t k = 0; k < K; ++k) {
-        const double q = h_dt[k] / h_dtsel[k];
-        int ex = 0;
-        const double m = frexp(q, &ex);
-        retry_factor[k] = (m == 0.5 && ex <= 1) ? q : 0.0;  // exact power of two (halvings), else 1e-3 reset
-        retried += h_dt[k] != h_dtsel[k];
-    }
-    QDFN_HIP(hipMemcpy2D(dK.data(), sizeof(double), SA.h_d + (K - 1), sizeof(double) * cap, sizeof(double), n,
-                         hipMemcpyDeviceToHost));
-    // vmax per step over the non-loading elements and its argmax (onset time)
     const int64_t nn = n - P.n_loading;
     std::vector<double> vmax(K, -INFINITY);
-    std::vector<int64_t> vmax_arg(K, 0);
-    {
+    std::vector<int64_t> vmax_arg(K, -1);
+    if (ckm) {
+        QDFN_HIP(hipMemcpy(h_time.data(), SA.t_told, sizeof(double) * K, hipMemcpyDeviceToHost));
+        QDFN_HIP(hipMemcpy(h_dt.data(), SA.t_dt, sizeof(double) * K, hipMemcpyDeviceToHost));
+        QDFN_HIP(hipMemcpy(h_dtsel.data(), SA.t_dtsel, sizeof(double) * K, hipMemcpyDeviceToHost));
+        QDFN_HIP(hipMemcpy(vmax.data(), SA.t_vmax, sizeof(double) * K, hipMemcpyDeviceToHost));
+        QDFN_HIP(hipMemcpy(dK.data(), SA.D + (int64_t)st.parity * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+    } else {
+        QDFN_HIP(hipMemcpy(h_time.data(), SA.h_time, sizeof(double) * K, hipMemcpyDeviceToHost));
+        QDFN_HIP(hipMemcpy(h_dt.data(), SA.h_dt, sizeof(double) * K, hipMemcpyDeviceToHost));
+        QDFN_HIP(hipMemcpy(h_dtsel.data(), SA.h_dtsel, sizeof(double) * K, hipMemcpyDeviceToHost));
+        QDFN_HIP(hipMemcpy2D(dK.data(), sizeof(double), SA.h_d + (K - 1), sizeof(double) * cap, sizeof(double), n,
+                             hipMemcpyDeviceToHost));
+        // vmax per step over the non-loading elements and its argmax (onset time)
         std::vector<double> hv((size_t)cap * nn);
         QDFN_HIP(hipMemcpy(hv.data(), SA.h_v, sizeof(double) * cap * nn, hipMemcpyDeviceToHost));
         for (int64_t i = 0; i < nn; ++i)
@@ -451,6 +470,14 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
                     vmax_arg[k] = i;
                 }
             }
+    }
+    int64_t retried = 0;
+    for (int64_t k = 0; k < K; ++k) {
+        const double q = h_dt[k] / h_dtsel[k];
+        int ex = 0;
+        const double m = frexp(q, &ex);
+        retry_factor[k] = (m == 0.5 && ex <= 1) ? q : 0.0;  // exact power of two (halvings), else 1e-3 reset
+        retried += h_dt[k] != h_dtsel[k];
     }
     double L = 0.0;
     std::vector<double> lam_dK(n, 0.0);
@@ -467,6 +494,8 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
             break;
         }
     double t_first = NAN;
+    // a seed's element (the step's argmax of V) is found from the tape when
+    // the backward sweep reaches its step (i = -2 until then; -1: time only)
     struct Seed { int64_t k; int64_t i; double lv; double lt; };
     std::vector<Seed> seeds;
     if (w_time > 0.0 && kon >= 1) {
@@ -483,8 +512,9 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
             df_dl1 = -(lt - l0) / ((l1 - l0) * (l1 - l0));
         }
         const double ln10 = log(10.0);
-        seeds.push_back({kon - 1, vmax_arg[kon - 1], g * (te1 - te0) * df_dl0 / (vmax[kon - 1] * ln10), g * (1.0 - f)});
-        seeds.push_back({kon, vmax_arg[kon], g * (te1 - te0) * df_dl1 / (vmax[kon] * ln10), g * f});
+        seeds.push_back({kon - 1, ckm ? -2 : vmax_arg[kon - 1], g * (te1 - te0) * df_dl0 / (vmax[kon - 1] * ln10),
+                         g * (1.0 - f)});
+        seeds.push_back({kon, ckm ? -2 : vmax_arg[kon], g * (te1 - te0) * df_dl1 / (vmax[kon] * ln10), g * f});
     } else if (w_time > 0.0 && kon == 0) {
         t_first = h_time[0] + h_dt[0];
         const double r = t_first - t_target;
@@ -523,6 +553,7 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
     A.a = SA.a; A.b = SA.b; A.dc = SA.dc; A.f0 = SA.f0; A.kself = SA.kself; A.omega = SA.omega; A.mass = SA.mass;
     A.tau0 = SA.tau0; A.sigma0 = SA.sigma0; A.lrate = SA.lrate; A.cand = SA.cand;
     A.v_init = init.as<double>(); A.th_init = init.as<double>() + n;
+    A.seg_v = A.v_init; A.seg_d = nullptr; A.seg_th = A.th_init;
     A.nt = SA.nt; A.ext_t = SA.ext_t; A.ext_n = SA.ext_n; A.ext_s = SA.ext_s;
     A.mu = SA.mu; A.rho = SA.rho; A.switch_v = SA.switch_v; A.dt_cut = SA.dt_cut; A.rupture_dt = SA.rupture_dt;
     A.maximum_dt = SA.maximum_dt; A.min_sig = SA.min_sig; A.v0 = SA.v0;
@@ -538,8 +569,51 @@ extern "C" int qdfn_sim_adjoint(qdfn_sim_t* s, const double* v_i, const double* 
     const qdfn_dense* op = s->op;
     const unsigned gb = (unsigned)nblk;
     int cur = 0;
+    ReplayRun rp;
+    int64_t seg_lo = K;  // first step of the recomputed segment (checkpoint mode)
+    if (ckm) {
+        rp.r = sim_clone(s, C);
+        if (!rp.r) return QDFN_ERR_NOMEM;
+    }
     for (int64_t k = K - 1; k >= 0; --k) {
-        A.k = k;
+        if (ckm && k < seg_lo) {
+            // recompute segment j from its checkpoint: every step recorded
+            const int64_t j = k / C, cnt = std::min(C, K - j * C);
+            seg_lo = j * C;
+            QDFN_HIP(hipStreamSynchronize(strm));
+            int rc = sim_restore(s, j, rp.r);
+            if (rc == QDFN_OK) rc = qdfn_sim_run(rp.r, cnt, 0, 32);
+            if (rc) return rc;
+            // the replay must retrace the forward's steps exactly
+            std::vector<double> rdt(cnt), rt(cnt);
+            QDFN_HIP(hipMemcpy(rdt.data(), rp.r->A.h_dt, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+            QDFN_HIP(hipMemcpy(rt.data(), rp.r->A.h_time, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+            for (int64_t q = 0; q < cnt; ++q)
+                if (rdt[q] != h_dt[seg_lo + q] || rt[q] != h_time[seg_lo + q])
+                    return fail(QDFN_ERR_STATE, "qdfn_sim_adjoint: the recomputed segment departs from the forward run");
+            for (Seed& sd : seeds)
+                if (sd.i == -2 && sd.k >= seg_lo && sd.k < seg_lo + cnt) {
+                    std::vector<double> vrow(nn);
+                    QDFN_HIP(hipMemcpy2D(vrow.data(), sizeof(double), rp.r->A.h_v + (sd.k - seg_lo), sizeof(double) * C,
+                                         sizeof(double), nn, hipMemcpyDeviceToHost));
+                    double vm = -INFINITY;
+                    for (int64_t i = 0; i < nn; ++i)
+                        if (vrow[i] > vm) {
+                            vm = vrow[i];
+                            sd.i = i;
+                        }
+                }
+            const SimArrays& RA = rp.r->A;
+            A.rec_cap = C;
+            A.h_time = RA.h_time; A.h_dt = RA.h_dt; A.h_v = RA.h_v; A.h_d = RA.h_d; A.h_th = RA.h_th;
+            A.h_sw = RA.h_sw;
+            const double* ck = s->ckpt.as<double>() + j * 11 * n;
+            const int64_t po = (int64_t)s->ck_state[j].parity * n;
+            A.seg_v = ck + po;
+            A.seg_d = ck + 2 * n + po;
+            A.seg_th = ck + 4 * n + po;
+        }
+        A.k = ckm ? k - seg_lo : k;
         A.lv_cur = lbuf[cur][0]; A.ld_cur = lbuf[cur][1]; A.lth_cur = lbuf[cur][2];
         A.lv_prev = lbuf[cur ^ 1][0]; A.ld_prev = lbuf[cur ^ 1][1]; A.lth_prev = lbuf[cur ^ 1][2];
         hipLaunchKernelGGL(k_adj_gather, dim3(gb), dim3(kAdjBlock), 0, strm, A);

@@ -37,5 +37,20 @@ struct qdfn_sim {
     double coup_ms_sum = 0.0;
     int64_t coup_count = 0;
     int64_t rec_flushed = 0;  // records already handed to a record callback
+    // checkpoints of qdfn_sim_run_checkpointed: the run state (V, D, TH, F,
+    // SIG both halves, Accel: 11 n doubles) and the state block before steps
+    // 0, every, 2 every, ...
+    qdfn::DevBuf ckpt;
+    std::vector<qdfn::SimState> ck_state;
+    int64_t ck_every = 0;
 };
+
+namespace qdfn {
+// A run on the same operator with the same constants, external tables and
+// parameters as s, recording every step into record_capacity records (no
+// trace): the replay run of checkpointed recomputation.
+qdfn_sim* sim_clone(const qdfn_sim* s, int64_t record_capacity);
+// Sets run r's state to checkpoint c of s (r a clone of s).
+int sim_restore(const qdfn_sim* s, int64_t c, qdfn_sim* r);
+}  // namespace qdfn
 

@@ -82,64 +82,22 @@ extern "C" void qdfn_sim_destroy(qdfn_sim_t* s) {
     if (st) (void)hipStreamDestroy(st);
 }
 
-// Shared set-up of main() / main_H(): kself = K_Self per element (host).
-static qdfn_sim* sim_create(const char* who, int64_t n_op, const double* kself_h, int normal_zero,
-                            const qdfn_sim_params* p, const double* mass, const double* a, const double* b,
-                            const double* dc, const double* theta_i, const double* v_i, const double* friction_i,
-                            const double* sigma0, const double* loading_rate) {
-    if (!p || !mass || !a || !b || !dc || !theta_i || !v_i || !friction_i || !sigma0 || !loading_rate) {
-        fail(QDFN_ERR_ARG, std::string(who) + ": null argument");
-        return nullptr;
-    }
-    if (p->n != n_op || p->n_loading < 0 || p->n_loading > p->n || p->record_capacity < 0 ||
-        p->trace_capacity < 0) {
-        fail(QDFN_ERR_ARG, std::string(who) + ": size mismatch");
-        return nullptr;
-    }
+// Device buffers of a run of p->n elements and their wiring into SimArrays
+// (values uploaded by the caller).  Returns nullptr after fail().
+static qdfn_sim* sim_alloc(const char* who, const qdfn_sim_params* p) {
     const int64_t n = p->n, L = p->n_loading;
     qdfn_sim* s = new qdfn_sim();
     s->P = *p;
     auto bail = [&](const char* msg) -> qdfn_sim_t* {
-        fail(QDFN_ERR_HIP, msg);
+        fail(QDFN_ERR_HIP, std::string(who) + ": " + msg);
         qdfn_sim_destroy(s);
         return nullptr;
     };
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) return bail("stream");
-    // host-side constants (QUAKEDFN_RunARG.jl / main():42-87)
-    const double V0 = 1e-9;
-    std::vector<double> kself(kself_h, kself_h + n), hc(10 * n);
-    std::vector<int8_t> cand(n, 0);
-    int64_t n_unstable = 0;
-    for (int64_t i = 0; i < n; ++i) n_unstable += (a[i] - b[i] < 0.0);
-    for (int64_t i = 0; i < n; ++i) cand[i] = n_unstable > 0 ? (a[i] - b[i] < 0.0) : (i < n - L);
-    double* C = hc.data();
-    for (int64_t i = 0; i < n; ++i) {
-        C[0 * n + i] = a[i];
-        C[1 * n + i] = b[i];
-        C[2 * n + i] = dc[i];
-        C[3 * n + i] = friction_i[i] - a[i] * log(v_i[i] / V0) - b[i] * log(theta_i[i] * V0 / dc[i]);
-        C[4 * n + i] = kself[i];
-        C[5 * n + i] = sqrt(kself[i] / mass[i]);
-        C[6 * n + i] = mass[i];
-        C[7 * n + i] = sigma0[i] * friction_i[i];
-        C[8 * n + i] = sigma0[i];
-        C[9 * n + i] = loading_rate[i];
-    }
-    if (s->consts.alloc(sizeof(double) * 10 * n) ||
-        hipMemcpy(s->consts.p, C, sizeof(double) * 10 * n, hipMemcpyHostToDevice))
-        return bail("constants upload");
+    // constants [a, b, Dc, f0, K_Self, Omega, Mass, tau0, sigma0, LoadingRate]
+    if (s->consts.alloc(sizeof(double) * 10 * n)) return bail("constants alloc");
     // state [V, D, TH, F, SIG] x 2 buffers
-    std::vector<double> h0(10 * n, 0.0);
-    for (int64_t i = 0; i < n; ++i) {
-        h0[0 * n + i] = v_i[i];        // V (old buffer 0)
-        h0[2 * n + i] = 0.0;           // D
-        h0[4 * n + i] = theta_i[i];    // TH
-        h0[6 * n + i] = friction_i[i]; // F
-        h0[8 * n + i] = sigma0[i];     // SIG (EffNormalStress_Old = InitialNormalStress)
-    }
-    if (s->state2.alloc(sizeof(double) * 10 * n) ||
-        hipMemcpy(s->state2.p, h0.data(), sizeof(double) * 10 * n, hipMemcpyHostToDevice))
-        return bail("state upload");
+    if (s->state2.alloc(sizeof(double) * 10 * n)) return bail("state alloc");
     // single: accel, tl, sigprod
     if (s->single.alloc(sizeof(double) * 3 * n) || hipMemset(s->single.p, 0, sizeof(double) * 3 * n))
         return bail("single alloc");
@@ -148,11 +106,11 @@ static qdfn_sim* sim_create(const char* who, int64_t n_op, const double* kself_h
         return bail("int alloc");
     const int64_t nblk = (n + kStepBlock - 1) / kStepBlock;
     if (s->parts.alloc(sizeof(BlockPart) * nblk)) return bail("partials alloc");
-    if (s->cand.alloc(n) || hipMemcpy(s->cand.p, cand.data(), n, hipMemcpyHostToDevice)) return bail("cand");
+    if (s->cand.alloc(n)) return bail("cand alloc");
     const int64_t rc_cap = p->record_capacity, tc = p->trace_capacity;
     if (s->hist.alloc(sizeof(double) * (3 * rc_cap + 4 * rc_cap * n + 1))) return bail("history alloc");
     if (s->histsw.alloc(rc_cap * n + 1)) return bail("history alloc");
-    if (s->trace.alloc(sizeof(double) * (3 * tc + 1))) return bail("trace alloc");
+    if (s->trace.alloc(sizeof(double) * (4 * tc + 1))) return bail("trace alloc");
     if (s->dstate.alloc(sizeof(SimState))) return bail("state alloc");
     s->d_st = s->dstate.as<SimState>();
     memset(&s->h_st, 0, sizeof(SimState));
@@ -184,15 +142,123 @@ static qdfn_sim* sim_create(const char* who, int64_t n_op, const double* kself_h
     A.h_th = A.h_d + rc_cap * n; A.h_sig = A.h_th + rc_cap * n; A.h_dtsel = A.h_sig + rc_cap * n;
     A.h_sw = s->histsw.as<int8_t>();
     double* tt = s->trace.as<double>();
-    A.t_told = tt; A.t_dt = tt + tc; A.t_vmax = tt + 2 * tc;
+    A.t_told = tt; A.t_dt = tt + tc; A.t_vmax = tt + 2 * tc; A.t_dtsel = tt + 3 * tc;
     A.mu = p->mu; A.rho = p->rho; A.switch_v = p->switch_v; A.dt_cut = p->dt_cut; A.rupture_dt = p->rupture_dt;
     A.maximum_dt = p->maximum_dt; A.min_sig = p->min_normal_stress; A.alpha_evo = p->alpha_evo;
-    A.conv = 1e-9; A.v0 = V0; A.dr = p->evolution_dr; A.planar = p->planar || normal_zero;
+    A.conv = 1e-9; A.v0 = 1e-9; A.dr = p->evolution_dr; A.planar = p->planar;
     A.ext_at_told = p->ext_at_told;
     A.hpart = nullptr;
     A.hkdiag = nullptr;
     A.hslots = 0;
     return s;
+}
+
+// Shared set-up of main() / main_H(): kself = K_Self per element (host).
+static qdfn_sim* sim_create(const char* who, int64_t n_op, const double* kself_h, int normal_zero,
+                            const qdfn_sim_params* p, const double* mass, const double* a, const double* b,
+                            const double* dc, const double* theta_i, const double* v_i, const double* friction_i,
+                            const double* sigma0, const double* loading_rate) {
+    if (!p || !mass || !a || !b || !dc || !theta_i || !v_i || !friction_i || !sigma0 || !loading_rate) {
+        fail(QDFN_ERR_ARG, std::string(who) + ": null argument");
+        return nullptr;
+    }
+    if (p->n != n_op || p->n_loading < 0 || p->n_loading > p->n || p->record_capacity < 0 ||
+        p->trace_capacity < 0) {
+        fail(QDFN_ERR_ARG, std::string(who) + ": size mismatch");
+        return nullptr;
+    }
+    const int64_t n = p->n, L = p->n_loading;
+    qdfn_sim* s = sim_alloc(who, p);
+    if (!s) return nullptr;
+    auto bail = [&](const char* msg) -> qdfn_sim_t* {
+        fail(QDFN_ERR_HIP, msg);
+        qdfn_sim_destroy(s);
+        return nullptr;
+    };
+    // host-side constants (QUAKEDFN_RunARG.jl / main():42-87)
+    const double V0 = 1e-9;
+    std::vector<double> kself(kself_h, kself_h + n), hc(10 * n);
+    std::vector<int8_t> cand(n, 0);
+    int64_t n_unstable = 0;
+    for (int64_t i = 0; i < n; ++i) n_unstable += (a[i] - b[i] < 0.0);
+    for (int64_t i = 0; i < n; ++i) cand[i] = n_unstable > 0 ? (a[i] - b[i] < 0.0) : (i < n - L);
+    double* C = hc.data();
+    for (int64_t i = 0; i < n; ++i) {
+        C[0 * n + i] = a[i];
+        C[1 * n + i] = b[i];
+        C[2 * n + i] = dc[i];
+        C[3 * n + i] = friction_i[i] - a[i] * log(v_i[i] / V0) - b[i] * log(theta_i[i] * V0 / dc[i]);
+        C[4 * n + i] = kself[i];
+        C[5 * n + i] = sqrt(kself[i] / mass[i]);
+        C[6 * n + i] = mass[i];
+        C[7 * n + i] = sigma0[i] * friction_i[i];
+        C[8 * n + i] = sigma0[i];
+        C[9 * n + i] = loading_rate[i];
+    }
+    if (hipMemcpy(s->consts.p, C, sizeof(double) * 10 * n, hipMemcpyHostToDevice)) return bail("constants upload");
+    std::vector<double> h0(10 * n, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+        h0[0 * n + i] = v_i[i];        // V (old buffer 0)
+        h0[2 * n + i] = 0.0;           // D
+        h0[4 * n + i] = theta_i[i];    // TH
+        h0[6 * n + i] = friction_i[i]; // F
+        h0[8 * n + i] = sigma0[i];     // SIG (EffNormalStress_Old = InitialNormalStress)
+    }
+    if (hipMemcpy(s->state2.p, h0.data(), sizeof(double) * 10 * n, hipMemcpyHostToDevice)) return bail("state upload");
+    if (hipMemcpy(s->cand.p, cand.data(), n, hipMemcpyHostToDevice)) return bail("cand");
+    s->A.planar = p->planar || normal_zero;
+    return s;
+}
+
+qdfn_sim* qdfn::sim_clone(const qdfn_sim* s, int64_t record_capacity) {
+    qdfn_sim_params p = s->P;
+    p.record_step = 1;
+    p.record_capacity = record_capacity;
+    p.trace_capacity = 0;
+    qdfn_sim* r = sim_alloc("sim_clone", &p);
+    if (!r) return nullptr;
+    const int64_t n = p.n;
+    auto bail = [&](const char* msg) -> qdfn_sim* {
+        fail(QDFN_ERR_HIP, std::string("sim_clone: ") + msg);
+        qdfn_sim_destroy(r);
+        return nullptr;
+    };
+    if (hipMemcpy(r->consts.p, s->consts.p, sizeof(double) * 10 * n, hipMemcpyDeviceToDevice) ||
+        hipMemcpy(r->cand.p, s->cand.p, n, hipMemcpyDeviceToDevice))
+        return bail("constants copy");
+    if (s->A.nt > 0) {
+        const size_t bytes = sizeof(double) * (s->A.nt + 2 * s->A.nt * n);
+        if (r->ext.alloc(bytes) || hipMemcpy(r->ext.p, s->ext.p, bytes, hipMemcpyDeviceToDevice)) return bail("ext");
+        double* d = r->ext.as<double>();
+        r->A.nt = s->A.nt;
+        r->A.ext_t = d;
+        r->A.ext_n = d + s->A.nt;
+        r->A.ext_s = d + s->A.nt + s->A.nt * n;
+    }
+    r->A.conv = s->A.conv;
+    r->A.planar = s->A.planar;
+    r->op = s->op;
+    r->hop = s->hop;
+    r->A.hpart = s->A.hpart;
+    r->A.hkdiag = s->A.hkdiag;
+    r->A.hslots = s->A.hslots;
+    r->started = true;
+    return r;
+}
+
+int qdfn::sim_restore(const qdfn_sim* s, int64_t c, qdfn_sim* r) {
+    const int64_t n = s->P.n;
+    if (c < 0 || c >= (int64_t)s->ck_state.size()) return fail(QDFN_ERR_ARG, "sim_restore: no such checkpoint");
+    const double* ck = s->ckpt.as<double>() + c * 11 * n;
+    QDFN_HIP(hipMemcpy(r->state2.p, ck, sizeof(double) * 10 * n, hipMemcpyDeviceToDevice));
+    QDFN_HIP(hipMemcpy(r->A.accel, ck + 10 * n, sizeof(double) * n, hipMemcpyDeviceToDevice));
+    SimState st = s->ck_state[c];
+    st.rec = 0;
+    st.ntrace = 0;
+    st.done = 1;
+    r->h_st = st;
+    QDFN_HIP(hipMemcpy(r->d_st, &st, sizeof(SimState), hipMemcpyHostToDevice));
+    return QDFN_OK;
 }
 
 extern "C" qdfn_sim_t* qdfn_sim_create_dense(qdfn_dense_t* op, const qdfn_sim_params* p, const double* mass,
@@ -392,6 +458,42 @@ extern "C" int qdfn_sim_run_until(qdfn_sim_t* s, double t_end, int64_t max_steps
                                   qdfn_record_cb cb, void* user) {
     if (!s || max_steps < 0 || !(t_end > 0.0)) return fail(QDFN_ERR_ARG, "qdfn_sim_run_until: bad args");
     return run_impl(s, "qdfn_sim_run_until", max_steps, t_end, 0, graph_attempts, cb, user);
+}
+
+extern "C" int qdfn_sim_set_convergence(qdfn_sim_t* s, double conv_crit) {
+    if (!s || !(conv_crit > 0.0) || !(conv_crit < 1.0))
+        return fail(QDFN_ERR_ARG, "qdfn_sim_set_convergence: need a run and 0 < conv_crit < 1");
+    if (s->started) return fail(QDFN_ERR_STATE, "qdfn_sim_set_convergence: run already started");
+    s->A.conv = conv_crit;
+    return QDFN_OK;
+}
+
+extern "C" int qdfn_sim_run_checkpointed(qdfn_sim_t* s, int64_t n_steps, int64_t every, int32_t graph_attempts) {
+    if (!s || n_steps < 1 || every < 1) return fail(QDFN_ERR_ARG, "qdfn_sim_run_checkpointed: bad args");
+    if (s->hop) return fail(QDFN_ERR_ARG, "qdfn_sim_run_checkpointed: dense runs only");
+    QDFN_HIP(hipStreamSynchronize(s->stream));
+    QDFN_HIP(hipMemcpy(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost));
+    if (s->started || s->h_st.step != 0)
+        return fail(QDFN_ERR_STATE, "qdfn_sim_run_checkpointed: the run must start here");
+    if (s->P.trace_capacity < n_steps)
+        return fail(QDFN_ERR_ARG, "qdfn_sim_run_checkpointed: trace_capacity must hold every step");
+    const int64_t n = s->P.n, nck = (n_steps + every - 1) / every;
+    s->ck_state.clear();
+    s->ck_every = 0;
+    QDFN_HIP(s->ckpt.alloc(sizeof(double) * 11 * n * nck));
+    for (int64_t c = 0; c < nck; ++c) {
+        double* ck = s->ckpt.as<double>() + c * 11 * n;
+        QDFN_HIP(hipMemcpyAsync(ck, s->state2.p, sizeof(double) * 10 * n, hipMemcpyDeviceToDevice, s->stream));
+        QDFN_HIP(hipMemcpyAsync(ck + 10 * n, s->A.accel, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+        QDFN_HIP(hipMemcpyAsync(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost, s->stream));
+        QDFN_HIP(hipStreamSynchronize(s->stream));
+        s->ck_state.push_back(s->h_st);
+        const int64_t seg = std::min(every, n_steps - c * every);
+        const int rc = run_impl(s, "qdfn_sim_run_checkpointed", seg, 0.0, 0, graph_attempts, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    s->ck_every = every;
+    return QDFN_OK;
 }
 
 extern "C" int qdfn_sim_get_stats(qdfn_sim_t* s, qdfn_sim_stats* o) {

@@ -76,7 +76,7 @@ struct SimArrays {
     double *h_time, *h_dt, *h_v, *h_d, *h_th, *h_sig;
     double* h_dtsel;  // selected Dt of each recorded step (before any retry)
     int8_t* h_sw;     // solver of each element in the recorded step's accepted attempt (1: HighV)
-    double *t_told, *t_dt, *t_vmax;
+    double *t_told, *t_dt, *t_vmax, *t_dtsel;  // per-step trace (t_dtsel: the step's selected Dt)
     // H operator (null otherwise): its product's unit slots, hslots x [2][n]
     // row sums that this kernel adds in slot order (TL = (-sum_S) / K_S,ii,
     // SigmaProd = sum_N, the loading form of the product)
@@ -268,6 +268,7 @@ __device__ inline void decide(const SimArrays& A, SimState* __restrict__ st, con
         A.t_told[S.ntrace] = told;
         A.t_dt[S.ntrace] = dta;
         A.t_vmax[S.ntrace] = VMAX;
+        A.t_dtsel[S.ntrace] = S.dt_first;
         st->ntrace = S.ntrace + 1;
     }
     if (record) {

@@ -358,4 +358,108 @@ function SimAdjoint(sim, V_i, Theta_i, obs, target; w_time=0.0, t_target=0.0, v_
     return loss[], ga, gb, gd, info[]
 end
 
+# ConvergenceCrit of the local solves (1e-9 in main()); before the run starts.
+SimSetConvergence(sim, conv_crit) =
+    check(ccall((:qdfn_sim_set_convergence, LIB), Cint, (Ptr{Cvoid}, Float64), sim, conv_crit))
+
+# Forward of n_steps from the run's start keeping a checkpoint every `every` steps
+# (p.trace_capacity >= n_steps); SimAdjoint then recomputes each segment from its checkpoint.
+SimRunCheckpointed(sim, n_steps, every; graph_attempts=32) =
+    check(ccall((:qdfn_sim_run_checkpointed, LIB), Cint, (Ptr{Cvoid}, Int64, Int64, Int32), sim, n_steps, every,
+                Int32(graph_attempts)))
+
+# Slip / event-time misfit of a dense run of K steps as a function of the per-element
+# a, b, Dc (the other inputs as main() takes them):
+#   m = CycleMisfit(op, p, FaultMass, Theta_i, V_i, Friction_i, NormalStress, V_Const, K, obs, target)
+#   L = m(a, b, Dc)
+# With ChainRulesCore available, rrule(m, a, b, Dc) runs the forward with a checkpoint every
+# m.every steps and its pullback returns the adjoint gradient (qdfn_sim_adjoint), so Zygote or
+# any ChainRules-aware AD takes gradients through the GPU loop.
+struct CycleMisfit
+    op::Ptr{Cvoid}; p::SimParams
+    FaultMass::Vector{Float64}; Theta_i::Vector{Float64}; V_i::Vector{Float64}
+    Friction_i::Vector{Float64}; NormalStress::Vector{Float64}; V_Const::Vector{Float64}
+    K::Int64; obs::Vector{Int64}; target::Vector{Float64}
+    w_time::Float64; t_target::Float64; v_onset::Float64; every::Int64
+end
+CycleMisfit(op, p, FaultMass, Theta_i, V_i, Friction_i, NormalStress, V_Const, K, obs, target;
+            w_time=0.0, t_target=0.0, v_onset=1e-3, every=50) =
+    CycleMisfit(op, p, map(x -> Vector{Float64}(vec(x)), (FaultMass, Theta_i, V_i, Friction_i, NormalStress,
+                                                            V_Const))..., Int64(K), Vector{Int64}(vec(obs)),
+                Vector{Float64}(vec(target)), Float64(w_time), Float64(t_target), Float64(v_onset), Int64(every))
+
+function _misfit_run(m::CycleMisfit, a, b, Dc)
+    p = SimParams(m.p.n, m.p.n_loading, m.p.mu, m.p.rho, m.p.switch_v, m.p.dt_cut, m.p.rupture_dt, m.p.maximum_dt,
+                  m.p.min_normal_stress, m.p.alpha_evo, m.p.evolution_dr, m.p.planar, m.p.ext_at_told, Int32(0),
+                  0, 0, m.K)
+    sim = SimCreateDense(m.op, p, m.FaultMass, a, b, Dc, m.Theta_i, m.V_i, m.Friction_i, m.NormalStress, m.V_Const)
+    try
+        SimRunCheckpointed(sim, m.K, m.every)
+    catch
+        SimDestroy(sim); rethrow()
+    end
+    return sim
+end
+
+# the loss from the run's final Disp and its per-step trace (first onset of max V >= v_onset,
+# interpolated in log10 V between step end times), as qdfn_sim_adjoint defines it
+function _misfit_value(m::CycleMisfit, sim)
+    n = m.p.n; K = m.K
+    D = zeros(n)
+    GC.@preserve D check(ccall((:qdfn_sim_get_state, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        sim, C_NULL, D, C_NULL, C_NULL, C_NULL, C_NULL))
+    L = sum(abs2, D[m.obs] .- m.target)
+    if m.w_time > 0
+        told = zeros(K); dt = zeros(K); vmax = zeros(K)
+        GC.@preserve told dt vmax check(ccall((:qdfn_sim_get_trace, LIB), Cint,
+            (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}), sim, K, told, dt, vmax))
+        k = findfirst(>=(m.v_onset), vmax)
+        if k !== nothing
+            te = told .+ dt
+            tf = if k == 1
+                te[1]
+            else
+                l0, l1, lt = log10(vmax[k-1]), log10(vmax[k]), log10(m.v_onset)
+                f = l1 != l0 ? (lt - l0) / (l1 - l0) : 1.0
+                te[k-1] + f * (te[k] - te[k-1])
+            end
+            L += m.w_time * (tf - m.t_target)^2
+        end
+    end
+    return L
+end
+
+function (m::CycleMisfit)(a, b, Dc)
+    sim = _misfit_run(m, a, b, Dc)
+    try
+        return _misfit_value(m, sim)
+    finally
+        SimDestroy(sim)
+    end
+end
+
+if Base.find_package("ChainRulesCore") !== nothing
+    @eval import ChainRulesCore
+    @eval function ChainRulesCore.rrule(m::CycleMisfit, a, b, Dc)
+        sim = _misfit_run(m, a, b, Dc)
+        L = try
+            _misfit_value(m, sim)
+        catch
+            SimDestroy(sim); rethrow()
+        end
+        function cycle_misfit_pullback(dL)
+            try
+                _, ga, gb, gd, _ = SimAdjoint(sim, m.V_i, m.Theta_i, m.obs, m.target; w_time=m.w_time,
+                                             t_target=m.t_target, v_onset=m.v_onset)
+                d = ChainRulesCore.unthunk(dL)
+                return (ChainRulesCore.NoTangent(), d .* ga, d .* gb, d .* gd)
+            finally
+                SimDestroy(sim)
+            end
+        end
+        return L, cycle_misfit_pullback
+    end
+end
+
 end # module

@@ -123,6 +123,8 @@ SIGNATURES = {
     "qdfn_sim_set_external": (C.c_int, [VP, I64, PD, PD, PD]),
     "qdfn_sim_run": (C.c_int, [VP, I64, I32, I32]),
     "qdfn_sim_run_until": (C.c_int, [VP, D, I64, I32, VP, VP]),
+    "qdfn_sim_set_convergence": (C.c_int, [VP, D]),
+    "qdfn_sim_run_checkpointed": (C.c_int, [VP, I64, I64, I32]),
     "qdfn_sim_get_stats": (C.c_int, [VP, C.POINTER(SimStats)]),
     "qdfn_sim_get_state": (C.c_int, [VP, PD, PD, PD, PD, PD, PD]),
     "qdfn_sim_get_history": (C.c_int, [VP, I64, PD, PD, PD, PD, PD, PD]),

@@ -60,8 +60,15 @@ def forward_loss(op, prob, steps, obs, d_target, w_time=0.0, t_target=0.0, v_ons
         sim.close()
 
 
-def loss_and_grad(op, prob, steps, obs, d_target, w_time=0.0, t_target=0.0, v_onset=1e-3):
-    sim = _run(op, prob, steps)
+def loss_and_grad(op, prob, steps, obs, d_target, w_time=0.0, t_target=0.0, v_onset=1e-3, checkpoint_every=None):
+    """Loss, gradients and info of a fresh run of `steps` steps; with
+    checkpoint_every the forward keeps a checkpoint every that many steps and
+    the backward sweep recomputes each segment from its checkpoint."""
+    if checkpoint_every:
+        sim = Simulation(op, prob, record_step=0, record_capacity=0, trace_capacity=steps)
+        sim.run_checkpointed(steps, checkpoint_every)
+    else:
+        sim = _run(op, prob, steps)
     try:
         return sim.adjoint(prob["v_i"], prob["theta_i"], obs, d_target, w_time, t_target, v_onset)
     finally:

@@ -147,6 +147,8 @@ class Simulation:
             if en.shape != (t.size, n) or es.shape != (t.size, n):
                 raise ValueError("external stress tables must be (len(t), n)")
             check(lib().qdfn_sim_set_external(self._h, t.size, ptr(t), ptr(en), ptr(es)))
+        if prob.get("conv_crit") is not None:  # ConvergenceCrit of the local solves (1e-9 by default)
+            check(lib().qdfn_sim_set_convergence(self._h, float(prob["conv_crit"])))
 
     def run(self, n_steps, timed=False, graph_attempts=32):
         """Advance n_steps accepted steps.  timed=False: replay a captured graph
@@ -154,6 +156,15 @@ class Simulation:
         coupling kernel (fills stats()["coupling_ms_avg"])."""
         mode = 1 if timed else 0
         rc = lib().qdfn_sim_run(self._h, int(n_steps), mode, int(graph_attempts))
+        if rc < 0:
+            raise _lib.QdfnError(rc, _lib.last_error())
+        return self.stats()
+
+    def run_checkpointed(self, n_steps, every=50, graph_attempts=32):
+        """Advance a fresh dense run n_steps accepted steps, keeping its state
+        every `every` steps for the adjoint's checkpointed recomputation
+        (qdfn_sim_run_checkpointed; trace_capacity >= n_steps)."""
+        rc = lib().qdfn_sim_run_checkpointed(self._h, int(n_steps), int(every), int(graph_attempts))
         if rc < 0:
             raise _lib.QdfnError(rc, _lib.last_error())
         return self.stats()
@@ -209,7 +220,8 @@ class Simulation:
         """Loss and gradients (a, b, Dc) of
         L = sum_o (Disp[obs_o] - d_target_o)^2 + w_time (t_first - t_target)^2
         over every step this run has taken (see qdfn_sim_adjoint); the run must
-        have been created with record_step=1 and record_capacity >= its steps.
+        have been created with record_step=1 and record_capacity >= its steps,
+        or advanced by run_checkpointed.
         Returns (loss, {"a": ..., "b": ..., "dc": ...}, info dict)."""
         obs = np.ascontiguousarray(obs, dtype=np.int64).ravel()
         tgt = _f64(d_target)
