@@ -123,8 +123,18 @@ int64_t aca_rank_cap(int64_t m, int64_t k, bool fp16_tail = false);
 // 0 when the factors were left as they were (r < 2 or no budget)
 double recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V, double budget,
                   bool fp16_tail);
-// first FP16 component of a part in SVD form (see hmat.hip)
-int64_t fp16_tail_start(const double* sig, int64_t r, double slack, int64_t m, int64_t k);
+// Rounding of the components of a low-rank part in SVD form (U m x r, column
+// a of norm sig; V r x k with leading dimension ldv, orthonormal rows), each
+// relative to its own scale: eu32 / ev32 of U's column / V's row copied to
+// FP32, eu16 / ev16 of the FP16 unit vectors of QDFN_PREC_MIXED
+struct CompRound {
+    double sig, eu32, ev32, eu16, ev16;
+};
+std::vector<CompRound> component_rounding(const double* U, const double* V, int64_t m, int64_t k, int64_t r,
+                                          int64_t ldv);
+// first FP16 component of a part in SVD form whose stored factors stay
+// within `slack` of the exact ones (see hmat.hip)
+int64_t fp16_tail_start(const std::vector<CompRound>& c, int64_t r, double slack);
 
 // part w of block q, held in device memory (column-major, leading dimension ld)
 struct DeviceBlock {

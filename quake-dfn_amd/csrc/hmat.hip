@@ -713,7 +713,7 @@ void choose_fp16_tail(HBlock& b, int w, std::vector<double>& fu, std::vector<dou
         for (int64_t i = 0; i < m; ++i) q += U[i + a * m] * U[i + a * m];
         sig[a] = std::sqrt(q);
     }
-    const int64_t r_hi = fp16_tail_start(sig.data(), r, b.slack[w], m, k);
+    const int64_t r_hi = fp16_tail_start(component_rounding(U.data(), V.data(), m, k, r, r), r, b.slack[w]);
     b.r_hi[w] = r_hi;
     fu.assign(r - r_hi, 0.0);
     fv.assign(r - r_hi, 0.0);
@@ -1290,33 +1290,89 @@ static void svd_jacobi(int64_t r, std::vector<double> C, std::vector<double>& W,
     }
 }
 
-// U (m x r), V (r x k) -> the shortest factors whose product differs from U V
-// by at most `budget` in the Frobenius norm.
-// First FP16 component of an m x k low-rank part in SVD form with singular
-// values sig[0..r) (decreasing) and rounding budget `slack`.  Stored component
-// a contributes sigma_a (u_a + du)(v_a + dv)^T: rounding a unit vector of
-// length l to FP16 after scaling its largest entry into [0.5, 1) errs by at
-// most 2^-11 + sqrt(l) 2^-24 in the 2-norm (relative rounding above 2^-14,
-// the subnormal quantum below); rounding U's column and V's row to FP32 by
-// 2^-24 each.  The trailing components move to FP16, from the smallest up,
-// while the sum over all components of sigma_a (|du| + |dv| + |du| |dv|)
-// stays within the slack (none when even FP32 rounding alone exceeds it).
-int64_t fp16_tail_start(const double* sig, int64_t r, double slack, int64_t m, int64_t k) {
-    const double e32 = 2 * 0x1p-24 + 0x1p-48;
-    const double du = 0x1p-11 + std::sqrt((double)m) * 0x1p-24, dv = 0x1p-11 + std::sqrt((double)k) * 0x1p-24;
-    const double e16 = du + dv + du * dv;
-    double E = 0.0;
-    for (int64_t a = 0; a < r; ++a) E += e32 * sig[a];
+std::vector<CompRound> component_rounding(const double* U, const double* V, int64_t m, int64_t k, int64_t r,
+                                          int64_t ldv) {
+    std::vector<CompRound> out(r);
+    auto fp16_value = [](double x) {
+        const uint16_t b = to_fp16(x);
+        _Float16 f;
+        std::memcpy(&f, &b, 2);
+        return (double)f;
+    };
+    for (int64_t a = 0; a < r; ++a) {
+        const double* u = U + a * m;
+        double s2 = 0.0, mu = 0.0, mv = 0.0;
+        for (int64_t i = 0; i < m; ++i) {
+            s2 += u[i] * u[i];
+            mu = std::max(mu, std::fabs(u[i]));
+        }
+        for (int64_t c = 0; c < k; ++c) mv = std::max(mv, std::fabs(V[a + c * ldv]));
+        CompRound& o = out[a];
+        o.sig = std::sqrt(s2);
+        o.eu32 = o.ev32 = o.eu16 = o.ev16 = 0.0;
+        if (!(o.sig > 0.0)) continue;  // a zero component stays zero in either pool
+        // FP16: the unit vectors scaled so their largest entry lies in [0.5, 1)
+        // (choose_fp16_tail); errors measured in the unit vectors' scale
+        int eu, ev;
+        std::frexp(mu / o.sig, &eu);
+        std::frexp(mv, &ev);
+        const double fu = std::ldexp(1.0 / o.sig, -eu), fv = std::ldexp(1.0, -ev);
+        double du32 = 0.0, du16 = 0.0, dv32 = 0.0, dv16 = 0.0;
+        for (int64_t i = 0; i < m; ++i) {
+            const double d32 = (double)(float)u[i] - u[i];
+            const double x = u[i] * fu, d16 = fp16_value(x) - x;
+            du32 += d32 * d32;
+            du16 += d16 * d16;
+        }
+        for (int64_t c = 0; c < k; ++c) {
+            const double v = V[a + c * ldv];
+            const double d32 = (double)(float)v - v;
+            const double x = v * fv, d16 = fp16_value(x) - x;
+            dv32 += d32 * d32;
+            dv16 += d16 * d16;
+        }
+        o.eu32 = std::sqrt(du32) / o.sig;
+        o.ev32 = std::sqrt(dv32);
+        o.eu16 = std::sqrt(du16) / (fu * o.sig);
+        o.ev16 = std::sqrt(dv16) / fv;
+    }
+    return out;
+}
+
+// The stored factors of a part in SVD form are sum_a sigma_a (u_a + du_a)
+// (v_a + dv_a)^T with u_a orthonormal columns and v_a orthonormal rows, so
+// their error is at most sqrt(sum sigma_a^2 |du_a|^2) + sqrt(sum sigma_a^2
+// |dv_a|^2) + sum sigma_a |du_a| |dv_a| in the Frobenius norm (the first two
+// terms exactly, by the orthonormality; the last by the triangle
+// inequality), with the measured rounding of each component in its pool.
+// The trailing components move to FP16, from the smallest up, while that
+// bound stays within the slack (none when even FP32 rounding exceeds it).
+int64_t fp16_tail_start(const std::vector<CompRound>& c, int64_t r, double slack) {
+    double su = 0.0, sv = 0.0, sx = 0.0;
+    for (int64_t a = 0; a < r; ++a) {
+        su += c[a].sig * c[a].sig * c[a].eu32 * c[a].eu32;
+        sv += c[a].sig * c[a].sig * c[a].ev32 * c[a].ev32;
+        sx += c[a].sig * c[a].eu32 * c[a].ev32;
+    }
+    if (!(std::sqrt(su) + std::sqrt(sv) + sx <= slack)) return r;
     int64_t r_hi = r;
     while (r_hi > 0) {
-        const double E2 = E + (e16 - e32) * sig[r_hi - 1];
-        if (!(E2 <= slack)) break;
-        E = E2;
+        const CompRound& q = c[r_hi - 1];
+        const double s2 = q.sig * q.sig;
+        const double su2 = su + s2 * (q.eu16 * q.eu16 - q.eu32 * q.eu32);
+        const double sv2 = sv + s2 * (q.ev16 * q.ev16 - q.ev32 * q.ev32);
+        const double sx2 = sx + q.sig * (q.eu16 * q.ev16 - q.eu32 * q.ev32);
+        if (!(std::sqrt(std::max(su2, 0.0)) + std::sqrt(std::max(sv2, 0.0)) + sx2 <= slack)) break;
+        su = su2;
+        sv = sv2;
+        sx = sx2;
         --r_hi;
     }
     return r_hi;
 }
 
+// U (m x r), V (r x k) -> the shortest factors whose product differs from U V
+// by at most `budget` in the Frobenius norm.
 double recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V,
                   double budget, bool fp16_tail) {
     if (r < 2 || budget <= 0) return 0.0;
@@ -1346,19 +1402,35 @@ double recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std:
         tail += s2;
         --keep;
     }
+    // components in SVD form (U = Q_u W Sigma, V = Z^T Q_v^T); with FP16
+    // trailing components, keeping a few more of them leaves more of the
+    // budget for rounding: the count whose storage (4 bytes per FP32 entry, 2
+    // per FP16 entry) is smallest, by the measured rounding of each component
+    const int64_t kmax = fp16_tail ? std::min<int64_t>(r, keep + 32) : keep;
+    std::vector<double> U2(m * kmax, 0.0), V2(kmax * k, 0.0);
+    for (int64_t a = 0; a < kmax; ++a) {
+        const int64_t j = ord[a];
+        for (int64_t l = 0; l < r; ++l) {
+            const double wl = W[l + j * r] * sig[j];
+            if (wl != 0.0)
+                for (int64_t i = 0; i < m; ++i) U2[i + a * m] += Qu[i + l * m] * wl;
+            const double zl = Z[l + j * r];
+            if (zl != 0.0)
+                for (int64_t c = 0; c < k; ++c) V2[a + c * kmax] += Qv[c + l * k] * zl;
+        }
+    }
     if (fp16_tail) {
-        // with FP16 trailing components, keeping a few more components leaves
-        // more of the budget for rounding: keep the count whose storage
-        // (4 bytes per FP32 entry, 2 per FP16 entry) is smallest
-        std::vector<double> ss(r), tail2(r + 1, 0.0);
-        for (int64_t a = 0; a < r; ++a) ss[a] = sig[ord[a]];
-        for (int64_t a = r - 1; a >= 0; --a) tail2[a] = tail2[a + 1] + ss[a] * ss[a];
+        // tail2[kk]: squared Frobenius norm of the components dropped when kk are kept
+        std::vector<double> tail2(kmax + 1, 0.0);
+        for (int64_t a = r - 1; a >= kmax; --a) tail2[kmax] += sig[ord[a]] * sig[ord[a]];
+        for (int64_t a = kmax - 1; a >= 0; --a) tail2[a] = tail2[a + 1] + sig[ord[a]] * sig[ord[a]];
+        const std::vector<CompRound> cr = component_rounding(U2.data(), V2.data(), m, k, kmax, kmax);
         auto bytes = [&](int64_t kk) {
-            const int64_t rh = fp16_tail_start(ss.data(), kk, budget - std::sqrt(tail2[kk]), m, k);
+            const int64_t rh = fp16_tail_start(cr, kk, budget - std::sqrt(tail2[kk]));
             return (m + k) * (4 * rh + 2 * (kk - rh));
         };
         int64_t best = keep, best_b = bytes(keep);
-        for (int64_t kk = keep + 1; kk <= std::min<int64_t>(r, keep + 32); ++kk) {
+        for (int64_t kk = keep + 1; kk <= kmax; ++kk) {
             const int64_t b = bytes(kk);
             if (b < best_b) {
                 best = kk;
@@ -1367,19 +1439,12 @@ double recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std:
         }
         keep = best;
         tail = tail2[keep];
-    }
-    // SVD form even when nothing is dropped: the storage of QDFN_PREC_MIXED
-    // rounds components by their singular values
-    std::vector<double> U2(m * keep, 0.0), V2(keep * k, 0.0);
-    for (int64_t a = 0; a < keep; ++a) {
-        const int64_t j = ord[a];
-        for (int64_t l = 0; l < r; ++l) {
-            const double wl = W[l + j * r] * sig[j];
-            if (wl != 0.0)
-                for (int64_t i = 0; i < m; ++i) U2[i + a * m] += Qu[i + l * m] * wl;
-            const double zl = Z[l + j * r];
-            if (zl != 0.0)
-                for (int64_t c = 0; c < k; ++c) V2[a + c * keep] += Qv[c + l * k] * zl;
+        if (keep < kmax) {
+            std::vector<double> V3(keep * k);
+            for (int64_t c = 0; c < k; ++c)
+                for (int64_t a = 0; a < keep; ++a) V3[a + c * keep] = V2[a + c * kmax];
+            V2.swap(V3);
+            U2.resize(m * keep);
         }
     }
     U.swap(U2);
@@ -1596,15 +1661,9 @@ int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, con
                 if (b.kind[w] != QDFN_HBLK_LOWRANK) continue;
                 const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1, r = b.rank[w];
                 int64_t r_hi = r;
-                if (r >= 2 && b.slack[w] > 0.0) {
-                    std::vector<double> sg(r);
-                    for (int64_t a = 0; a < r; ++a) {
-                        double t2 = 0.0;
-                        for (int64_t i = 0; i < m; ++i) t2 += b.a[w][i + a * m] * b.a[w][i + a * m];
-                        sg[a] = std::sqrt(t2);
-                    }
-                    r_hi = fp16_tail_start(sg.data(), r, b.slack[w], m, k);
-                }
+                if (r >= 2 && b.slack[w] > 0.0)
+                    r_hi = fp16_tail_start(component_rounding(b.a[w].data(), b.b[w].data(), m, k, r, r), r,
+                                           b.slack[w]);
                 if ((m + k) * (4 * r_hi + 2 * (r - r_hi)) >= 4 * m * k) {
                     dense_items.push_back({lr[q].src, lr[q].ld, (int32_t)m, (int32_t)k, 0});
                     dense_q.push_back(q);
