@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ 
 // Phase 1 of QDFN_PREC_MIXED: FP32 sub-panels (leading components) and FP16
 // sub-panels (unit vectors of the trailing components, their t rows scaled)
 // in one launch; the task's flag picks the body (uniform per wavefront).
-__global__ __launch_bounds__(kVThreads) void k_vpanel_mixed(const VTask* __restrict__ tasks, int32_t ntasks,
+__global__ __launch_bounds__(kVThreads, 6) void k_vpanel_mixed(const VTask* __restrict__ tasks, int32_t ntasks,
                                                             const float* __restrict__ pool,
                                                             const fp16_t* __restrict__ pool_lo,
                                                             const double* __restrict__ tscale,
