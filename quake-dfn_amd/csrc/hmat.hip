@@ -1017,8 +1017,46 @@ int hmat_layout(qdfn_hmat* h) {
     auto unit_bytes_of = [&](const HUnit& U) {
         return (int64_t)U.ld * U.k * (U.lo ? (int64_t)sizeof(uint16_t) : es);
     };
-    std::stable_sort(units.begin(), units.end(),
-                     [&](const HUnit& a, const HUnit& b) { return unit_bytes_of(a) > unit_bytes_of(b); });
+    // XCD-aware order: the dispatcher hands workgroup i to XCD i mod 8, each
+    // XCD with its own L2.  Row groups are split into 8 contiguous row ranges
+    // of equal panel bytes, one per XCD, so that an XCD's units gather the x
+    // and t entries of their own receiver range (the blocks below the top
+    // levels of the tree lie within one range) instead of every XCD fetching
+    // all of x and t into its L2.  Position p of the launch takes the next
+    // largest unit of XCD p mod 8's list (another list's once it is empty).
+    {
+        constexpr int kXcds = 8;
+        int64_t total = 0;
+        for (const HUnit& U : units) total += unit_bytes_of(U);
+        std::vector<int64_t> rowbytes(n + 1, 0);  // panel bytes of the units starting at each row
+        for (const HUnit& U : units) rowbytes[U.out] += unit_bytes_of(U);
+        std::vector<int8_t> xcd_of_row(n, 0);
+        int64_t acc = 0;
+        for (int64_t r = 0; r < n; ++r) {
+            xcd_of_row[r] = (int8_t)std::min<int64_t>(kXcds - 1, total > 0 ? acc * kXcds / total : 0);
+            acc += rowbytes[r];
+        }
+        std::vector<std::vector<HUnit>> lists(kXcds);
+        for (const HUnit& U : units) lists[xcd_of_row[U.out]].push_back(U);
+        for (auto& L : lists)
+            std::stable_sort(L.begin(), L.end(),
+                             [&](const HUnit& a, const HUnit& b) { return unit_bytes_of(a) > unit_bytes_of(b); });
+        std::vector<size_t> next(kXcds, 0);
+        const size_t nu = units.size();
+        units.clear();
+        for (size_t pos = 0; pos < nu; ++pos) {
+            int x = (int)(pos % kXcds);
+            if (next[x] == lists[x].size()) {  // this XCD's list is done: the largest unit left anywhere
+                int best = -1;
+                for (int y = 0; y < kXcds; ++y)
+                    if (next[y] < lists[y].size() &&
+                        (best < 0 || unit_bytes_of(lists[y][next[y]]) > unit_bytes_of(lists[best][next[best]])))
+                        best = y;
+                x = best;
+            }
+            units.push_back(lists[x][next[x]++]);
+        }
+    }
     h->n_units = (int64_t)units.size();
     h->n_slots = slots;
     int64_t slot_rows = 0;  // rows the units write to their slots
