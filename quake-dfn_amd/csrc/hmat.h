@@ -117,6 +117,9 @@ int hmat_layout(qdfn_hmat* h);
 // rank cap of the cross approximation of an m x k block (fp16_tail: the
 // break-even rank of FP32 + FP16-tail factors against the dense FP32 block)
 int64_t aca_rank_cap(int64_t m, int64_t k, bool fp16_tail = false);
+// share of a block's tolerance the full build's cross approximation may use
+// (fp16_tail: QDFN_PREC_MIXED storage)
+double aca_share(bool fp16_tail);
 // U V (m x r, r x k) -> factors in SVD form within `budget` (Frobenius): the
 // shortest, or with fp16_tail those whose FP32 + FP16-tail storage is
 // smallest; returns the unused budget (budget - ||dropped tail||_F, >= 0), or

@@ -1536,6 +1536,20 @@ static void set_dense(HBlock& b, int w, const double* a) {
 // stores large admissible blocks dense: 256 made the 20,000-triangle operator
 // 811 MB instead of 632 MB (ranks up to ~1000 at ACA 1e-6 with
 // DistDiamRatioCrit 1.0); 2048 and 4096 give the 1024 operator there.
+double aca_share(bool fp16_tail) {
+    static const double share = [] {
+        const char* e = getenv("QDFN_HMAT_ACA_SHARE");
+        const double v = e ? atof(e) : 0.0;
+        return v >= 0.05 && v <= 0.95 ? v : 0.0;
+    }();
+    // FP16 tails: a smaller share leaves more of the budget to round
+    // components to FP16 (20,000 triangles, crit 1.5: 0.5 / 0.3 / 0.2 / 0.1 ->
+    // 589.8 / 558.6 / 546.1 / 539.1 MB per product, 7,441 / 7,820 / 7,921 /
+    // 7,737 RHS evals/s); FP32 storage gains nothing from it (more parts
+    // reach the rank ceiling and are stored dense)
+    return share > 0.0 ? share : (fp16_tail ? 0.2 : 0.5);
+}
+
 int64_t aca_rank_cap(int64_t m, int64_t k, bool fp16_tail) {
     static const int64_t ceiling = [] {
         const char* e = getenv("QDFN_HMAT_RANK_CAP");
@@ -1622,8 +1636,10 @@ int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, con
             J.k = (int32_t)k;
             J.rmax = (int32_t)rmax;
             J.pad = 0;
-            // half of the budget for the cross approximation, the rest for recompression
-            J.rtol = 0.5 * rtol;
+            // a share of the budget for the cross approximation (aca_share,
+            // or QDFN_HMAT_ACA_SHARE in [0.05, 0.95]), the rest for
+            // recompression and rounding
+            J.rtol = aca_share(tail) * rtol;
             J.atol = atol;
             jobs.push_back(J);
         }
