@@ -119,6 +119,16 @@ void qdfn_dense_destroy(qdfn_dense_t* op);
 int qdfn_dense_kself(qdfn_dense_t* op, double* kself, double* diag);
 /* 1 if every K_N entry is zero (the reference's NormalStiffnessZero). */
 int qdfn_dense_normal_zero(qdfn_dense_t* op);
+/* Column splits of the coupling product (no reference counterpart: the
+ * reference's product is one BLAS call).  By default an operator takes the
+ * split count that best fills the GPU when it runs alone; runs advanced
+ * together (qdfn_ensemble_*) fill it with their number instead, and fewer
+ * splits mean fewer partial sums written and read per product.  splits = 0
+ * restores the default.  Sums keep a fixed order for any count, but a run's
+ * arithmetic depends on it: change it before runs on the operator start.
+ * Returns the split count taken (>= 1), or an error for an FP64 operator of
+ * 6,144 or more elements, whose split-tiled layout is fixed at its build. */
+int qdfn_dense_set_splits(qdfn_dense_t* op, int32_t splits);
 
 /* Coupling product (Functions_RSFDFN3DMain_D.jl:199-202).  loading_form=1:
  * yS = E*(0-x) (Elastic_Load_Disp), yN = K_N*x.  loading_form=0: yS = K_S*x,

@@ -423,6 +423,23 @@ extern "C" int qdfn_dense_kself(qdfn_dense_t* op, double* kself, double* diag) {
 
 extern "C" int qdfn_dense_normal_zero(qdfn_dense_t* op) { return op ? op->normal_zero : QDFN_ERR_ARG; }
 
+extern "C" int qdfn_dense_set_splits(qdfn_dense_t* op, int32_t splits) {
+    if (!op || splits < 0) return fail(QDFN_ERR_ARG, "qdfn_dense_set_splits: bad arguments");
+    if (op->tiled)
+        return fail(QDFN_ERR_ARG, "qdfn_dense_set_splits: the split-tiled layout is fixed at the operator's build");
+    QDFN_HIP(hipStreamSynchronize(op->stream));
+    if (splits == 0) {
+        choose_splits(op);
+    } else {
+        const int64_t sp = std::min<int64_t>(splits, op->n);
+        op->cols_per_split = round_up((op->n + sp - 1) / sp, kUnroll);
+        op->splits = (int32_t)std::max<int64_t>(1, (op->n + op->cols_per_split - 1) / op->cols_per_split);
+    }
+    QDFN_HIP(op->partS.alloc(sizeof(double) * op->splits * op->ld));
+    QDFN_HIP(op->partN.alloc(sizeof(double) * op->splits * op->ld));
+    return op->splits;
+}
+
 __global__ void k_full_shear(const double* __restrict__ tl, const double* __restrict__ x,
                              const double* __restrict__ diag, int64_t n, double* __restrict__ y) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -329,31 +329,34 @@ int enqueue_attempt(qdfn_ensemble* e) {
 }
 
 // Autotuner: the group size (right-hand sides per pass) with the fastest
-// forced coupling sweep of the whole ensemble.
+// forced coupling sweep of the whole ensemble: each candidate timed over
+// three sweeps in each of three interleaved rounds, its fastest round kept
+// (a single round picked a slower group size now and then).
 int autotune(qdfn_ensemble* e, int32_t* best) {
     hipEvent_t a, b;
     QDFN_HIP(hipEventCreate(&a));
     QDFN_HIP(hipEventCreate(&b));
-    float best_ms = 0.f;
-    *best = 1;
+    float best_ms[kEnsMaxR + 1];
+    for (float& v : best_ms) v = 1e30f;
     int rc = QDFN_OK;
-    for (int rmax = 1; rmax <= kEnsMaxR && rc == QDFN_OK; rmax *= 2) {
-        if ((rc = build_groups(e, rmax))) break;
-        if ((rc = launch_coupling_all(e, 1))) break;  // warm-up
-        float ms = 0.f;
-        if (hipEventRecord(a, e->stream) != hipSuccess) { rc = fail(QDFN_ERR_HIP, "autotune: event"); break; }
-        for (int k = 0; k < 3 && rc == QDFN_OK; ++k) rc = launch_coupling_all(e, 1);
-        if (rc) break;
-        if (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
-            hipEventElapsedTime(&ms, a, b) != hipSuccess) {
-            rc = fail(QDFN_ERR_HIP, "autotune: timing");
-            break;
+    for (int round = 0; round < 3 && rc == QDFN_OK; ++round)
+        for (int rmax = 1; rmax <= kEnsMaxR && rc == QDFN_OK; rmax *= 2) {
+            if ((rc = build_groups(e, rmax))) break;
+            if ((rc = launch_coupling_all(e, 1))) break;  // warm-up
+            float ms = 0.f;
+            if (hipEventRecord(a, e->stream) != hipSuccess) { rc = fail(QDFN_ERR_HIP, "autotune: event"); break; }
+            for (int k = 0; k < 3 && rc == QDFN_OK; ++k) rc = launch_coupling_all(e, 1);
+            if (rc) break;
+            if (hipEventRecord(b, e->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+                rc = fail(QDFN_ERR_HIP, "autotune: timing");
+                break;
+            }
+            best_ms[rmax] = std::min(best_ms[rmax], ms);
         }
-        if (rmax == 1 || ms < best_ms) {
-            best_ms = ms;
-            *best = rmax;
-        }
-    }
+    *best = 1;
+    for (int rmax = 2; rmax <= kEnsMaxR; rmax *= 2)
+        if (best_ms[rmax] < best_ms[*best]) *best = rmax;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     return rc;

@@ -261,6 +261,10 @@ function DenseCreateRect(Input_Segment, ShearModulus, PoissonRatio; prec=1)
     return h
 end
 
+# Column splits of an operator's coupling product (0: the single-run default); operators of
+# runs advanced together by EnsembleRun take about 1.5 MB of matrix per split (qdfn.ensemble).
+DenseSetSplits(op, splits) = check(ccall((:qdfn_dense_set_splits, LIB), Cint, (Ptr{Cvoid}, Int32), op, Int32(splits)))
+
 # main()'s set-up on the device: element arrays as QUAKEDFN_RunARG.jl passes them.
 function SimCreateDense(op, p::SimParams, FaultMass, Fault_a, Fault_b, Fault_Dc, Fault_Theta_i, Fault_V_i,
                         Fault_Friction_i, Fault_NormalStress, Fault_V_Const)

@@ -93,6 +93,7 @@ SIGNATURES = {
     "qdfn_dense_destroy": (None, [VP]),
     "qdfn_dense_kself": (C.c_int, [VP, PD, PD]),
     "qdfn_dense_normal_zero": (C.c_int, [VP]),
+    "qdfn_dense_set_splits": (C.c_int, [VP, I32]),
     "qdfn_matvec": (C.c_int, [VP, PD, PD, PD, I32]),
     "qdfn_matvec_device_noalloc": (C.c_int, [VP, VP, VP, VP, I32, VP]),
     "qdfn_dense_time_coupling": (C.c_int, [VP, I32, PD]),

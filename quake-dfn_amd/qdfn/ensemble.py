@@ -17,6 +17,24 @@ from ._lib import check, check_handle, lib
 from .sim import DenseOperator, Simulation
 
 
+# Matrix megabytes per coupling workgroup of an ensemble member's operator.
+# Alone, an operator of 500-4,000 elements is split into up to 256
+# workgroups to fill the GPU, and its split partial sums (written by the
+# coupling, read by the finish) cost about a third of the matrix bytes; in an
+# ensemble the members fill the GPU together.  The 256-member ensemble with
+# four right-hand sides per pass: 263k RHS evals/s with the single-run splits,
+# 283k / 301k / 310k / 300k / 235k with 0.5 / 1 / 1.5 / 2 / 3.5 MB.
+SPLIT_MB = 1.5
+
+
+def ensemble_splits(n, split_mb=SPLIT_MB):
+    """Column splits giving about split_mb MB of FP32 E and K_N per coupling
+    workgroup (1,024 rows each) for an n-element operator."""
+    ld = (n + 63) // 64 * 64
+    rb = (ld + 1023) // 1024
+    return max(1, int(round(2 * 4 * ld * n / (rb * split_mb * 1e6))))
+
+
 class Ensemble:
     def __init__(self, sims, max_rhs=0):
         """sims: Simulation objects on FP32 DenseOperators (borrowed; kept alive
@@ -28,14 +46,19 @@ class Ensemble:
         self.ops = []
 
     @classmethod
-    def from_members(cls, members, problem_fn, max_rhs=0, record_step=10, record_capacity=0, trace_capacity=0):
+    def from_members(cls, members, problem_fn, max_rhs=0, record_step=10, record_capacity=0, trace_capacity=0,
+                     split_mb=SPLIT_MB):
         """Build one FP32 operator per distinct geometry (member["grid"]), one run
-        per member with problem_fn(member, op) as its problem, and the ensemble."""
+        per member with problem_fn(member, op) as its problem, and the ensemble.
+        Each operator's coupling is split into workgroups of about split_mb MB
+        of matrix (ensemble_splits; None keeps the single-run default)."""
         ops, sims = {}, []
         for mem in members:
             key = mem["grid"]
             if key not in ops:
                 ops[key] = DenseOperator.from_rect_segments(mem["seg"], mem["mu"], mem["nu"], _lib.PREC_FP32)
+                if split_mb:
+                    ops[key].set_splits(ensemble_splits(ops[key].n, split_mb))
             op = ops[key]
             sims.append(Simulation(op, problem_fn(mem, op), record_step=record_step,
                                    record_capacity=record_capacity, trace_capacity=trace_capacity))

@@ -65,6 +65,14 @@ class DenseOperator:
         check(lib().qdfn_dense_kself(self.handle, ptr(ks), ptr(dg)))
         return ks, dg
 
+    def set_splits(self, splits):
+        """Column splits of the coupling product (qdfn_dense_set_splits; 0 = the
+        default that fills the GPU for a run alone); returns the count taken."""
+        rc = lib().qdfn_dense_set_splits(self.handle, int(splits))
+        if rc < 0:
+            raise _lib.QdfnError(rc, _lib.last_error())
+        return rc
+
     def normal_zero(self):
         return bool(check(lib().qdfn_dense_normal_zero(self.handle)))
 

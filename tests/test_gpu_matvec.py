@@ -93,3 +93,36 @@ def test_tiled_layout_bitwise_equal(qd, monkeypatch):
         for a, b in zip(out[0], out[1]):
             np.testing.assert_array_equal(a[0], b[0])
             np.testing.assert_array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("prec", ["PREC_FP32", "PREC_FP64"])
+def test_split_count_keeps_the_product(qd, orc, prec):
+    """qdfn_dense_set_splits: any split count gives the oracle's product to the
+    storage's bar; the default count (0) restores the default product bit for
+    bit; the split-tiled FP64 layout (6,144 or more elements) refuses."""
+    n = 2048
+    KS, KN = _mats(n, 17)
+    x = np.random.default_rng(18).standard_normal(n)
+    E, _ = orc.dense_prepare(KS)
+    tlo, spo = orc.dense_coupling(E, KN, x)
+    scaleS = np.abs(E) @ np.abs(x) + 1e-300
+    bar = 1e-12 if prec == "PREC_FP64" else 2e-7
+    op = qd.DenseOperator.from_matrices(KS, KN, getattr(qd, prec))
+    tl0, sp0 = op.matvec(x, loading_form=True)
+    for sp in (1, 3, 64, n):
+        taken = op.set_splits(sp)
+        assert 1 <= taken <= sp
+        tl, s = op.matvec(x, loading_form=True)
+        assert (np.abs(tl - tlo) / scaleS).max() <= bar
+    op.set_splits(0)
+    tl, s = op.matvec(x, loading_form=True)
+    np.testing.assert_array_equal(tl, tl0)
+    np.testing.assert_array_equal(s, sp0)
+    op.close()
+    big = 6144
+    rng = np.random.default_rng(19)
+    KS2 = np.asfortranarray(rng.standard_normal((big, big)))
+    op2 = qd.DenseOperator.from_matrices(KS2, KS2, qd.PREC_FP64)
+    with pytest.raises(qd.QdfnError):
+        op2.set_splits(4)
+    op2.close()
