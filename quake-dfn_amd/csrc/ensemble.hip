@@ -62,10 +62,8 @@ struct EnsBlk {
 // (4 rows per lane, columns in order, FP64 accumulation) with one
 // accumulator set per run.  force = 1 ignores the gates (autotuning).
 template <int RG>
-__global__ __launch_bounds__(256) void k_ens_coupling(const EnsTile* __restrict__ tiles,
-                                                      const EnsGroup* __restrict__ groups,
-                                                      const EnsInst* __restrict__ insts, int32_t force) {
-    const EnsTile tl = tiles[blockIdx.x];
+__device__ __forceinline__ void ens_tile(const EnsTile& tl, const EnsGroup* __restrict__ groups,
+                                         const EnsInst* __restrict__ insts, int32_t force) {
     const EnsGroup G = groups[tl.group];
     int run[RG];
     int any = 0;
@@ -169,6 +167,25 @@ __global__ __launch_bounds__(256) void k_ens_coupling(const EnsTile* __restrict_
             pn[q] = aN[r][q];
         }
     }
+}
+
+// All groups' tiles in one launch: tiles of groups of 4 runs first, then 3,
+// 2 and 1 (t[r] holds the tiles of groups of r + 1 runs, n[r] of them).
+struct EnsTiles {
+    const EnsTile* t[kEnsMaxR];
+    int32_t n[kEnsMaxR];
+};
+
+__global__ __launch_bounds__(256) void k_ens_coupling(EnsTiles T, const EnsGroup* __restrict__ groups,
+                                                      const EnsInst* __restrict__ insts, int32_t force) {
+    int32_t b = (int32_t)blockIdx.x;
+    if (b < T.n[3]) return ens_tile<4>(T.t[3][b], groups, insts, force);
+    b -= T.n[3];
+    if (b < T.n[2]) return ens_tile<3>(T.t[2][b], groups, insts, force);
+    b -= T.n[2];
+    if (b < T.n[1]) return ens_tile<2>(T.t[1][b], groups, insts, force);
+    b -= T.n[1];
+    if (b < T.n[0]) ens_tile<1>(T.t[0][b], groups, insts, force);
 }
 
 // k_coupling_finish of each run: 64 rows per workgroup, its 4 wavefronts sum
@@ -297,21 +314,16 @@ int build_groups(qdfn_ensemble* e, int rmax) {
 }
 
 int launch_coupling_all(qdfn_ensemble* e, int32_t force) {
-    const EnsGroup* g = e->groups.as<EnsGroup>();
-    const EnsInst* in = e->insts.as<EnsInst>();
-    hipStream_t s = e->stream;
-    if (e->ntiles[0])
-        hipLaunchKernelGGL(k_ens_coupling<1>, dim3((unsigned)e->ntiles[0]), dim3(256), 0, s,
-                           e->tiles[0].as<EnsTile>(), g, in, force);
-    if (e->ntiles[1])
-        hipLaunchKernelGGL(k_ens_coupling<2>, dim3((unsigned)e->ntiles[1]), dim3(256), 0, s,
-                           e->tiles[1].as<EnsTile>(), g, in, force);
-    if (e->ntiles[2])
-        hipLaunchKernelGGL(k_ens_coupling<3>, dim3((unsigned)e->ntiles[2]), dim3(256), 0, s,
-                           e->tiles[2].as<EnsTile>(), g, in, force);
-    if (e->ntiles[3])
-        hipLaunchKernelGGL(k_ens_coupling<4>, dim3((unsigned)e->ntiles[3]), dim3(256), 0, s,
-                           e->tiles[3].as<EnsTile>(), g, in, force);
+    EnsTiles T;
+    int64_t total = 0;
+    for (int r = 0; r < kEnsMaxR; ++r) {
+        T.t[r] = e->tiles[r].as<EnsTile>();
+        T.n[r] = (int32_t)e->ntiles[r];
+        total += e->ntiles[r];
+    }
+    if (total > 0)
+        hipLaunchKernelGGL(k_ens_coupling, dim3((unsigned)total), dim3(256), 0, e->stream, T,
+                           e->groups.as<EnsGroup>(), e->insts.as<EnsInst>(), force);
     QDFN_HIP(hipGetLastError());
     return QDFN_OK;
 }
