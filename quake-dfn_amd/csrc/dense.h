@@ -25,6 +25,7 @@ struct qdfn_dense {
     qdfn::DevBuf x, yS, yN;     // scratch vectors for qdfn_matvec
     hipStream_t stream = nullptr;
     int device = 0;             // HIP device the operator's memory lives on
+    int32_t users = 0;          // runs created on it (their step kernels read partS / partN)
 };
 
 namespace qdfn {

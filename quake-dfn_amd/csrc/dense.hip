@@ -427,6 +427,9 @@ extern "C" int qdfn_dense_set_splits(qdfn_dense_t* op, int32_t splits) {
     if (!op || splits < 0) return fail(QDFN_ERR_ARG, "qdfn_dense_set_splits: bad arguments");
     if (op->tiled)
         return fail(QDFN_ERR_ARG, "qdfn_dense_set_splits: the split-tiled layout is fixed at the operator's build");
+    if (op->users > 0)
+        return fail(QDFN_ERR_STATE, "qdfn_dense_set_splits: runs exist on this operator (set the splits before "
+                                    "creating them)");
     QDFN_HIP(hipStreamSynchronize(op->stream));
     if (splits == 0) {
         choose_splits(op);

@@ -419,6 +419,9 @@ extern "C" qdfn_ensemble_t* qdfn_ensemble_create(int64_t m, qdfn_sim_t* const* s
         const qdfn_dense* op = s->op;
         (void)hipStreamSynchronize(s->stream);
         arrays[k] = s->A;
+        // TL and SigProd come from k_ens_finish here (with the whole ensemble's
+        // elements the separate finish costs less than a longer step kernel)
+        arrays[k].dpartS = arrays[k].dpartN = nullptr;
         states[k] = s->d_st;
         const int64_t n = s->P.n;
         nblk[k] = (uint32_t)((n + kStepBlock - 1) / kStepBlock);

@@ -64,8 +64,6 @@ static int enqueue_attempt(qdfn_sim* s, int slot, hipEvent_t ev0, hipEvent_t ev1
         rc = launch_coupling(s->op, s->A.D, g, s->stream);
         if (rc) return rc;
         if (ev1) QDFN_HIP(hipEventRecord(ev1, s->stream));
-        rc = launch_coupling_finish(s->op, s->A.tl, s->A.sigprod, g.gate, s->stream);
-        if (rc) return rc;
     }
     hipLaunchKernelGGL(k_step, dim3((unsigned)((n + kStepBlock - 1) / kStepBlock)), dim3(kStepBlock), 0, s->stream,
                        s->A, s->d_st);
@@ -76,6 +74,7 @@ static int enqueue_attempt(qdfn_sim* s, int slot, hipEvent_t ev0, hipEvent_t ev1
 extern "C" void qdfn_sim_destroy(qdfn_sim_t* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->op) s->op->users -= 1;
     s->graph.reset();
     hipStream_t st = s->stream;
     delete s;
@@ -150,6 +149,10 @@ static qdfn_sim* sim_alloc(const char* who, const qdfn_sim_params* p) {
     A.hpart = nullptr;
     A.hkdiag = nullptr;
     A.hslots = 0;
+    A.dpartS = A.dpartN = nullptr;
+    A.dld = 0;
+    A.dsplits = 0;
+    A.dnormal_zero = 0;
     return s;
 }
 
@@ -242,6 +245,12 @@ qdfn_sim* qdfn::sim_clone(const qdfn_sim* s, int64_t record_capacity) {
     r->A.hpart = s->A.hpart;
     r->A.hkdiag = s->A.hkdiag;
     r->A.hslots = s->A.hslots;
+    r->A.dpartS = s->A.dpartS;
+    r->A.dpartN = s->A.dpartN;
+    r->A.dld = s->A.dld;
+    r->A.dsplits = s->A.dsplits;
+    r->A.dnormal_zero = s->A.dnormal_zero;
+    if (r->op) r->op->users += 1;
     r->started = true;
     return r;
 }
@@ -276,7 +285,16 @@ extern "C" qdfn_sim_t* qdfn_sim_create_dense(qdfn_dense_t* op, const qdfn_sim_pa
     }
     qdfn_sim* s = sim_create("qdfn_sim_create_dense", op->n, kself.data(), op->normal_zero, p, mass, a, b, dc,
                              theta_i, v_i, friction_i, sigma0, loading_rate);
-    if (s) s->op = op;
+    if (s) {
+        // the step kernel adds the coupling's split partials itself
+        s->op = op;
+        op->users += 1;
+        s->A.dpartS = op->partS.as<double>();
+        s->A.dpartN = op->partN.as<double>();
+        s->A.dld = op->ld;
+        s->A.dsplits = op->splits;
+        s->A.dnormal_zero = op->normal_zero;
+    }
     return s;
 }
 

@@ -83,6 +83,14 @@ struct SimArrays {
     const double* hpart;
     const double* hkdiag;
     int64_t hslots;
+    // dense operator (null otherwise): its product's split partial sums,
+    // dsplits x dld per matrix, that this kernel adds in the order of the
+    // coupling finish (splits k = q, q + 4, ... summed per quarter q, then the
+    // quarters in order): TL = -sum_S, SigmaProd = sum_N (0 when the
+    // operator's K_N is zero)
+    const double *dpartS, *dpartN;
+    int64_t dld;
+    int32_t dsplits, dnormal_zero;
     // parameters
     double mu, rho, switch_v, dt_cut, rupture_dt, maximum_dt, min_sig, alpha_evo, conv, v0;
     int32_t dr, planar, ext_at_told, pad;
@@ -341,6 +349,30 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
             A.tl[i] = tl_i;
             A.sigprod[i] = sN;
             sp_i = A.planar ? 0.0 : sN;
+        } else if (A.dpartS) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0;
+            const double* pS = A.dpartS + i;
+            const double* pN = A.dpartN + i;
+            const int64_t ld = A.dld;
+            int32_t k = 0;
+            for (; k + 4 <= A.dsplits; k += 4) {
+                s0 += pS[k * ld];
+                s1 += pS[(k + 1) * ld];
+                s2 += pS[(k + 2) * ld];
+                s3 += pS[(k + 3) * ld];
+                n0 += pN[k * ld];
+                n1 += pN[(k + 1) * ld];
+                n2 += pN[(k + 2) * ld];
+                n3 += pN[(k + 3) * ld];
+            }
+            if (k < A.dsplits) { s0 += pS[k * ld]; n0 += pN[k * ld]; }
+            if (k + 1 < A.dsplits) { s1 += pS[(k + 1) * ld]; n1 += pN[(k + 1) * ld]; }
+            if (k + 2 < A.dsplits) { s2 += pS[(k + 2) * ld]; n2 += pN[(k + 2) * ld]; }
+            tl_i = -(((s0 + s1) + s2) + s3);
+            const double sn = A.dnormal_zero ? 0.0 : ((n0 + n1) + n2) + n3;
+            A.tl[i] = tl_i;
+            A.sigprod[i] = sn;
+            sp_i = A.planar ? 0.0 : sn;
         } else {
             tl_i = A.tl[i];
             sp_i = A.planar ? 0.0 : A.sigprod[i];
