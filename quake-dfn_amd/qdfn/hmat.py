@@ -264,7 +264,8 @@ def main_H(ShearModulus, FaultCount, LoadingFaultCount, Mass, NormalStiffnessZer
     """main_H of scripts/Functions_RSFDFN3DMain_H.jl:2 on the GPU.  The blocks
     are given as ShearStiffness_H / NormalStiffness_H lists (dense arrays or
     (U, V) pairs); Ranks_* and ThreadCount are accepted for the signature only.
-    Returns the History_* arrays (saved as .npz when SaveResultFileName is a str)."""
+    Returns the History_* arrays (saved to SaveResultFileName when it is a str:
+    JLD2 with the reference's Result keys for a .jld2 name, else .npz)."""
     from .sim import Simulation
     H = HMatrix.from_blocks(FaultCount, ElementRange_SR, Admissible, ShearStiffness_H, NormalStiffness_H, prec)
     prob = dict(n=FaultCount, n_loading=LoadingFaultCount, mass=Mass, a=a, b=b, dc=Dc, theta_i=ThetaI, v_i=Vini,
@@ -277,7 +278,11 @@ def main_H(ShearModulus, FaultCount, LoadingFaultCount, Mass, NormalStiffnessZer
     sim.run(TotalStep)
     hist = sim.history()
     if isinstance(SaveResultFileName, str) and SaveResultFileName:
-        np.savez(SaveResultFileName, **hist)
+        if SaveResultFileName.endswith(".jld2"):  # the reference's Result.jld2 keys (qdfn.jld2)
+            from . import jld2
+            jld2.save_result(SaveResultFileName, hist)
+        else:
+            np.savez(SaveResultFileName, **hist)
     sim.close()
     H.close()
     return hist

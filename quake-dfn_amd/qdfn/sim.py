@@ -287,7 +287,8 @@ def main(StiffnessMatrixShear, StiffnessMatrixNormal, NormalStiffnessZero, Shear
     """main() of scripts/Functions_RSFDFN3DMain_D.jl:2 on the GPU.  The
     reference reads Input_ExternalStressChange.jld2 from the working directory;
     here the tables are passed as external={"t","n","s"}.  Returns the History_*
-    arrays (and writes them to SaveResultFileName as .npz when it is a str)."""
+    arrays (and writes them to SaveResultFileName when it is a str: the reference's
+    Result.jld2 keys for a .jld2 name, else .npz)."""
     op = DenseOperator.from_matrices(StiffnessMatrixShear, StiffnessMatrixNormal, prec)
     prob = dict(n=FaultCount, n_loading=LoadingFaultCount, mass=Mass, a=a, b=b, dc=Dc, theta_i=ThetaI, v_i=Vini,
                 friction_i=FrictionI, sigma0=InitialNormalStress, loading_rate=LoadingRate, mu=ShearModulus,
@@ -299,7 +300,11 @@ def main(StiffnessMatrixShear, StiffnessMatrixNormal, NormalStiffnessZero, Shear
     sim.run(TotalStep)
     hist = sim.history()
     if isinstance(SaveResultFileName, str) and SaveResultFileName:
-        np.savez(SaveResultFileName, **hist)
+        if SaveResultFileName.endswith(".jld2"):  # the reference's Result.jld2 keys (qdfn.jld2)
+            from . import jld2
+            jld2.save_result(SaveResultFileName, hist)
+        else:
+            np.savez(SaveResultFileName, **hist)
     sim.close()
     op.close()
     return hist

@@ -73,3 +73,27 @@ def test_staging_buffer_too_small_is_refused(fault, qd):
         sim.run_until(1e7, 1000, on_records=lambda f, h: False, graph_attempts=32)
     assert e.value.code == -1 and "record_capacity" in str(e.value)
     sim.close()
+
+
+def test_periodic_result_file(fault, qd, tmp_path):
+    """The reference's save(SaveResultFileName, "History_V", ...) every SaveStep:
+    the streamed records appended to host histories and written as Result.jld2
+    at each batch; the last file holds every record of the run, bitwise."""
+    from qdfn import jld2
+    path = tmp_path / "Result.jld2"
+    acc = {}
+
+    def on_records(first, h):
+        for k, v in h.items():
+            acc[k] = v if k not in acc else np.concatenate([acc[k], v])
+        jld2.save_result(path, acc)
+        return False
+
+    sim = qd.Simulation(fault["op"], fault["prob"], record_step=10, record_capacity=8, trace_capacity=0)
+    st = sim.run_until(0.5 * 3.15576e7, 10 ** 6, on_records=on_records, graph_attempts=32)
+    sim.close()
+    V, D, T = jld2.load(path, "History_V", "History_Disp", "History_Time")
+    assert T.shape == (st["steps"] // 10, 1) and V.shape == (st["steps"] // 10, fault["op"].n)
+    np.testing.assert_array_equal(V, acc["History_V"])
+    np.testing.assert_array_equal(D, acc["History_Disp"])
+    np.testing.assert_array_equal(T[:, 0], acc["History_Time"])
