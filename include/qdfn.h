@@ -274,6 +274,14 @@ int qdfn_hmat_get_split(qdfn_hmat_t* h, int64_t* rhi_S, int64_t* rhi_N);
  * Functions_Hmatrix.jl:459); diag = K_S,ii (0 where no diagonal block). */
 int qdfn_hmat_kself(qdfn_hmat_t* h, double* kself, double* diag);
 int qdfn_hmat_normal_zero(qdfn_hmat_t* h);
+/* ReduceTooStrongInteraction_Hmat (scripts/Functions_Solvers.jl:546-579,
+ * applied at QUAKEDFN_RunARG.jl:126-131 when StrongInteractionCriteriaMultiple
+ * > 0): in every inadmissible block, each entry of the loading-form shear
+ * matrix (diagonal of diagonal blocks zeroed) whose magnitude exceeds
+ * multiple * |K_Self| of its receiver has its shear and normal entries scaled
+ * by multiple * |K_Self| / |entry|; the operator's product layout is rebuilt.
+ * n_pairs (optional): entries changed.  Before runs are created on h. */
+int qdfn_hmat_reduce_strong_interaction(qdfn_hmat_t* h, double multiple, int64_t* n_pairs);
 /* HmatSolver_Pararllel (Functions_Hmatrix.jl:631) for both matrices.
  * loading_form=1: yS = (H_load * (0 - x)) ./ (-K_Self) (main_H :241-243),
  * yN = H_N * x; loading_form=0: yS = H_S * x, yN = H_N * x. */

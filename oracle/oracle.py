@@ -17,6 +17,8 @@ Function names follow the reference functions they restate:
   sim_dense / sim_hmat  main / main_H              Functions_RSFDFN3DMain_{D,H}.jl
   event_fragments / merge_fragments
                         get_event_fragments / merge_fragments  scripts/Functions_Plot.jl:430-579
+  reduce_too_strong_interaction(_hmat)
+                        ReduceTooStrongInteraction(_Hmat)      Functions_Solvers.jl:546-607
 """
 import ctypes as C
 import os
@@ -375,6 +377,45 @@ def hmat_matvec(H, x):
     y = np.zeros(int(H["n"]))
     lib().orc_hmat_matvec(C.byref(s), _p(x), _p(y))
     return y
+
+
+def reduce_too_strong_interaction_hmat(multiple, admissible, HS, HN):
+    """ReduceTooStrongInteraction_Hmat (Functions_Solvers.jl:546-579), loop for
+    loop on copies of the oracle's block dicts: returns (HS', HN', pairs) with
+    pairs the (source, receiver) 1-based list in the reference's order."""
+    load, ks = hmat_transition_to_loading(HS)
+    S = dict(HS, a=[None if a is None else np.array(a, dtype=np.float64, copy=True) for a in HS["a"]])
+    N = dict(HN, a=[None if a is None else np.array(a, dtype=np.float64, copy=True) for a in HN["a"]])
+    pairs = []
+    for blk in range(len(admissible)):
+        if admissible[blk] != 0:
+            continue
+        s0, s1, r0, r1 = (int(v) for v in np.asarray(HS["range"])[blk])
+        for js, src in enumerate(range(s0, s1 + 1)):
+            for ir, rcv in enumerate(range(r0, r1 + 1)):
+                L = load["a"][blk][ir, js]
+                if abs(ks[rcv - 1]) * multiple < abs(L):
+                    ratio = abs(ks[rcv - 1]) * multiple / abs(L)
+                    S["a"][blk][ir, js] = S["a"][blk][ir, js] * ratio
+                    N["a"][blk][ir, js] = N["a"][blk][ir, js] * ratio
+                    pairs.append((src, rcv))
+    return S, N, pairs
+
+
+def reduce_too_strong_interaction(multiple, n, KS, KN):
+    """ReduceTooStrongInteraction (Functions_Solvers.jl:586-607), loop for loop
+    on copies of the dense matrices: returns (KS', KN', pair count)."""
+    KS = np.array(KS, dtype=np.float64, copy=True)
+    KN = np.array(KN, dtype=np.float64, copy=True)
+    count = 0
+    for src in range(n):
+        for rcv in range(n):
+            if src != rcv and abs(KS[rcv, rcv]) * multiple < abs(KS[rcv, src]):
+                ratio = abs(KS[rcv, rcv]) * multiple / abs(KS[rcv, src])
+                KS[rcv, src] = KS[rcv, src] * ratio
+                KN[rcv, src] = KN[rcv, src] * ratio
+                count += 1
+    return KS, KN, count
 
 
 def hmat_transition_to_loading(H):

@@ -83,6 +83,7 @@ struct qdfn_hmat {
     // column), outputs (phase 2)
     int64_t p1_entries = 0, p1_other = 0, p2_entries = 0, p2_index = 0, p2_gather = 0, p2_out = 0;
     hipStream_t stream = nullptr;
+    int32_t users = 0;                    // runs created on it (their step kernels read part and kdiag)
     // construction statistics
     int64_t build_entries = 0, build_steps = 0, build_sampled = 0, build_fallback = 0;
     double build_ms = 0.0;

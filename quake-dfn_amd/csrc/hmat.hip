@@ -2109,6 +2109,39 @@ extern "C" int qdfn_hmat_kself(qdfn_hmat_t* h, double* kself, double* diag) {
 
 extern "C" int qdfn_hmat_normal_zero(qdfn_hmat_t* h) { return h ? h->normal_zero : QDFN_ERR_ARG; }
 
+// ReduceTooStrongInteraction_Hmat (scripts/Functions_Solvers.jl:546-579): in
+// every inadmissible block, an entry of the loading-form shear matrix (the
+// diagonal of diagonal blocks zeroed) larger in magnitude than multiple |K_Self|
+// of its receiver scales its shear and normal entries by multiple |K_Self| /
+// |entry|; then the product layout is rebuilt from the changed blocks.
+extern "C" int qdfn_hmat_reduce_strong_interaction(qdfn_hmat_t* h, double multiple, int64_t* n_pairs) {
+    if (!h || !(multiple > 0.0)) return fail(QDFN_ERR_ARG, "qdfn_hmat_reduce_strong_interaction: bad arguments");
+    if (h->users > 0)
+        return fail(QDFN_ERR_STATE, "qdfn_hmat_reduce_strong_interaction: runs exist on this operator");
+    int64_t pairs = 0;
+    for (HBlock& b : h->blocks) {
+        if (b.adm != 0) continue;
+        if (b.kind[0] != QDFN_HBLK_DENSE || b.kind[1] != QDFN_HBLK_DENSE)
+            return fail(QDFN_ERR_STATE, "qdfn_hmat_reduce_strong_interaction: an inadmissible block is not dense");
+        const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+        const bool diag = b.s0 == b.r0 && b.s1 == b.r1;
+        for (int64_t c = 0; c < k; ++c)
+            for (int64_t i = 0; i < m; ++i) {
+                if (diag && i == c) continue;  // zero in the loading form
+                const double e = b.a[0][i + c * m];
+                const double lim = std::fabs(-h->kdiag_h[b.r0 + i]) * multiple;
+                if (lim < std::fabs(e)) {
+                    const double ratio = lim / std::fabs(e);
+                    b.a[0][i + c * m] = e * ratio;
+                    b.a[1][i + c * m] = b.a[1][i + c * m] * ratio;
+                    ++pairs;
+                }
+            }
+    }
+    if (n_pairs) *n_pairs = pairs;
+    return hmat_layout(h);
+}
+
 extern "C" int qdfn_hmat_matvec(qdfn_hmat_t* h, const double* x, double* yS, double* yN, int32_t loading_form) {
     if (!h || !x) return fail(QDFN_ERR_ARG, "qdfn_hmat_matvec: null argument");
     hipStream_t s = h->stream;

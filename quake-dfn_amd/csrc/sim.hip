@@ -75,6 +75,7 @@ extern "C" void qdfn_sim_destroy(qdfn_sim_t* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->op) s->op->users -= 1;
+    if (s->hop) s->hop->users -= 1;
     s->graph.reset();
     hipStream_t st = s->stream;
     delete s;
@@ -251,6 +252,7 @@ qdfn_sim* qdfn::sim_clone(const qdfn_sim* s, int64_t record_capacity) {
     r->A.dsplits = s->A.dsplits;
     r->A.dnormal_zero = s->A.dnormal_zero;
     if (r->op) r->op->users += 1;
+    if (r->hop) r->hop->users += 1;
     r->started = true;
     return r;
 }
@@ -313,6 +315,7 @@ extern "C" qdfn_sim_t* qdfn_sim_create_hmat(qdfn_hmat_t* h, const qdfn_sim_param
                              theta_i, v_i, friction_i, sigma0, loading_rate);
     if (s) {
         s->hop = h;
+        h->users += 1;
         s->A.hpart = h->part.as<double>();
         s->A.hkdiag = h->kdiag.as<double>();
         s->A.hslots = h->n_slots;

@@ -188,6 +188,16 @@ end
 HmatDestroy(h) = ccall((:qdfn_hmat_destroy, LIB), Cvoid, (Ptr{Cvoid},), h)
 
 # HmatSolver_Pararllel (Functions_Hmatrix.jl:631) for both matrices: (shear, normal) products.
+# ReduceTooStrongInteraction_Hmat (Functions_Solvers.jl:546) on a device H operator h, in place of the
+# block arrays (QUAKEDFN_RunARG.jl:126-131); before runs are created on h.
+function ReduceTooStrongInteraction_Hmat(StrongInteractionCriteriaMultiple, h, FaultCount)
+    cnt = Ref{Int64}(0)
+    check(ccall((:qdfn_hmat_reduce_strong_interaction, LIB), Cint, (Ptr{Cvoid}, Float64, Ref{Int64}), h,
+                StrongInteractionCriteriaMultiple, cnt))
+    println(cnt[], "/", FaultCount^2, " interaction is smoothened")
+    return h
+end
+
 function HmatSolver(h, x, n; loading_form=false)
     xv = Vector{Float64}(vec(x)); yS = zeros(n); yN = zeros(n)
     GC.@preserve xv yS yN check(ccall((:qdfn_hmat_matvec, LIB), Cint,

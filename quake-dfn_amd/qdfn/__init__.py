@@ -8,11 +8,11 @@ the first call does, and raises if the library has not been built.
 from . import _lib
 from ._lib import PREC_BF16_FAR, PREC_FP32, PREC_FP64, PREC_MIXED, QdfnError, device_info, lib
 from .assembly import (BuildMatrixByParts_Even_Rec, BuildMatrixByParts_Even_Tri, Okada_DC3D_Vector,
-                       RotVerts_UnitVectors, StiffnessMatrix_ByParts_Calculation_Rec,
+                       ReduceTooStrongInteraction, RotVerts_UnitVectors, StiffnessMatrix_ByParts_Calculation_Rec,
                        StiffnessMatrix_ByParts_Calculation_Tri, TDstressHS)
 from .catalog import EventCatalog, fault_area, get_event_fragments, merge_fragments
 from .ensemble import Ensemble
-from .hmat import HMatrix, hmat_structure, main_H
+from .hmat import HMatrix, ReduceTooStrongInteraction_Hmat, hmat_structure, main_H
 from .inputs import BulkToSegment, LRtoRake, ReadBulkInput, ReadSegmentInput
 from .sim import DenseOperator, Simulation, main
 from .solvers import SolveOneTimeStep
@@ -22,4 +22,5 @@ __all__ = ["PREC_BF16_FAR", "PREC_FP32", "PREC_FP64", "PREC_MIXED", "QdfnError",
            "RotVerts_UnitVectors", "StiffnessMatrix_ByParts_Calculation_Tri", "BuildMatrixByParts_Even_Tri",
            "DenseOperator", "Simulation", "main", "SolveOneTimeStep", "HMatrix", "hmat_structure", "main_H",
            "BulkToSegment", "LRtoRake", "ReadBulkInput", "ReadSegmentInput",
-           "EventCatalog", "fault_area", "get_event_fragments", "merge_fragments", "Ensemble"]
+           "EventCatalog", "fault_area", "get_event_fragments", "merge_fragments", "Ensemble",
+           "ReduceTooStrongInteraction", "ReduceTooStrongInteraction_Hmat"]

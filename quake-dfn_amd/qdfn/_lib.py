@@ -117,6 +117,7 @@ SIGNATURES = {
     "qdfn_hmat_bytes": (C.c_int, [VP, PI64]),
     "qdfn_hmat_kself": (C.c_int, [VP, PD, PD]),
     "qdfn_hmat_normal_zero": (C.c_int, [VP]),
+    "qdfn_hmat_reduce_strong_interaction": (C.c_int, [VP, D, C.POINTER(I64)]),
     "qdfn_hmat_matvec": (C.c_int, [VP, PD, PD, PD, I32]),
     "qdfn_hmat_time_matvec": (C.c_int, [VP, I32, PD]),
     "qdfn_sim_create_dense": (VP, [VP, C.POINTER(SimParams)] + [PD] * 9),

@@ -109,3 +109,26 @@ def BuildMatrixByParts_Even_Tri(P1, P2, P3, Rake, FaultCenter, UnitVector_Normal
     lam = 2 * ShearModulus * PoissonRatio / (1 - 2 * PoissonRatio)
     return StiffnessMatrix_ByParts_Calculation_Tri(P1, P2, P3, Rake, FaultCenter, ShearModulus, lam,
                                                    UnitVector_Normal, UnitVector_Slip)
+
+
+def ReduceTooStrongInteraction(StrongInteractionCriteriaMultiple, FaultCount, StiffnessMatrixShear,
+                               StiffnessMatrixNormal):
+    """ReduceTooStrongInteraction (scripts/Functions_Solvers.jl:586-607) on the
+    dense matrices before they go to the device: every off-diagonal shear entry
+    larger in magnitude than StrongInteractionCriteriaMultiple |K_S,ii| of its
+    receiver row scales its shear and normal entries by that bound over its
+    magnitude (the same arithmetic, elementwise).  Modifies and returns the
+    matrices, as the reference does."""
+    KS, KN = StiffnessMatrixShear, StiffnessMatrixNormal
+    n = int(FaultCount)
+    c = float(StrongInteractionCriteriaMultiple)
+    lim = np.abs(np.diagonal(KS)[:n]).reshape(-1, 1) * c  # |K_ii| * multiple, per receiver row
+    sub = KS[:n, :n]
+    hit = lim < np.abs(sub)
+    np.fill_diagonal(hit, False)
+    ratio = np.ones_like(sub)
+    ratio[hit] = (np.broadcast_to(lim, sub.shape)[hit]) / np.abs(sub[hit])
+    KS[:n, :n] = np.where(hit, sub * ratio, sub)
+    KN[:n, :n] = np.where(hit, KN[:n, :n] * ratio, KN[:n, :n])
+    print(f"{int(hit.sum())}/{n ** 2} interaction is smoothened")
+    return KS, KN

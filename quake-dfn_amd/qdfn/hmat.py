@@ -222,6 +222,14 @@ class HMatrix:
         check(lib().qdfn_hmat_kself(self.handle, ptr(ks), ptr(dg)))
         return ks, dg
 
+    def reduce_too_strong_interaction(self, multiple):
+        """ReduceTooStrongInteraction_Hmat (Functions_Solvers.jl:546-579) on this
+        operator's inadmissible blocks (qdfn_hmat_reduce_strong_interaction);
+        returns the number of entries scaled.  Before runs are created on it."""
+        cnt = _lib.I64()
+        check(lib().qdfn_hmat_reduce_strong_interaction(self.handle, float(multiple), C.byref(cnt)))
+        return cnt.value
+
     def normal_zero(self):
         return bool(check(lib().qdfn_hmat_normal_zero(self.handle)))
 
@@ -286,3 +294,14 @@ def main_H(ShearModulus, FaultCount, LoadingFaultCount, Mass, NormalStiffnessZer
     sim.close()
     H.close()
     return hist
+
+
+def ReduceTooStrongInteraction_Hmat(StrongInteractionCriteriaMultiple, H):
+    """ReduceTooStrongInteraction_Hmat (scripts/Functions_Solvers.jl:546-579, called
+    at QUAKEDFN_RunARG.jl:126-131 when StrongInteractionCriteriaMultiple > 0) on
+    an HMatrix built on the device: the reference takes and returns the
+    ShearStiffness_H / NormalStiffness_H block arrays, here the operator holds
+    them.  Prints the reference's count line and returns H."""
+    cnt = H.reduce_too_strong_interaction(StrongInteractionCriteriaMultiple)
+    print(f"{cnt}/{H.n ** 2} interaction is smoothened")
+    return H
