@@ -91,10 +91,16 @@ int qdfn_rotverts_unitvectors(int64_t n, double* P1, double* P2, double* P3, con
  * block in FP32; its trailing components (the smallest singular values) as
  * FP16 unit vectors with an FP64 scale each, as many as fit in the part of
  * the block's error budget (aca_rtol ||K_S,b||_F) that the ACA residual and
- * the truncation leave unused, counting FP32 and FP16 rounding at their unit
- * round-off.  FP64 accumulation.  Blocks that were not recompressed (rank
+ * the truncation leave unused, by the measured FP32 and FP16 rounding of each
+ * component.  FP64 accumulation.  Blocks that were not recompressed (rank
  * below 2, aca_rtol = 0, or imported through qdfn_hmat_create) stay FP32. */
 #define QDFN_PREC_MIXED 3
+/* H operators only: QDFN_PREC_MIXED with a third tier: the smallest trailing
+ * components as FP8 (E4M3) vectors scaled to a largest entry in [128, 256)
+ * with an FP64 scale each, then FP16, then FP32, as many in each as the
+ * block's unused error budget admits by the measured rounding of each
+ * component.  FP64 accumulation. */
+#define QDFN_PREC_MIXED8 4
 
 typedef struct qdfn_dense qdfn_dense_t;
 
@@ -270,6 +276,13 @@ int qdfn_hmat_get_block_stored(qdfn_hmat_t* h, int64_t b, int32_t which, double*
 /* Per block, the first low-rank component held in the 16-bit pool (r_hi;
  * rank: none, 0: all). */
 int qdfn_hmat_get_split(qdfn_hmat_t* h, int64_t* rhi_S, int64_t* rhi_N);
+/* Per block, the first low-rank component held in the 8-bit pool
+ * (QDFN_PREC_MIXED8; rank: none).  No reference counterpart. */
+int qdfn_hmat_get_split8(qdfn_hmat_t* h, int64_t* r8_S, int64_t* r8_N);
+/* y[i] = x[i] rounded to FP8 E4M3 (OCP: bias 7, no infinities, largest
+ * finite 448; round to nearest even, saturating), the storage of the 8-bit
+ * pool of QDFN_PREC_MIXED8.  Host only.  No reference counterpart. */
+int qdfn_fp8_round(const double* x, double* y, int64_t count);
 /* K_Self = -K_S,ii from the diagonal blocks (StiffnessTransitionToLoading,
  * Functions_Hmatrix.jl:459); diag = K_S,ii (0 where no diagonal block). */
 int qdfn_hmat_kself(qdfn_hmat_t* h, double* kself, double* diag);

@@ -23,19 +23,21 @@ struct HBlock {
     // its Frobenius error budget the truncation left unused (0: not in SVD form)
     double slack[2] = {0.0, 0.0};
     // components [0, r_hi) of a low-rank part are stored in the FP64 / FP32
-    // pool, [r_hi, rank) in the 16-bit pool (set by hmat_layout); with
-    // QDFN_PREC_MIXED, FP16 component r_hi + j is U's column times fu[j] and
-    // V's row times fv[j], its t row scaled by fc[j] (choose_fp16_tail)
+    // pool, [r_hi, r_8) in the 16-bit pool, [r_8, rank) in the 8-bit pool (set
+    // by hmat_layout); with QDFN_PREC_MIXED / _MIXED8, tail component r_hi + j
+    // is U's column times fu[j] and V's row times fv[j], its t row scaled by
+    // fc[j] (choose_tails)
     int64_t r_hi[2] = {0, 0};
+    int64_t r_8[2] = {0, 0};
     std::vector<double> fu[2], fv[2], fc[2];
 };
 
 // Phase-2 unit (one workgroup): columns [0, k) of one panel of a row group,
 // the group being rows [out, out + m) of one matrix (kind 1 shear, 2
 // normal).  The columns are column-major at off in the FP64 / FP32 pool
-// (lo 0; column stride ld = m rounded up to four) or in the 16-bit pool (lo
-// 1; ld = m rounded up to eight); column c multiplies w[idx[z + c]], w = [x;
-// t].  Unit u of its group writes its row sums to slot u.
+// (lo 0; column stride ld = m rounded up to four), in the 16-bit pool (lo 1)
+// or in the 8-bit pool (lo 2; ld = m rounded up to eight in both); column c
+// multiplies w[idx[z + c]], w = [x; t].  Unit u of its group writes its row sums to slot u.
 struct HUnit {
     int64_t off, z;
     int32_t k, ld, lo, out, m, kind, u, pad;
@@ -45,7 +47,7 @@ struct HUnit {
 // data_off (column stride ld, a multiple of the entries per lane load; K
 // columns multiplying x[xoff + c]); row q writes t[out + q], or vpart[part + q]
 // when the sub-panel is split into column segments (part >= 0); lpr = lanes per
-// column; lo: the sub-panel is in the 16-bit pool
+// column; lo: the sub-panel is in the 16-bit (1) or 8-bit (2) pool
 struct VTask {
     int64_t data_off;
     int32_t xoff, out, m, K, ld, lpr, part, lo;
@@ -69,7 +71,8 @@ struct qdfn_hmat {
     qdfn::DevBuf kdiag;
     qdfn::DevBuf pool, idx, t;            // panel entries (V sub-panels then row-group panels), gather indices, t
     qdfn::DevBuf pool_lo;                 // 16-bit V sub-panels, then row groups' U panels (BF16_FAR: BF16; MIXED: FP16)
-    qdfn::DevBuf tscale;                  // QDFN_PREC_MIXED: scale of each t row (sigma of FP16 components, else 1)
+    qdfn::DevBuf pool_8;                  // QDFN_PREC_MIXED8: FP8 (E4M3) V sub-panels, then row groups' U panels
+    qdfn::DevBuf tscale;                  // QDFN_PREC_MIXED(8): scale of each t row (16- and 8-bit components, else 1)
     qdfn::DevBuf units, part;             // phase-2 units, their slots (n_slots x [y_S; y_N] row sums)
     qdfn::DevBuf vtasks, vfins, vpart;    // phase-1 wavefront tasks, split-panel finish, partial sums
     qdfn::DevBuf x, yS, yN;               // vectors for qdfn_hmat_matvec
@@ -122,23 +125,34 @@ int64_t aca_rank_cap(int64_t m, int64_t k, bool fp16_tail = false);
 // (fp16_tail: QDFN_PREC_MIXED storage)
 double aca_share(bool fp16_tail);
 // U V (m x r, r x k) -> factors in SVD form within `budget` (Frobenius): the
-// shortest, or with fp16_tail those whose FP32 + FP16-tail storage is
-// smallest; returns the unused budget (budget - ||dropped tail||_F, >= 0), or
+// shortest, or with tail storage (tail_mode > 0) those whose tiered storage
+// is smallest; returns the unused budget (budget - ||dropped tail||_F, >= 0), or
 // 0 when the factors were left as they were (r < 2 or no budget)
 double recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V, double budget,
-                  bool fp16_tail);
+                  int tail);
 // Rounding of the components of a low-rank part in SVD form (U m x r, column
 // a of norm sig; V r x k with leading dimension ldv, orthonormal rows), each
 // relative to its own scale: eu32 / ev32 of U's column / V's row copied to
 // FP32, eu16 / ev16 of the FP16 unit vectors of QDFN_PREC_MIXED
 struct CompRound {
-    double sig, eu32, ev32, eu16, ev16;
+    double sig, eu32, ev32, eu16, ev16, eu8, ev8;
 };
 std::vector<CompRound> component_rounding(const double* U, const double* V, int64_t m, int64_t k, int64_t r,
                                           int64_t ldv);
-// first FP16 component of a part in SVD form whose stored factors stay
-// within `slack` of the exact ones (see hmat.hip)
-int64_t fp16_tail_start(const std::vector<CompRound>& c, int64_t r, double slack);
+// Storage tiers of a part in SVD form whose stored factors stay within
+// `slack` of the exact ones (see hmat.hip): components [0, r_hi) in FP32,
+// [r_hi, r_8) in FP16, [r_8, r) in FP8 (fp8 false: r_8 = r)
+void tail_tiers(const std::vector<CompRound>& c, int64_t r, double slack, bool fp8, int64_t& r_hi, int64_t& r_8);
+// the bytes of an m x k part of r components stored in those tiers
+inline int64_t tier_bytes(int64_t m, int64_t k, int64_t r, int64_t r_hi, int64_t r_8) {
+    return (m + k) * (4 * r_hi + 2 * (r_8 - r_hi) + (r - r_8));
+}
+// tail storage of a precision: 0 none, 1 FP16 tails (QDFN_PREC_MIXED), 2 FP16
+// and FP8 tails (QDFN_PREC_MIXED8)
+inline int tail_mode(int32_t prec) { return prec == QDFN_PREC_MIXED ? 1 : prec == QDFN_PREC_MIXED8 ? 2 : 0; }
+// FP8 E4M3 (OCP) rounding of the host layout, and its value
+uint8_t to_fp8(double d);
+double from_fp8(uint8_t b);
 
 // part w of block q, held in device memory (column-major, leading dimension ld)
 struct DeviceBlock {

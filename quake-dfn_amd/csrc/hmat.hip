@@ -98,6 +98,11 @@ struct bf16_t {
 struct fp16_t {
     uint16_t u;
 };
+// FP8 entry (OCP E4M3: 3 mantissa bits, exponent bias 7, largest 448, quantum
+// 2^-9 below 2^-6), decoded by the gfx950 packed conversion
+struct fp8_t {
+    uint8_t u;
+};
 
 // the two 16-bit entries of a 32-bit word (entry 0 in the low half) as FP32
 template <class T>
@@ -179,6 +184,24 @@ template <>
 struct RowLoad<bf16_t> : RowLoad16<bf16_t> {};
 template <>
 struct RowLoad<fp16_t> : RowLoad16<fp16_t> {};
+// eight FP8 entries per 8-byte load (entry j in byte j)
+typedef unsigned int nt_u2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+template <>
+struct RowLoad<fp8_t> {
+    static constexpr int W = 8;
+    uint2 v;
+    __device__ void load(const fp8_t* p) {
+        const nt_u2 q = __builtin_nontemporal_load(reinterpret_cast<const nt_u2*>(p));
+        v = make_uint2(q.x, q.y);
+    }
+    __device__ double get(int j) const {
+        const uint32_t w = j < 4 ? v.x : v.y;
+        const f32x2 pr = (j & 2) ? __builtin_amdgcn_cvt_pk_f32_fp8((int)w, true)
+                                 : __builtin_amdgcn_cvt_pk_f32_fp8((int)w, false);
+        return (double)((j & 1) ? pr.y : pr.x);
+    }
+};
 
 // x moved between lanes by the DPP control CTRL (both 32-bit halves)
 template <int CTRL>
@@ -346,10 +369,29 @@ __device__ __forceinline__ void p2_slice(const T* __restrict__ panel, const int3
 // for the normal matrix).  Slots of units a group does not have stay zero, so
 // a row is the sum over every slot in slot order (k_hsum, or the step kernel;
 // no atomics, results the same run to run).  T0: panel-0 entries (float,
-// double), T1: panel-1 entries (16-bit; unused when no unit is in panel 1).
-template <class T0, class T1>
+// double), T1: panel-1 entries (16-bit; unused when no unit is in panel 1),
+// F8: units of panel 2 (FP8) exist.
+template <class T>
+__device__ __forceinline__ void hunit_slice(const HUnit& R, const T* __restrict__ pool,
+                                            const int32_t* __restrict__ idx, int32_t c0, int32_t c1,
+                                            const double* __restrict__ x, const double* __restrict__ t, int64_t n,
+                                            double (*zs)[kP2Chunk], double* __restrict__ red) {
+    constexpr int W = RowLoad<T>::W;
+    const int lane = threadIdx.x & 63;
+    double acc[W];
+#pragma unroll
+    for (int v = 0; v < W; ++v) acc[v] = 0.0;
+    if (c1 > c0) p2_slice<T>(pool + R.off + (int64_t)c0 * R.ld, idx + R.z + c0, c1 - c0, R.ld, x, t, n, zs, acc);
+    if (lane < R.ld / W) {
+#pragma unroll
+        for (int v = 0; v < W; ++v) red[W * lane + v] = acc[v];
+    }
+}
+
+template <class T0, class T1, bool F8>
 __global__ __launch_bounds__(kP2Threads, 6) void k_hunit(const HUnit* __restrict__ units,
                                                         const T0* __restrict__ pool, const T1* __restrict__ pool_lo,
+                                                        const fp8_t* __restrict__ pool_8,
                                                         const int32_t* __restrict__ idx,
                                                         const double* __restrict__ x0, const double* __restrict__ t,
                                                         int64_t n, double* __restrict__ part, CouplingGate g) {
@@ -358,32 +400,15 @@ __global__ __launch_bounds__(kP2Threads, 6) void k_hunit(const HUnit* __restrict
     __shared__ double zs[kP2Waves][2][kP2Chunk];
     const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
     const HUnit R = units[blockIdx.x];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wave = threadIdx.x >> 6;
     const int32_t c0 = (int32_t)((int64_t)R.k * wave / kP2Waves), c1 = (int32_t)((int64_t)R.k * (wave + 1) / kP2Waves);
     // lanes of lane group 0 hold the slice's row sums (a slice with no columns: zeros)
-    if (R.lo) {
-        constexpr int W = RowLoad<T1>::W;
-        double acc[W];
-#pragma unroll
-        for (int v = 0; v < W; ++v) acc[v] = 0.0;
-        if (c1 > c0)
-            p2_slice<T1>(pool_lo + R.off + (int64_t)c0 * R.ld, idx + R.z + c0, c1 - c0, R.ld, x, t, n, zs[wave], acc);
-        if (lane < R.ld / W) {
-#pragma unroll
-            for (int v = 0; v < W; ++v) red[wave][W * lane + v] = acc[v];
-        }
-    } else {
-        constexpr int W = RowLoad<T0>::W;
-        double acc[W];
-#pragma unroll
-        for (int v = 0; v < W; ++v) acc[v] = 0.0;
-        if (c1 > c0)
-            p2_slice<T0>(pool + R.off + (int64_t)c0 * R.ld, idx + R.z + c0, c1 - c0, R.ld, x, t, n, zs[wave], acc);
-        if (lane < R.ld / W) {
-#pragma unroll
-            for (int v = 0; v < W; ++v) red[wave][W * lane + v] = acc[v];
-        }
-    }
+    if (F8 && R.lo == 2)
+        hunit_slice<fp8_t>(R, pool_8, idx, c0, c1, x, t, n, zs[wave], red[wave]);
+    else if (R.lo)
+        hunit_slice<T1>(R, pool_lo, idx, c0, c1, x, t, n, zs[wave], red[wave]);
+    else
+        hunit_slice<T0>(R, pool, idx, c0, c1, x, t, n, zs[wave], red[wave]);
     __syncthreads();
     const int row = (int)threadIdx.x;
     if (row < R.m) {
@@ -512,12 +537,15 @@ __global__ __launch_bounds__(kVThreads) void k_vpanel(const VTask* __restrict__ 
     vcol_body<T>(tasks[ti], pool, x, t, vpart, nullptr);
 }
 
-// Phase 1 of QDFN_PREC_MIXED: FP32 sub-panels (leading components) and FP16
-// sub-panels (unit vectors of the trailing components, their t rows scaled)
-// in one launch; the task's flag picks the body (uniform per wavefront).
+// Phase 1 of QDFN_PREC_MIXED(8): FP32 sub-panels (leading components), FP16
+// and (F8) FP8 sub-panels (scaled unit vectors of the trailing components,
+// their t rows scaled back) in one launch; the task's flag picks the body
+// (uniform per wavefront).
+template <bool F8>
 __global__ __launch_bounds__(kVThreads, 6) void k_vpanel_mixed(const VTask* __restrict__ tasks, int32_t ntasks,
                                                             const float* __restrict__ pool,
                                                             const fp16_t* __restrict__ pool_lo,
+                                                            const fp8_t* __restrict__ pool_8,
                                                             const double* __restrict__ tscale,
                                                             const double* __restrict__ x0, double* __restrict__ t,
                                                             double* __restrict__ vpart, CouplingGate g) {
@@ -526,7 +554,9 @@ __global__ __launch_bounds__(kVThreads, 6) void k_vpanel_mixed(const VTask* __re
     if (ti >= ntasks) return;
     const double* x = g.parity ? x0 + (int64_t)ctrl_load(g.parity) * g.parity_stride : x0;
     const VTask tk = tasks[ti];
-    if (tk.lo)
+    if (F8 && tk.lo == 2)
+        vcol_body<fp8_t>(tk, pool_8, x, t, vpart, tscale);
+    else if (tk.lo)
         vcol_body<fp16_t>(tk, pool_lo, x, t, vpart, tscale);
     else
         vcol_body<float>(tk, pool, x, t, vpart, nullptr);
@@ -698,12 +728,50 @@ uint16_t to_fp16(double d) {
     return b;
 }
 
-// QDFN_PREC_MIXED: the FP16 tail of low-rank part w of b, which is in SVD form
-// (column a of U has norm sigma_a, decreasing; V has orthonormal rows), within
-// b.slack[w] (fp16_tail_start); b.r_hi[w] is its first component.  Per FP16
-// component: fu, fv scale its U column and V row to unit vectors whose largest
-// entry lies in [0.5, 1), fc = 1 / (fu fv) scales its t row.
-void choose_fp16_tail(HBlock& b, int w, std::vector<double>& fu, std::vector<double>& fv, std::vector<double>& fc) {
+// d rounded once to FP8 E4M3 (OCP: 4 significant bits, ties to even; quantum
+// 2^-9 below 2^-6), for |d| < 448; the byte the gfx950 conversion decodes
+uint8_t to_fp8(double d) {
+    const uint8_t sgn = std::signbit(d) ? 0x80 : 0x00;
+    const double a = std::fabs(d);
+    if (a < 0x1p-6) {
+        const double q = std::nearbyint(std::ldexp(a, 9));  // 0..8 (8 = the smallest normal)
+        if (q >= 8.0) return sgn | 0x08;
+        return sgn | (uint8_t)q;
+    }
+    int e;
+    std::frexp(a, &e);  // a = f 2^e, 0.5 <= f < 1: the leading bit weighs 2^(e-1)
+    double m = std::nearbyint(std::ldexp(a, 4 - e));  // 8..16: four significant bits
+    if (m >= 16.0) {
+        m = 8.0;
+        ++e;
+    }
+    const int ef = (e - 1) + 7;  // biased exponent of 2^(e-1)
+    if (ef > 15 || (ef == 15 && m - 8.0 > 6.0)) return sgn | 0x7E;  // saturate at 448
+    return sgn | (uint8_t)(ef << 3) | (uint8_t)(m - 8.0);
+}
+
+extern "C" int qdfn_fp8_round(const double* x, double* y, int64_t count) {
+    if ((!x || !y) && count > 0) return fail(QDFN_ERR_ARG, "qdfn_fp8_round: null array");
+    for (int64_t i = 0; i < count; ++i) y[i] = from_fp8(to_fp8(x[i]));
+    return QDFN_OK;
+}
+
+double from_fp8(uint8_t b) {
+    const int e = (b >> 3) & 15, mt = b & 7;
+    double v;
+    if (e == 15 && mt == 7) return NAN;
+    if (e == 0) v = std::ldexp((double)mt, -9);
+    else v = std::ldexp(1.0 + mt / 8.0, e - 7);
+    return (b & 0x80) ? -v : v;
+}
+
+// QDFN_PREC_MIXED / _MIXED8: the storage tiers of low-rank part w of b, which
+// is in SVD form (column a of U has norm sigma_a, decreasing; V has
+// orthonormal rows), within b.slack[w] (tail_tiers): b.r_hi[w] and b.r_8[w].
+// Per tail component: fu, fv scale its U column and V row to unit vectors
+// whose largest entry lies in [0.5, 1) (FP16) or [128, 256) (FP8), fc =
+// 1 / (fu fv) scales its t row.
+void choose_tails(HBlock& b, int w, bool fp8) {
     const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1, r = b.rank[w];
     const std::vector<double>& U = b.a[w];
     const std::vector<double>& V = b.b[w];
@@ -713,8 +781,13 @@ void choose_fp16_tail(HBlock& b, int w, std::vector<double>& fu, std::vector<dou
         for (int64_t i = 0; i < m; ++i) q += U[i + a * m] * U[i + a * m];
         sig[a] = std::sqrt(q);
     }
-    const int64_t r_hi = fp16_tail_start(component_rounding(U.data(), V.data(), m, k, r, r), r, b.slack[w]);
+    int64_t r_hi, r_8;
+    tail_tiers(component_rounding(U.data(), V.data(), m, k, r, r), r, b.slack[w], fp8, r_hi, r_8);
     b.r_hi[w] = r_hi;
+    b.r_8[w] = r_8;
+    std::vector<double>& fu = b.fu[w];
+    std::vector<double>& fv = b.fv[w];
+    std::vector<double>& fc = b.fc[w];
     fu.assign(r - r_hi, 0.0);
     fv.assign(r - r_hi, 0.0);
     fc.assign(r - r_hi, 0.0);
@@ -726,9 +799,10 @@ void choose_fp16_tail(HBlock& b, int w, std::vector<double>& fu, std::vector<dou
         int eu, ev;
         std::frexp(mu / sig[a], &eu);  // largest entry of the unit column = f 2^eu, f in [0.5, 1)
         std::frexp(mv, &ev);
-        fu[a - r_hi] = std::ldexp(1.0 / sig[a], -eu);
-        fv[a - r_hi] = std::ldexp(1.0, -ev);
-        fc[a - r_hi] = std::ldexp(sig[a], eu + ev);
+        const int up = a >= r_8 ? 8 : 0;  // FP8: largest entry in [128, 256)
+        fu[a - r_hi] = std::ldexp(1.0 / sig[a], up - eu);
+        fv[a - r_hi] = std::ldexp(1.0, up - ev);
+        fc[a - r_hi] = std::ldexp(sig[a], eu + ev - 2 * up);
     }
 }
 
@@ -753,30 +827,40 @@ int hmat_layout(qdfn_hmat* h) {
     // entries of the FP64 / FP32 panels (pool) and of the 16-bit panels (lo):
     // BF16 low-rank factors (QDFN_PREC_BF16_FAR: every component of every
     // low-rank part) or FP16 unit vectors (QDFN_PREC_MIXED: the trailing
-    // components of each recompressed part, choose_fp16_tail)
-    const bool lo_far = h->prec == QDFN_PREC_BF16_FAR, mixed = h->prec == QDFN_PREC_MIXED;
+    // components of each recompressed part, choose_tails) and FP8 unit vectors
+    // (QDFN_PREC_MIXED8: the smallest of those components)
+    const int tmode = tail_mode(h->prec);
+    const bool lo_far = h->prec == QDFN_PREC_BF16_FAR, mixed = tmode > 0, fp8 = tmode == 2;
     const bool has_lo = lo_far || mixed;
-    std::vector<double> pool, lo;
+    const int ntier = fp8 ? 3 : 2;  // pools: FP64 / FP32, 16-bit, 8-bit
+    std::vector<double> pool, lo, lo8;
     std::vector<int32_t> idx;
     for (HBlock& b : B)
         for (int w = 0; w < 2; ++w) {
             b.r_hi[w] = lo_far ? 0 : b.rank[w];
+            b.r_8[w] = b.rank[w];
             b.fu[w].clear();
             b.fv[w].clear();
             b.fc[w].clear();
             if (mixed && b.kind[w] == QDFN_HBLK_LOWRANK && b.rank[w] >= 2 && b.slack[w] > 0.0)
-                choose_fp16_tail(b, w, b.fu[w], b.fv[w], b.fc[w]);
+                choose_tails(b, w, fp8);
         }
+    // components of tier glo of a low-rank part: [0, r_hi), [r_hi, r_8), [r_8, rank)
+    auto tier_span = [&](const HBlock& b, int w, int glo, int64_t& a0, int64_t& a1) {
+        a0 = glo == 0 ? 0 : glo == 1 ? b.r_hi[w] : b.r_8[w];
+        a1 = glo == 0 ? b.r_hi[w] : glo == 1 ? b.r_8[w] : b.rank[w];
+    };
     std::vector<HUnit> units;
     int64_t slots = 1, n_gid = 0;  // unit slots (most units of a row group), row groups
     std::vector<VTask> vtasks;
     std::vector<VFin> vfins;
-    std::vector<double> tsc;  // scale of each t row (QDFN_PREC_MIXED)
-    int64_t v_entries = 0, v_entries_lo = 0, v_entries_lo_rows = 0, v_cols = 0, vpart_total = 0;
+    std::vector<double> tsc;  // scale of each t row (QDFN_PREC_MIXED / _MIXED8)
+    int64_t v_entries = 0, v_entries_lo = 0, v_entries_8 = 0, v_entries_lo_rows = 0, v_cols = 0, vpart_total = 0;
     // phase 1: V panels, one per source range and pool (components [0, r_hi)
-    // in FP64 / FP32, [r_hi, rank) in 16 bits), in order of first appearance;
-    // toff / toff_lo: first t row of each part's components in either pool
-    std::vector<int64_t> toff(2 * B.size(), -1), toff_lo(2 * B.size(), -1);
+    // in FP64 / FP32, [r_hi, r_8) in 16 bits, [r_8, rank) in 8 bits), in order
+    // of first appearance; toffs[tier]: first t row of each part's components
+    // in that tier's pool
+    std::vector<std::vector<int64_t>> toffs(ntier, std::vector<int64_t>(2 * B.size(), -1));
     {
         std::vector<std::tuple<int64_t, int64_t, int>> keys;
         std::vector<std::vector<std::pair<size_t, int>>> members;
@@ -785,9 +869,10 @@ int hmat_layout(qdfn_hmat* h) {
             for (int w = 0; w < 2; ++w) {
                 const HBlock& b = B[bi];
                 if (b.kind[w] != QDFN_HBLK_LOWRANK || b.rank[w] == 0) continue;
-                for (int glo = 0; glo < 2; ++glo) {
-                    const int64_t rows = glo ? b.rank[w] - b.r_hi[w] : b.r_hi[w];
-                    if (rows == 0) continue;
+                for (int glo = 0; glo < ntier; ++glo) {
+                    int64_t a0, a1;
+                    tier_span(b, w, glo, a0, a1);
+                    if (a1 == a0) continue;
                     const auto key = std::make_tuple(b.s0, b.s1, glo);
                     auto it = key_index.find(key);
                     size_t gk;
@@ -819,17 +904,18 @@ int hmat_layout(qdfn_hmat* h) {
         for (size_t gk = 0; gk < keys.size(); ++gk) {
             const int64_t s0 = std::get<0>(keys[gk]), k = std::get<1>(keys[gk]) - s0 + 1;
             const int glo = std::get<2>(keys[gk]);
-            // 16-bit panels are read eight entries per lane load, the others four
+            // 16- and 8-bit panels are read eight entries per lane load, the others four
             const int64_t al = glo ? 8 : 4;
             auto round_al = [&](int64_t v) { return (v + al - 1) / al * al; };
-            const int64_t task_entries = task_env ? task_env : (glo ? 2 * kVTaskEntries : kVTaskEntries);
-            std::vector<double>& vdst = glo ? lo : pool;
+            const int64_t task_entries = task_env ? task_env : (kVTaskEntries << glo);  // about the same bytes
+            std::vector<double>& vdst = glo == 2 ? lo8 : glo ? lo : pool;
             rowsrc.clear();
             for (size_t f = 0; f < members[gk].size(); ++f) {
                 const HBlock& b = B[members[gk][f].first];
                 const int w = members[gk][f].second;
-                (glo ? toff_lo : toff)[2 * members[gk][f].first + w] = tbase + (int64_t)rowsrc.size();
-                const int64_t a0 = glo ? b.r_hi[w] : 0, a1 = glo ? b.rank[w] : b.r_hi[w];
+                toffs[glo][2 * members[gk][f].first + w] = tbase + (int64_t)rowsrc.size();
+                int64_t a0, a1;
+                tier_span(b, w, glo, a0, a1);
                 for (int64_t a = a0; a < a1; ++a) rowsrc.push_back({f, a});
             }
             const int64_t rows = (int64_t)rowsrc.size();
@@ -845,7 +931,7 @@ int hmat_layout(qdfn_hmat* h) {
                     const auto& m = members[gk][rowsrc[ra + q].first];
                     const HBlock& b = B[m.first];
                     const int w = m.second;
-                    const int64_t r = b.rank[w], a = rowsrc[ra + q].second, a0 = glo ? b.r_hi[w] : 0;
+                    const int64_t r = b.rank[w], a = rowsrc[ra + q].second, a0 = b.r_hi[w];
                     const double mult = (glo && mixed) ? b.fv[w][a - a0] : 1.0;
                     const std::vector<double>& V = b.b[w];
                     for (int64_t c = 0; c < k; ++c) vdst[off + c * ld + q] = V[a + c * r] * mult;
@@ -883,7 +969,7 @@ int hmat_layout(qdfn_hmat* h) {
                     vpart_total += nseg * rs;
                 }
             }
-            (glo ? v_entries_lo : v_entries) += rows * k;
+            (glo == 2 ? v_entries_8 : glo ? v_entries_lo : v_entries) += rows * k;
             if (glo) v_entries_lo_rows += rows;
             v_cols += k;
             tbase += rows;
@@ -922,18 +1008,18 @@ int hmat_layout(qdfn_hmat* h) {
             const int64_t m = g0[gi + 1] - g0[gi], row0 = g0[gi];
             out_rows2 += m;
             // the row group's panels (column-major, column stride ld): in the
-            // FP64 / FP32 pool the columns of its dense blocks and the components
-            // [0, r_hi) of its low-rank parts, in the 16-bit pool (lopanel) the
-            // components [r_hi, rank); returns (offset, first index, columns)
-            auto panel = [&](bool lopanel, std::vector<double>& dst, int64_t& off, int64_t& i0) {
-                const int64_t ld = lopanel ? (m + 7) / 8 * 8 : round4(m);
+            // FP64 / FP32 pool (tier 0) the columns of its dense blocks and the
+            // components [0, r_hi) of its low-rank parts, in the 16-bit pool
+            // (tier 1) the components [r_hi, r_8), in the 8-bit pool (tier 2)
+            // [r_8, rank); returns (offset, first index, columns)
+            auto panel = [&](int tier, std::vector<double>& dst, int64_t& off, int64_t& i0) {
+                const int64_t ld = tier ? (m + 7) / 8 * 8 : round4(m);
                 auto span = [&](const HBlock& b, int64_t& a0, int64_t& a1) {
                     if (b.kind[w] == QDFN_HBLK_DENSE) {
                         a0 = 0;
-                        a1 = lopanel ? 0 : b.s1 - b.s0 + 1;
+                        a1 = tier ? 0 : b.s1 - b.s0 + 1;
                     } else {
-                        a0 = lopanel ? b.r_hi[w] : 0;
-                        a1 = lopanel ? b.rank[w] : b.r_hi[w];
+                        tier_span(b, w, tier, a0, a1);
                     }
                 };
                 i0 = (int64_t)idx.size();
@@ -945,8 +1031,7 @@ int hmat_layout(qdfn_hmat* h) {
                         if (b.kind[w] == QDFN_HBLK_DENSE)
                             idx.push_back((int32_t)(b.s0 + c));
                         else
-                            idx.push_back((int32_t)(n + (lopanel ? toff_lo[2 * bi + w] + (c - a0)
-                                                                 : toff[2 * bi + w] + c)));
+                            idx.push_back((int32_t)(n + toffs[tier][2 * bi + w] + (c - a0)));
                     }
                 }
                 const int64_t K = (int64_t)idx.size() - i0;
@@ -960,47 +1045,48 @@ int hmat_layout(qdfn_hmat* h) {
                     if (a1 == a0) continue;
                     const int64_t mb = b.r1 - b.r0 + 1, roff = row0 - b.r0;
                     const bool diag = (w == 0 && b.s0 == b.r0 && b.s1 == b.r1 && b.kind[0] == QDFN_HBLK_DENSE);
-                    const bool scaled = lopanel && mixed && b.kind[w] == QDFN_HBLK_LOWRANK;
+                    const bool scaled = tier && mixed && b.kind[w] == QDFN_HBLK_LOWRANK;
                     const std::vector<double>& su = b.fu[w];
                     const std::vector<double>& A = b.a[w];
                     for (int64_t c = a0; c < a1; ++c)
                         for (int64_t i = 0; i < m; ++i) {
                             double v = A[(roff + i) + c * mb];
                             if (diag && row0 + i == b.s0 + c) v = 0.0;  // loading form: zero diagonal
-                            if (scaled) v *= su[c - a0];
+                            if (scaled) v *= su[c - b.r_hi[w]];
                             dst[off + (col + (c - a0)) * ld + i] = v;
                         }
                     col += a1 - a0;
                 }
                 return K;
             };
-            const int64_t ld0 = round4(m), ld1 = (m + 7) / 8 * 8;
-            int64_t off0, z0, off1 = 0, z1 = 0;
-            const int64_t k0 = panel(false, pool, off0, z0);
-            const int64_t k1 = has_lo ? panel(true, lo, off1, z1) : 0;
+            const int64_t ldt[3] = {round4(m), (m + 7) / 8 * 8, (m + 7) / 8 * 8};
+            const int64_t est[3] = {es, (int64_t)sizeof(uint16_t), 1};  // entry bytes per tier
+            int64_t offt[3] = {0, 0, 0}, zt[3] = {0, 0, 0}, kt[3] = {0, 0, 0};
+            kt[0] = panel(0, pool, offt[0], zt[0]);
+            if (has_lo) kt[1] = panel(1, lo, offt[1], zt[1]);
+            if (fp8) kt[2] = panel(2, lo8, offt[2], zt[2]);
             // units of at most unit_bytes panel bytes (at least one per group);
-            // unit u of the group writes slot u
-            const int64_t B0 = ld0 * k0 * es, B1 = ld1 * k1 * (int64_t)sizeof(uint16_t);
-            int64_t ub = unit_bytes, S0, S1;
+            // unit u of the group writes slot u; the tiers' units in tier order
+            int64_t ub = unit_bytes, St[3];
             for (;;) {
-                S0 = (B0 + ub - 1) / ub;
-                S1 = (B1 + ub - 1) / ub;
-                if (S0 + S1 <= kP2MaxSlots) break;
+                int64_t S = 0;
+                for (int q = 0; q < 3; ++q) S += St[q] = (ldt[q] * kt[q] * est[q] + ub - 1) / ub;
+                if (S <= kP2MaxSlots) break;
                 ub += ub / 4;
             }
-            if (S0 + S1 == 0) S0 = 1;
-            const int64_t S = S0 + S1;
+            if (St[0] + St[1] + St[2] == 0) St[0] = 1;
+            const int64_t S = St[0] + St[1] + St[2];
             slots = std::max(slots, S);
             for (int64_t u = 0; u < S; ++u) {
-                const bool in1 = u >= S0;
-                const int64_t Sp = in1 ? S1 : S0, up = in1 ? u - S0 : u, kp = in1 ? k1 : k0;
-                const int64_t ca = kp * up / Sp, cb = kp * (up + 1) / Sp, ldp = in1 ? ld1 : ld0;
+                const int tr = u < St[0] ? 0 : u < St[0] + St[1] ? 1 : 2;
+                const int64_t Sp = St[tr], up = u - (tr > 0 ? St[0] : 0) - (tr > 1 ? St[1] : 0), kp = kt[tr];
+                const int64_t ca = kp * up / Sp, cb = kp * (up + 1) / Sp, ldp = ldt[tr];
                 HUnit U;
-                U.off = (in1 ? off1 : off0) + ca * ldp;
-                U.z = (in1 ? z1 : z0) + ca;
+                U.off = offt[tr] + ca * ldp;
+                U.z = zt[tr] + ca;
                 U.k = (int32_t)(cb - ca);
                 U.ld = (int32_t)ldp;
-                U.lo = in1 ? 1 : 0;
+                U.lo = tr;
                 U.out = (int32_t)row0;
                 U.m = (int32_t)m;
                 U.kind = 1 + w;
@@ -1009,13 +1095,13 @@ int hmat_layout(qdfn_hmat* h) {
                 units.push_back(U);
             }
             ++n_gid;
-            b2 += m * (k0 * es + k1 * (int64_t)sizeof(uint16_t));
-            k2 += k0 + k1;
+            b2 += m * (kt[0] * es + kt[1] * (int64_t)sizeof(uint16_t) + kt[2]);
+            k2 += kt[0] + kt[1] + kt[2];
         }
     // largest units first (the dispatcher hands out workgroups in order, so
     // the tail of a launch is made of small units)
     auto unit_bytes_of = [&](const HUnit& U) {
-        return (int64_t)U.ld * U.k * (U.lo ? (int64_t)sizeof(uint16_t) : es);
+        return (int64_t)U.ld * U.k * (U.lo == 2 ? 1 : U.lo ? (int64_t)sizeof(uint16_t) : es);
     };
     // XCD-aware order: the dispatcher hands workgroup i to XCD i mod 8, each
     // XCD with its own L2.  Row groups are split into 8 contiguous row ranges
@@ -1095,6 +1181,12 @@ int hmat_layout(qdfn_hmat* h) {
         QDFN_HIP(h->pool_lo.alloc(sizeof(uint16_t) * b.size()));
         QDFN_HIP(hipMemcpy(h->pool_lo.p, b.data(), sizeof(uint16_t) * b.size(), hipMemcpyHostToDevice));
     }
+    if (fp8) {
+        std::vector<uint8_t> b(std::max<size_t>(lo8.size(), 8), 0);
+        for (size_t q = 0; q < lo8.size(); ++q) b[q] = to_fp8(lo8[q]);
+        QDFN_HIP(h->pool_8.alloc(b.size()));
+        QDFN_HIP(hipMemcpy(h->pool_8.p, b.data(), b.size(), hipMemcpyHostToDevice));
+    }
     if (mixed) {
         tsc.resize(std::max<int64_t>(h->n_t, 1), 1.0);
         QDFN_HIP(h->tscale.alloc(sizeof(double) * tsc.size()));
@@ -1128,7 +1220,7 @@ int hmat_layout(qdfn_hmat* h) {
     QDFN_HIP(h->yS.alloc(sizeof(double) * n));
     QDFN_HIP(h->yN.alloc(sizeof(double) * n));
     // HBM bytes of one product: panel entries, indices, descriptors, t (write + read), x, yS, yN
-    h->phase_bytes = (int64_t)(es * pool.size() + sizeof(uint16_t) * lo.size() + sizeof(int32_t) * idx.size() +
+    h->phase_bytes = (int64_t)(es * pool.size() + sizeof(uint16_t) * lo.size() + lo8.size() + sizeof(int32_t) * idx.size() +
                                sizeof(HUnit) * units.size() + sizeof(VTask) * vtasks.size() +
                                sizeof(VFin) * vfins.size() + 8 * slot_rows + 8 * slots * 2 * n +
                                16 * vpart_total + 16 * h->n_t +
@@ -1137,7 +1229,7 @@ int hmat_layout(qdfn_hmat* h) {
     // algorithmic bytes of each phase's main kernel: stored entries (unpadded),
     // index list (phase 2), gathered right-hand side, outputs
     // phase 1: V entries, each panel's x slice once, t
-    h->p1_entries = v_entries * es + v_entries_lo * (int64_t)sizeof(uint16_t);
+    h->p1_entries = v_entries * es + v_entries_lo * (int64_t)sizeof(uint16_t) + v_entries_8;
     h->p1_other = 8 * v_cols + 8 * h->n_t + (mixed ? 8 * v_entries_lo_rows : 0);
     h->p2_entries = b2;
     h->p2_index = 4 * k2;
@@ -1152,16 +1244,21 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
                 double* d_yN, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, hipEvent_t ev2, bool finish) {
     const double* kd = h->kdiag.as<double>();
     double* t = h->t.as<double>();
-    const bool f32 = h->prec == QDFN_PREC_FP32, bf = h->prec == QDFN_PREC_BF16_FAR, mx = h->prec == QDFN_PREC_MIXED;
+    const bool f32 = h->prec == QDFN_PREC_FP32, bf = h->prec == QDFN_PREC_BF16_FAR;
+    const bool f8 = h->prec == QDFN_PREC_MIXED8, mx = h->prec == QDFN_PREC_MIXED || f8;
     const double* tsc = mx ? h->tscale.as<double>() : nullptr;
+    const fp8_t* p8 = h->pool_8.as<fp8_t>();
     if (ev0) QDFN_HIP(hipEventRecord(ev0, s));
     if (h->n_vtasks > 0) {
         const unsigned grid = (unsigned)((h->n_vtasks + kVThreads / 64 - 1) / (kVThreads / 64));
         const VTask* vt = h->vtasks.as<VTask>();
         double* vp = h->vpart.as<double>();
-        if (mx)
-            hipLaunchKernelGGL(k_vpanel_mixed, dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
-                               h->pool.as<float>(), h->pool_lo.as<fp16_t>(), tsc, d_x, t, vp, g);
+        if (f8)
+            hipLaunchKernelGGL(k_vpanel_mixed<true>, dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
+                               h->pool.as<float>(), h->pool_lo.as<fp16_t>(), p8, tsc, d_x, t, vp, g);
+        else if (mx)
+            hipLaunchKernelGGL(k_vpanel_mixed<false>, dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
+                               h->pool.as<float>(), h->pool_lo.as<fp16_t>(), p8, tsc, d_x, t, vp, g);
         else if (f32)
             hipLaunchKernelGGL((k_vpanel<float>), dim3(grid), dim3(kVThreads), 0, s, vt, (int32_t)h->n_vtasks,
                                h->pool.as<float>(), d_x, t, vp, g);
@@ -1186,17 +1283,20 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
     double* pt = h->part.as<double>();
     if (nu == 0) return fail(QDFN_ERR_STATE, "H-matrix: no phase-2 units");
     if (bf)
-        hipLaunchKernelGGL((k_hunit<float, bf16_t>), dim3(nu), dim3(kP2Threads), 0, s, un, h->pool.as<float>(),
-                           h->pool_lo.as<bf16_t>(), ix, d_x, t, h->n, pt, g);
+        hipLaunchKernelGGL((k_hunit<float, bf16_t, false>), dim3(nu), dim3(kP2Threads), 0, s, un,
+                           h->pool.as<float>(), h->pool_lo.as<bf16_t>(), p8, ix, d_x, t, h->n, pt, g);
+    else if (f8)
+        hipLaunchKernelGGL((k_hunit<float, fp16_t, true>), dim3(nu), dim3(kP2Threads), 0, s, un,
+                           h->pool.as<float>(), h->pool_lo.as<fp16_t>(), p8, ix, d_x, t, h->n, pt, g);
     else if (mx)
-        hipLaunchKernelGGL((k_hunit<float, fp16_t>), dim3(nu), dim3(kP2Threads), 0, s, un, h->pool.as<float>(),
-                           h->pool_lo.as<fp16_t>(), ix, d_x, t, h->n, pt, g);
+        hipLaunchKernelGGL((k_hunit<float, fp16_t, false>), dim3(nu), dim3(kP2Threads), 0, s, un,
+                           h->pool.as<float>(), h->pool_lo.as<fp16_t>(), p8, ix, d_x, t, h->n, pt, g);
     else if (f32)
-        hipLaunchKernelGGL((k_hunit<float, fp16_t>), dim3(nu), dim3(kP2Threads), 0, s, un, h->pool.as<float>(),
-                           (const fp16_t*)nullptr, ix, d_x, t, h->n, pt, g);
+        hipLaunchKernelGGL((k_hunit<float, fp16_t, false>), dim3(nu), dim3(kP2Threads), 0, s, un,
+                           h->pool.as<float>(), (const fp16_t*)nullptr, p8, ix, d_x, t, h->n, pt, g);
     else
-        hipLaunchKernelGGL((k_hunit<double, fp16_t>), dim3(nu), dim3(kP2Threads), 0, s, un, h->pool.as<double>(),
-                           (const fp16_t*)nullptr, ix, d_x, t, h->n, pt, g);
+        hipLaunchKernelGGL((k_hunit<double, fp16_t, false>), dim3(nu), dim3(kP2Threads), 0, s, un,
+                           h->pool.as<double>(), (const fp16_t*)nullptr, p8, ix, d_x, t, h->n, pt, g);
     QDFN_HIP(hipGetLastError());
     if (ev2) QDFN_HIP(hipEventRecord(ev2, s));
     if (finish) {
@@ -1211,7 +1311,7 @@ int launch_hmat(const qdfn_hmat* h, const double* d_x, const CouplingGate& g, in
 
 qdfn_hmat* new_hmat(int64_t n, int32_t prec) {
     if (n <= 0 || (prec != QDFN_PREC_FP64 && prec != QDFN_PREC_FP32 && prec != QDFN_PREC_BF16_FAR &&
-                   prec != QDFN_PREC_MIXED)) {
+                   prec != QDFN_PREC_MIXED && prec != QDFN_PREC_MIXED8)) {
         fail(QDFN_ERR_ARG, "H-matrix: bad size or precision");
         return nullptr;
     }
@@ -1347,32 +1447,40 @@ std::vector<CompRound> component_rounding(const double* U, const double* V, int6
         for (int64_t c = 0; c < k; ++c) mv = std::max(mv, std::fabs(V[a + c * ldv]));
         CompRound& o = out[a];
         o.sig = std::sqrt(s2);
-        o.eu32 = o.ev32 = o.eu16 = o.ev16 = 0.0;
-        if (!(o.sig > 0.0)) continue;  // a zero component stays zero in either pool
-        // FP16: the unit vectors scaled so their largest entry lies in [0.5, 1)
-        // (choose_fp16_tail); errors measured in the unit vectors' scale
+        o.eu32 = o.ev32 = o.eu16 = o.ev16 = o.eu8 = o.ev8 = 0.0;
+        if (!(o.sig > 0.0)) continue;  // a zero component stays zero in any pool
+        // FP16: the unit vectors scaled so their largest entry lies in [0.5, 1);
+        // FP8: in [128, 256) (choose_tails); errors measured in the unit
+        // vectors' scale
         int eu, ev;
         std::frexp(mu / o.sig, &eu);
         std::frexp(mv, &ev);
         const double fu = std::ldexp(1.0 / o.sig, -eu), fv = std::ldexp(1.0, -ev);
-        double du32 = 0.0, du16 = 0.0, dv32 = 0.0, dv16 = 0.0;
+        const double fu8 = std::ldexp(fu, 8), fv8 = std::ldexp(fv, 8);
+        double du32 = 0.0, du16 = 0.0, dv32 = 0.0, dv16 = 0.0, du8 = 0.0, dv8 = 0.0;
         for (int64_t i = 0; i < m; ++i) {
             const double d32 = (double)(float)u[i] - u[i];
             const double x = u[i] * fu, d16 = fp16_value(x) - x;
+            const double y = u[i] * fu8, d8 = from_fp8(to_fp8(y)) - y;
             du32 += d32 * d32;
             du16 += d16 * d16;
+            du8 += d8 * d8;
         }
         for (int64_t c = 0; c < k; ++c) {
             const double v = V[a + c * ldv];
             const double d32 = (double)(float)v - v;
             const double x = v * fv, d16 = fp16_value(x) - x;
+            const double y = v * fv8, d8 = from_fp8(to_fp8(y)) - y;
             dv32 += d32 * d32;
             dv16 += d16 * d16;
+            dv8 += d8 * d8;
         }
         o.eu32 = std::sqrt(du32) / o.sig;
         o.ev32 = std::sqrt(dv32);
         o.eu16 = std::sqrt(du16) / (fu * o.sig);
         o.ev16 = std::sqrt(dv16) / fv;
+        o.eu8 = std::sqrt(du8) / (fu8 * o.sig);
+        o.ev8 = std::sqrt(dv8) / fv8;
     }
     return out;
 }
@@ -1385,34 +1493,38 @@ std::vector<CompRound> component_rounding(const double* U, const double* V, int6
 // inequality), with the measured rounding of each component in its pool.
 // The trailing components move to FP16, from the smallest up, while that
 // bound stays within the slack (none when even FP32 rounding exceeds it).
-int64_t fp16_tail_start(const std::vector<CompRound>& c, int64_t r, double slack) {
+void tail_tiers(const std::vector<CompRound>& c, int64_t r, double slack, bool fp8, int64_t& r_hi, int64_t& r_8) {
     double su = 0.0, sv = 0.0, sx = 0.0;
     for (int64_t a = 0; a < r; ++a) {
         su += c[a].sig * c[a].sig * c[a].eu32 * c[a].eu32;
         sv += c[a].sig * c[a].sig * c[a].ev32 * c[a].ev32;
         sx += c[a].sig * c[a].eu32 * c[a].ev32;
     }
-    if (!(std::sqrt(su) + std::sqrt(sv) + sx <= slack)) return r;
-    int64_t r_hi = r;
-    while (r_hi > 0) {
-        const CompRound& q = c[r_hi - 1];
+    r_hi = r_8 = r;
+    if (!(std::sqrt(su) + std::sqrt(sv) + sx <= slack)) return;
+    // component a moves from FP32 to the tier with errors (eu, ev) if the bound holds
+    auto try_move = [&](const CompRound& q, double eu, double ev) {
         const double s2 = q.sig * q.sig;
-        const double su2 = su + s2 * (q.eu16 * q.eu16 - q.eu32 * q.eu32);
-        const double sv2 = sv + s2 * (q.ev16 * q.ev16 - q.ev32 * q.ev32);
-        const double sx2 = sx + q.sig * (q.eu16 * q.ev16 - q.eu32 * q.ev32);
-        if (!(std::sqrt(std::max(su2, 0.0)) + std::sqrt(std::max(sv2, 0.0)) + sx2 <= slack)) break;
+        const double su2 = su + s2 * (eu * eu - q.eu32 * q.eu32);
+        const double sv2 = sv + s2 * (ev * ev - q.ev32 * q.ev32);
+        const double sx2 = sx + q.sig * (eu * ev - q.eu32 * q.ev32);
+        if (!(std::sqrt(std::max(su2, 0.0)) + std::sqrt(std::max(sv2, 0.0)) + sx2 <= slack)) return false;
         su = su2;
         sv = sv2;
         sx = sx2;
-        --r_hi;
-    }
-    return r_hi;
+        return true;
+    };
+    // the smallest components to FP8, then the next ones to FP16
+    if (fp8)
+        while (r_8 > 0 && try_move(c[r_8 - 1], c[r_8 - 1].eu8, c[r_8 - 1].ev8)) --r_8;
+    r_hi = r_8;
+    while (r_hi > 0 && try_move(c[r_hi - 1], c[r_hi - 1].eu16, c[r_hi - 1].ev16)) --r_hi;
 }
 
 // U (m x r), V (r x k) -> the shortest factors whose product differs from U V
 // by at most `budget` in the Frobenius norm.
 double recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std::vector<double>& V,
-                  double budget, bool fp16_tail) {
+                  double budget, int tail_mode) {
     if (r < 2 || budget <= 0) return 0.0;
     std::vector<double> Vt(k * r);
     for (int64_t i = 0; i < r; ++i)
@@ -1440,11 +1552,12 @@ double recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std:
         tail += s2;
         --keep;
     }
-    // components in SVD form (U = Q_u W Sigma, V = Z^T Q_v^T); with FP16
-    // trailing components, keeping a few more of them leaves more of the
+    // components in SVD form (U = Q_u W Sigma, V = Z^T Q_v^T); with FP16 (and
+    // FP8) trailing components, keeping a few more of them leaves more of the
     // budget for rounding: the count whose storage (4 bytes per FP32 entry, 2
-    // per FP16 entry) is smallest, by the measured rounding of each component
-    const int64_t kmax = fp16_tail ? std::min<int64_t>(r, keep + 32) : keep;
+    // per FP16 entry, 1 per FP8 entry) is smallest, by the measured rounding of
+    // each component
+    const int64_t kmax = tail_mode ? std::min<int64_t>(r, keep + 32) : keep;
     std::vector<double> U2(m * kmax, 0.0), V2(kmax * k, 0.0);
     for (int64_t a = 0; a < kmax; ++a) {
         const int64_t j = ord[a];
@@ -1457,15 +1570,16 @@ double recompress(int64_t m, int64_t k, int64_t& r, std::vector<double>& U, std:
                 for (int64_t c = 0; c < k; ++c) V2[a + c * kmax] += Qv[c + l * k] * zl;
         }
     }
-    if (fp16_tail) {
+    if (tail_mode) {
         // tail2[kk]: squared Frobenius norm of the components dropped when kk are kept
         std::vector<double> tail2(kmax + 1, 0.0);
         for (int64_t a = r - 1; a >= kmax; --a) tail2[kmax] += sig[ord[a]] * sig[ord[a]];
         for (int64_t a = kmax - 1; a >= 0; --a) tail2[a] = tail2[a + 1] + sig[ord[a]] * sig[ord[a]];
         const std::vector<CompRound> cr = component_rounding(U2.data(), V2.data(), m, k, kmax, kmax);
         auto bytes = [&](int64_t kk) {
-            const int64_t rh = fp16_tail_start(cr, kk, budget - std::sqrt(tail2[kk]));
-            return (m + k) * (4 * rh + 2 * (kk - rh));
+            int64_t rh, r8;
+            tail_tiers(cr, kk, budget - std::sqrt(tail2[kk]), tail_mode == 2, rh, r8);
+            return tier_bytes(m, k, kk, rh, r8);
         };
         int64_t best = keep, best_b = bytes(keep);
         for (int64_t kk = keep + 1; kk <= kmax; ++kk) {
@@ -1590,7 +1704,8 @@ int block_norms(const std::vector<const double*>& src, const std::vector<int64_t
 int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, const std::vector<double>& bnorm,
                            double rtol, double atol) {
     const double rc0 = h->build_phase_ms[3];
-    const bool tail = h->prec == QDFN_PREC_MIXED;
+    const int tmode = tail_mode(h->prec);
+    const bool tail = tmode > 0;
     double total = 0.0;  // full-pivot time: this scope minus the recompression inside it
     {
         ScopeMs whole(&total);
@@ -1705,8 +1820,9 @@ int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, con
         }
         recompress_parallel(h, lr, to_recompress, budgets);
         if (tail) {
-            // parts whose FP32 + FP16-tail factors would take at least the dense
-            // block's FP32 bytes are stored dense (from the assembled source)
+            // parts whose tiered factors (FP32 and FP16 / FP8 tails) would take at
+            // least the dense block's FP32 bytes are stored dense (from the
+            // assembled source)
             std::vector<PackItem> dense_items;
             std::vector<size_t> dense_q;
             for (size_t q = pos; q < end; ++q) {
@@ -1714,11 +1830,11 @@ int compress_device_blocks(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, con
                 const int w = lr[q].w;
                 if (b.kind[w] != QDFN_HBLK_LOWRANK) continue;
                 const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1, r = b.rank[w];
-                int64_t r_hi = r;
+                int64_t r_hi = r, r_8 = r;
                 if (r >= 2 && b.slack[w] > 0.0)
-                    r_hi = fp16_tail_start(component_rounding(b.a[w].data(), b.b[w].data(), m, k, r, r), r,
-                                           b.slack[w]);
-                if ((m + k) * (4 * r_hi + 2 * (r - r_hi)) >= 4 * m * k) {
+                    tail_tiers(component_rounding(b.a[w].data(), b.b[w].data(), m, k, r, r), r, b.slack[w],
+                               tmode == 2, r_hi, r_8);
+                if (tier_bytes(m, k, r, r_hi, r_8) >= 4 * m * k) {
                     dense_items.push_back({lr[q].src, lr[q].ld, (int32_t)m, (int32_t)k, 0});
                     dense_q.push_back(q);
                 }
@@ -1752,7 +1868,7 @@ void recompress_parallel(qdfn_hmat* h, const std::vector<DeviceBlock>& lr, const
             HBlock& b = h->blocks[lr[items[i]].q];
             const int w = lr[items[i]].w;
             const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
-            b.slack[w] = recompress(m, k, b.rank[w], b.a[w], b.b[w], budgets[i], h->prec == QDFN_PREC_MIXED);
+            b.slack[w] = recompress(m, k, b.rank[w], b.a[w], b.b[w], budgets[i], tail_mode(h->prec));
             ++done;
             if (main_thread && verbose &&
                 std::chrono::steady_clock::now() - t_last > std::chrono::seconds(10)) {
@@ -2060,22 +2176,22 @@ extern "C" int qdfn_hmat_get_block_stored(qdfn_hmat_t* h, int64_t bi, int32_t wh
                     a[i + c * m] = (diag && i == c) ? b.a[w][i + c * m] : hi(b.a[w][i + c * m]);
         return QDFN_OK;
     }
-    const int64_t r = b.rank[w], r_hi = b.r_hi[w];
+    const int64_t r = b.rank[w], r_hi = b.r_hi[w], r_8 = b.r_8[w];
     for (int64_t c = 0; c < r; ++c) {
-        const bool lo = c >= r_hi;
+        const bool lo = c >= r_hi, f8 = c >= r_8;
         const bool scaled = lo && !bf;
         const double mu = scaled ? b.fu[w][c - r_hi] : 1.0, mv = scaled ? b.fv[w][c - r_hi] : 1.0;
         const double sc = scaled ? b.fc[w][c - r_hi] : 1.0;
+        // the value one stored entry v of this component stands for
+        auto stored = [&](double v, double mult) {
+            if (!lo) return hi(v);
+            if (f8) return from_fp8(to_fp8(v * mult));
+            return from_lo(bf ? to_bf16(v) : to_fp16(v * mult), bf);
+        };
         if (a)
-            for (int64_t i = 0; i < m; ++i) {
-                const double v = b.a[w][i + c * m];
-                a[i + c * m] = lo ? from_lo(bf ? to_bf16(v) : to_fp16(v * mu), bf) * sc : hi(v);
-            }
+            for (int64_t i = 0; i < m; ++i) a[i + c * m] = lo ? stored(b.a[w][i + c * m], mu) * sc : hi(b.a[w][i + c * m]);
         if (bv)
-            for (int64_t j = 0; j < k; ++j) {
-                const double v = b.b[w][c + j * r];
-                bv[c + j * r] = lo ? from_lo(bf ? to_bf16(v) : to_fp16(v * mv), bf) : hi(v);
-            }
+            for (int64_t j = 0; j < k; ++j) bv[c + j * r] = stored(b.b[w][c + j * r], mv);
     }
     return QDFN_OK;
 }
@@ -2085,6 +2201,15 @@ extern "C" int qdfn_hmat_get_split(qdfn_hmat_t* h, int64_t* rhi_S, int64_t* rhi_
     for (size_t q = 0; q < h->blocks.size(); ++q) {
         if (rhi_S) rhi_S[q] = h->blocks[q].r_hi[0];
         if (rhi_N) rhi_N[q] = h->blocks[q].r_hi[1];
+    }
+    return QDFN_OK;
+}
+
+extern "C" int qdfn_hmat_get_split8(qdfn_hmat_t* h, int64_t* r8_S, int64_t* r8_N) {
+    if (!h) return fail(QDFN_ERR_ARG, "qdfn_hmat_get_split8: null handle");
+    for (size_t q = 0; q < h->blocks.size(); ++q) {
+        if (r8_S) r8_S[q] = h->blocks[q].r_8[0];
+        if (r8_N) r8_N[q] = h->blocks[q].r_8[1];
     }
     return QDFN_OK;
 }

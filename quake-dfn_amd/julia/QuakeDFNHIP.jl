@@ -137,7 +137,8 @@ end
 # U = F[:Q] and V with V[:, F[:p]] = F[:R].  prec: 0 FP64, 1 FP32 storage, 2 BF16 low-rank
 # factors with FP32 dense blocks (QDFN_PREC_BF16_FAR), 3 FP32 with the trailing components of each
 # recompressed low-rank block as FP16 unit vectors within the block's ACA budget (QDFN_PREC_MIXED;
-# imported blocks are not recompressed and stay FP32); FP64 accumulation in every case.
+# imported blocks are not recompressed and stay FP32), 4 the same with the smallest of those
+# components in FP8 (QDFN_PREC_MIXED8); FP64 accumulation in every case.
 function HmatCreate(FaultCount, ElementRange_SR, Admissible, ShearStiffness_H, NormalStiffness_H; prec=0)
     nb = size(ElementRange_SR, 1)
     rng = Matrix{Int64}(permutedims(ElementRange_SR))          # 4 x nb = nb x 4 row-major

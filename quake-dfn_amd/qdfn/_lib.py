@@ -30,6 +30,7 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16_FAR = 2  # H operators: BF16 low-rank factors, FP32 dense blocks
 PREC_MIXED = 3  # H operators: FP32, trailing low-rank components FP16 within each block's error budget
+PREC_MIXED8 = 4  # H operators: PREC_MIXED with the smallest trailing components in FP8 (E4M3)
 
 
 class QdfnError(RuntimeError):
@@ -114,6 +115,8 @@ SIGNATURES = {
     "qdfn_hmat_get_block": (C.c_int, [VP, I64, I32, PD, PD]),
     "qdfn_hmat_get_block_stored": (C.c_int, [VP, I64, I32, PD, PD]),
     "qdfn_hmat_get_split": (C.c_int, [VP, PI64, PI64]),
+    "qdfn_hmat_get_split8": (C.c_int, [VP, PI64, PI64]),
+    "qdfn_fp8_round": (C.c_int, [PD, PD, I64]),
     "qdfn_hmat_bytes": (C.c_int, [VP, PI64]),
     "qdfn_hmat_kself": (C.c_int, [VP, PD, PD]),
     "qdfn_hmat_normal_zero": (C.c_int, [VP]),

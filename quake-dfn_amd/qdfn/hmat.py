@@ -189,6 +189,14 @@ class HMatrix:
         check(lib().qdfn_hmat_get_split(self.handle, s.ctypes.data_as(PI64), nn.ctypes.data_as(PI64)))
         return s, nn
 
+    def split8(self):
+        """Per block, the first low-rank component in the 8-bit pool (shear, normal;
+        the rank where there is none)."""
+        nb = self.info()["blocks"]
+        s, nn = np.zeros(nb, dtype=np.int64), np.zeros(nb, dtype=np.int64)
+        check(lib().qdfn_hmat_get_split8(self.handle, s.ctypes.data_as(PI64), nn.ctypes.data_as(PI64)))
+        return s, nn
+
     def blocks(self, range_SR, stored=False):
         """The blocks in the oracle's dict format, one dict per matrix
         (n, range, kind, rank, a, b): FP64 values as built, or (stored=True)

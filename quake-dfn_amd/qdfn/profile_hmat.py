@@ -1,7 +1,7 @@
 """Launch the H-matrix product alone, for kernel traces and counter collection.
 
     python -m qdfn.profile_hmat [--reps 50] [--crit 1.5] [--aca-tol 1e-6] [--no-calibration] [--sampled]
-                                [--prec fp32|mixed|bf16|fp64]
+                                [--prec fp32|mixed|mixed8|bf16|fp64]
 
 Builds the 20,000-triangle rough-fault H operator (storage --prec: FP32; FP32
 with FP16 trailing low-rank components; BF16 low-rank factors; FP64) on the device
@@ -27,7 +27,7 @@ def main():
     ap.add_argument("--aca-tol", type=float, default=1e-6)
     ap.add_argument("--no-calibration", action="store_true")
     ap.add_argument("--sampled", action="store_true")
-    ap.add_argument("--prec", choices=("fp32", "mixed", "bf16", "fp64"), default="mixed")
+    ap.add_argument("--prec", choices=("fp32", "mixed", "mixed8", "bf16", "fp64"), default="mixed")
     a = ap.parse_args()
     import qdfn
     from qdfn import configs
@@ -46,8 +46,8 @@ def main():
     p = st["perm"]
     geo = [x[p] for x in (P1, P2, P3, m["rake"], ctr, nrm, slip)]
     h = qdfn.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=a.aca_tol,
-                               prec={"fp32": qdfn.PREC_FP32, "mixed": qdfn.PREC_MIXED, "bf16": qdfn.PREC_BF16_FAR,
-                                     "fp64": qdfn.PREC_FP64}[a.prec], sampled=a.sampled)
+                               prec={"fp32": qdfn.PREC_FP32, "mixed": qdfn.PREC_MIXED, "mixed8": qdfn.PREC_MIXED8,
+                                     "bf16": qdfn.PREC_BF16_FAR, "fp64": qdfn.PREC_FP64}[a.prec], sampled=a.sampled)
     print(f"build: {h.build_info()}", flush=True)
     info = h.info()
     t1, t2, tot = h.time_phases(a.reps)

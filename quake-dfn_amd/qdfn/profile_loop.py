@@ -2,7 +2,7 @@
 kernel traces of a whole attempt (product phases, local solves, decision).
 
     python -m qdfn.profile_loop [--steps 1000] [--warmup 100] [--aca-tol 1e-6]
-                                [--prec fp32|mixed|bf16|fp64] [--crit 1.5]
+                                [--prec fp32|mixed|mixed8|bf16|fp64] [--crit 1.5]
 
 Builds the H operator (storage --prec: FP32; FP32 with FP16 trailing low-rank
 components; BF16 low-rank factors; FP64 — leaves of 64, DistDiamRatioCrit
@@ -22,7 +22,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--aca-tol", type=float, default=1e-6)
-    ap.add_argument("--prec", choices=("fp32", "mixed", "bf16", "fp64"), default="fp32")
+    ap.add_argument("--prec", choices=("fp32", "mixed", "mixed8", "bf16", "fp64"), default="fp32")
     ap.add_argument("--crit", type=float, default=1.5)
     a = ap.parse_args()
     import qdfn
@@ -38,8 +38,8 @@ def main():
     p = st["perm"]
     geo = [x[p] for x in (P1, P2, P3, m["rake"], ctr, nrm, slip)]
     h = qdfn.HMatrix.build_tri(*geo, m["mu"], lam, st["range_SR"], st["admissible"], aca_rtol=a.aca_tol,
-                               prec={"fp32": qdfn.PREC_FP32, "mixed": qdfn.PREC_MIXED, "bf16": qdfn.PREC_BF16_FAR,
-                                     "fp64": qdfn.PREC_FP64}[a.prec])
+                               prec={"fp32": qdfn.PREC_FP32, "mixed": qdfn.PREC_MIXED, "mixed8": qdfn.PREC_MIXED8,
+                                     "bf16": qdfn.PREC_BF16_FAR, "fp64": qdfn.PREC_FP64}[a.prec])
     ks, _ = h.kself()
     hs, hn = h.matvec(np.ones(n), loading_form=False)
     prob = configs.problem_from_triangles(m, ks, hs, hn, order=p)

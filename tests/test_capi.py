@@ -86,5 +86,5 @@ def test_precision_checks_without_device():
     B = (PD * 1)()
     import ctypes as C
     args = [kind.ctypes.data_as(PI32), rank.ctypes.data_as(PI64), C.cast(A, C.POINTER(PD)), C.cast(B, C.POINTER(PD))]
-    assert L.qdfn_hmat_create(4, 1, rng.ctypes.data_as(PI64), adm.ctypes.data_as(PI64), *args, *args, 4) is None
+    assert L.qdfn_hmat_create(4, 1, rng.ctypes.data_as(PI64), adm.ctypes.data_as(PI64), *args, *args, 5) is None
     assert "precision" in _lib.last_error()

@@ -1,16 +1,18 @@
 """GPU: H operators with QDFN_PREC_MIXED storage — FP32 dense blocks and
 leading low-rank components, the trailing components of each recompressed
 low-rank part as FP16 unit vectors with an FP64 scale, as many as fit in the
-part's unused error budget (no reference counterpart: the reference stores
-FP64; HmatSolver_Pararllel, scripts/Functions_Hmatrix.jl:631-646, is the
-product, main_H, Functions_RSFDFN3DMain_H.jl:241-248, the loop).
+part's unused error budget — and QDFN_PREC_MIXED8, whose smallest trailing
+components are FP8 (E4M3) unit vectors scaled to [128, 256) (no reference
+counterpart: the reference stores FP64; HmatSolver_Pararllel,
+scripts/Functions_Hmatrix.jl:631-646, is the product, main_H,
+Functions_RSFDFN3DMain_H.jl:241-248, the loop).
 
 On the 2,400-triangle rough fault (ACA tolerance 1e-6):
 * every block as stored (rounded) is within the ACA bound of the stiffness
   assembled on the device: ||H_b - K_b||_F <= rtol ||K_S,b||_F, for both
   matrices — the FP16 tail costs no accuracy class;
-* a large share of the factor entries is FP16, and the operator moves fewer
-  bytes than FP32 storage;
+* a large share of the factor entries is FP16 (and FP8), and the operator
+  moves fewer bytes than FP32 storage (MIXED8: fewer than MIXED);
 * the product equals the oracle's HmatSolver on the stored blocks to 1e-12
   (both forms);
 * main_H on the stored operator tracks the oracle's main_H on the same stored
@@ -37,12 +39,18 @@ def _mesh(qd, m):
 
 
 @pytest.fixture(scope="module")
-def rough(qd):
+def rough_mesh(qd):
     from qdfn import configs
-    c = _mesh(qd, configs.rough_fault(nx=40, nz=30, h=50.0))
+    return _mesh(qd, configs.rough_fault(nx=40, nz=30, h=50.0))
+
+
+@pytest.fixture(scope="module", params=["mixed", "mixed8"])
+def rough(qd, rough_mesh, request):
+    c = dict(rough_mesh)
     st = c["st"]
+    c["prec"] = {"mixed": qd.PREC_MIXED, "mixed8": qd.PREC_MIXED8}[request.param]
     c["H"] = qd.HMatrix.build_tri(*c["geo"], c["m"]["mu"], c["lam"], st["range_SR"], st["admissible"],
-                                  aca_rtol=RTOL, prec=qd.PREC_MIXED)
+                                  aca_rtol=RTOL, prec=c["prec"])
     yield c
     c["H"].close()
 
@@ -54,13 +62,14 @@ def test_mixed_blocks_within_tolerance(rough, qd):
                                                          rough["lam"], geo[5], geo[6])
     BS, BN = H.blocks(st["range_SR"], stored=True)
     sS, sN = H.split()
-    lo_entries = lr_entries = 0
+    eS, eN = H.split8()
+    lo_entries = lr_entries = lo8_entries = 0
     worst = 0.0
     for q, r in enumerate(st["range_SR"]):
         sl = (slice(r[2] - 1, r[3]), slice(r[0] - 1, r[1]))
         tol = RTOL * np.linalg.norm(KS[sl])
         mb, kb = r[3] - r[2] + 1, r[1] - r[0] + 1
-        for B, K, rhi in ((BS, KS, sS), (BN, KN, sN)):
+        for B, K, rhi, r8 in ((BS, KS, sS, eS), (BN, KN, sN, eN)):
             if B["kind"][q] == 0:
                 approx = B["a"][q]
             else:
@@ -68,13 +77,20 @@ def test_mixed_blocks_within_tolerance(rough, qd):
                 approx = B["a"][q] @ B["b"][q] if rk > 0 else np.zeros((mb, kb))
                 lr_entries += rk * (mb + kb)
                 lo_entries += (rk - rhi[q]) * (mb + kb)
+                lo8_entries += (rk - r8[q]) * (mb + kb)
+                assert rhi[q] <= r8[q] <= rk
             err = np.linalg.norm(approx - K[sl])
             worst = max(worst, err / tol if tol > 0 else 0.0)
             # FP32 rounding of dense blocks is outside the ACA budget: 2^-24 per entry
             allow = tol * (1 + 1e-9) if B["kind"][q] == 1 else tol + 2.0 ** -24 * np.linalg.norm(K[sl])
             assert err <= allow + 1e-9 * np.abs(K).max(), (q, err, tol)
-    print(f"[mixed] worst block error / tolerance {worst:.3f}; FP16 share of factor entries {lo_entries / lr_entries:.3f}")
+    print(f"[mixed] worst block error / tolerance {worst:.3f}; 16- and 8-bit share of factor entries "
+          f"{lo_entries / lr_entries:.3f}, 8-bit {lo8_entries / lr_entries:.3f}")
     assert lo_entries > 0.3 * lr_entries
+    if rough["prec"] == qd.PREC_MIXED8:
+        assert lo8_entries > 0.05 * lr_entries
+    else:
+        assert lo8_entries == 0
 
 
 def test_mixed_fewer_bytes(rough, qd):
@@ -83,8 +99,15 @@ def test_mixed_fewer_bytes(rough, qd):
                                aca_rtol=RTOL, prec=qd.PREC_FP32)
     b32, bmx = H32.info()["bytes_per_product"], rough["H"].info()["bytes_per_product"]
     H32.close()
-    print(f"[mixed] bytes per product: FP32 {b32 / 1e6:.1f} MB, mixed {bmx / 1e6:.1f} MB")
+    print(f"[mixed] bytes per product: FP32 {b32 / 1e6:.1f} MB, precision {rough['prec']} {bmx / 1e6:.1f} MB")
     assert bmx < 0.9 * b32
+    if rough["prec"] == qd.PREC_MIXED8:
+        H16 = qd.HMatrix.build_tri(*rough["geo"], rough["m"]["mu"], rough["lam"], st["range_SR"],
+                                   st["admissible"], aca_rtol=RTOL, prec=qd.PREC_MIXED)
+        b16 = H16.info()["bytes_per_product"]
+        H16.close()
+        print(f"[mixed] bytes per product: MIXED {b16 / 1e6:.1f} MB")
+        assert bmx < b16
 
 
 def test_mixed_product_matches_oracle(rough, orc):
@@ -103,13 +126,15 @@ def test_mixed_product_matches_oracle(rough, orc):
     assert scale > 0
 
 
-def test_mixed_main_h_tracks_oracle(qd, orc):
+@pytest.mark.parametrize("tiered", ["mixed", "mixed8"])
+def test_mixed_main_h_tracks_oracle(qd, orc, tiered):
     from qdfn import configs
     c = _mesh(qd, configs.rough_fault_patch(nx=40, nz=30))
     st = c["st"]
     steps = 1200
     runs = {}
-    for prec in (qd.PREC_MIXED, qd.PREC_FP64):
+    mp = {"mixed": qd.PREC_MIXED, "mixed8": qd.PREC_MIXED8}[tiered]
+    for prec in (mp, qd.PREC_FP64):
         H = qd.HMatrix.build_tri(*c["geo"], c["m"]["mu"], c["lam"], st["range_SR"], st["admissible"],
                                  aca_rtol=1e-7, prec=prec)
         ks, _ = H.kself()
@@ -121,7 +146,7 @@ def test_mixed_main_h_tracks_oracle(qd, orc):
         tr = sim.trace()
         sim.close()
         runs[prec] = tr
-        if prec == qd.PREC_MIXED:
+        if prec == mp:
             BS, BN = H.blocks(st["range_SR"], stored=True)
             status, ref = orc.sim_hmat(prob, BS, BN, total_step=steps, record_step=0)
             assert status == 0
@@ -132,6 +157,6 @@ def test_mixed_main_h_tracks_oracle(qd, orc):
             assert abs(e_gpu - e_ref) <= 1e-9 * e_ref
         H.close()
     on = {p: configs.first_event_time(t["told"], t["dt"], t["vmax"]) for p, t in runs.items()}
-    rel = abs(on[qd.PREC_MIXED] - on[qd.PREC_FP64]) / on[qd.PREC_FP64]
-    print(f"[mixed] onset mixed {on[qd.PREC_MIXED]:.9e} FP64 {on[qd.PREC_FP64]:.9e} rel {rel:.2e}")
+    rel = abs(on[mp] - on[qd.PREC_FP64]) / on[qd.PREC_FP64]
+    print(f"[mixed] onset {tiered} {on[mp]:.9e} FP64 {on[qd.PREC_FP64]:.9e} rel {rel:.2e}")
     assert rel <= 1e-6
