@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kStepBlock) void k_ens_step(const EnsBlk* __restric
                                                          SimState* const* __restrict__ states,
                                                          const uint32_t* __restrict__ nblk) {
     const EnsBlk b = blks[blockIdx.x];
-    step_body(arrays[b.inst], states[b.inst], (unsigned)b.blk, nblk[b.inst]);
+    step_body<true>(arrays[b.inst], states[b.inst], (unsigned)b.blk, nblk[b.inst]);
 }
 
 __global__ __launch_bounds__(kSetupBlock) void k_ens_setup(const SimArrays* __restrict__ arrays,

@@ -65,7 +65,7 @@ constexpr int kP2Threads = 64 * kP2Waves;
 constexpr int kP2Depth = 4;    // phase 2: steps in flight per wavefront
 constexpr int kVThreads = 256; // phase 1: four independent wavefronts per workgroup
 constexpr int kVDepth = 4;     // phase 1: steps in flight per wavefront
-constexpr int kVTaskEntries = 4096;   // phase 1: target entries per task (FP32 and FP64; twice in 16 bits)
+constexpr int kVTaskEntries = 12288;  // phase 1: target entries per task (any pool)
 
 int64_t round4(int64_t v) { return (v + 3) / 4 * 4; }
 
@@ -907,7 +907,7 @@ int hmat_layout(qdfn_hmat* h) {
             // 16- and 8-bit panels are read eight entries per lane load, the others four
             const int64_t al = glo ? 8 : 4;
             auto round_al = [&](int64_t v) { return (v + al - 1) / al * al; };
-            const int64_t task_entries = task_env ? task_env : (kVTaskEntries << glo);  // about the same bytes
+            const int64_t task_entries = task_env ? task_env : kVTaskEntries;
             std::vector<double>& vdst = glo == 2 ? lo8 : glo ? lo : pool;
             rowsrc.clear();
             for (size_t f = 0; f < members[gk].size(); ++f) {

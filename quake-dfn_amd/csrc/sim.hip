@@ -2,19 +2,21 @@
 // scripts/Functions_RSFDFN3DMain_D.jl:2-324 (and main_H's identical control
 // logic, scripts/Functions_RSFDFN3DMain_H.jl:180-383) without host round trips.
 //
-// One "attempt" = three launches on the run's stream:
-//   k_coupling        split coupling product of DispOld (gated: only on the
-//                     first attempt of a step; retries reuse it, as the
-//                     reference does), reading the parity-selected buffer
-//   k_coupling_finish fixed-order sum of the splits -> TL, SigProd (gated)
-//   k_step            one lane per element: HighV / LowV secant solve (FP64),
-//                     then per-workgroup partial reductions; the last
-//                     workgroup to arrive takes the retry-or-accept decision
-//                     of main() (:240-258), records history / trace, swaps
-//                     Old<-New by flipping the buffer parity, and selects the
-//                     next step's Dt (:162-187) from quantities every lane
-//                     computed on the assumption that its attempt is accepted
-//                     (Accel, V/Accel, SolverSwitch inputs).
+// One "attempt" = launches on the run's stream:
+//   coupling          the product of DispOld (gated: only on the first
+//                     attempt of a step; retries reuse it, as the reference
+//                     does), reading the parity-selected buffer: k_coupling
+//                     (dense: split partial sums) or the two H phases (unit
+//                     slots)
+//   k_step            one lane per element: the split / slot sums -> TL,
+//                     SigProd, then HighV / LowV secant solve (FP64), then
+//                     per-workgroup partial reductions
+//   k_decide          one workgroup: the retry-or-accept decision of main()
+//                     (:240-258) from the parts, records history / trace,
+//                     swaps Old<-New by flipping the buffer parity, and
+//                     selects the next step's Dt (:162-187) from quantities
+//                     every lane computed on the assumption that its attempt
+//                     is accepted (Accel, V/Accel, SolverSwitch inputs).
 // All decisions live in a device SimState block, so attempts can be captured
 // once into a HIP graph and replayed; the host reads the state once per
 // replay.  Kernels return at once when the state says the requested number of
@@ -39,7 +41,12 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SimArrays A, SimState* __
 
 // One attempt over the whole run (one workgroup per kStepBlock elements).
 __global__ __launch_bounds__(kStepBlock) void k_step(SimArrays A, SimState* __restrict__ st) {
-    step_body(A, st, blockIdx.x, gridDim.x);
+    step_body<false>(A, st, blockIdx.x, gridDim.x);
+}
+
+// The attempt's decision from k_step's nblk workgroup parts.
+__global__ __launch_bounds__(kStepBlock) void k_decide(SimArrays A, SimState* __restrict__ st, unsigned nblk) {
+    decide_body(A, st, nblk);
 }
 
 }  // namespace qdfn
@@ -65,8 +72,10 @@ static int enqueue_attempt(qdfn_sim* s, int slot, hipEvent_t ev0, hipEvent_t ev1
         if (rc) return rc;
         if (ev1) QDFN_HIP(hipEventRecord(ev1, s->stream));
     }
-    hipLaunchKernelGGL(k_step, dim3((unsigned)((n + kStepBlock - 1) / kStepBlock)), dim3(kStepBlock), 0, s->stream,
-                       s->A, s->d_st);
+    const unsigned nblk = (unsigned)((n + kStepBlock - 1) / kStepBlock);
+    hipLaunchKernelGGL(k_step, dim3(nblk), dim3(kStepBlock), 0, s->stream, s->A, s->d_st);
+    QDFN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(kStepBlock), 0, s->stream, s->A, s->d_st, nblk);
     QDFN_HIP(hipGetLastError());
     return QDFN_OK;
 }

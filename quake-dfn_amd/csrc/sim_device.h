@@ -309,6 +309,10 @@ __device__ inline void decide(const SimArrays& A, SimState* __restrict__ st, con
     st->rhs_evals = S.rhs_evals + 1;
 }
 
+// kLastDecides: the last workgroup to arrive takes the decision (after an
+// agent-scope fence); otherwise the workgroups only write their parts and
+// decide_body, in the next kernel on the stream, takes it.
+template <bool kLastDecides>
 __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restrict__ st, unsigned blk, unsigned nblk) {
     __shared__ BlockPart shp[kStepBlock / 64];
     __shared__ int s_last;
@@ -435,6 +439,10 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
     mine.ok = ok; mine.any4 = any4; mine.found = found; mine.pad = 0;
     mine.mindt = mindt; mine.vmax = vmax; mine.vmax_next = vmax_next; mine.dtmin_next = dtmin_next;
     const BlockPart bpart = block_reduce_part<kStepBlock>(mine, shp);
+    if (!kLastDecides) {
+        if (threadIdx.x == 0) A.parts[blk] = bpart;
+        return;
+    }
     if (threadIdx.x == 0) {
         A.parts[blk] = bpart;
         __threadfence();
@@ -460,6 +468,23 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
     all = block_reduce_part<kStepBlock>(all, shp);
     if (threadIdx.x != 0) return;
     *A.arrive = 0u;
+    decide(A, st, S, all);
+}
+
+// The decision of step_body<false>'s attempt, one workgroup: the parts of the
+// nblk workgroups folded in the same order as the last workgroup folds them.
+// The kernel boundary orders the parts' writes before these reads, so no
+// agent-scope fence (an L2 write-back on every XCD) is needed.
+__device__ __forceinline__ void decide_body(const SimArrays& A, SimState* __restrict__ st, unsigned nblk) {
+    __shared__ BlockPart shp[kStepBlock / 64];
+    const SimState S = load_state(st);
+    if (S.done) return;
+    BlockPart all;
+    all.ok = 1; all.any4 = 0; all.found = 0; all.pad = 0;
+    all.mindt = INFINITY; all.vmax = -INFINITY; all.vmax_next = -INFINITY; all.dtmin_next = INFINITY;
+    for (unsigned k = threadIdx.x; k < nblk; k += kStepBlock) fold_part(all, A.parts[k]);
+    all = block_reduce_part<kStepBlock>(all, shp);
+    if (threadIdx.x != 0) return;
     decide(A, st, S, all);
 }
 
