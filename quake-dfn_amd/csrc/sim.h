@@ -32,7 +32,11 @@ struct qdfn_sim {
     qdfn::SimState* d_st = nullptr;
     qdfn::SimState h_st{};
     bool started = false;
-    GraphSlot graph;  // captured attempt sequence
+    // captured attempt sequences of 1, 2, 4, ... attempts and of graph_k (the
+    // run's batch size), so that a run near its step target replays no more
+    // attempts than it can use
+    std::vector<GraphSlot> graphs;
+    int graph_k = 0;
     // timing of coupling kernels (mode 1)
     double coup_ms_sum = 0.0;
     int64_t coup_count = 0;

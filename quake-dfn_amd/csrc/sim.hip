@@ -85,7 +85,8 @@ extern "C" void qdfn_sim_destroy(qdfn_sim_t* s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->op) s->op->users -= 1;
     if (s->hop) s->hop->users -= 1;
-    s->graph.reset();
+    for (GraphSlot& g : s->graphs) g.reset();
+    s->graphs.clear();
     hipStream_t st = s->stream;
     delete s;
     if (st) (void)hipStreamDestroy(st);
@@ -421,26 +422,39 @@ static int run_impl(qdfn_sim* s, const char* who, int64_t n_steps, double t_end,
     if (mode != 0 && mode != 1) return fail(QDFN_ERR_ARG, "qdfn_sim_run: mode must be 0 or 1");
     if (mode == 0) {
         const int K = std::max(1, graph_attempts > 0 ? graph_attempts : 32);
-        GraphSlot& gs = s->graph;
-        if (!gs.exec || gs.attempts != K) {
-            gs.reset();
+        if (s->graph_k != K) {
+            for (GraphSlot& g : s->graphs) g.reset();
+            s->graphs.clear();
+            std::vector<int> sizes;
+            for (int k = 1; k < K; k *= 2) sizes.push_back(k);
+            sizes.push_back(K);
             QDFN_HIP(hipStreamSynchronize(s->stream));
-            QDFN_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-            int rc = QDFN_OK;
-            for (int k = 0; k < K && rc == QDFN_OK; ++k) rc = enqueue_attempt(s, -1, nullptr, nullptr);
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(s->stream, &g);
-            if (rc != QDFN_OK) return rc;
-            if (e != hipSuccess) return fail(QDFN_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-            gs.graph = g;
-            QDFN_HIP(hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0));
-            gs.attempts = K;
+            for (int size : sizes) {
+                s->graphs.emplace_back();
+                GraphSlot& gs = s->graphs.back();
+                QDFN_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+                int rc = QDFN_OK;
+                for (int k = 0; k < size && rc == QDFN_OK; ++k) rc = enqueue_attempt(s, -1, nullptr, nullptr);
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(s->stream, &g);
+                if (rc != QDFN_OK) return rc;
+                if (e != hipSuccess) return fail(QDFN_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+                gs.graph = g;
+                QDFN_HIP(hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0));
+                gs.attempts = size;
+            }
+            s->graph_k = K;
         }
         for (;;) {
             int rc = maybe_flush(K, false);
             if (rc) return rc;
             if (cb_rc) break;
-            QDFN_HIP(hipGraphLaunch(gs.exec, s->stream));
+            // the largest batch that the steps still wanted can use (each
+            // attempt accepts at most one step; runs to t_end take K)
+            const int64_t left = t_end > 0.0 ? K : std::max<int64_t>(1, s->h_st.target - s->h_st.step);
+            size_t pick = 0;
+            while (pick + 1 < s->graphs.size() && s->graphs[pick + 1].attempts <= left) ++pick;
+            QDFN_HIP(hipGraphLaunch(s->graphs[pick].exec, s->stream));
             QDFN_HIP(hipMemcpyAsync(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost, s->stream));
             QDFN_HIP(hipStreamSynchronize(s->stream));
             if (s->h_st.done) break;

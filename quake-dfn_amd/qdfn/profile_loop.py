@@ -3,12 +3,14 @@ kernel traces of a whole attempt (product phases, local solves, decision).
 
     python -m qdfn.profile_loop [--steps 1000] [--warmup 100] [--aca-tol 1e-6]
                                 [--prec fp32|mixed|mixed8|bf16|fp64] [--crit 1.5]
+                                [--repeat 1]
 
 Builds the H operator (storage --prec: FP32; FP32 with FP16 trailing low-rank
 components; BF16 low-rank factors; FP64 — leaves of 64, DistDiamRatioCrit
 --crit) and the run's arrays the way the benchmark does, replays the captured
 attempt graph for `steps` accepted steps and prints the RHS evaluations per
-second.
+second (with --repeat R: R such runs one after another, each timed, and the
+rates of every run).
 """
 import argparse
 import sys
@@ -24,6 +26,7 @@ def main():
     ap.add_argument("--aca-tol", type=float, default=1e-6)
     ap.add_argument("--prec", choices=("fp32", "mixed", "mixed8", "bf16", "fp64"), default="fp32")
     ap.add_argument("--crit", type=float, default=1.5)
+    ap.add_argument("--repeat", type=int, default=1)
     a = ap.parse_args()
     import qdfn
     from qdfn import configs
@@ -44,13 +47,19 @@ def main():
     hs, hn = h.matvec(np.ones(n), loading_form=False)
     prob = configs.problem_from_triangles(m, ks, hs, hn, order=p)
     prob["ext_at_told"] = 1
-    sim = qdfn.Simulation(h, prob, record_step=0, record_capacity=0, trace_capacity=a.steps + a.warmup)
+    sim = qdfn.Simulation(h, prob, record_step=0, record_capacity=0,
+                          trace_capacity=a.steps * a.repeat + a.warmup)
     sim.run(a.warmup)
-    s0 = sim.stats()
-    t0 = time.perf_counter()
-    sim.run(a.steps)
-    el = time.perf_counter() - t0
-    s1 = sim.stats()
+    rates = []
+    for _ in range(a.repeat):
+        s0 = sim.stats()
+        t0 = time.perf_counter()
+        sim.run(a.steps)
+        el = time.perf_counter() - t0
+        s1 = sim.stats()
+        rates.append((s1["rhs_evals"] - s0["rhs_evals"]) / el)
+    if a.repeat > 1:
+        print("rates of the runs (RHS evals/s): " + " ".join(f"{r:.0f}" for r in rates), flush=True)
     sim.close()
     info = h.info()
     ph = h.time_phases(20)
