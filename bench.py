@@ -661,7 +661,7 @@ def main():
         "adjoint_4096": adj,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
-                     "kernel": "k_hunit<float, fp16_t> (H product phase 2)",
+                     "kernel": "k_hunit<float, fp16_t, false> (H product phase 2)",
                      "bytes": "stored panel entries only", "bytes_per_launch": hb["phase2_entries"],
                      "index_and_gather_bytes": hb["phase2_index"] + hb["phase2_gather"]},
         "cpu_baseline": cpu,
