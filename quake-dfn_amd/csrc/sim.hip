@@ -449,12 +449,15 @@ static int run_impl(qdfn_sim* s, const char* who, int64_t n_steps, double t_end,
             int rc = maybe_flush(K, false);
             if (rc) return rc;
             if (cb_rc) break;
-            // the largest batch that the steps still wanted can use (each
-            // attempt accepts at most one step; runs to t_end take K)
-            const int64_t left = t_end > 0.0 ? K : std::max<int64_t>(1, s->h_st.target - s->h_st.step);
-            size_t pick = 0;
-            while (pick + 1 < s->graphs.size() && s->graphs[pick + 1].attempts <= left) ++pick;
-            QDFN_HIP(hipGraphLaunch(s->graphs[pick].exec, s->stream));
+            // a batch of min(K, steps still wanted) attempts (each attempt
+            // accepts at most one step; runs to t_end take K), composed of the
+            // captured sizes largest first, then one read of the run state
+            int64_t left = t_end > 0.0 ? K : std::min<int64_t>(K, std::max<int64_t>(1, s->h_st.target - s->h_st.step));
+            for (size_t g = s->graphs.size(); g-- > 0 && left > 0;)
+                while (s->graphs[g].attempts <= left) {
+                    QDFN_HIP(hipGraphLaunch(s->graphs[g].exec, s->stream));
+                    left -= s->graphs[g].attempts;
+                }
             QDFN_HIP(hipMemcpyAsync(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost, s->stream));
             QDFN_HIP(hipStreamSynchronize(s->stream));
             if (s->h_st.done) break;
