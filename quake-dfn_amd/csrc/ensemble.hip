@@ -216,12 +216,12 @@ __global__ __launch_bounds__(256) void k_ens_finish(const EnsBlk* __restrict__ f
     }
 }
 
-__global__ __launch_bounds__(kStepBlock) void k_ens_step(const EnsBlk* __restrict__ blks,
+__global__ __launch_bounds__(kEnsStepBlock) void k_ens_step(const EnsBlk* __restrict__ blks,
                                                          const SimArrays* __restrict__ arrays,
                                                          SimState* const* __restrict__ states,
                                                          const uint32_t* __restrict__ nblk) {
     const EnsBlk b = blks[blockIdx.x];
-    step_body<true>(arrays[b.inst], states[b.inst], (unsigned)b.blk, nblk[b.inst]);
+    step_body<true, kEnsStepBlock>(arrays[b.inst], states[b.inst], (unsigned)b.blk, nblk[b.inst]);
 }
 
 __global__ __launch_bounds__(kSetupBlock) void k_ens_setup(const SimArrays* __restrict__ arrays,
@@ -333,7 +333,7 @@ int enqueue_attempt(qdfn_ensemble* e) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_ens_finish, dim3((unsigned)e->nfins), dim3(256), 0, e->stream, e->fins.as<EnsBlk>(),
                        e->insts.as<EnsInst>(), 0);
-    hipLaunchKernelGGL(k_ens_step, dim3((unsigned)e->nsteps), dim3(kStepBlock), 0, e->stream,
+    hipLaunchKernelGGL(k_ens_step, dim3((unsigned)e->nsteps), dim3(kEnsStepBlock), 0, e->stream,
                        e->steps.as<EnsBlk>(), e->arrays.as<SimArrays>(), e->states.as<SimState*>(),
                        e->nblk.as<uint32_t>());
     QDFN_HIP(hipGetLastError());
@@ -424,7 +424,7 @@ extern "C" qdfn_ensemble_t* qdfn_ensemble_create(int64_t m, qdfn_sim_t* const* s
         arrays[k].dpartS = arrays[k].dpartN = nullptr;
         states[k] = s->d_st;
         const int64_t n = s->P.n;
-        nblk[k] = (uint32_t)((n + kStepBlock - 1) / kStepBlock);
+        nblk[k] = (uint32_t)((n + kEnsStepBlock - 1) / kEnsStepBlock);
         EnsInst& I = insts[k];
         I.gate = &s->d_st->need_coupling;
         I.parity = &s->d_st->parity;

@@ -39,13 +39,13 @@ namespace qdfn {
 // Run-start step setup (:148-187).
 __global__ __launch_bounds__(kSetupBlock) void k_setup(SimArrays A, SimState* __restrict__ st) { setup_body(A, st); }
 
-// One attempt over the whole run (one workgroup per kStepBlock elements).
-__global__ __launch_bounds__(kStepBlock) void k_step(SimArrays A, SimState* __restrict__ st) {
-    step_body<false>(A, st, blockIdx.x, gridDim.x);
+// One attempt over the whole run (one workgroup per kRunStepBlock elements).
+__global__ __launch_bounds__(kRunStepBlock) void k_step(SimArrays A, SimState* __restrict__ st) {
+    step_body<false, kRunStepBlock>(A, st, blockIdx.x, gridDim.x);
 }
 
 // The attempt's decision from k_step's nblk workgroup parts.
-__global__ __launch_bounds__(kStepBlock) void k_decide(SimArrays A, SimState* __restrict__ st, unsigned nblk) {
+__global__ __launch_bounds__(kDecideBlock) void k_decide(SimArrays A, SimState* __restrict__ st, unsigned nblk) {
     decide_body(A, st, nblk);
 }
 
@@ -72,10 +72,10 @@ static int enqueue_attempt(qdfn_sim* s, int slot, hipEvent_t ev0, hipEvent_t ev1
         if (rc) return rc;
         if (ev1) QDFN_HIP(hipEventRecord(ev1, s->stream));
     }
-    const unsigned nblk = (unsigned)((n + kStepBlock - 1) / kStepBlock);
-    hipLaunchKernelGGL(k_step, dim3(nblk), dim3(kStepBlock), 0, s->stream, s->A, s->d_st);
+    const unsigned nblk = (unsigned)((n + kRunStepBlock - 1) / kRunStepBlock);
+    hipLaunchKernelGGL(k_step, dim3(nblk), dim3(kRunStepBlock), 0, s->stream, s->A, s->d_st);
     QDFN_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_decide, dim3(1), dim3(kStepBlock), 0, s->stream, s->A, s->d_st, nblk);
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(kDecideBlock), 0, s->stream, s->A, s->d_st, nblk);
     QDFN_HIP(hipGetLastError());
     return QDFN_OK;
 }
@@ -114,7 +114,7 @@ static qdfn_sim* sim_alloc(const char* who, const qdfn_sim_params* p) {
     // sw[n] + arrival counter
     if (s->ints.alloc(sizeof(int32_t) * (n + 1)) || hipMemset(s->ints.p, 0, sizeof(int32_t) * (n + 1)))
         return bail("int alloc");
-    const int64_t nblk = (n + kStepBlock - 1) / kStepBlock;
+    const int64_t nblk = (n + kRunStepBlock - 1) / kRunStepBlock;
     if (s->parts.alloc(sizeof(BlockPart) * nblk)) return bail("partials alloc");
     if (s->cand.alloc(n)) return bail("cand alloc");
     const int64_t rc_cap = p->record_capacity, tc = p->trace_capacity;

@@ -134,7 +134,13 @@ __device__ inline void ext_interp_at(const double* __restrict__ ext_t, const dou
     }
 }
 
-constexpr int kStepBlock = 256;
+// threads per step workgroup: a single run's k_step (its decision in
+// k_decide; smaller workgroups spread the latency-bound solves over more CUs)
+// and the ensemble's k_ens_step (decision by the last workgroup of each run);
+// k_decide's one workgroup
+constexpr int kRunStepBlock = 64;
+constexpr int kEnsStepBlock = 256;
+constexpr int kDecideBlock = 256;
 constexpr int kSetupBlock = 1024;
 constexpr int kTimedBatch = 64;
 
@@ -247,7 +253,7 @@ __device__ __forceinline__ void setup_body(const SimArrays& A, SimState* __restr
 
 // One attempt of SolveOneTimeStep (Functions_Solvers.jl:334-380) with the
 // effective normal stress of main() (:212-226), plus the step decision, for
-// workgroup blk of the nblk (kStepBlock threads each) that cover the run.
+// workgroup blk of the nblk (kBlock threads each) that cover the run.
 // The retry-or-accept decision of main() (:240-319) from the reduction `all`
 // of every workgroup's part, taken by one thread: the run state S as the
 // attempt read it.
@@ -312,12 +318,12 @@ __device__ inline void decide(const SimArrays& A, SimState* __restrict__ st, con
 // kLastDecides: the last workgroup to arrive takes the decision (after an
 // agent-scope fence); otherwise the workgroups only write their parts and
 // decide_body, in the next kernel on the stream, takes it.
-template <bool kLastDecides>
+template <bool kLastDecides, int kBlock>
 __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restrict__ st, unsigned blk, unsigned nblk) {
-    __shared__ BlockPart shp[kStepBlock / 64];
+    __shared__ BlockPart shp[kBlock / 64];
     __shared__ int s_last;
     const int64_t n = A.n;
-    const int64_t i = (int64_t)blk * kStepBlock + threadIdx.x;
+    const int64_t i = (int64_t)blk * kBlock + threadIdx.x;
     // element constants, issued before the run state is read (their latency
     // overlaps the state's)
     const int64_t ic = i < n ? i : n - 1;
@@ -438,7 +444,7 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
     BlockPart mine;
     mine.ok = ok; mine.any4 = any4; mine.found = found; mine.pad = 0;
     mine.mindt = mindt; mine.vmax = vmax; mine.vmax_next = vmax_next; mine.dtmin_next = dtmin_next;
-    const BlockPart bpart = block_reduce_part<kStepBlock>(mine, shp);
+    const BlockPart bpart = block_reduce_part<kBlock>(mine, shp);
     if (!kLastDecides) {
         if (threadIdx.x == 0) A.parts[blk] = bpart;
         return;
@@ -458,14 +464,14 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
     BlockPart all;
     all.ok = 1; all.any4 = 0; all.found = 0; all.pad = 0;
     all.mindt = INFINITY; all.vmax = -INFINITY; all.vmax_next = -INFINITY; all.dtmin_next = INFINITY;
-    for (unsigned k = threadIdx.x; k < nblk; k += kStepBlock) {
+    for (unsigned k = threadIdx.x; k < nblk; k += kBlock) {
         const volatile BlockPart* q = A.parts + k;
         BlockPart b;
         b.ok = q->ok; b.any4 = q->any4; b.found = q->found; b.pad = 0;
         b.mindt = q->mindt; b.vmax = q->vmax; b.vmax_next = q->vmax_next; b.dtmin_next = q->dtmin_next;
         fold_part(all, b);
     }
-    all = block_reduce_part<kStepBlock>(all, shp);
+    all = block_reduce_part<kBlock>(all, shp);
     if (threadIdx.x != 0) return;
     *A.arrive = 0u;
     decide(A, st, S, all);
@@ -476,14 +482,14 @@ __device__ __forceinline__ void step_body(const SimArrays& A, SimState* __restri
 // The kernel boundary orders the parts' writes before these reads, so no
 // agent-scope fence (an L2 write-back on every XCD) is needed.
 __device__ __forceinline__ void decide_body(const SimArrays& A, SimState* __restrict__ st, unsigned nblk) {
-    __shared__ BlockPart shp[kStepBlock / 64];
+    __shared__ BlockPart shp[kDecideBlock / 64];
     const SimState S = load_state(st);
     if (S.done) return;
     BlockPart all;
     all.ok = 1; all.any4 = 0; all.found = 0; all.pad = 0;
     all.mindt = INFINITY; all.vmax = -INFINITY; all.vmax_next = -INFINITY; all.dtmin_next = INFINITY;
-    for (unsigned k = threadIdx.x; k < nblk; k += kStepBlock) fold_part(all, A.parts[k]);
-    all = block_reduce_part<kStepBlock>(all, shp);
+    for (unsigned k = threadIdx.x; k < nblk; k += kDecideBlock) fold_part(all, A.parts[k]);
+    all = block_reduce_part<kDecideBlock>(all, shp);
     if (threadIdx.x != 0) return;
     decide(A, st, S, all);
 }
