@@ -113,10 +113,26 @@ def onsets(told, dt, vmax, threshold=1e-3, rearm=1e-5):
     return out
 
 
+def first_event_peak(vmax, threshold=1e-3, rearm=1e-5):
+    """Peak V of the first event: the largest max-V of the accepted steps from
+    its onset (max V >= threshold) until max V falls back below `rearm`;
+    (peak, ended) with ended False when the run stops inside the event, None
+    without an event."""
+    v = np.asarray(vmax)
+    on = np.nonzero(v >= threshold)[0]
+    if on.size == 0:
+        return None, False
+    k = int(on[0])
+    below = np.nonzero(v[k:] < rearm)[0]
+    end = k + int(below[0]) if below.size else v.size
+    return float(v[k:end].max()), bool(below.size)
+
+
 def run_window(qd, op, prob, years, stop_at_event, max_steps, chunk=2000, graph_attempts=32):
     """One run from t = 0 to `years` simulated years (or, with stop_at_event,
-    to its first event), max_steps at most, launched in chunks of accepted
-    steps replayed from the captured attempt graph."""
+    through its first event: until max V is back below 1e-5 m/s), max_steps at
+    most, launched in chunks of accepted steps replayed from the captured
+    attempt graph."""
     from qdfn import configs
     sim = qd.Simulation(op, prob, record_step=0, record_capacity=0, trace_capacity=max_steps)
     t_end = years * YEAR
@@ -133,14 +149,16 @@ def run_window(qd, op, prob, years, stop_at_event, max_steps, chunk=2000, graph_
         if stop_at_event:
             tr = sim.trace()
             ev = configs.first_event_time(tr["told"], tr["dt"], tr["vmax"])
-            if ev is not None:
+            if ev is not None and first_event_peak(tr["vmax"])[1]:
                 break
     s = sim.stats()
     tr = sim.trace()
     sim.close()
     if ev is None:
         ev = configs.first_event_time(tr["told"], tr["dt"], tr["vmax"])
-    return dict(first_event_s=ev, steps=s["steps"], sim_seconds=s["t"], wall_s=wall, rhs=s["rhs_evals"],
+    peak, ended = first_event_peak(tr["vmax"])
+    return dict(first_event_s=ev, peak_v=peak, event_ended=ended, steps=s["steps"], sim_seconds=s["t"], wall_s=wall,
+                rhs=s["rhs_evals"],
                 retries=s["retry_passes"], onsets=onsets(tr["told"], tr["dt"], tr["vmax"]), status=s["status"],
                 sim_years_per_hour=(s["t"] / YEAR) / (wall / 3600.0) if wall > 0 else None,
                 rhs_evals_per_s=s["rhs_evals"] / wall if wall > 0 else None)
@@ -152,16 +170,23 @@ def event_accuracy(qd, ops, probs, years, n_pert, max_steps):
     1 + 1e-15 xi (xi standard normal, seeds 1..n_pert: rounding-level
     perturbations of the reference's own path) and of each listed mixed run."""
     base = run_window(qd, ops["fp64_dense"], probs["fp64_dense"], years, True, max_steps)
-    log(f"[bench] event: fp64_dense {fmt_ev(base['first_event_s'])} in {base['steps']} steps, {base['wall_s']:.1f}s")
-    pert = []
+    log(f"[bench] event: fp64_dense {fmt_ev(base['first_event_s'])}, peak V {base['peak_v']}, in {base['steps']} "
+        f"steps, {base['wall_s']:.1f}s")
+    pert, pert_peak = [], []
     for seed in range(1, n_pert + 1):
         p = dict(probs["fp64_dense"])
         xi = np.random.default_rng(seed).standard_normal(p["theta_i"].size)
         p["theta_i"] = p["theta_i"] * (1.0 + 1e-15 * xi)
         r = run_window(qd, ops["fp64_dense"], p, years, True, max_steps)
         pert.append(r["first_event_s"])
-        log(f"[bench] event: fp64_dense perturbed (seed {seed}) {fmt_ev(r['first_event_s'])}")
-    out = {"window_years": years, "fp64_plain_s": base["first_event_s"], "fp64_perturbed_s": pert}
+        pert_peak.append(r["peak_v"])
+        log(f"[bench] event: fp64_dense perturbed (seed {seed}) {fmt_ev(r['first_event_s'])}, peak V {r['peak_v']}")
+    out = {"window_years": years, "fp64_plain_s": base["first_event_s"], "fp64_perturbed_s": pert,
+           "peak_v_definition": "largest max-V of the accepted steps from the first onset (1e-3 m/s) until max V "
+                                "is back below 1e-5 m/s",
+           "fp64_plain_peak_v": base["peak_v"], "fp64_perturbed_peak_v": pert_peak}
+    pv = [x for x in pert_peak + [base["peak_v"]] if x is not None]
+    out["fp64_peak_v_rel_spread"] = (max(pv) - min(pv)) / base["peak_v"] if pv and base["peak_v"] else None
     evs = [t for t in pert + [base["first_event_s"]] if t is not None]
     out["fp64_spread"] = {"min_s": min(evs) if evs else None, "max_s": max(evs) if evs else None,
                           "runs_without_event": sum(1 for t in pert + [base["first_event_s"]] if t is None),
@@ -173,7 +198,9 @@ def event_accuracy(qd, ops, probs, years, n_pert, max_steps):
             continue
         r = run_window(qd, ops[name], probs[name], years, True, max_steps)
         t, t64 = r["first_event_s"], base["first_event_s"]
-        entry = {"first_event_s": t}
+        entry = {"first_event_s": t, "peak_v": r["peak_v"],
+                 "peak_v_rel_err": abs(r["peak_v"] - base["peak_v"]) / base["peak_v"]
+                 if r["peak_v"] is not None and base["peak_v"] else None}
         if t is not None and t64 is not None:
             entry["rel_err"] = abs(t - t64) / t64
         else:
@@ -186,7 +213,7 @@ def event_accuracy(qd, ops, probs, years, n_pert, max_steps):
             (t is None and none_seen) or (t is not None and lo is not None and lo <= t <= (np.inf if none_seen else hi)))
         out[name] = entry
         log(f"[bench] event: {name} {fmt_ev(t)}; rel. error {entry['rel_err']}; within the FP64 spread "
-            f"{entry['within_fp64_perturbation_range']}")
+            f"{entry['within_fp64_perturbation_range']}; peak V rel. error {entry['peak_v_rel_err']}")
     return out, base
 
 
@@ -643,6 +670,7 @@ def main():
                                     "runs perturbed at 1e-15); the configuration's element-random friction gives "
                                     "an ill-conditioned first event (event_time: its FP64 runs perturbed at 1e-15 "
                                     "scatter by the spread shown, runs without an event included)",
+        "peak_v_rel_err": acc_patch["mixed_h"]["peak_v_rel_err"],
         "event_time_patch": acc_patch,
         "event_time": acc,
         "mixed_over_duration": {"years": full["sim_seconds"] / YEAR, "steps": full["steps"],
