@@ -1491,8 +1491,10 @@ std::vector<CompRound> component_rounding(const double* U, const double* V, int6
 // |dv_a|^2) + sum sigma_a |du_a| |dv_a| in the Frobenius norm (the first two
 // terms exactly, by the orthonormality; the last by the triangle
 // inequality), with the measured rounding of each component in its pool.
-// The trailing components move to FP16, from the smallest up, while that
-// bound stays within the slack (none when even FP32 rounding exceeds it).
+// With fp8 the trailing components first move to FP8, from the smallest up,
+// while that bound stays within the slack; then the components before them
+// move to FP16 the same way (none when even FP32 rounding exceeds it):
+// [0, r_hi) FP32, [r_hi, r_8) FP16, [r_8, r) FP8.
 void tail_tiers(const std::vector<CompRound>& c, int64_t r, double slack, bool fp8, int64_t& r_hi, int64_t& r_8) {
     double su = 0.0, sv = 0.0, sx = 0.0;
     for (int64_t a = 0; a < r; ++a) {
