@@ -93,7 +93,11 @@ int qdfn_rotverts_unitvectors(int64_t n, double* P1, double* P2, double* P3, con
  * the block's error budget (aca_rtol ||K_S,b||_F) that the ACA residual and
  * the truncation leave unused, by the measured FP32 and FP16 rounding of each
  * component.  FP64 accumulation.  Blocks that were not recompressed (rank
- * below 2, aca_rtol = 0, or imported through qdfn_hmat_create) stay FP32. */
+ * below 2, aca_rtol = 0, or imported through qdfn_hmat_create) stay FP32.
+ * The device builds (qdfn_hmat_build_rect/_tri) also compress the inadmissible
+ * blocks off the diagonal (near-field neighbours, which the reference keeps
+ * dense) where their tiered factors take fewer bytes than the dense block,
+ * within the same bound; diagonal blocks stay dense. */
 #define QDFN_PREC_MIXED 3
 /* H operators only: QDFN_PREC_MIXED with a third tier: the smallest trailing
  * components as FP8 (E4M3) vectors scaled to a largest entry in [128, 256)
@@ -293,7 +297,9 @@ int qdfn_hmat_normal_zero(qdfn_hmat_t* h);
  * matrix (diagonal of diagonal blocks zeroed) whose magnitude exceeds
  * multiple * |K_Self| of its receiver has its shear and normal entries scaled
  * by multiple * |K_Self| / |entry|; the operator's product layout is rebuilt.
- * n_pairs (optional): entries changed.  Before runs are created on h. */
+ * An inadmissible block stored low-rank (QDFN_PREC_MIXED / _MIXED8) is first
+ * expanded to its FP64 product U V.  n_pairs (optional): entries changed.
+ * Before runs are created on h. */
 int qdfn_hmat_reduce_strong_interaction(qdfn_hmat_t* h, double multiple, int64_t* n_pairs);
 /* HmatSolver_Pararllel (Functions_Hmatrix.jl:631) for both matrices.
  * loading_form=1: yS = (H_load * (0 - x)) ./ (-K_Self) (main_H :241-243),

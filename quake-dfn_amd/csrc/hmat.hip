@@ -1908,10 +1908,16 @@ void init_blocks(qdfn_hmat* h, int64_t nblk, const int64_t* range, const int64_t
 }
 
 // Compress / read out every block from full device matrices KS, KN (n x n, ld n).
+// Admissible blocks are compressed; with tail storage (QDFN_PREC_MIXED / _MIXED8)
+// so are the inadmissible blocks off the diagonal (near-field neighbours, kept
+// low-rank only where the tiered factors take fewer bytes than the dense block,
+// like any other part, and within the same bound rtol ||K_S,b||_F); diagonal
+// blocks stay dense (K_S,ii and the loading form's zero diagonal).
 static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, int64_t nblk, const int64_t* range,
                             const int64_t* adm, double rtol, double atol) {
     const int64_t n = h->n;
     init_blocks(h, nblk, range, adm);
+    const bool near_lr = tail_mode(h->prec) > 0;
     const double* full[2] = {dKS, dKN};
     // dense blocks (one packed copy)
     std::vector<DeviceBlock> lr;  // parts to compress
@@ -1922,7 +1928,9 @@ static int blocks_from_full(qdfn_hmat* h, const double* dKS, const double* dKN, 
             HBlock& b = h->blocks[q];
             const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
             const double* src = full[w] + b.r0 + b.s0 * n;
-            if (!b.adm || aca_rank_cap(m, k) < 1 || m * k > 0x7fffffffLL) {
+            const bool diag = b.s0 == b.r0 && b.s1 == b.r1;
+            const bool compress = b.adm || (near_lr && !diag);
+            if (!compress || aca_rank_cap(m, k) < 1 || m * k > 0x7fffffffLL) {
                 items.push_back({src, n, (int32_t)m, (int32_t)k, 0});
                 dense_parts.push_back({q, w});
             } else {
@@ -2240,7 +2248,9 @@ extern "C" int qdfn_hmat_normal_zero(qdfn_hmat_t* h) { return h ? h->normal_zero
 // every inadmissible block, an entry of the loading-form shear matrix (the
 // diagonal of diagonal blocks zeroed) larger in magnitude than multiple |K_Self|
 // of its receiver scales its shear and normal entries by multiple |K_Self| /
-// |entry|; then the product layout is rebuilt from the changed blocks.
+// |entry|; then the product layout is rebuilt from the changed blocks.  An
+// inadmissible block stored low-rank (tail storage) is first expanded to its
+// dense product U V (FP64), which the rule then scales.
 extern "C" int qdfn_hmat_reduce_strong_interaction(qdfn_hmat_t* h, double multiple, int64_t* n_pairs) {
     if (!h || !(multiple > 0.0)) return fail(QDFN_ERR_ARG, "qdfn_hmat_reduce_strong_interaction: bad arguments");
     if (h->users > 0)
@@ -2248,9 +2258,19 @@ extern "C" int qdfn_hmat_reduce_strong_interaction(qdfn_hmat_t* h, double multip
     int64_t pairs = 0;
     for (HBlock& b : h->blocks) {
         if (b.adm != 0) continue;
-        if (b.kind[0] != QDFN_HBLK_DENSE || b.kind[1] != QDFN_HBLK_DENSE)
-            return fail(QDFN_ERR_STATE, "qdfn_hmat_reduce_strong_interaction: an inadmissible block is not dense");
         const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
+        for (int w = 0; w < 2; ++w) {
+            if (b.kind[w] == QDFN_HBLK_DENSE) continue;
+            const int64_t r = b.rank[w];
+            std::vector<double> d(m * k, 0.0);
+            for (int64_t c = 0; c < k; ++c)
+                for (int64_t a = 0; a < r; ++a) {
+                    const double v = b.b[w][a + c * r];
+                    for (int64_t i = 0; i < m; ++i) d[i + c * m] += b.a[w][i + a * m] * v;
+                }
+            set_dense(b, w, d.data());
+            b.slack[w] = 0.0;
+        }
         const bool diag = b.s0 == b.r0 && b.s1 == b.r1;
         for (int64_t c = 0; c < k; ++c)
             for (int64_t i = 0; i < m; ++i) {

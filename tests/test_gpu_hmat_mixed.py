@@ -84,6 +84,12 @@ def test_mixed_blocks_within_tolerance(rough, qd):
             # FP32 rounding of dense blocks is outside the ACA budget: 2^-24 per entry
             allow = tol * (1 + 1e-9) if B["kind"][q] == 1 else tol + 2.0 ** -24 * np.linalg.norm(K[sl])
             assert err <= allow + 1e-9 * np.abs(K).max(), (q, err, tol)
+    # near-field (inadmissible, off-diagonal) blocks are compressed too where that saves bytes;
+    # diagonal blocks stay dense
+    rs, adm, rk = st["range_SR"], st["admissible"], H.ranks()
+    offdiag = (rs[:, 0] != rs[:, 2]) | (rs[:, 1] != rs[:, 3])
+    assert ((adm == 0) & offdiag & (rk["kind_S"] == 1)).any()
+    assert not ((adm == 0) & ~offdiag & (rk["kind_S"] == 1)).any()
     print(f"[mixed] worst block error / tolerance {worst:.3f}; 16- and 8-bit share of factor entries "
           f"{lo_entries / lr_entries:.3f}, 8-bit {lo8_entries / lr_entries:.3f}")
     assert lo_entries > 0.3 * lr_entries

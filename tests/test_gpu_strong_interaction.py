@@ -51,3 +51,55 @@ def test_hmat_reduction_matches_oracle(qd, orc):
         assert H.reduce_too_strong_interaction(mult) == 0  # nothing left above the bound
     finally:
         H.close()
+
+
+def test_hmat_reduction_expands_low_rank_near_field(qd, orc):
+    """Tail storage compresses inadmissible blocks off the diagonal too; the
+    reduction first expands such a block to its FP64 product U V (components
+    added in order) and then scales it, bit for bit as the oracle's rule on
+    the expanded blocks."""
+    from qdfn import configs
+    m = configs.rough_fault(nx=40, nz=30, h=50.0)
+    n = m["P1"].shape[0]
+    P1, P2, P3, nrm, _, _, slip = qd.RotVerts_UnitVectors(np.hstack([m["P1"], m["P2"], m["P3"]]), n, m["rake"])
+    ctr = (P1 + P2 + P3) / 3
+    st = qd.hmat_structure(ctr, P1, min_elements_to_cut=64, max_levels=40, dist_diam_ratio=1.0,
+                           arrange_point=(0.0, 0.0, 0.0))
+    p = st["perm"]
+    geo = [a[p] for a in (P1, P2, P3, m["rake"], ctr, nrm, slip)]
+    lam = 2 * m["mu"] * m["nu"] / (1 - 2 * m["nu"])
+    rs, adm = st["range_SR"], st["admissible"]
+    H = qd.HMatrix.build_tri(*geo, m["mu"], lam, rs, adm, aca_rtol=1e-6, prec=qd.PREC_MIXED)
+    try:
+        rk = H.ranks()
+        offdiag = (rs[:, 0] != rs[:, 2]) | (rs[:, 1] != rs[:, 3])
+        near_lr = (adm == 0) & offdiag & (rk["kind_S"] == 1)
+        assert near_lr.any(), "no near-field block was stored low-rank"
+        assert not ((adm == 0) & ~offdiag & (rk["kind_S"] == 1)).any()  # diagonal blocks stay dense
+        BS, BN = H.blocks(rs)
+        for B in (BS, BN):
+            for q in np.nonzero(adm == 0)[0]:
+                if B["kind"][q] == 1:
+                    U, V = B["a"][q], B["b"][q]
+                    d = np.zeros((U.shape[0], V.shape[1]))
+                    for a in range(U.shape[1]):
+                        d += np.outer(U[:, a], V[a, :])
+                    B["a"][q], B["b"][q], B["kind"][q], B["rank"][q] = d, None, 0, 0
+        mult = 0.15
+        S2, N2, pairs = orc.reduce_too_strong_interaction_hmat(mult, adm, BS, BN)
+        cnt = H.reduce_too_strong_interaction(mult)
+        assert cnt == len(pairs) > 0
+        CS, CN = H.blocks(rs)
+        for q in np.nonzero(adm == 0)[0]:
+            for got, ref in ((CS, S2), (CN, N2)):
+                assert got["kind"][q] == 0
+                np.testing.assert_array_equal(got["a"][q], ref["a"][q])
+        SS, SN = H.blocks(rs, stored=True)
+        x = np.random.default_rng(5).standard_normal(n)
+        yS, yN = H.matvec(x, loading_form=False)
+        rS, rN = orc.hmat_matvec(SS, x), orc.hmat_matvec(SN, x)
+        scale = max(np.abs(rS).max(), np.abs(rN).max())
+        assert np.abs(yS - rS).max() <= 1e-12 * scale
+        assert np.abs(yN - rN).max() <= 1e-12 * scale
+    finally:
+        H.close()
