@@ -1659,11 +1659,12 @@ double aca_share(bool fp16_tail) {
         return v >= 0.05 && v <= 0.95 ? v : 0.0;
     }();
     // FP16 tails: a smaller share leaves more of the budget to round
-    // components to FP16 (20,000 triangles, crit 1.5: 0.5 / 0.3 / 0.2 / 0.1 ->
-    // 589.8 / 558.6 / 546.1 / 539.1 MB per product, 7,441 / 7,820 / 7,921 /
-    // 7,737 RHS evals/s); FP32 storage gains nothing from it (more parts
-    // reach the rank ceiling and are stored dense)
-    return share > 0.0 ? share : (fp16_tail ? 0.2 : 0.5);
+    // components to FP16 (20,000 triangles, crit 1.5, near-field blocks
+    // compressed: 0.4 / 0.3 / 0.2 / 0.15 / 0.12 / 0.1 -> 530.3 / 515.0 / 501.7 /
+    // 495.6 / 492.3 / 494.1 MB per product, 8,151 / 8,451 / 8,629 / 8,762 /
+    // 8,780 / 8,552 RHS evals/s); FP32 storage gains nothing from it (more
+    // parts reach the rank ceiling and are stored dense)
+    return share > 0.0 ? share : (fp16_tail ? 0.15 : 0.5);
 }
 
 int64_t aca_rank_cap(int64_t m, int64_t k, bool fp16_tail) {
