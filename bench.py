@@ -45,12 +45,13 @@ YEAR = 3.15576e7
 CHECKPOINT_EVERY = 50  # adjoint workload: forward checkpoints, segments recomputed in the backward sweep
 
 
-def measured_traffic():
-    """HBM bytes per launch of the headline phase-2 kernel from the committed
+def measured_traffic(key="traffic_bytes_per_launch"):
+    """HBM bytes per launch of a headline product kernel (phase 2 by default,
+    key traffic_phase1_bytes_per_launch for phase 1) from the committed
     rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, calibrated); None if absent."""
     try:
         for line in open(PROFILE):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
@@ -601,6 +602,8 @@ def main():
     hb = rf["hinfo"]["mixed_h"]["bytes_by_part"]
     t2 = mixed["coupling_ms"] * 1e-3  # phase-2 kernel, HIP events on the run's stream
     achieved = hb["phase2_entries"] / t2 / 1e9
+    t1 = rf["phases"]["mixed_h"][0] * 1e-6  # phase-1 kernel (us), HIP events around it (standalone product)
+    achieved1 = hb["phase1_entries"] / t1 / 1e9
     # accuracy: fixed-duration runs, independent of --steps
     ev_ops = {k: rf["ops"][k] for k in rf["ops"] if k in ("fp64_dense", "mixed_h", "fp32_h", "mixed_h_crit1")}
     acc, base = event_accuracy(qdfn, ev_ops, rf["probs"], args.event_years, args.perturbations, 200000)
@@ -691,7 +694,11 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "kernel": "k_hunit<float, fp16_t, false> (H product phase 2)",
                      "bytes": "stored panel entries only", "bytes_per_launch": hb["phase2_entries"],
-                     "index_and_gather_bytes": hb["phase2_index"] + hb["phase2_gather"]},
+                     "index_and_gather_bytes": hb["phase2_index"] + hb["phase2_gather"],
+                     "phase1": {"kernel": "k_vpanel_mixed<false> (H product phase 1)", "us": round(t1 * 1e6, 2),
+                                "achieved": round(achieved1, 1), "frac": round(achieved1 / HBM_PEAK_GBS, 4),
+                                "bytes_per_launch": hb["phase1_entries"], "bytes": "stored V entries only",
+                                "traffic": measured_traffic("traffic_phase1_bytes_per_launch")}},
         "cpu_baseline": cpu,
     }
     for o in rf["ops"].values():
