@@ -8,9 +8,9 @@
 // ToLoading, Functions_Hmatrix.jl:459-477) divided by -K_Self.
 //
 // Build on the device: the full K_S and K_N are assembled in HBM (FP64), dense
-// blocks are read out of them and admissible blocks are compressed by an
-// adaptive cross approximation with full pivoting, one workgroup per block and
-// matrix.  The residual is held explicitly, so the stopping test is exact:
+// blocks are read out of them and admissible blocks (with tail storage also
+// the near-field blocks off the diagonal) are compressed by an adaptive cross
+// approximation with full pivoting, one workgroup per block and matrix.  The residual is held explicitly, so the stopping test is exact:
 // ||residual||_F <= rtol ||K_S,b||_F or max |residual| <= atol, K_S,b the
 // block's shear entries (for the normal part too: both are stresses per unit
 // slip, and the normal entries of a near-planar fault are far smaller, so a
@@ -27,15 +27,17 @@
 //            (contiguous) range of x, stored as column-major sub-panels;
 //   phase 2: y = panel * [x; t] per row group, the right-hand side gathered
 //            through an index list.
-// Phase 1 runs one wavefront per task (k_vpanel), phase 2 one workgroup per
-// row group and matrix (k_hgroup).  Both stream contiguous column runs with
-// wide loads and FP64 accumulation; sums are combined over fixed shuffle trees
-// and (phase 2) through LDS in wavefront order, with no atomics, so results
-// are the same run to run.
+// Phase 1 runs one wavefront per task (k_vpanel*), phase 2 one workgroup per
+// unit, a column range of one panel of one row group and matrix (k_hunit),
+// whose row sums go to the unit's slot; the slots are added in slot order
+// (k_hsum, or the step kernel).  Both phases stream contiguous column runs
+// with wide loads and FP64 accumulation; sums are combined over fixed shuffle
+// trees and (phase 2) through LDS in wavefront order, with no atomics, so
+// results are the same run to run.
 // Storage: FP64, FP32, FP32 with BF16 low-rank factors (QDFN_PREC_BF16_FAR),
-// or FP32 with FP16 trailing components (QDFN_PREC_MIXED); a row group with
-// 16-bit columns has a second panel in the 16-bit pool, streamed by some of
-// its workgroup's wavefronts.
+// FP32 with FP16 trailing components (QDFN_PREC_MIXED), and with FP8 ones as
+// well (QDFN_PREC_MIXED8); each tier's columns form their own panels in their
+// own pool.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
