@@ -2,10 +2,11 @@
 // (scripts/Functions_RSFDFN3DMain_D.jl:2-324), each with its own dense FP32
 // operator, friction, adaptive Dt and retries, advanced together.  One
 // attempt of every run is three batched launches:
-//   k_ens_coupling<RG>  split coupling products of every run that needs one;
-//                       runs sharing an operator (same geometry) are read as
-//                       one multi-right-hand-side product, so the matrices
-//                       stream once per RG runs
+//   k_ens_coupling      split coupling products of every run that needs one
+//                       (all groups in one launch); runs sharing an operator
+//                       (same geometry) are read as one multi-right-hand-side
+//                       product of RG <= 4 runs, so the matrices stream once
+//                       per RG runs
 //   k_ens_finish        fixed-order split sums -> TL, SigProd of each run
 //   k_ens_step          the per-element attempt and per-run decision of main()
 // and k_ens_count counts the runs still going.  Attempts are captured into
