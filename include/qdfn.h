@@ -94,11 +94,11 @@ int qdfn_rotverts_unitvectors(int64_t n, double* P1, double* P2, double* P3, con
  * the truncation leave unused, by the measured FP32 and FP16 rounding of each
  * component.  FP64 accumulation.  Blocks that were not recompressed (rank
  * below 2, aca_rtol = 0, or imported through qdfn_hmat_create) stay FP32.
- * The full device builds (qdfn_hmat_build_rect/_tri; not the sampled ones)
- * also compress the inadmissible blocks off the diagonal (near-field
- * neighbours, which the reference keeps dense) where their tiered factors take
- * fewer bytes than the dense block, within the same bound; diagonal blocks
- * stay dense. */
+ * The device builds (qdfn_hmat_build_*, the sampled ones too, which evaluate
+ * such blocks in full) also compress the inadmissible blocks off the diagonal
+ * (near-field neighbours, which the reference keeps dense) where their tiered
+ * factors take fewer bytes than the dense block, within the same bound;
+ * diagonal blocks stay dense. */
 #define QDFN_PREC_MIXED 3
 /* H operators only: QDFN_PREC_MIXED with a third tier: the smallest trailing
  * components as FP8 (E4M3) vectors scaled to a largest entry in [128, 256)

@@ -710,14 +710,21 @@ int build_sampled(qdfn_hmat* h, const EntrySource& es, int64_t nblk, const int64
     const auto t0 = std::chrono::steady_clock::now();
     init_blocks(h, nblk, range, adm);
     BuildStats bs;
-    std::vector<int64_t> dense, lowr;
+    // with tail storage the inadmissible blocks off the diagonal (near field)
+    // are evaluated in full and compressed like the full build's
+    const bool near_lr = tail_mode(h->prec) > 0;
+    std::vector<int64_t> dense, lowr, near;
     for (int64_t q = 0; q < nblk; ++q) {
         const HBlock& b = h->blocks[q];
         const int64_t m = b.r1 - b.r0 + 1, k = b.s1 - b.s0 + 1;
         if (!b.adm || aca_rank_cap(m, k) < 1) {
             if (m * k > 0x7fffffffLL)
                 return fail(QDFN_ERR_ARG, "sampled H build: an inadmissible block exceeds 2^31 - 1 entries");
-            dense.push_back(q);
+            const bool diag = b.s0 == b.r0 && b.s1 == b.r1;
+            if (near_lr && !b.adm && !diag && aca_rank_cap(m, k) >= 1 && m * k <= kFallbackMaxEntries)
+                near.push_back(q);
+            else
+                dense.push_back(q);
         } else {
             lowr.push_back(q);  // sampled whatever its size
         }
@@ -728,9 +735,16 @@ int build_sampled(qdfn_hmat* h, const EntrySource& es, int64_t nblk, const int64
         rc = dense_blocks(es, h, dense, bs);
     }
     if (rc) return rc;
-    vlog("inadmissible blocks: %zu evaluated in full; %zu admissible", dense.size(), lowr.size());
-    // shear parts: the threshold scale is each block's running norm estimate
     std::vector<double> ref(nblk, -1.0), estS(nblk, 0.0), estN(nblk, 0.0), shear_norm(nblk, 0.0);
+    if (!near.empty()) {
+        const int64_t fb0 = bs.fallback;  // not a fallback of the sampling
+        rc = fallback_blocks(es, h, near, true, rtol, atol, shear_norm, bs);
+        if (rc) return rc;
+        bs.fallback = fb0;
+    }
+    vlog("inadmissible blocks: %zu evaluated in full and stored dense, %zu near-field ones compressed; %zu "
+         "admissible", dense.size(), near.size(), lowr.size());
+    // shear parts: the threshold scale is each block's running norm estimate
     std::vector<int64_t> fbS, fbN;
     rc = sampled_parts(es, h, lowr, 0, ref, rtol, atol, estS, fbS, bs);
     if (rc) return rc;

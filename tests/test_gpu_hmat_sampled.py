@@ -185,3 +185,46 @@ def test_main_h_on_sampled_operator(orc, qd):
     assert abs(first[1] - first[0]) <= 1e-6 * first[0], first
     Hf.close()
     Hs.close()
+
+
+def test_tri_mixed_near_field_as_full_build(orc, qd):
+    """Tail storage (QDFN_PREC_MIXED): both builds compress the inadmissible
+    blocks off the diagonal from the same fully evaluated entries, so every
+    inadmissible block (dense or near-field low-rank) is bitwise the full
+    build's; diagonal blocks stay dense; the sampled operator stays within
+    its bound of the assembled stiffness."""
+    rtol = 1e-6
+    geo, mu, lam, st, KS, KN = _tri_case(orc, qd, 40, 30, 64, oracle_entries=False)
+    rs, ad = st["range_SR"], st["admissible"]
+    Hf = qd.HMatrix.build_tri(*geo, mu, lam, rs, ad, aca_rtol=rtol, prec=qd.PREC_MIXED)
+    Hs = qd.HMatrix.build_tri(*geo, mu, lam, rs, ad, aca_rtol=rtol, prec=qd.PREC_MIXED, sampled=True)
+    try:
+        offdiag = (rs[:, 0] != rs[:, 2]) | (rs[:, 1] != rs[:, 3])
+        rkf, rks = Hf.ranks(), Hs.ranks()
+        assert ((ad == 0) & offdiag & (rks["kind_S"] == 1)).any()
+        assert not ((ad == 0) & ~offdiag & (rks["kind_S"] == 1)).any()
+        BF, BS = Hf.blocks(rs), Hs.blocks(rs)
+        for w in range(2):
+            for q in np.nonzero(ad == 0)[0]:
+                assert BF[w]["kind"][q] == BS[w]["kind"][q]
+                np.testing.assert_array_equal(BF[w]["a"][q], BS[w]["a"][q])
+                if BF[w]["kind"][q] == 1 and BF[w]["rank"][q] > 0:
+                    np.testing.assert_array_equal(BF[w]["b"][q], BS[w]["b"][q])
+        SS, SN = Hs.blocks(rs, stored=True)
+        worst = 0.0
+        for q in np.nonzero(ad == 0)[0]:
+            r = rs[q]
+            sl = (slice(r[2] - 1, r[3]), slice(r[0] - 1, r[1]))
+            tol = rtol * np.linalg.norm(KS[sl])
+            for B, K in ((SS, KS), (SN, KN)):
+                approx = B["a"][q] if B["kind"][q] == 0 else (
+                    B["a"][q] @ B["b"][q] if B["rank"][q] > 0 else np.zeros(K[sl].shape))
+                err = np.linalg.norm(approx - K[sl])
+                # dense blocks: FP32 rounding (2^-24 per entry) on top
+                allow = tol * (1 + 1e-9) if B["kind"][q] == 1 else tol + 2.0 ** -24 * np.linalg.norm(K[sl])
+                assert err <= allow + 1e-9 * np.abs(K).max(), (q, err, tol)
+                worst = max(worst, err / tol if tol > 0 else 0.0)
+        print(f"[sampled mixed] worst near-field block error / tolerance {worst:.3f}")
+    finally:
+        Hf.close()
+        Hs.close()
