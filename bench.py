@@ -648,7 +648,9 @@ def main():
         "data": "synthetic (seeded rough fault, back-slip loading)",
         "config": {"workload": "rough_fault_20000", "n_elements": rf["n"],
                    "operator": "H-matrix: FP32 dense blocks and leading low-rank components, FP16 trailing "
-                               "components within each block's ACA budget, FP64 accumulation",
+                               "components within each block's ACA budget (admissible blocks and the "
+                               "near-field blocks off the diagonal compressed, diagonal blocks dense), FP64 "
+                               "accumulation",
                    "leaf": 64, "admissibility_eta": 2, "dist_diam_ratio": args.crit, "aca_tol": args.aca_tol,
                    "rank_cap": 1024, "plain": "dense FP64", "steps_timed": args.steps},
         "fp64_plain": {"rhs_evals_per_s": round(plain["evals_per_s"], 2),
