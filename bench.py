@@ -590,8 +590,8 @@ def main():
     ap.add_argument("--bf16-far", action="store_true", help="also build and run BF16 low-rank factors")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU layout is "
-              "design-only: see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU layout, "
+              "receiver-row shards with an all-gather of the slip per step, is a design only)", file=sys.stderr)
         sys.exit(2)
 
     import qdfn
