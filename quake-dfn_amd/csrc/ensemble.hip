@@ -230,14 +230,18 @@ __global__ __launch_bounds__(kSetupBlock) void k_ens_setup(const SimArrays* __re
     setup_body(arrays[blockIdx.x], states[blockIdx.x]);
 }
 
-// number of runs still going -> *out
+// number of runs still going -> *out, and which ones -> mask[k]
 __global__ __launch_bounds__(256) void k_ens_count(SimState* const* __restrict__ states, int32_t m,
-                                                   int32_t* __restrict__ out) {
+                                                   int32_t* __restrict__ out, int32_t* __restrict__ mask) {
     __shared__ int sh[4];
     int c = 0;
     // the flags were written by earlier kernels: volatile (vector) loads, never
     // the scalar cache, which a one-workgroup kernel can find holding an old line
-    for (int k = threadIdx.x; k < m; k += 256) c += ctrl_load(&states[k]->done) ? 0 : 1;
+    for (int k = threadIdx.x; k < m; k += 256) {
+        const int going = ctrl_load(&states[k]->done) ? 0 : 1;
+        mask[k] = going;
+        c += going;
+    }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
     __syncthreads();
@@ -253,11 +257,12 @@ struct qdfn_ensemble {
     std::vector<qdfn_sim*> sims;
     hipStream_t stream = nullptr;
     int32_t max_rhs = 0;
-    DevBuf arrays, states, nblk, insts, groups, fins, steps, parts, live;
+    DevBuf arrays, states, nblk, insts, groups, fins, steps, parts, live, livemask;
     DevBuf tiles[kEnsMaxR];
     int64_t ntiles[kEnsMaxR] = {0, 0, 0, 0};
     int64_t ngroups = 0, nfins = 0, nsteps = 0;
     int64_t bytes_per_sweep = 0;  // matrix bytes when every run needs its coupling
+    int64_t listed = 0;           // runs in the current work lists (all, or those still going)
     GraphSlot graph;
     ~qdfn_ensemble() {
         graph.reset();
@@ -267,16 +272,19 @@ struct qdfn_ensemble {
 
 namespace {
 
-// Groups of at most rmax runs per operator and their tiles.
-int build_groups(qdfn_ensemble* e, int rmax) {
+// Groups of at most rmax of the listed runs per operator and their tiles
+// (runs: every run when null).
+int build_groups(qdfn_ensemble* e, int rmax, const std::vector<int32_t>* runs = nullptr) {
     std::vector<EnsGroup> groups;
     std::vector<EnsTile> tiles[kEnsMaxR];
     std::map<const qdfn_dense*, std::vector<int32_t>> by_op;
     std::vector<const qdfn_dense*> order;
-    for (size_t k = 0; k < e->sims.size(); ++k) {
+    const size_t nr = runs ? runs->size() : e->sims.size();
+    for (size_t i = 0; i < nr; ++i) {
+        const int32_t k = runs ? (*runs)[i] : (int32_t)i;
         const qdfn_dense* op = e->sims[k]->op;
         if (!by_op.count(op)) order.push_back(op);
-        by_op[op].push_back((int32_t)k);
+        by_op[op].push_back(k);
     }
     int64_t bytes = 0;
     for (const qdfn_dense* op : order) {
@@ -309,8 +317,29 @@ int build_groups(qdfn_ensemble* e, int rmax) {
                                hipMemcpyHostToDevice));
     }
     e->ngroups = (int64_t)groups.size();
-    e->bytes_per_sweep = bytes;
+    if (!runs) e->bytes_per_sweep = bytes;
     e->max_rhs = rmax;
+    return QDFN_OK;
+}
+
+// The finish and step blocks of the listed runs (every run when null).
+int build_blocks(qdfn_ensemble* e, const std::vector<int32_t>* runs = nullptr) {
+    std::vector<EnsBlk> fins, steps;
+    const size_t nr = runs ? runs->size() : e->sims.size();
+    for (size_t i = 0; i < nr; ++i) {
+        const int32_t k = runs ? (*runs)[i] : (int32_t)i;
+        const int64_t n = e->sims[k]->P.n;
+        for (int64_t b = 0; b < (n + 63) / 64; ++b) fins.push_back(EnsBlk{k, (int32_t)b});
+        for (int64_t b = 0; b < (n + kEnsStepBlock - 1) / kEnsStepBlock; ++b) steps.push_back(EnsBlk{k, (int32_t)b});
+    }
+    e->nfins = (int64_t)fins.size();
+    e->nsteps = (int64_t)steps.size();
+    QDFN_HIP(e->fins.alloc(sizeof(EnsBlk) * std::max<size_t>(fins.size(), 1)));
+    QDFN_HIP(e->steps.alloc(sizeof(EnsBlk) * std::max<size_t>(steps.size(), 1)));
+    if (!fins.empty()) QDFN_HIP(hipMemcpy(e->fins.p, fins.data(), sizeof(EnsBlk) * fins.size(), hipMemcpyHostToDevice));
+    if (!steps.empty())
+        QDFN_HIP(hipMemcpy(e->steps.p, steps.data(), sizeof(EnsBlk) * steps.size(), hipMemcpyHostToDevice));
+    e->listed = (int64_t)nr;
     return QDFN_OK;
 }
 
@@ -409,7 +438,6 @@ extern "C" qdfn_ensemble_t* qdfn_ensemble_create(int64_t m, qdfn_sim_t* const* s
     std::vector<SimState*> states((size_t)m);
     std::vector<uint32_t> nblk((size_t)m);
     std::vector<EnsInst> insts((size_t)m);
-    std::vector<EnsBlk> fins, steps;
     int64_t part_total = 0;
     for (int64_t k = 0; k < m; ++k) part_total += 2 * (int64_t)sims[k]->op->splits * sims[k]->op->ld;
     if (e->parts.alloc(sizeof(double) * part_total))
@@ -439,23 +467,17 @@ extern "C" qdfn_ensemble_t* qdfn_ensemble_create(int64_t m, qdfn_sim_t* const* s
         I.ld = op->ld;
         I.splits = op->splits;
         I.planar = op->normal_zero;
-        for (int64_t b = 0; b < (n + 63) / 64; ++b) fins.push_back(EnsBlk{(int32_t)k, (int32_t)b});
-        for (uint32_t b = 0; b < nblk[k]; ++b) steps.push_back(EnsBlk{(int32_t)k, (int32_t)b});
     }
-    e->nfins = (int64_t)fins.size();
-    e->nsteps = (int64_t)steps.size();
     if (e->arrays.alloc(sizeof(SimArrays) * m) || e->states.alloc(sizeof(SimState*) * m) ||
         e->nblk.alloc(sizeof(uint32_t) * m) || e->insts.alloc(sizeof(EnsInst) * m) ||
-        e->fins.alloc(sizeof(EnsBlk) * fins.size()) || e->steps.alloc(sizeof(EnsBlk) * steps.size()) ||
-        e->live.alloc(sizeof(int32_t)))
+        e->live.alloc(sizeof(int32_t)) || e->livemask.alloc(sizeof(int32_t) * m))
         return bail(QDFN_ERR_NOMEM, "qdfn_ensemble_create: tables");
     if (hipMemcpy(e->arrays.p, arrays.data(), sizeof(SimArrays) * m, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(e->states.p, states.data(), sizeof(SimState*) * m, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(e->nblk.p, nblk.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(e->insts.p, insts.data(), sizeof(EnsInst) * m, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(e->fins.p, fins.data(), sizeof(EnsBlk) * fins.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(e->steps.p, steps.data(), sizeof(EnsBlk) * steps.size(), hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(e->insts.p, insts.data(), sizeof(EnsInst) * m, hipMemcpyHostToDevice) != hipSuccess)
         return bail(QDFN_ERR_HIP, "qdfn_ensemble_create: table upload");
+    if (build_blocks(e)) return bail(QDFN_OK, nullptr);
     int32_t rmax = max_rhs;
     if (rmax == 0 && autotune(e, &rmax)) return bail(QDFN_OK, nullptr);
     if (build_groups(e, rmax)) return bail(QDFN_OK, nullptr);
@@ -489,7 +511,8 @@ extern "C" int qdfn_ensemble_run(qdfn_ensemble_t* e, int64_t n_steps, double t_e
     QDFN_HIP(hipGetLastError());
     const int K = std::max(1, graph_attempts > 0 ? graph_attempts : 16);
     GraphSlot& gs = e->graph;
-    if (!gs.exec || gs.attempts != K) {
+    // K attempts of the listed runs, then the count of the runs still going
+    auto capture = [&]() -> int {
         gs.reset();
         QDFN_HIP(hipStreamSynchronize(e->stream));
         QDFN_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
@@ -497,7 +520,7 @@ extern "C" int qdfn_ensemble_run(qdfn_ensemble_t* e, int64_t n_steps, double t_e
         for (int k = 0; k < K && rc == QDFN_OK; ++k) rc = enqueue_attempt(e);
         if (rc == QDFN_OK)
             hipLaunchKernelGGL(k_ens_count, dim3(1), dim3(256), 0, e->stream, e->states.as<SimState*>(), (int32_t)m,
-                               e->live.as<int32_t>());
+                               e->live.as<int32_t>(), e->livemask.as<int32_t>());
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(e->stream, &g);
         if (rc != QDFN_OK) return rc;
@@ -505,12 +528,39 @@ extern "C" int qdfn_ensemble_run(qdfn_ensemble_t* e, int64_t n_steps, double t_e
         gs.graph = g;
         QDFN_HIP(hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0));
         gs.attempts = K;
+        return QDFN_OK;
+    };
+    // every run is listed at the start of a call (an earlier call may have
+    // left only the runs that were still going then)
+    if (e->listed != m) {
+        int rc = build_groups(e, e->max_rhs);
+        if (!rc) rc = build_blocks(e);
+        if (rc) return rc;
+        gs.reset();
     }
+    if (!gs.exec || gs.attempts != K) {
+        const int rc = capture();
+        if (rc) return rc;
+    }
+    // when half the listed runs have finished, the work lists are rebuilt
+    // from those still going (a finished run's workgroups return at once, but
+    // they are still dispatched); each run's own arithmetic is unchanged
     int32_t live = 1;
+    std::vector<int32_t> mask((size_t)m);
     while (live > 0) {
         QDFN_HIP(hipGraphLaunch(gs.exec, e->stream));
         QDFN_HIP(hipMemcpyAsync(&live, e->live.p, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
         QDFN_HIP(hipStreamSynchronize(e->stream));
+        if (live > 0 && 2 * (int64_t)live <= e->listed) {
+            QDFN_HIP(hipMemcpy(mask.data(), e->livemask.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+            std::vector<int32_t> going;
+            for (int64_t k = 0; k < m; ++k)
+                if (mask[k]) going.push_back((int32_t)k);
+            int rc = build_groups(e, e->max_rhs, &going);
+            if (!rc) rc = build_blocks(e, &going);
+            if (!rc) rc = capture();
+            if (rc) return rc;
+        }
     }
     for (qdfn_sim* s : e->sims) {
         QDFN_HIP(hipMemcpy(&s->h_st, s->d_st, sizeof(SimState), hipMemcpyDeviceToHost));

@@ -83,5 +83,11 @@ def test_ensemble_time_limit(qd, members):
             # the step before the last started short of t_end
             if st["steps"] >= 2 and st["t"] >= t_end:
                 assert r["trace"]["told"][-1] < t_end
+        # the runs reach t_end after different step counts, so the work lists
+        # are rebuilt from the runs still going before the last ones finish;
+        # every run is still bitwise its standalone run
+        assert len({r["stats"]["steps"] for r in res}) > 1
+        for k, (s, prob) in enumerate(zip(e.sims, probs)):
+            _same(res[k], _standalone(qd, s.op, prob, 5000, t_end=t_end))
     finally:
         e.close()
