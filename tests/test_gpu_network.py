@@ -9,6 +9,8 @@ scripts/Functions_RSFDFN3DMain_D.jl:2-324 at its full size.
   state): its first event within 5e-5 of the FP64 run's (measured on MI355X:
   2.85e-5).  FP32 entries err by 2^-24, which the loop turns into this shift;
   a 1e-6 bar would need more than FP32 storage.
+* The same bar for a mixed-storage H operator of the network (FP32 + FP16
+  tails, ACA 1e-6, DistDiamRatioCrit 1.0).
 """
 import numpy as np
 import pytest
@@ -63,3 +65,31 @@ def test_network_event_time_tolerance(net, qd):
           f"FP32 rel. {abs(t32 - t64) / t64:.2e}, perturbation rel. {abs(tp - t64) / t64:.2e}")
     assert abs(tp - t64) <= 1e-8 * t64
     assert abs(t32 - t64) <= 5e-5 * t64
+
+
+def test_network_mixed_h_event_time(net, qd):
+    """The network on a mixed-storage H operator (DistDiamRatioCrit 1.0, the
+    reference default; leaves of 64; ACA 1e-6): its first event within the
+    same 5e-5 of the FP64 dense run's (measured on MI355X: 2.4e-5), at about a
+    tenth of the FP32 dense operator's bytes per product."""
+    from qdfn import configs
+    cfg = configs.fault_network()
+    seg = cfg["seg"]
+    st = qd.hmat_structure(seg[:, :3], seg[:, :3], min_elements_to_cut=64, max_levels=40, dist_diam_ratio=1.0,
+                           arrange_point=(0.0, 0.0, 0.0))
+    p = st["perm"]
+    segs = np.ascontiguousarray(seg[p])
+    H = qd.HMatrix.build_rect(segs, cfg["mu"], cfg["nu"], st["range_SR"], st["admissible"], aca_rtol=1e-6,
+                              prec=qd.PREC_MIXED)
+    try:
+        assert H.info()["bytes_per_product"] < 0.15 * 2 * 4 * seg.shape[0] ** 2
+        ks, _ = net["ops"][qd.PREC_FP64].kself()
+        # the same problem in the cluster tree's element order
+        prob = configs.problem_from_segments(dict(cfg, seg=segs), ks[p], net["KS"].sum(1)[p], net["KN"].sum(1)[p])
+        th = _onset(qd, H, prob)
+    finally:
+        H.close()
+    t64 = _onset(qd, net["ops"][qd.PREC_FP64], net["prob"])
+    assert th is not None and t64 is not None
+    print(f"[network] first event FP64 dense {t64:.9e} s, mixed H {th:.9e} s: rel. {abs(th - t64) / t64:.2e}")
+    assert abs(th - t64) <= 5e-5 * t64
